@@ -1,0 +1,250 @@
+/*
+ * sylver_amd.h -- entry points of the MI355X numeric factorization beyond the
+ * reference's public header.
+ *
+ * Three groups:
+ *  1. The internal numeric-tree seam of the reference (the C ABI that the
+ *     reference's Fortran factor_core and solve call into), with the same
+ *     names and argument meaning; these are where the device path plugs in.
+ *  2. Library-owned extension knobs (precision tier, refinement) and
+ *     statistics, kept out of sylver_options_t so its ABI is unchanged.
+ *  3. Read-back of the symbolic structures (host copies and the device
+ *     resident copies) and the batched dense front factorization, used by
+ *     the parity tests and the benchmark.
+ * All functions are plain C: pointers, sizes, no exceptions.
+ */
+#ifndef SYLVER_AMD_EXT_H
+#define SYLVER_AMD_EXT_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "sylver.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- flag codes (src/sylver_datatypes_mod.F90:14-45) ---- */
+enum {
+   SYLVER_SUCCESS = 0,
+   SYLVER_ERROR_CALL_SEQUENCE = -1,
+   SYLVER_ERROR_A_N_OOR = -2,
+   SYLVER_ERROR_A_PTR = -3,
+   SYLVER_ERROR_A_ALL_OOR = -4,
+   SYLVER_ERROR_SINGULAR = -5,
+   SYLVER_ERROR_NOT_POS_DEF = -6,
+   SYLVER_ERROR_PTR_ROW = -7,
+   SYLVER_ERROR_ORDER = -8,
+   SYLVER_ERROR_VAL = -9,
+   SYLVER_ERROR_X_SIZE = -10,
+   SYLVER_ERROR_JOB_OOR = -11,
+   SYLVER_ERROR_NOT_LLT = -13,
+   SYLVER_ERROR_NOT_LDLT = -14,
+   SYLVER_ERROR_NO_SAVED_SCALING = -15,
+   SYLVER_ERROR_ALLOCATION = -50,
+   SYLVER_ERROR_CUDA_UNKNOWN = -51, /* HIP runtime failure (or no HIP device) */
+   SYLVER_ERROR_CUBLAS_UNKNOWN = -52,
+   SYLVER_ERROR_UNIMPLEMENTED = -98,
+   SYLVER_ERROR_UNKNOWN = -99,
+   SYLVER_WARNING_IDX_OOR = 1,
+   SYLVER_WARNING_DUP_IDX = 2,
+   SYLVER_WARNING_DUP_AND_OOR = 3,
+   SYLVER_WARNING_MISSING_DIAGONAL = 4,
+   SYLVER_WARNING_MISS_DIAG_OORDUP = 5,
+   SYLVER_WARNING_ANAL_SINGULAR = 6,
+   SYLVER_WARNING_FACT_SINGULAR = 7,
+   SYLVER_WARNING_MATCH_ORD_NO_SCALE = 8
+};
+
+/* ---- 1. internal numeric-tree seam ----
+ * Reference: src/sylver_ciface.hxx:39-55 (options_c) and :73-86 (inform_c);
+ * Fortran mirror src/sylver_ciface_mod.F90:9-32. */
+typedef struct {
+   int print_level;
+   bool action;
+   double small;
+   double u;
+   double multiplier;
+   long small_subtree_threshold;
+   int nb;
+   int pivot_method;
+   int failed_pivot_method;
+   int cpu_topology;
+} sylver_options_c;
+
+typedef struct {
+   int flag;
+   int num_delay;
+   int num_neg;
+   int num_two;
+   int num_zero;
+   int maxfront;
+   int not_first_pass;
+   int not_second_pass;
+} sylver_inform_c;
+
+/* Reference: src/SymbolicTree.cxx:110-137. Builds the symbolic tree from the
+ * analyse outputs (1-based arrays, kept by pointer: they must outlive the
+ * tree). Subtree partitioning is a CPU-runtime concept of the reference; on
+ * this path every node runs on the device, so nsubtrees may be 0. */
+void *spldlt_create_symbolic_tree(void *akeep, int n, int nnodes,
+                                  int const *sptr, int const *sparent,
+                                  long const *rptr, int const *rlist,
+                                  long const *nptr, long const *nlist,
+                                  int nsubtrees, int const *subtrees,
+                                  int const *small, int const *contrib_dest,
+                                  int const *exec_loc);
+void spldlt_destroy_symbolic_tree(void *symbolic_tree);
+
+/* Reference: src/NumericTreePosdef.cxx:19-35 and src/NumericTree.cxx:27-44.
+ * Runs the whole numeric factorization on the device; returns the numeric
+ * tree, or NULL with inform->flag set. aval are the values of the (cleaned)
+ * input matrix addressed by nlist; scaling may be NULL. child_contrib is
+ * unused (no CPU subtrees on this path). */
+void *spldlt_create_numeric_tree_posdef_dbl(void *fkeep, void *symbolic_tree,
+                                            double *aval, double const *scaling,
+                                            void **child_contrib,
+                                            sylver_options_c *options,
+                                            sylver_inform_c *inform);
+void *spldlt_create_numeric_tree_dbl(bool posdef, void *fkeep,
+                                     void *symbolic_tree, double *aval,
+                                     double const *scaling, void **child_contrib,
+                                     sylver_options_c *options,
+                                     sylver_inform_c *inform);
+void spldlt_destroy_numeric_tree_posdef_dbl(void *numeric_tree);
+void spldlt_destroy_numeric_tree_dbl(bool posdef, void *numeric_tree);
+
+/* Reference: src/NumericTreePosdef.cxx:46-83, src/NumericTree.cxx:56-136.
+ * x is in pivot order (already permuted), column-major ldx x nrhs, host
+ * memory. Return a flag. */
+int spldlt_tree_solve_fwd_posdef_dbl(void const *numeric_tree, int nrhs, double *x, int ldx);
+int spldlt_tree_solve_bwd_posdef_dbl(void const *numeric_tree, int nrhs, double *x, int ldx);
+int spldlt_tree_solve_fwd_dbl(bool posdef, void const *numeric_tree, int nrhs, double *x, int ldx);
+int spldlt_tree_solve_bwd_dbl(bool posdef, void const *numeric_tree, int nrhs, double *x, int ldx);
+int spldlt_tree_solve_diag_dbl(bool posdef, void const *numeric_tree, int nrhs, double *x, int ldx);
+int spldlt_tree_solve_diag_bwd_dbl(bool posdef, void const *numeric_tree, int nrhs, double *x, int ldx);
+
+/* ---- 2. extension knobs and statistics (no counterpart in the reference) ---- */
+enum {
+   SYLVER_AMD_PREC_FP64 = 0,   /* fronts, updates and solves in FP64 */
+   SYLVER_AMD_PREC_MIXED = 1,  /* FP32 fronts; BF16 (power-of-two block scaled) updates
+                                  on fronts wider than bf16_min_ncol; FP32 accumulation;
+                                  FP64 refinement in spldlt_solve */
+   SYLVER_AMD_PREC_FP32 = 2    /* FP32 fronts and updates; FP64 refinement */
+};
+enum {
+   SYLVER_AMD_REFINE_NONE = 0,
+   SYLVER_AMD_REFINE_IR = 1,     /* classical iterative refinement */
+   SYLVER_AMD_REFINE_GMRES = 2,  /* GMRES-IR (FP64 GMRES preconditioned by the factors) */
+   SYLVER_AMD_REFINE_AUTO = 3    /* IR, falling back to GMRES-IR when IR stalls */
+};
+
+typedef struct {
+   int precision;        /* SYLVER_AMD_PREC_* (default FP64) */
+   int bf16_min_ncol;    /* BF16 updates on fronts with more pivots than this (1024) */
+   int refine;           /* SYLVER_AMD_REFINE_* (default AUTO; ignored for FP64 factors) */
+   double refine_tol;    /* target scaled backward error (1e-15) */
+   int refine_max_iter;  /* (30) */
+   int use_graph;        /* capture the level schedule in a hipGraph (1) */
+   int tile_nb;          /* block size of the large-front path (0 = auto) */
+   int small_front_max;  /* fronts with nrow <= this use the one-workgroup kernel (0 = auto) */
+   int record_timing;    /* record per-level device timings (0) */
+   int reserved[7];
+} sylver_amd_options_t;
+
+void sylver_amd_default_options(sylver_amd_options_t *opt);
+/* Process-wide extension options used by later spldlt_factorize/solve calls. */
+void sylver_amd_set_options(sylver_amd_options_t const *opt);
+void sylver_amd_get_options(sylver_amd_options_t *opt);
+
+/* Written into sylver_inform_t.unused by spldlt_factorize and spldlt_solve. */
+typedef struct {
+   double factor_ms;      /* device time of the numeric factorization */
+   double solve_ms;       /* device time of the last solve (incl. refinement) */
+   double gflops;         /* num_flops / factor time */
+   double bwd_err;        /* last scaled backward error seen by refinement (or -1) */
+   int refine_iters;      /* refinement iterations of the last solve */
+   int refine_mode;       /* SYLVER_AMD_REFINE_* actually used */
+   int precision;         /* SYLVER_AMD_PREC_* of the factors */
+   int nlevels;           /* levels of the device schedule */
+   double upload_ms;      /* host->device copy of the values */
+   char pad[16];
+} sylver_amd_inform_tail_t;
+
+typedef struct {
+   double factor_ms;
+   double upload_ms;
+   double solve_ms;
+   double flops;           /* reference flop model over all nodes */
+   int nlevels;
+   int nlaunches;
+   int precision;
+   int refine_iters;
+   double bwd_err;
+   /* Per launch kind (0 zero, 1 scatter A, 2 extend-add, 3 one-workgroup
+    * fronts, 4 diagonal POTRF/TRTRI, 5 panel TRSM, 6 Schur update, 7 inverse
+    * of small-front diagonal blocks), filled when the factorization ran with
+    * record_timing != 0 (eager launches bracketed by HIP events on the
+    * library's stream): summed device time, algorithmic flops, launches. */
+   double kind_ms[16];
+   double kind_flops[16];
+   double kind_bytes[16];
+   int kind_launches[16];
+} sylver_amd_stats_t;
+
+int sylver_amd_fkeep_stats(void const *fkeep, sylver_amd_stats_t *stats);
+
+/* ---- 3. read-back for parity tests ---- */
+/* Sizes: n, nnodes, length of rlist, number of nlist pairs. */
+int sylver_amd_akeep_sizes(void const *akeep, int *n, int *nnodes, long *nrlist, long *nnlist);
+/* Host copies of the analyse outputs (any pointer may be NULL). */
+int sylver_amd_akeep_symbolic(void const *akeep, int *order, int *invp, int *sptr,
+                              int *sparent, long *rptr, int *rlist, long *nptr,
+                              long *nlist);
+/* Copies of the device-resident structures, read back from HBM: sptr,
+ * sparent, rptr, rlist, nptr, nlist (same sizes as above) and the child
+ * maps (cmap, length sum over nodes of nrow-ncol; cmap_off, nnodes+1). */
+int sylver_amd_fkeep_device_symbolic(void const *fkeep, int *sptr, int *sparent,
+                                     long *rptr, int *rlist, long *nptr, long *nlist,
+                                     int *cmap, long *cmap_off);
+/* Factor of node `node` (0-based): L (nrow x ncol, column-major, ld = nrow) as
+ * FP64 on the host (converted from FP32 storage if needed). */
+int sylver_amd_fkeep_node_factor(void const *fkeep, int node, double *l);
+/* Contribution-block memory of node `node` before its parent consumed it is
+ * not kept; the assembled front of a node (nrow x nrow, before
+ * factorization) is returned when the factorization ran with
+ * record_timing == 2 (debug mode). */
+int sylver_amd_fkeep_node_assembled(void const *fkeep, int node, double *front);
+
+/* Batched dense front factorization: for each front f, the nrow[f] x nrow[f]
+ * column-major block at data + offset[f] (elements, ld = nrow[f], device
+ * memory) is partially Cholesky factorized over its first ncol[f] columns:
+ * POTRF of the leading block, TRSM of the rows below, SYRK of the trailing
+ * block (the contribution block, lower triangle updated in place).
+ * dtype: 0 = double, 1 = float. info (device, nfronts ints) receives 0 or the
+ * 1-based column of the first non-positive pivot. stream may be NULL.
+ * Returns 0 or a SYLVER_ERROR_* code. */
+int sylver_amd_batched_front_potrf(int nfronts, int const *nrow_host, int const *ncol_host,
+                                   long const *offset_host, void *data_dev, int dtype,
+                                   int *info_dev, void *stream);
+
+/* Number of HIP devices visible (0 when none); never fails. */
+int sylver_amd_device_count(void);
+
+/* Device memory helpers for callers without their own HIP runtime (the
+ * tests and benchmark use them instead of a second framework's allocator).
+ * Return 0 or SYLVER_ERROR_CUDA_UNKNOWN. */
+int sylver_amd_dev_alloc(void **ptr, size_t bytes);
+int sylver_amd_dev_free(void *ptr);
+int sylver_amd_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int sylver_amd_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* Device time (ms, HIP events) of the last sylver_amd_batched_front_potrf call. */
+double sylver_amd_last_batched_ms(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYLVER_AMD_EXT_H */

@@ -1,0 +1,135 @@
+"""CPU oracle (test infrastructure only): ctypes access to oracle/liboracle.so.
+
+This package restates the reference's posdef numeric phase on the CPU in
+FP64. Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+may use it; the library never does. Build it with ``python oracle/build.py``.
+
+Which function each oracle entry restates is stated in oracle/oracle.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            from . import build as _b  # noqa: F401
+            raise RuntimeError(f"oracle library missing: {LIB}; run `python oracle/build.py`")
+        L = C.CDLL(LIB)
+        i64p, ip, dp = C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.oracle_build_map.argtypes = [C.c_int, i64p, ip, ip, ip, C.c_int, ip, i64p, ip, i64p, i64p]
+        L.oracle_symbolic_tree.argtypes = [C.c_int, ip, ip, i64p, ip, ip, ip, ip, ip]
+        L.oracle_symbolic_tree.restype = C.c_int64
+        L.oracle_child_maps.argtypes = [C.c_int, C.c_int, ip, ip, i64p, ip, i64p, ip]
+        L.oracle_factor_posdef.argtypes = [C.c_int, C.c_int, ip, ip, i64p, ip, i64p, i64p, dp, dp,
+                                           C.c_int, C.c_int, C.c_char_p, i64p, dp]
+        L.oracle_factor_size.argtypes = [C.c_int, ip, i64p]
+        L.oracle_factor_size.restype = C.c_int64
+        L.oracle_solve_posdef.argtypes = [C.c_int, C.c_int, ip, i64p, ip, i64p, dp, C.c_int, dp,
+                                          C.c_int]
+        L.oracle_front_potrf.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int]
+        L.oracle_bwd_err.argtypes = [C.c_int, i64p, ip, dp, C.c_int, dp, C.c_int, dp, C.c_int]
+        L.oracle_bwd_err.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def build_map(A, order, invp, sym):
+    nn = sym["nnodes"]
+    nptr = np.zeros(nn + 1, np.int64)
+    nlist = np.zeros(2 * max(A.nnz, 1), np.int64)
+    lib().oracle_build_map(A.n, _p(A.ptr, C.c_int64), _p(A.row, C.c_int),
+                           _p(np.ascontiguousarray(order, np.int32), C.c_int),
+                           _p(np.ascontiguousarray(invp, np.int32), C.c_int), nn,
+                           _p(sym["sptr"], C.c_int), _p(sym["rptr"], C.c_int64),
+                           _p(sym["rlist"], C.c_int), _p(nptr, C.c_int64), _p(nlist, C.c_int64))
+    return nptr, nlist[:2 * (nptr[-1] - 1)]
+
+
+def symbolic_tree(sym):
+    nn = sym["nnodes"]
+    out = {k: np.zeros(nn + 1, np.int32) for k in ("parent", "first_child", "next_child", "least_desc")}
+    mf = C.c_int()
+    nf = lib().oracle_symbolic_tree(nn, _p(sym["sptr"], C.c_int), _p(sym["sparent"], C.c_int),
+                                    _p(sym["rptr"], C.c_int64), _p(out["parent"], C.c_int),
+                                    _p(out["first_child"], C.c_int), _p(out["next_child"], C.c_int),
+                                    _p(out["least_desc"], C.c_int), C.byref(mf))
+    out["nfactor"] = int(nf)
+    out["maxfront"] = mf.value
+    return out
+
+
+def child_maps(n, sym):
+    nn = sym["nnodes"]
+    q = (np.diff(sym["rptr"]) - np.diff(sym["sptr"])).astype(np.int64)
+    off = np.zeros(nn + 1, np.int64)
+    cmap = np.zeros(max(int(q.sum()), 1), np.int32)
+    lib().oracle_child_maps(n, nn, _p(sym["sptr"], C.c_int), _p(sym["sparent"], C.c_int),
+                            _p(sym["rptr"], C.c_int64), _p(sym["rlist"], C.c_int),
+                            _p(off, C.c_int64), _p(cmap, C.c_int))
+    return off, cmap[:int(q.sum())]
+
+
+def factor_posdef(sym, aval, scaling=None, nb=256, nthreads=0, node_mask=None):
+    """Returns (status, lcol_off, lcol)."""
+    nn = sym["nnodes"]
+    size = lib().oracle_factor_size(nn, _p(sym["sptr"], C.c_int), _p(sym["rptr"], C.c_int64))
+    lcol = np.zeros(max(size, 1))
+    off = np.zeros(nn + 1, np.int64)
+    mask = None
+    if node_mask is not None:
+        mask = np.ascontiguousarray(node_mask, dtype=np.int8).tobytes()
+    aval = np.ascontiguousarray(aval, np.float64)
+    sc = None if scaling is None else np.ascontiguousarray(scaling, np.float64)
+    st = lib().oracle_factor_posdef(sym["n"], nn, _p(sym["sptr"], C.c_int),
+                                    _p(sym["sparent"], C.c_int), _p(sym["rptr"], C.c_int64),
+                                    _p(sym["rlist"], C.c_int), _p(sym["nptr"], C.c_int64),
+                                    _p(sym["nlist"], C.c_int64), _p(aval, C.c_double),
+                                    _p(sc, C.c_double), nb, nthreads, mask, _p(off, C.c_int64),
+                                    _p(lcol, C.c_double))
+    return st, off, lcol
+
+
+def node_factor(sym, off, lcol, node):
+    m = int(sym["rptr"][node + 1] - sym["rptr"][node])
+    n = int(sym["sptr"][node + 1] - sym["sptr"][node])
+    return lcol[off[node]:off[node] + m * n].reshape((m, n), order="F")
+
+
+def solve_posdef(sym, off, lcol, b):
+    """b: (n,) or (n, nrhs) in pivot order; returns x in pivot order."""
+    x = np.asfortranarray(np.array(b, dtype=np.float64, copy=True).reshape(sym["n"], -1))
+    lib().oracle_solve_posdef(sym["n"], sym["nnodes"], _p(sym["sptr"], C.c_int),
+                              _p(sym["rptr"], C.c_int64), _p(sym["rlist"], C.c_int),
+                              _p(off, C.c_int64), _p(lcol, C.c_double), x.shape[1],
+                              _p(x, C.c_double), sym["n"])
+    return x
+
+
+def front_potrf(a, ncol, nb=256):
+    """Partial Cholesky of a dense front (copy returned, Fortran order)."""
+    a = np.asfortranarray(np.array(a, dtype=np.float64, copy=True))
+    m = a.shape[0]
+    info = lib().oracle_front_potrf(m, ncol, _p(a, C.c_double), m, nb)
+    return info, a
+
+
+def bwd_err(A, x, b):
+    x = np.asfortranarray(np.asarray(x, np.float64).reshape(A.n, -1))
+    b = np.asfortranarray(np.asarray(b, np.float64).reshape(A.n, -1))
+    return float(lib().oracle_bwd_err(A.n, _p(A.ptr, C.c_int64), _p(A.row, C.c_int),
+                                      _p(A.val, C.c_double), x.shape[1], _p(x, C.c_double), A.n,
+                                      _p(b, C.c_double), A.n))

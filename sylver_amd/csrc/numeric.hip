@@ -1,0 +1,1105 @@
+// Device numeric phase: the multifrontal factorization as a device-resident
+// level schedule on one GPU, its solves and FP64 refinement.
+//
+// The reference walks the assembly tree in postorder on the CPU and submits
+// per-tile StarPU tasks (src/NumericTreePosdef.hxx:133-353). Here the tree is
+// cut into levels by height (leaves are level 0); every level runs as a fixed
+// sequence of batched launches over all its fronts:
+//   zero -> scatter A -> extend-add (one launch per child rank)
+//   -> one-workgroup factorization of small fronts
+//   -> per block column of large fronts: diagonal POTRF+TRTRI, TRSM as GEMM,
+//      MFMA Schur update of the trailing block (contribution block included).
+// The launch list and all task descriptors are built once per pattern and
+// live in HBM; the list is captured in a hipGraph and replayed per factorize.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "numeric.hpp"
+
+namespace sylver_amd {
+
+namespace {
+
+struct HipError {
+   hipError_t err;
+   int line;
+};
+
+#define HIPCHECK(x)                                   \
+   do {                                              \
+      hipError_t e_ = (x);                           \
+      if (e_ != hipSuccess) throw HipError{e_, __LINE__}; \
+   } while (0)
+
+template <typename T>
+T* dalloc(size_t n) {
+   void* p = nullptr;
+   if (n == 0) n = 1;
+   HIPCHECK(hipMalloc(&p, n * sizeof(T)));
+   return static_cast<T*>(p);
+}
+
+template <typename T>
+T* dupload(const std::vector<T>& v) {
+   T* p = dalloc<T>(v.size());
+   if (!v.empty()) HIPCHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+   return p;
+}
+
+void dfree(void* p) {
+   if (p) (void)hipFree(p);
+}
+
+enum LaunchKind {
+   L_ZERO,
+   L_INIT,
+   L_EA,
+   L_SMALL,
+   L_DIAG,
+   L_TRSM,
+   L_UPDATE,
+   L_DINV,
+   L_FGATHER,
+   L_FSTEP,
+   L_BGATHER,
+   L_BCB,
+   L_BSTEP
+};
+
+struct Launch {
+   int kind;
+   int count;
+   int64_t off;
+   double flops = 0;  // algorithmic flops of the launch
+   double words = 0;  // algorithmic memory traffic in front elements
+   double bytes = 0;  // algorithmic memory traffic independent of the element size
+};
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+sylver_amd_options_t& ext_options() {
+   static sylver_amd_options_t o = [] {
+      sylver_amd_options_t d;
+      std::memset(&d, 0, sizeof(d));
+      d.precision = SYLVER_AMD_PREC_FP64;
+      d.bf16_min_ncol = 1024;
+      d.refine = SYLVER_AMD_REFINE_AUTO;
+      d.refine_tol = 1e-15;
+      d.refine_max_iter = 30;
+      d.use_graph = 1;
+      return d;
+   }();
+   return o;
+}
+
+int device_count() {
+   int c = 0;
+   if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+   return c;
+}
+
+// ---------------------------------------------------------------------------
+// Plan: pattern-dependent device state.
+// ---------------------------------------------------------------------------
+struct DevicePlan {
+   int n = 0, nnodes = 0, nlevels = 0;
+   int small_max = 256;
+   std::vector<NodeDesc> hnodes;
+   std::vector<std::vector<int>> levels;
+   std::vector<int> hcmap;
+   std::vector<int64_t> hcmap_off;
+   int64_t arena_elems = 0, dinv_elems = 0, solve_rows = 0;
+   // bit-exact device copies of the analyse outputs
+   int* d_sptr = nullptr;
+   int* d_sparent = nullptr;
+   int64_t* d_rptr = nullptr;
+   int* d_rlist = nullptr;
+   int64_t* d_nptr = nullptr;
+   int64_t* d_nlist = nullptr;
+   int* d_cmap = nullptr;
+   int64_t* d_cmap_off = nullptr;
+   NodeDesc* d_nodes = nullptr;
+   // task lists
+   std::vector<ColTask> zero;
+   std::vector<InitTask> init;
+   std::vector<EaTask> ea;
+   std::vector<int> nlistv;  // node lists (small fronts, gathers)
+   std::vector<DiagTask> diag;
+   std::vector<GemmTask> gemm;
+   std::vector<SolveTask> stask;
+   ColTask* d_zero = nullptr;
+   InitTask* d_init = nullptr;
+   EaTask* d_ea = nullptr;
+   int* d_nlistv = nullptr;
+   DiagTask* d_diag = nullptr;
+   GemmTask* d_gemm = nullptr;
+   SolveTask* d_stask = nullptr;
+   std::vector<Launch> fprog, fwdprog, bwdprog;
+   int64_t num_flops = 0;
+   // CSR of the whole matrix in pivot order, values gathered per factorize
+   int64_t csr_nnz = 0;
+   int64_t* d_csr_ptr = nullptr;
+   int* d_csr_col = nullptr;
+   int64_t* d_csr_src = nullptr;
+   int64_t clean_nnz = 0;
+
+   ~DevicePlan() {
+      for (void* p : {(void*)d_sptr, (void*)d_sparent, (void*)d_rptr, (void*)d_rlist,
+                      (void*)d_nptr, (void*)d_nlist, (void*)d_cmap, (void*)d_cmap_off,
+                      (void*)d_nodes, (void*)d_zero, (void*)d_init, (void*)d_ea,
+                      (void*)d_nlistv, (void*)d_diag, (void*)d_gemm, (void*)d_stask,
+                      (void*)d_csr_ptr, (void*)d_csr_col, (void*)d_csr_src})
+         dfree(p);
+   }
+};
+
+struct DeviceFactors {
+   const DevicePlan* plan = nullptr;
+   int precision = SYLVER_AMD_PREC_FP64;
+   bool posdef = true;
+   void* arena = nullptr;
+   void* dinv = nullptr;
+   double* d_aval = nullptr;
+   double* d_scaling = nullptr;
+   double* d_csr_val = nullptr;
+   int* d_info = nullptr;
+   double anorm = 0;
+   hipStream_t stream = nullptr;
+   hipGraphExec_t graph = nullptr;
+   bool has_scaling = false;
+   // solve workspaces
+   int ws_nrhs = 0;
+   double* d_xloc = nullptr;
+   double* d_b = nullptr;
+   double* d_x = nullptr;
+   double* d_r = nullptr;
+   double* d_sol = nullptr;
+   unsigned long long* d_norms = nullptr;
+   // statistics
+   double factor_ms = 0, upload_ms = 0, solve_ms = 0, bwd_err = -1;
+   int refine_iters = 0;
+   int nlaunches = 0;
+   double kind_ms[16] = {0};
+   double kind_flops[16] = {0};
+   double kind_bytes[16] = {0};
+   int kind_launches[16] = {0};
+
+   ~DeviceFactors() {
+      if (graph) (void)hipGraphExecDestroy(graph);
+      for (void* p : {arena, dinv, (void*)d_aval, (void*)d_scaling, (void*)d_csr_val,
+                      (void*)d_info, (void*)d_xloc, (void*)d_b, (void*)d_x, (void*)d_r,
+                      (void*)d_sol, (void*)d_norms})
+         dfree(p);
+      if (stream) (void)hipStreamDestroy(stream);
+   }
+};
+
+void destroy_plan(DevicePlan* p) { delete p; }
+void destroy_factors(DeviceFactors* f) { delete f; }
+
+namespace {
+
+DevicePlan* build_plan(const Analysis& a, int small_max) {
+   auto* P = new DevicePlan();
+   P->n = a.n;
+   P->nnodes = a.nnodes;
+   P->small_max = small_max;
+   const int nn = a.nnodes;
+   P->hnodes.resize(nn);
+   std::vector<int> height(nn + 1, 0);
+   for (int i = 0; i < nn; ++i) {
+      const SymbolicFront& f = a.fronts[i];
+      NodeDesc& d = P->hnodes[i];
+      d.m = f.nrow;
+      d.n = f.ncol;
+      d.parent = f.parent;
+      d.ld = f.nrow;
+      d.rlist_off = f.rlist_off;
+      d.amap_off = f.amap_off;
+      d.num_a = f.num_a;
+      d.first_child = f.first_child;
+      d.next_child = f.next_child;
+      if (f.parent < nn) height[f.parent] = std::max(height[f.parent], height[i] + 1);
+      P->num_flops += node_flops(f.nrow, f.ncol);
+   }
+   int nlev = 0;
+   for (int i = 0; i < nn; ++i) nlev = std::max(nlev, height[i] + 1);
+   P->nlevels = nlev;
+   P->levels.assign(nlev, {});
+   for (int i = 0; i < nn; ++i) P->levels[height[i]].push_back(i);
+   // arena, dinv and solve offsets: level by level
+   int64_t off = 0, doff = 0, soff = 0;
+   for (int L = 0; L < nlev; ++L)
+      for (int i : P->levels[L]) {
+         NodeDesc& d = P->hnodes[i];
+         d.off = off;
+         off += align_up((int64_t)d.m * d.ld, 64);
+         d.dinv_off = doff;
+         if (d.n > 0) {
+            int64_t nblk = (d.n + kNB - 1) / kNB;
+            int64_t last = d.n - (nblk - 1) * kNB;
+            doff += align_up((nblk - 1) * kNB * kNB + last * last, 64);
+         }
+         d.solve_off = soff;
+         soff += d.m;
+      }
+   P->arena_elems = off;
+   P->dinv_elems = doff;
+   P->solve_rows = soff;
+   // child maps: position of each CB row in the parent front
+   // (reference src/NumericTreePosdef.hxx:219-238)
+   P->hcmap_off.assign(nn + 1, 0);
+   for (int i = 0; i < nn; ++i) P->hcmap_off[i + 1] = P->hcmap_off[i] + (P->hnodes[i].m - P->hnodes[i].n);
+   P->hcmap.assign(P->hcmap_off[nn], 0);
+   {
+      std::vector<int> map(a.n + 1, 0);
+      for (int p = 0; p < nn; ++p) {
+         const SymbolicFront& fp = a.fronts[p];
+         if (fp.first_child == -1) continue;
+         const int* rl = &a.rlist[fp.rlist_off];
+         for (int i = 0; i < fp.ncol; ++i) map[rl[i]] = i;
+         for (int i = fp.ncol; i < fp.nrow; ++i) map[rl[i]] = i;  // no delays on this path
+         for (int c = fp.first_child; c != -1; c = a.fronts[c].next_child) {
+            const SymbolicFront& fc = a.fronts[c];
+            const int* rc = &a.rlist[fc.rlist_off];
+            int q = fc.nrow - fc.ncol;
+            for (int k = 0; k < q; ++k) P->hcmap[P->hcmap_off[c] + k] = map[rc[fc.ncol + k]];
+         }
+      }
+   }
+   for (int i = 0; i < nn; ++i) P->hnodes[i].cmap_off = P->hcmap_off[i];
+
+   // ---- factor program ----
+   auto push = [](std::vector<Launch>& prog, int kind, int64_t off, int64_t count) {
+      if (count > 0) {
+         Launch l;
+         l.kind = kind;
+         l.count = (int)count;
+         l.off = off;
+         prog.push_back(l);
+      }
+   };
+   for (int L = 0; L < nlev; ++L) {
+      const std::vector<int>& lv = P->levels[L];
+      // zero
+      int64_t z0 = P->zero.size();
+      for (int i : lv) {
+         const NodeDesc& d = P->hnodes[i];
+         int chunk = std::max(4, std::min(256, 16384 / std::max(d.m, 1)));
+         for (int c = 0; c < d.m; c += chunk) P->zero.push_back(ColTask{i, c, std::min(d.m, c + chunk), 0});
+      }
+      push(P->fprog, L_ZERO, z0, P->zero.size() - z0);
+      // scatter A
+      int64_t i0 = P->init.size();
+      for (int i : lv) {
+         const NodeDesc& d = P->hnodes[i];
+         for (int64_t p = 0; p < d.num_a; p += 4096)
+            P->init.push_back(InitTask{i, 0, p, std::min<int64_t>(d.num_a, p + 4096)});
+      }
+      push(P->fprog, L_INIT, i0, P->init.size() - i0);
+      // extend-add by child rank
+      for (int rank = 0;; ++rank) {
+         int64_t e0 = P->ea.size();
+         bool any = false;
+         for (int i : lv) {
+            int c = P->hnodes[i].first_child;
+            for (int r = 0; r < rank && c != -1; ++r) c = P->hnodes[c].next_child;
+            if (c == -1) continue;
+            any = true;
+            const NodeDesc& dc = P->hnodes[c];
+            int q = dc.m - dc.n;
+            if (q <= 0) continue;
+            int chunk = std::max(4, std::min(q, 16384 / std::max(q, 1)));
+            for (int j = 0; j < q; j += chunk) P->ea.push_back(EaTask{c, j, std::min(q, j + chunk), 0});
+         }
+         if (!any) break;
+         push(P->fprog, L_EA, e0, P->ea.size() - e0);
+      }
+      // small fronts
+      int64_t s0 = P->nlistv.size();
+      std::vector<int> large;
+      for (int i : lv) {
+         const NodeDesc& d = P->hnodes[i];
+         if (d.n == 0) continue;
+         if (d.m <= small_max)
+            P->nlistv.push_back(i);
+         else
+            large.push_back(i);
+      }
+      push(P->fprog, L_SMALL, s0, P->nlistv.size() - s0);
+      // large fronts, block column by block column
+      int maxn = 0;
+      for (int i : large) maxn = std::max(maxn, P->hnodes[i].n);
+      for (int k0 = 0; k0 < maxn; k0 += kNB) {
+         int64_t d0 = P->diag.size();
+         for (int i : large) {
+            const NodeDesc& d = P->hnodes[i];
+            if (k0 >= d.n) continue;
+            P->diag.push_back(DiagTask{i, k0, std::min(kNB, d.n - k0), 0});
+         }
+         push(P->fprog, L_DIAG, d0, P->diag.size() - d0);
+         int64_t t0 = P->gemm.size();
+         for (int i : large) {
+            const NodeDesc& d = P->hnodes[i];
+            if (k0 >= d.n) continue;
+            int kb = std::min(kNB, d.n - k0);
+            for (int r = k0 + kb; r < d.m; r += 64) {
+               GemmTask g{};
+               g.a_off = d.off + r + (int64_t)k0 * d.ld;
+               g.b_off = d.dinv_off + (int64_t)(k0 / kNB) * kNB * kNB;
+               g.c_off = g.a_off;
+               g.lda = d.ld;
+               g.ldb = kb;
+               g.ldc = d.ld;
+               g.M = std::min(64, d.m - r);
+               g.N = kb;
+               g.K = kb;
+               P->gemm.push_back(g);
+            }
+         }
+         push(P->fprog, L_TRSM, t0, P->gemm.size() - t0);
+         int64_t u0 = P->gemm.size();
+         for (int i : large) {
+            const NodeDesc& d = P->hnodes[i];
+            if (k0 >= d.n) continue;
+            int kb = std::min(kNB, d.n - k0);
+            int r0 = k0 + kb;
+            int R = d.m - r0;
+            int nt = (R + 127) / 128;
+            for (int ti = 0; ti < nt; ++ti)
+               for (int tj = 0; tj <= ti; ++tj) {
+                  GemmTask g{};
+                  g.a_off = d.off + r0 + ti * 128 + (int64_t)k0 * d.ld;
+                  g.b_off = d.off + r0 + tj * 128 + (int64_t)k0 * d.ld;
+                  g.c_off = d.off + (r0 + ti * 128) + (int64_t)(r0 + tj * 128) * d.ld;
+                  g.lda = d.ld;
+                  g.ldb = d.ld;
+                  g.ldc = d.ld;
+                  g.M = std::min(128, R - ti * 128);
+                  g.N = std::min(128, R - tj * 128);
+                  g.K = kb;
+                  P->gemm.push_back(g);
+               }
+         }
+         push(P->fprog, L_UPDATE, u0, P->gemm.size() - u0);
+      }
+   }
+   // inverted diagonal blocks of the small fronts (the large path writes its own)
+   {
+      int64_t d0 = P->diag.size();
+      for (int L = 0; L < nlev; ++L)
+         for (int i : P->levels[L]) {
+            const NodeDesc& d = P->hnodes[i];
+            if (d.n == 0 || d.m > small_max) continue;
+            for (int k0 = 0; k0 < d.n; k0 += kNB)
+               P->diag.push_back(DiagTask{i, k0, std::min(kNB, d.n - k0), 0});
+         }
+      push(P->fprog, L_DINV, d0, P->diag.size() - d0);
+   }
+   // algorithmic work of every factor launch (for the roofline figures)
+   for (Launch& L : P->fprog) {
+      for (int64_t t = L.off; t < L.off + L.count; ++t) {
+         switch (L.kind) {
+            case L_ZERO: {
+               const ColTask& c = P->zero[t];
+               const NodeDesc& d = P->hnodes[c.node];
+               for (int col = c.c0; col < c.c1; ++col) L.words += d.m - col;
+               break;
+            }
+            case L_INIT: {
+               const InitTask& it = P->init[t];
+               L.words += (double)(it.p1 - it.p0);
+               L.bytes += 24.0 * (double)(it.p1 - it.p0);
+               break;
+            }
+            case L_EA: {
+               const EaTask& e = P->ea[t];
+               const NodeDesc& d = P->hnodes[e.child];
+               const double q = d.m - d.n;
+               double cnt = 0;
+               for (int j = e.j0; j < e.j1; ++j) cnt += q - j;
+               L.words += 3.0 * cnt;  // read child, read+write parent
+               L.bytes += 4.0 * cnt;  // row map
+               break;
+            }
+            case L_SMALL: {
+               const NodeDesc& d = P->hnodes[P->nlistv[t]];
+               L.flops += (double)node_flops(d.m, d.n);
+               L.words += (double)d.m * (d.m + 1);  // lower triangle read + written once
+               break;
+            }
+            case L_DIAG:
+            case L_DINV: {
+               const double kb = P->diag[t].kb;
+               L.flops += (L.kind == L_DIAG ? 2.0 : 1.0) * kb * kb * kb / 3.0;
+               L.words += 2.0 * kb * kb;
+               break;
+            }
+            case L_TRSM: {
+               const GemmTask& g = P->gemm[t];
+               L.flops += (double)g.M * g.K * g.K;
+               L.words += 2.0 * g.M * g.K + 0.5 * g.K * g.K;
+               break;
+            }
+            case L_UPDATE: {
+               const GemmTask& g = P->gemm[t];
+               if (g.a_off == g.b_off)
+                  L.flops += (double)g.M * (g.M + 1) * g.K;
+               else
+                  L.flops += 2.0 * g.M * g.N * g.K;
+               L.words += 2.0 * g.M * g.N + (double)(g.M + g.N) * g.K;
+               break;
+            }
+            default:
+               break;
+         }
+      }
+   }
+
+   // ---- solve programs ----
+   for (int L = 0; L < nlev; ++L) {
+      const std::vector<int>& lv = P->levels[L];
+      int64_t g0 = P->nlistv.size();
+      for (int i : lv) P->nlistv.push_back(i);
+      push(P->fwdprog, L_FGATHER, g0, lv.size());
+      int maxblk = 0;
+      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
+      for (int s = 0; s < maxblk; ++s) {
+         int64_t t0 = P->stask.size();
+         for (int i : lv) {
+            const NodeDesc& d = P->hnodes[i];
+            int k0 = s * kNB;
+            if (k0 >= d.n) continue;
+            int kb = std::min(kNB, d.n - k0);
+            int rb = k0 + kb;
+            if (rb >= d.m) {
+               P->stask.push_back(SolveTask{i, k0, kb, rb, rb, 1});
+            } else {
+               for (int r = rb; r < d.m; r += kThreads)
+                  P->stask.push_back(SolveTask{i, k0, kb, r, std::min(d.m, r + kThreads), r == rb ? 1 : 0});
+            }
+         }
+         push(P->fwdprog, L_FSTEP, t0, P->stask.size() - t0);
+      }
+   }
+   for (int L = nlev - 1; L >= 0; --L) {
+      const std::vector<int>& lv = P->levels[L];
+      int64_t g0 = P->nlistv.size();
+      for (int i : lv) P->nlistv.push_back(i);
+      push(P->bwdprog, L_BGATHER, g0, lv.size());
+      int64_t c0 = P->stask.size();
+      for (int i : lv) {
+         const NodeDesc& d = P->hnodes[i];
+         if (d.m - d.n <= 0 || d.n == 0) continue;
+         for (int j = 0; j < d.n; j += 16)
+            P->stask.push_back(SolveTask{i, 0, 0, j, std::min(d.n, j + 16), 0});
+      }
+      push(P->bwdprog, L_BCB, c0, P->stask.size() - c0);
+      int maxblk = 0;
+      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
+      for (int s = maxblk - 1; s >= 0; --s) {
+         int64_t t0 = P->stask.size();
+         for (int i : lv) {
+            const NodeDesc& d = P->hnodes[i];
+            int k0 = s * kNB;
+            if (k0 >= d.n) continue;
+            int kb = std::min(kNB, d.n - k0);
+            if (k0 == 0) {
+               P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
+            } else {
+               for (int j = 0; j < k0; j += 32)
+                  P->stask.push_back(SolveTask{i, k0, kb, j, std::min(k0, j + 32), j == 0 ? 1 : 0});
+            }
+         }
+         push(P->bwdprog, L_BSTEP, t0, P->stask.size() - t0);
+      }
+   }
+
+   // ---- CSR of the full matrix in pivot order ----
+   {
+      int n = a.n;
+      const int64_t* cptr = a.ptr.data();
+      const int* crow = a.row.data();
+      P->clean_nnz = n > 0 ? cptr[n] - 1 : 0;
+      std::vector<int64_t> rp(n + 1, 0);
+      for (int j = 0; j < n; ++j)
+         for (int64_t p = cptr[j] - 1; p < cptr[j + 1] - 1; ++p) {
+            int i = crow[p] - 1;
+            rp[a.order[i]]++;
+            if (i != j) rp[a.order[j]]++;
+         }
+      for (int i = 0; i < n; ++i) rp[i + 1] += rp[i];
+      std::vector<int> col(rp[n]);
+      std::vector<int64_t> src(rp[n]);
+      std::vector<int64_t> pos(rp.begin(), rp.end() - 1);
+      for (int j = 0; j < n; ++j)
+         for (int64_t p = cptr[j] - 1; p < cptr[j + 1] - 1; ++p) {
+            int i = crow[p] - 1;
+            int pi = a.order[i] - 1, pj = a.order[j] - 1;
+            col[pos[pi]] = pj;
+            src[pos[pi]++] = p;
+            if (i != j) {
+               col[pos[pj]] = pi;
+               src[pos[pj]++] = p;
+            }
+         }
+      P->csr_nnz = rp[n];
+      P->d_csr_ptr = dupload(rp);
+      P->d_csr_col = dupload(col);
+      P->d_csr_src = dupload(src);
+   }
+
+   // ---- device copies ----
+   P->d_sptr = dupload(a.sptr);
+   P->d_sparent = dupload(a.sparent);
+   P->d_rptr = dupload(a.rptr);
+   P->d_rlist = dupload(a.rlist);
+   P->d_nptr = dupload(a.nptr);
+   P->d_nlist = dupload(a.nlist);
+   P->d_cmap = dupload(P->hcmap);
+   P->d_cmap_off = dupload(P->hcmap_off);
+   P->d_nodes = dupload(P->hnodes);
+   P->d_zero = dupload(P->zero);
+   P->d_init = dupload(P->init);
+   P->d_ea = dupload(P->ea);
+   P->d_nlistv = dupload(P->nlistv);
+   P->d_diag = dupload(P->diag);
+   P->d_gemm = dupload(P->gemm);
+   P->d_stask = dupload(P->stask);
+   return P;
+}
+
+inline unsigned grid1(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+template <typename T>
+void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
+                        std::vector<hipEvent_t>* events = nullptr) {
+   T* arena = static_cast<T*>(F->arena);
+   T* dinv = static_cast<T*>(F->dinv);
+   size_t li = 0;
+   for (const Launch& L : P->fprog) {
+      dim3 g(L.count), b(kThreads);
+      if (events) HIPCHECK(hipEventRecord((*events)[2 * li], s));
+      switch (L.kind) {
+         case L_ZERO:
+            k_zero<T><<<g, b, 0, s>>>(P->d_zero + L.off, P->d_nodes, arena);
+            break;
+         case L_INIT:
+            k_init_a<T><<<g, b, 0, s>>>(P->d_init + L.off, P->d_nodes, P->d_nlist, P->d_rlist,
+                                        F->d_aval, F->has_scaling ? F->d_scaling : nullptr, arena);
+            break;
+         case L_EA:
+            k_extend_add<T><<<g, b, 0, s>>>(P->d_ea + L.off, P->d_nodes, P->d_cmap, arena);
+            break;
+         case L_SMALL:
+            k_small_front<T><<<g, b, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, F->d_info);
+            break;
+         case L_DIAG:
+            k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, F->d_info, 1);
+            break;
+         case L_TRSM:
+            k_gemm_nt<T, 64, 128><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, dinv, arena, T(1), T(0));
+            break;
+         case L_UPDATE:
+            k_gemm_nt<T, 128, 128><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, arena, arena, T(-1), T(1));
+            break;
+         case L_DINV:
+            k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, F->d_info, 0);
+            break;
+         default:
+            break;
+      }
+      if (events) HIPCHECK(hipEventRecord((*events)[2 * li + 1], s));
+      ++li;
+   }
+   HIPCHECK(hipGetLastError());
+}
+
+template <typename T>
+void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<Launch>& prog,
+                       const double* b, double* xsol, int nrhs, hipStream_t s) {
+   const T* arena = static_cast<const T*>(F->arena);
+   const T* dinv = static_cast<const T*>(F->dinv);
+   const int n = P->n;
+   for (const Launch& L : prog) {
+      dim3 g(L.count), bl(kThreads);
+      switch (L.kind) {
+         case L_FGATHER:
+            k_fwd_gather<<<g, bl, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, P->d_rlist, P->d_cmap, b,
+                                          n, F->d_xloc, nrhs);
+            break;
+         case L_FSTEP:
+            k_fwd_step<T><<<g, bl, 0, s>>>(P->d_stask + L.off, P->d_nodes, arena, dinv, P->d_rlist,
+                                           F->d_xloc, xsol, n, nrhs);
+            break;
+         case L_BGATHER:
+            k_bwd_gather<<<g, bl, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, P->d_rlist, xsol, n,
+                                          F->d_xloc, nrhs);
+            break;
+         case L_BCB:
+            k_bwd_cb<T><<<g, bl, 0, s>>>(P->d_stask + L.off, P->d_nodes, arena, F->d_xloc, nrhs);
+            break;
+         case L_BSTEP:
+            k_bwd_step<T><<<g, bl, 0, s>>>(P->d_stask + L.off, P->d_nodes, arena, dinv, P->d_rlist,
+                                           F->d_xloc, xsol, n, nrhs);
+            break;
+         default:
+            break;
+      }
+   }
+   HIPCHECK(hipGetLastError());
+}
+
+void ensure_solve_ws(const DevicePlan* P, DeviceFactors* F, int nrhs) {
+   if (F->ws_nrhs >= nrhs) return;
+   for (void* p : {(void*)F->d_xloc, (void*)F->d_b, (void*)F->d_x, (void*)F->d_r, (void*)F->d_sol,
+                   (void*)F->d_norms})
+      dfree(p);
+   F->d_xloc = dalloc<double>((size_t)P->solve_rows * nrhs);
+   F->d_b = dalloc<double>((size_t)P->n * nrhs);
+   F->d_x = dalloc<double>((size_t)P->n * nrhs);
+   F->d_r = dalloc<double>((size_t)P->n * nrhs);
+   F->d_sol = dalloc<double>((size_t)P->n * nrhs);
+   F->d_norms = dalloc<unsigned long long>(3 * (size_t)nrhs);
+   F->ws_nrhs = nrhs;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Factorization driver
+// ---------------------------------------------------------------------------
+FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors** fact, bool posdef,
+                           const double* aval, const double* scaling, const sylver_options_t& opts) {
+   FactorResult res;
+   res.matrix_rank = a.n;
+   if (device_count() <= 0) {
+      res.flag = SYLVER_ERROR_CUDA_UNKNOWN;
+      res.hip_error = (int)hipErrorNoDevice;
+      return res;
+   }
+   if (!posdef) {
+      // The APP LDL^T device path is not part of this build yet.
+      res.flag = SYLVER_ERROR_UNIMPLEMENTED;
+      return res;
+   }
+   const sylver_amd_options_t& xo = ext_options();
+   (void)opts;
+   try {
+      int small_max = xo.small_front_max > 0 ? xo.small_front_max : 256;
+      if (*plan && (*plan)->small_max != small_max) {
+         if (*fact && (*fact)->plan == *plan) {
+            destroy_factors(*fact);
+            *fact = nullptr;
+         }
+         destroy_plan(*plan);
+         *plan = nullptr;
+      }
+      if (!*plan) *plan = build_plan(a, small_max);
+      DevicePlan* P = *plan;
+      int prec = xo.precision;
+      if (prec != SYLVER_AMD_PREC_FP64 && prec != SYLVER_AMD_PREC_FP32 && prec != SYLVER_AMD_PREC_MIXED)
+         prec = SYLVER_AMD_PREC_FP64;
+      // Mixed precision without the BF16 update kernels runs the FP32 path.
+      const bool low = prec != SYLVER_AMD_PREC_FP64;
+      DeviceFactors* F = *fact;
+      if (F && (F->plan != P || F->precision != prec)) {
+         destroy_factors(F);
+         F = nullptr;
+      }
+      if (!F) {
+         F = new DeviceFactors();
+         *fact = F;
+         F->plan = P;
+         F->precision = prec;
+         size_t es = low ? sizeof(float) : sizeof(double);
+         HIPCHECK(hipMalloc(&F->arena, std::max<int64_t>(P->arena_elems, 1) * es));
+         HIPCHECK(hipMalloc(&F->dinv, std::max<int64_t>(P->dinv_elems, 1) * es));
+         F->d_aval = dalloc<double>(std::max<int64_t>(P->clean_nnz, 1));
+         F->d_scaling = dalloc<double>(std::max(a.n, 1));
+         F->d_csr_val = dalloc<double>(std::max<int64_t>(P->csr_nnz, 1));
+         F->d_info = dalloc<int>(std::max(P->nnodes, 1));
+         HIPCHECK(hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking));
+      }
+      F->posdef = posdef;
+      hipStream_t s = F->stream;
+      hipEvent_t e0, e1, e2;
+      HIPCHECK(hipEventCreate(&e0));
+      HIPCHECK(hipEventCreate(&e1));
+      HIPCHECK(hipEventCreate(&e2));
+      HIPCHECK(hipEventRecord(e0, s));
+      if (P->clean_nnz > 0)
+         HIPCHECK(hipMemcpyAsync(F->d_aval, aval, P->clean_nnz * sizeof(double), hipMemcpyHostToDevice, s));
+      F->has_scaling = scaling != nullptr;
+      if (scaling) HIPCHECK(hipMemcpyAsync(F->d_scaling, scaling, a.n * sizeof(double), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemsetAsync(F->d_info, 0, std::max(P->nnodes, 1) * sizeof(int), s));
+      HIPCHECK(hipEventRecord(e1, s));
+      const bool record = xo.record_timing != 0;
+      bool use_graph = xo.use_graph != 0 && !F->has_scaling && !record;
+      std::vector<hipEvent_t> events;
+      if (record) {
+         events.resize(2 * P->fprog.size());
+         for (auto& e : events) HIPCHECK(hipEventCreate(&e));
+      }
+      if (use_graph && !F->graph) {
+         hipGraph_t g;
+         HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+         if (low)
+            run_factor_program<float>(P, F, s);
+         else
+            run_factor_program<double>(P, F, s);
+         HIPCHECK(hipStreamEndCapture(s, &g));
+         HIPCHECK(hipGraphInstantiate(&F->graph, g, nullptr, nullptr, 0));
+         (void)hipGraphDestroy(g);
+      }
+      if (use_graph && F->graph) {
+         HIPCHECK(hipGraphLaunch(F->graph, s));
+      } else if (low) {
+         run_factor_program<float>(P, F, s, record ? &events : nullptr);
+      } else {
+         run_factor_program<double>(P, F, s, record ? &events : nullptr);
+      }
+      HIPCHECK(hipEventRecord(e2, s));
+      // FP64 values of the whole matrix for the refinement residual
+      if (P->csr_nnz > 0)
+         k_gather_vals<<<grid1(P->csr_nnz), kThreads, 0, s>>>(P->csr_nnz, P->d_csr_src, F->d_aval,
+                                                               nullptr, nullptr, F->d_csr_val);
+      std::vector<int> info(std::max(P->nnodes, 1));
+      HIPCHECK(hipMemcpyAsync(info.data(), F->d_info, info.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      float t1 = 0, t2 = 0;
+      HIPCHECK(hipEventElapsedTime(&t1, e0, e1));
+      HIPCHECK(hipEventElapsedTime(&t2, e1, e2));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      (void)hipEventDestroy(e2);
+      F->upload_ms = t1;
+      F->factor_ms = t2;
+      if (record) {
+         const double es = low ? 4.0 : 8.0;
+         for (int k = 0; k < 16; ++k) {
+            F->kind_ms[k] = F->kind_flops[k] = F->kind_bytes[k] = 0;
+            F->kind_launches[k] = 0;
+         }
+         for (size_t i = 0; i < P->fprog.size(); ++i) {
+            float t = 0;
+            HIPCHECK(hipEventElapsedTime(&t, events[2 * i], events[2 * i + 1]));
+            const Launch& L = P->fprog[i];
+            F->kind_ms[L.kind] += t;
+            F->kind_flops[L.kind] += L.flops;
+            F->kind_bytes[L.kind] += L.words * es + L.bytes;
+            F->kind_launches[L.kind] += 1;
+         }
+         for (auto& e : events) (void)hipEventDestroy(e);
+      }
+      F->nlaunches = (int)P->fprog.size();
+      res.upload_ms = t1;
+      res.factor_ms = t2;
+      for (int i = 0; i < P->nnodes; ++i)
+         if (info[i] != 0) {
+            res.flag = SYLVER_ERROR_NOT_POS_DEF;
+            break;
+         }
+      // ||A||_inf with full symmetric row sums (scaled matrix when scaling given)
+      {
+         std::vector<double> rs(a.n, 0.0);
+         for (int j = 0; j < a.n; ++j)
+            for (int64_t p = a.ptr[j] - 1; p < a.ptr[j + 1] - 1; ++p) {
+               int i = a.row[p] - 1;
+               double v = std::fabs(aval[p]);
+               rs[i] += v;
+               if (i != j) rs[j] += v;
+            }
+         double an = 0;
+         for (double v : rs) an = std::max(an, v);
+         F->anorm = an;
+      }
+   } catch (const HipError& e) {
+      res.flag = SYLVER_ERROR_CUDA_UNKNOWN;
+      res.hip_error = (int)e.err;
+      if (opts.print_level > 0)
+         std::fprintf(stderr, "sylver_amd: HIP error %d (%s) at numeric.hip:%d\n", (int)e.err,
+                      hipGetErrorString(e.err), e.line);
+   }
+   return res;
+}
+
+// ---------------------------------------------------------------------------
+// Solve driver
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+void solve_once(const DevicePlan* P, DeviceFactors* F, int job, const double* b, double* xsol,
+                int nrhs, hipStream_t s) {
+   if (job == 0 || job == 1) run_solve_program<T>(P, F, P->fwdprog, b, xsol, nrhs, s);
+   if (job == 3) {
+      // backward only: the input is the current solution vector
+      if (b != xsol)
+         k_copy<<<grid1((int64_t)P->n * nrhs), kThreads, 0, s>>>((int64_t)P->n * nrhs, b, xsol);
+   }
+   if (job == 0 || job == 3) run_solve_program<T>(P, F, P->bwdprog, xsol, xsol, nrhs, s);
+   if (job == 2 || job == 4) {
+      // posdef: D = I
+      if (job == 2 && b != xsol)
+         k_copy<<<grid1((int64_t)P->n * nrhs), kThreads, 0, s>>>((int64_t)P->n * nrhs, b, xsol);
+      if (job == 4) {
+         if (b != xsol)
+            k_copy<<<grid1((int64_t)P->n * nrhs), kThreads, 0, s>>>((int64_t)P->n * nrhs, b, xsol);
+         run_solve_program<T>(P, F, P->bwdprog, xsol, xsol, nrhs, s);
+      }
+   }
+}
+
+// Scaled backward error of x for the system A x = b, worst over columns.
+double device_bwd_err(const DevicePlan* P, DeviceFactors* F, const double* b, const double* x,
+                      double* r, int nrhs, hipStream_t s) {
+   const int n = P->n;
+   k_residual<<<grid1(n), kThreads, 0, s>>>(n, P->d_csr_ptr, P->d_csr_col, F->d_csr_val, x, b, r, nrhs);
+   k_fill_u64<<<1, 64, 0, s>>>(F->d_norms, 3 * nrhs, 0ull);
+   dim3 g(std::min<unsigned>(grid1(n), 256u), nrhs);
+   k_colmax<<<g, kThreads, 0, s>>>(n, r, nrhs, F->d_norms);
+   k_colmax<<<g, kThreads, 0, s>>>(n, x, nrhs, F->d_norms + nrhs);
+   k_colmax<<<g, kThreads, 0, s>>>(n, b, nrhs, F->d_norms + 2 * nrhs);
+   std::vector<unsigned long long> h(3 * nrhs);
+   HIPCHECK(hipMemcpyAsync(h.data(), F->d_norms, h.size() * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s));
+   HIPCHECK(hipStreamSynchronize(s));
+   double worst = 0;
+   for (int c = 0; c < nrhs; ++c) {
+      double rn, xn, bn;
+      std::memcpy(&rn, &h[c], 8);
+      std::memcpy(&xn, &h[nrhs + c], 8);
+      std::memcpy(&bn, &h[2 * nrhs + c], 8);
+      double den = F->anorm * xn + bn;
+      double e = den > 0 ? rn / den : rn;
+      if (std::isnan(e)) return e;
+      worst = std::max(worst, e);
+   }
+   return worst;
+}
+
+}  // namespace
+
+SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int job, int nrhs,
+                         double* x, int ldx) {
+   SolveResult res;
+   if (!P || !F) {
+      res.flag = SYLVER_ERROR_CALL_SEQUENCE;
+      return res;
+   }
+   const sylver_amd_options_t& xo = ext_options();
+   try {
+      const int n = a.n;
+      ensure_solve_ws(P, F, nrhs);
+      hipStream_t s = F->stream;
+      hipEvent_t e0, e1;
+      HIPCHECK(hipEventCreate(&e0));
+      HIPCHECK(hipEventCreate(&e1));
+      HIPCHECK(hipMemcpy2DAsync(F->d_b, (size_t)n * sizeof(double), x, (size_t)ldx * sizeof(double),
+                                (size_t)n * sizeof(double), nrhs, hipMemcpyHostToDevice, s));
+      HIPCHECK(hipEventRecord(e0, s));
+      const bool low = F->precision != SYLVER_AMD_PREC_FP64;
+      const bool refine = low && job == 0 && xo.refine != SYLVER_AMD_REFINE_NONE && !F->has_scaling;
+      const int64_t nn = (int64_t)n * nrhs;
+      if (!refine) {
+         if (low)
+            solve_once<float>(P, F, job, F->d_b, F->d_sol, nrhs, s);
+         else
+            solve_once<double>(P, F, job, F->d_b, F->d_sol, nrhs, s);
+         res.refine_mode = SYLVER_AMD_REFINE_NONE;
+      } else {
+         // Classical iterative refinement in FP64: x0 = 0, r0 = b,
+         // x_{k+1} = x_k + solve(r_k), r_{k+1} = b - A x_{k+1}.
+         k_fill<<<grid1(nn), kThreads, 0, s>>>(nn, 0.0, F->d_x);
+         k_copy<<<grid1(nn), kThreads, 0, s>>>(nn, F->d_b, F->d_r);
+         double err = 1e300;
+         int it = 0;
+         for (; it < xo.refine_max_iter; ++it) {
+            solve_once<float>(P, F, 0, F->d_r, F->d_sol, nrhs, s);
+            k_axpy<<<grid1(nn), kThreads, 0, s>>>(nn, 1.0, F->d_sol, F->d_x);
+            double e = device_bwd_err(P, F, F->d_b, F->d_x, F->d_r, nrhs, s);
+            err = e;
+            if (!(e > xo.refine_tol)) {
+               ++it;
+               break;
+            }
+         }
+         k_copy<<<grid1(nn), kThreads, 0, s>>>(nn, F->d_x, F->d_sol);
+         res.refine_iters = it;
+         res.bwd_err = err;
+         res.refine_mode = SYLVER_AMD_REFINE_IR;
+      }
+      HIPCHECK(hipEventRecord(e1, s));
+      HIPCHECK(hipMemcpy2DAsync(x, (size_t)ldx * sizeof(double), F->d_sol, (size_t)n * sizeof(double),
+                                (size_t)n * sizeof(double), nrhs, hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      float t = 0;
+      HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      res.solve_ms = t;
+      F->solve_ms = t;
+      F->refine_iters = res.refine_iters;
+      F->bwd_err = res.bwd_err;
+   } catch (const HipError& e) {
+      res.flag = SYLVER_ERROR_CUDA_UNKNOWN;
+      res.hip_error = (int)e.err;
+   }
+   return res;
+}
+
+// ---------------------------------------------------------------------------
+// Statistics and read-back
+// ---------------------------------------------------------------------------
+int factors_stats(const DeviceFactors* F, sylver_amd_stats_t* st) {
+   if (!F || !st) return SYLVER_ERROR_CALL_SEQUENCE;
+   std::memset(st, 0, sizeof(*st));
+   st->factor_ms = F->factor_ms;
+   st->upload_ms = F->upload_ms;
+   st->solve_ms = F->solve_ms;
+   st->flops = F->plan ? (double)F->plan->num_flops : 0;
+   st->nlevels = F->plan ? F->plan->nlevels : 0;
+   st->nlaunches = F->nlaunches;
+   st->precision = F->precision;
+   st->refine_iters = F->refine_iters;
+   st->bwd_err = F->bwd_err;
+   for (int k = 0; k < 16; ++k) {
+      st->kind_ms[k] = F->kind_ms[k];
+      st->kind_flops[k] = F->kind_flops[k];
+      st->kind_bytes[k] = F->kind_bytes[k];
+      st->kind_launches[k] = F->kind_launches[k];
+   }
+   return 0;
+}
+
+int plan_device_symbolic(const DevicePlan* P, int* sptr, int* sparent, int64_t* rptr, int* rlist,
+                         int64_t* nptr, int64_t* nlist, int* cmap, int64_t* cmap_off) {
+   if (!P) return SYLVER_ERROR_CALL_SEQUENCE;
+   try {
+      int nn = P->nnodes;
+      auto cp = [](void* dst, const void* src, size_t bytes) {
+         if (dst && bytes) HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+      };
+      std::vector<int64_t> hr(nn + 1), hn(nn + 1);
+      HIPCHECK(hipMemcpy(hr.data(), P->d_rptr, (nn + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(hn.data(), P->d_nptr, (nn + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+      cp(sptr, P->d_sptr, (nn + 1) * sizeof(int));
+      cp(sparent, P->d_sparent, nn * sizeof(int));
+      if (rptr) std::memcpy(rptr, hr.data(), (nn + 1) * sizeof(int64_t));
+      cp(rlist, P->d_rlist, (hr[nn] - 1) * sizeof(int));
+      if (nptr) std::memcpy(nptr, hn.data(), (nn + 1) * sizeof(int64_t));
+      cp(nlist, P->d_nlist, 2 * (hn[nn] - 1) * sizeof(int64_t));
+      cp(cmap, P->d_cmap, P->hcmap.size() * sizeof(int));
+      cp(cmap_off, P->d_cmap_off, (nn + 1) * sizeof(int64_t));
+      return 0;
+   } catch (const HipError&) {
+      return SYLVER_ERROR_CUDA_UNKNOWN;
+   }
+}
+
+int factors_node_factor(const DeviceFactors* F, int node, double* l) {
+   if (!F || !F->plan || node < 0 || node >= F->plan->nnodes || !l) return SYLVER_ERROR_CALL_SEQUENCE;
+   try {
+      const NodeDesc& d = F->plan->hnodes[node];
+      size_t cnt = (size_t)d.m * d.n;
+      if (F->precision == SYLVER_AMD_PREC_FP64) {
+         HIPCHECK(hipMemcpy(l, static_cast<double*>(F->arena) + d.off, cnt * sizeof(double),
+                            hipMemcpyDeviceToHost));
+      } else {
+         std::vector<float> tmp(cnt);
+         HIPCHECK(hipMemcpy(tmp.data(), static_cast<float*>(F->arena) + d.off, cnt * sizeof(float),
+                            hipMemcpyDeviceToHost));
+         for (size_t i = 0; i < cnt; ++i) l[i] = tmp[i];
+      }
+      return 0;
+   } catch (const HipError&) {
+      return SYLVER_ERROR_CUDA_UNKNOWN;
+   }
+}
+
+int factors_node_assembled(const DeviceFactors* F, int node, double* front) {
+   (void)F;
+   (void)node;
+   (void)front;
+   return SYLVER_ERROR_UNIMPLEMENTED;
+}
+
+int dev_alloc(void** ptr, size_t bytes) {
+   if (!ptr) return SYLVER_ERROR_CALL_SEQUENCE;
+   return hipMalloc(ptr, bytes ? bytes : 1) == hipSuccess ? 0 : SYLVER_ERROR_CUDA_UNKNOWN;
+}
+int dev_free(void* ptr) { return hipFree(ptr) == hipSuccess ? 0 : SYLVER_ERROR_CUDA_UNKNOWN; }
+int dev_copy(void* dst, const void* src, size_t bytes, bool h2d) {
+   hipError_t e = hipMemcpy(dst, src, bytes, h2d ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost);
+   return e == hipSuccess ? 0 : SYLVER_ERROR_CUDA_UNKNOWN;
+}
+
+static double g_last_batched_ms = 0;
+double last_batched_ms() { return g_last_batched_ms; }
+
+// ---------------------------------------------------------------------------
+// Batched dense fronts (no tree): every front through the one-workgroup
+// kernel in one variable-size launch.
+// ---------------------------------------------------------------------------
+int batched_front_potrf(int nfronts, const int* nrow, const int* ncol, const int64_t* off, void* data,
+                        int dtype, int* info, void* stream) {
+   if (nfronts < 0 || (nfronts > 0 && (!nrow || !ncol || !off || !data || !info)))
+      return SYLVER_ERROR_CALL_SEQUENCE;
+   if (nfronts == 0) return 0;
+   if (device_count() <= 0) return SYLVER_ERROR_CUDA_UNKNOWN;
+   try {
+      std::vector<NodeDesc> nd(nfronts);
+      std::vector<int> list(nfronts);
+      for (int f = 0; f < nfronts; ++f) {
+         if (nrow[f] < ncol[f] || ncol[f] < 0) return SYLVER_ERROR_CALL_SEQUENCE;
+         std::memset(&nd[f], 0, sizeof(NodeDesc));
+         nd[f].m = nrow[f];
+         nd[f].n = ncol[f];
+         nd[f].ld = nrow[f];
+         nd[f].off = off[f];
+         nd[f].first_child = nd[f].next_child = -1;
+         list[f] = f;
+      }
+      hipStream_t s = static_cast<hipStream_t>(stream);
+      NodeDesc* dnd = dalloc<NodeDesc>(nfronts);
+      int* dl = dalloc<int>(nfronts);
+      HIPCHECK(hipMemcpyAsync(dnd, nd.data(), nfronts * sizeof(NodeDesc), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemcpyAsync(dl, list.data(), nfronts * sizeof(int), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemsetAsync(info, 0, nfronts * sizeof(int), s));
+      hipEvent_t e0, e1;
+      HIPCHECK(hipEventCreate(&e0));
+      HIPCHECK(hipEventCreate(&e1));
+      HIPCHECK(hipEventRecord(e0, s));
+      if (dtype == 0)
+         k_small_front<double><<<nfronts, kThreads, 0, s>>>(dl, dnd, static_cast<double*>(data), info);
+      else
+         k_small_front<float><<<nfronts, kThreads, 0, s>>>(dl, dnd, static_cast<float*>(data), info);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipEventRecord(e1, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      float t = 0;
+      HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+      g_last_batched_ms = t;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      dfree(dnd);
+      dfree(dl);
+      return 0;
+   } catch (const HipError&) {
+      return SYLVER_ERROR_CUDA_UNKNOWN;
+   }
+}
+
+}  // namespace sylver_amd

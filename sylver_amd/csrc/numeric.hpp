@@ -1,0 +1,82 @@
+// Device numeric phase: plan (device-resident symbolic structures and the
+// level schedule), multifrontal factorization, solves and refinement.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "analyse.hpp"
+#include "../../include/sylver_amd.h"
+
+namespace sylver_amd {
+
+struct DevicePlan;     // pattern-dependent device state (one per akeep)
+struct DeviceFactors;  // value-dependent device state (one per fkeep)
+
+struct FactorResult {
+   int flag = 0;
+   int hip_error = 0;
+   int num_neg = 0;
+   int num_two = 0;
+   int num_delay = 0;
+   int num_zero = 0;
+   int matrix_rank = 0;
+   int not_first_pass = 0;
+   int not_second_pass = 0;
+   double factor_ms = 0;
+   double upload_ms = 0;
+};
+
+struct SolveResult {
+   int flag = 0;
+   int hip_error = 0;
+   double solve_ms = 0;
+   int refine_iters = 0;
+   int refine_mode = 0;
+   double bwd_err = -1;
+};
+
+// Global extension options (process-wide).
+sylver_amd_options_t& ext_options();
+
+// Number of HIP devices (0 on a host without a GPU).
+int device_count();
+
+// Plan lifetime (owned by the akeep).
+void destroy_plan(DevicePlan* plan);
+// Factor lifetime (owned by the fkeep).
+void destroy_factors(DeviceFactors* f);
+
+// Numeric factorization on the device. `aval` are the cleaned matrix values
+// (host, FP64, addressed by nlist); `scaling` may be null. Creates or reuses
+// *plan and *fact.
+FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors** fact,
+                           bool posdef, const double* aval, const double* scaling,
+                           const sylver_options_t& opts);
+
+// Full solve (job 0) or partial solves (1 fwd, 2 diag, 3 bwd, 4 diag+bwd) of
+// nrhs right-hand sides held in pivot order on the host (column-major,
+// ld = ldx). With low-precision factors and job 0, FP64 refinement against
+// the original matrix is applied.
+SolveResult device_solve(const Analysis& a, DevicePlan* plan, DeviceFactors* fact,
+                         int job, int nrhs, double* x, int ldx);
+
+// Statistics and read-back.
+int factors_stats(const DeviceFactors* f, sylver_amd_stats_t* st);
+int plan_device_symbolic(const DevicePlan* plan, int* sptr, int* sparent, int64_t* rptr,
+                         int* rlist, int64_t* nptr, int64_t* nlist, int* cmap,
+                         int64_t* cmap_off);
+int factors_node_factor(const DeviceFactors* f, int node, double* l);
+int factors_node_assembled(const DeviceFactors* f, int node, double* front);
+
+int dev_alloc(void** ptr, size_t bytes);
+int dev_free(void* ptr);
+int dev_copy(void* dst, const void* src, size_t bytes, bool h2d);
+double last_batched_ms();
+
+int batched_front_potrf(int nfronts, const int* nrow, const int* ncol, const int64_t* off,
+                        void* data, int dtype, int* info, void* stream);
+
+}  // namespace sylver_amd

@@ -1,0 +1,204 @@
+"""Parity of the device posdef path against the CPU oracle (MI355X only).
+
+Bars: integer structures (device copies of the symbolic arrays, child maps)
+bit-exact; FP64 factors within 1e-12*max|L| of the oracle (the LL^T factor
+of an SPD front is unique, differences come only from summation order);
+scaled backward error <= 1e-15 in FP64; FP32 factors within 2e-4 relative;
+mixed precision reaches <= 1e-15 through FP64 refinement.
+"""
+import numpy as np
+import pytest
+
+from sylver_amd import matrices as M
+
+pytestmark = pytest.mark.gpu
+
+FP64_L_TOL = 1e-12
+BWD_TOL = 1e-15
+
+
+def _analyse(S, A, order=None, ordering=1):
+    opts = S.default_options()
+    opts.ordering = 0 if order is not None else ordering
+    akeep, inf, _ = S.spldlt_analyse(A.n, A.ptr, A.row, A.val, order=order, options=opts)
+    assert inf.flag >= 0
+    return akeep, opts, S.akeep_symbolic(akeep)
+
+
+def _factor(S, A, akeep, opts, posdef=True):
+    fkeep, inf = S.spldlt_factorize(posdef, A.val, akeep, options=opts)
+    return fkeep, inf
+
+
+def _compare_factors(S, O, sym, fkeep, A, tol):
+    st, off, lcol = O.factor_posdef(sym, A.val)
+    assert st == 0
+    worst = 0.0
+    for s in range(sym["nnodes"]):
+        m = int(sym["rptr"][s + 1] - sym["rptr"][s])
+        n = int(sym["sptr"][s + 1] - sym["sptr"][s])
+        ref = O.node_factor(sym, off, lcol, s)
+        got = np.tril(S.fkeep_node_factor(fkeep, s, m, n))
+        scale = max(1.0, np.abs(ref).max())
+        worst = max(worst, np.abs(got - np.tril(ref)).max() / scale)
+    assert worst <= tol, worst
+    return off, lcol
+
+
+@pytest.fixture(autouse=True)
+def _fp64_defaults(S):
+    S.set_ext_options(precision=S.PREC_FP64, small_front_max=0, use_graph=1)
+    yield
+    S.set_ext_options(precision=S.PREC_FP64, small_front_max=0, use_graph=1)
+
+
+def test_c_example_tridiagonal(S, gpu):
+    # examples/C/spldlt_simple_example_c.c:30-40 (solved as posdef here)
+    A = M.CSCLower(3, [1, 3, 5, 6], [1, 2, 2, 3, 3], [2.0, -1.0, 2.0, -1.0, 2.0])
+    akeep, opts, _ = _analyse(S, A)
+    fkeep, inf = _factor(S, A, akeep, opts)
+    assert inf.flag == 0
+    x = np.ones(3)
+    inf = S.spldlt_solve(x, akeep, fkeep, options=opts)
+    assert inf.flag == 0
+    np.testing.assert_allclose(x, [1.5, 2.0, 1.5], rtol=0, atol=1e-15)
+
+
+def test_simple_mat_known_answer(S, gpu):
+    # tests/sylver_test_mod.F90:120-155 (simple_mat), x = 1
+    A = M.CSCLower(4, [1, 4, 5, 7, 8], [1, 2, 4, 2, 3, 4, 4],
+                   [10.0, 2.0, 3.0, 10.0, 10.0, 4.0, 10.0])
+    b = A.to_dense() @ np.ones(4)
+    for order in (None, np.array([1, 2, 3, 4], np.int32), np.array([4, 3, 2, 1], np.int32)):
+        akeep, opts, _ = _analyse(S, A, order=order)
+        fkeep, inf = _factor(S, A, akeep, opts)
+        assert inf.flag == 0
+        x = b.copy()
+        assert S.spldlt_solve(x, akeep, fkeep, options=opts).flag == 0
+        np.testing.assert_allclose(x, np.ones(4), rtol=0, atol=1e-14)
+
+
+def test_laplacian_2d_100_vs_oracle(S, O, gpu):
+    """2D 5-point Laplacian on 100x100 (n=10,000), nested dissection, one RHS
+    uniform in [-1,1] from seed 0."""
+    A = M.laplacian_2d(100)
+    akeep, opts, sym = _analyse(S, A)
+    fkeep, inf = _factor(S, A, akeep, opts)
+    assert inf.flag == 0
+    _compare_factors(S, O, sym, fkeep, A, FP64_L_TOL)
+    b = M.random_rhs(A.n, 1, seed=0)
+    x = b.copy(order="F")
+    assert S.spldlt_solve(x, akeep, fkeep, options=opts).flag == 0
+    err = O.bwd_err(A, x, b)
+    assert err <= BWD_TOL, err
+
+
+def test_device_symbolic_bit_exact(S, O, gpu):
+    A = M.laplacian_3d(12)
+    akeep, opts, sym = _analyse(S, A)
+    fkeep, inf = _factor(S, A, akeep, opts)
+    assert inf.flag == 0
+    dev = S.fkeep_device_symbolic(fkeep, sym)
+    for k in ("sptr", "sparent", "rptr", "rlist", "nptr", "nlist"):
+        assert np.array_equal(dev[k], sym[k]), k
+    nptr, nlist = O.build_map(A, sym["order"], sym["invp"], sym)
+    assert np.array_equal(dev["nptr"], nptr) and np.array_equal(dev["nlist"], nlist)
+    off, cmap = O.child_maps(A.n, sym)
+    assert np.array_equal(dev["cmap_off"], off)
+    assert np.array_equal(dev["cmap"][:len(cmap)], cmap)
+
+
+@pytest.mark.parametrize("small_max", [0, 48])
+def test_laplacian_3d_large_front_path(S, O, gpu, small_max):
+    """20^3 Laplacian; small_front_max=48 routes most fronts through the
+    tiled large-front path (diagonal POTRF/TRTRI, TRSM-by-GEMM, MFMA update)."""
+    S.set_ext_options(small_front_max=small_max)
+    A = M.laplacian_3d(20)
+    akeep, opts, sym = _analyse(S, A)
+    fkeep, inf = _factor(S, A, akeep, opts)
+    assert inf.flag == 0
+    _compare_factors(S, O, sym, fkeep, A, FP64_L_TOL)
+    b = M.random_rhs(A.n, 3, seed=1)
+    x = b.copy(order="F")
+    assert S.spldlt_solve(x, akeep, fkeep, options=opts).flag == 0
+    assert O.bwd_err(A, x, b) <= BWD_TOL
+
+
+def test_geometric_nd_order(S, O, gpu):
+    A = M.laplacian_3d(16)
+    akeep, opts, sym = _analyse(S, A, order=M.grid_nd_order(16))
+    fkeep, inf = _factor(S, A, akeep, opts)
+    assert inf.flag == 0
+    _compare_factors(S, O, sym, fkeep, A, FP64_L_TOL)
+
+
+def test_fp32_factor_and_refinement(S, O, gpu):
+    S.set_ext_options(precision=S.PREC_FP32, refine=S.REFINE_IR, refine_tol=1e-15)
+    A = M.laplacian_3d(16)
+    akeep, opts, sym = _analyse(S, A)
+    fkeep, inf = _factor(S, A, akeep, opts)
+    assert inf.flag == 0
+    _compare_factors(S, O, sym, fkeep, A, 2e-4)
+    b = M.random_rhs(A.n, 4, seed=2)
+    x = b.copy(order="F")
+    inf = S.spldlt_solve(x, akeep, fkeep, options=opts)
+    assert inf.flag == 0
+    t = inf.tail()
+    assert t.refine_iters >= 2
+    assert O.bwd_err(A, x, b) <= BWD_TOL
+
+
+def test_not_posdef_flag(S, gpu):
+    # tests/sylver_test_mod.F90 simple_mat_indef factorized as posdef -> -6
+    A = M.CSCLower(4, [1, 4, 5, 7, 8], [1, 2, 4, 2, 3, 4, 4], [1.0, 2.0, 3.0, 1.0, 1.0, 4.0, 1.0])
+    akeep, opts, _ = _analyse(S, A)
+    _, inf = _factor(S, A, akeep, opts)
+    assert inf.flag == S.ERROR_NOT_POS_DEF
+
+
+def test_refactorize_deterministic(S, gpu):
+    """Same values twice -> bitwise identical factors (no atomics anywhere in
+    assembly or factorization)."""
+    A = M.laplacian_3d(14)
+    akeep, opts, sym = _analyse(S, A)
+    fkeep, inf = _factor(S, A, akeep, opts)
+    first = [S.fkeep_node_factor(fkeep, s, int(sym["rptr"][s + 1] - sym["rptr"][s]),
+                                 int(sym["sptr"][s + 1] - sym["sptr"][s])).copy()
+             for s in range(sym["nnodes"])]
+    fkeep, inf = S.spldlt_factorize(True, A.val, akeep, fkeep, options=opts)
+    assert inf.flag == 0
+    for s in range(sym["nnodes"]):
+        got = S.fkeep_node_factor(fkeep, s, *first[s].shape)
+        assert np.array_equal(np.tril(got), np.tril(first[s]))
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_batched_dense_fronts(S, O, gpu, dtype):
+    """Slice of the batched small-front workload: fronts with ncol
+    log-uniform in [8,256], nrow = ncol*(1+r), r in [0,2]; one
+    variable-size launch; every front checked against the oracle."""
+    nf = 48
+    nrow, ncol = M.dense_front_sizes(nf, seed=3)
+    offs = np.zeros(nf, np.int64)
+    tot = 0
+    fronts = []
+    for f in range(nf):
+        offs[f] = tot
+        a = M.dense_front_values(int(nrow[f]), seed=100 + f)
+        fronts.append(a)
+        tot += int(nrow[f]) ** 2
+    host = np.concatenate([a.ravel(order="F") for a in fronts]).astype(dtype)
+    dev = S.DeviceArray(host)
+    rc, info, _ = S.batched_front_potrf(dev, nrow, ncol, offs, dtype)
+    assert rc == 0
+    assert int(np.abs(info).sum()) == 0
+    out = dev.numpy().astype(np.float64)
+    dev.free()
+    tol = 1e-12 if dtype == "float64" else 2e-5
+    for f in range(nf):
+        m, n = int(nrow[f]), int(ncol[f])
+        got = out[offs[f]:offs[f] + m * m].reshape((m, m), order="F")
+        st, ref = O.front_potrf(fronts[f], n)
+        assert st == 0
+        scale = np.abs(ref).max()
+        assert np.abs(np.tril(got) - np.tril(ref)).max() <= tol * scale * max(1, m / 64), f
