@@ -116,7 +116,7 @@ class Stats(C.Structure):
     ]
 
 KIND_NAMES = ["zero", "scatter_a", "extend_add", "small_front", "diag_potrf", "panel_trsm",
-              "schur_update", "diag_inverse"]
+              "schur_update", "diag_inverse", "schur_update_wide"]
 
 
 _lib = None

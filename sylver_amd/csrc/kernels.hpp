@@ -149,99 +149,216 @@ __global__ __launch_bounds__(kThreads) void k_extend_add(const EaTask* __restric
 }
 
 // ---------------------------------------------------------------------------
-// In-LDS dense kernels on a kb x kb block (kb <= kNB), stored column-major
-// with leading dimension LD.
+// In-LDS dense kernels on the diagonal block of the large-front path. The
+// block (kb <= kNB) is held padded to a multiple of 16 with identity, so
+// every loop below works on whole 16x16 sub-blocks: the Cholesky factor and
+// the inverse of [A 0; 0 I] are [L 0; 0 I] and [inv(L) 0; 0 I].
 // ---------------------------------------------------------------------------
-// Unblocked right-looking Cholesky. Returns 0 or the 1-based column of the
-// first non-positive (or NaN) pivot. All threads must call it.
+// Cholesky of a 16x16 block at D (ld LD) by one wave: lane i < 16 holds row
+// i in registers, pivot columns are broadcast by cross-lane shuffles.
+// Returns 0 or the 1-based column of the first non-positive pivot.
 template <typename T, int LD>
-__device__ int lds_potrf(T* L, int kb) {
-   const int tid = threadIdx.x;
-   for (int j = 0; j < kb; ++j) {
-      __syncthreads();
-      T d = L[j + j * LD];
-      if (!(d > T(0))) return j + 1;
-      d = dsqrt(d);
-      __syncthreads();
-      if (tid == 0) L[j + j * LD] = d;
-      for (int i = j + 1 + tid; i < kb; i += kThreads) L[i + j * LD] /= d;
-      __syncthreads();
-      // trailing update: rows i (< kNB) on tid % kNB, columns strided
-      const int i = j + 1 + (tid % kNB);
-      if (i < kb) {
-         const T lij = L[i + j * LD];
-         for (int c = j + 1 + tid / kNB; c <= i; c += kThreads / kNB)
-            L[i + c * LD] -= lij * L[c + j * LD];
+__device__ int wave_potrf16(T* D) {
+   const int lane = threadIdx.x & 63;
+   const bool own = lane < 16;
+   T r[16];
+#pragma unroll
+   for (int c = 0; c < 16; ++c) r[c] = (own && c <= lane) ? D[lane + c * LD] : T(0);
+   int bad = 0;
+#pragma unroll
+   for (int j = 0; j < 16; ++j) {
+      T d = __shfl(r[j], j, 64);
+      if (bad == 0 && !(d > T(0))) bad = j + 1;
+      if (bad == 0) {
+         d = dsqrt(d);
+         const T inv = T(1) / d;
+         if (lane == j) r[j] = d;
+         if (lane > j) r[j] *= inv;
+#pragma unroll
+         for (int c = j + 1; c < 16; ++c) {
+            const T lcj = __shfl(r[j], c, 64);
+            if (lane >= c) r[c] -= r[j] * lcj;
+         }
       }
    }
-   __syncthreads();
+   if (own)
+#pragma unroll
+      for (int c = 0; c < 16; ++c)
+         if (c <= lane) D[lane + c * LD] = r[c];
+   return bad;
+}
+
+// In-place inverse of a 16x16 lower triangular block by one wave: lane j < 16
+// computes column j of the inverse by forward substitution.
+template <typename T, int LD>
+__device__ void wave_trtri16(T* D) {
+   const int lane = threadIdx.x & 63;
+   T x[16];
+#pragma unroll
+   for (int i = 0; i < 16; ++i) {
+      T s = (i == lane) ? T(1) : T(0);
+#pragma unroll
+      for (int k = 0; k < i; ++k) s -= D[i + k * LD] * x[k];
+      x[i] = (i >= lane) ? s / D[i + i * LD] : T(0);
+   }
+   __builtin_amdgcn_wave_barrier();
+   if (lane < 16)
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+         if (i >= lane) D[i + lane * LD] = x[i];
+}
+
+// C(16x16) += A(16x16) * B(16x16)^T or A*B, operands in LDS.
+// A element (r, k) at A[r + k*lda]; B element (k, c) at B[c + k*ldb] when
+// transB (B given as rows c), else at B[k + c*ldb].
+template <typename T, bool TRANSB>
+__device__ inline typename Mfma<T>::acc_t lds_mma16(const T* A, int lda, const T* B, int ldb,
+                                                    typename Mfma<T>::acc_t acc) {
+   const int lane = threadIdx.x & 63;
+#pragma unroll
+   for (int kc = 0; kc < 16; kc += 4) {
+      const int k = kc + (lane >> 4), rc = lane & 15;
+      const T a = A[rc + k * lda];
+      const T b = TRANSB ? B[rc + k * ldb] : B[k + rc * ldb];
+      acc = Mfma<T>::op(a, b, acc);
+   }
+   return acc;
+}
+
+// Blocked Cholesky of the padded kb16 x kb16 block in LDS. All threads call.
+template <typename T, int LD>
+__device__ int lds_potrf_blocked(T* L, int kb16, int* sh_bad) {
+   typedef typename Mfma<T>::acc_t acc_t;
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int nbk = kb16 / 16;
+   for (int s = 0; s < nbk; ++s) {
+      const int o = s * 16;
+      if (wave == 0) {
+         int bad = wave_potrf16<T, LD>(L + o + o * LD);
+         if (lane == 0) *sh_bad = bad ? o + bad : 0;
+      }
+      __syncthreads();
+      if (*sh_bad) return *sh_bad;
+      // panel rows below the diagonal sub-block: X = P * inv(D)^T, one row per thread
+      const int r = o + 16 + tid;
+      if (r < kb16) {
+         T x[16];
+#pragma unroll
+         for (int c = 0; c < 16; ++c) x[c] = L[r + (o + c) * LD];
+#pragma unroll
+         for (int c = 0; c < 16; ++c) {
+            T sum = x[c];
+#pragma unroll
+            for (int cc = 0; cc < c; ++cc) sum -= x[cc] * L[(o + c) + (o + cc) * LD];
+            x[c] = sum / L[(o + c) + (o + c) * LD];
+         }
+#pragma unroll
+         for (int c = 0; c < 16; ++c) L[r + (o + c) * LD] = x[c];
+      }
+      __syncthreads();
+      // trailing update on 16x16 tiles (lower), MFMA with operands in LDS
+      const int t0 = o + 16;
+      const int nt = (kb16 - t0) / 16;
+      const int ntiles = nt * (nt + 1) / 2;
+      for (int tile = wave; tile < ntiles; tile += 4) {
+         int ti = 0;
+         while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+         const int tj = tile - ti * (ti + 1) / 2;
+         acc_t acc = acc_t{0, 0, 0, 0};
+         acc = lds_mma16<T, true>(L + t0 + ti * 16 + o * LD, LD, L + t0 + tj * 16 + o * LD, LD, acc);
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int i = t0 + ti * 16 + Mfma<T>::row(lane, v);
+            const int j = t0 + tj * 16 + (lane & 15);
+            if (i >= j) L[i + j * LD] -= acc[v];
+         }
+      }
+      __syncthreads();
+   }
    return 0;
 }
 
-// In-place inverse of a lower triangular kb x kb block (upper part set to 0).
-// tmp holds kNB elements.
+// Blocked in-place inverse of the padded lower triangular kb16 x kb16 block
+// (block columns right to left; Tmp holds kNB x 16 elements, ld kNB).
 template <typename T, int LD>
-__device__ void lds_trtri(T* L, T* tmp, int kb) {
-   const int tid = threadIdx.x;
+__device__ void lds_trtri_blocked(T* L, T* Tmp, int kb16) {
+   typedef typename Mfma<T>::acc_t acc_t;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int nbk = kb16 / 16;
+   for (int b = wave; b < nbk; b += 4) wave_trtri16<T, LD>(L + b * 16 + b * 16 * LD);
    __syncthreads();
-   for (int j = kb - 1; j >= 0; --j) {
-      if (tid < kb - j - 1) tmp[tid] = L[j + 1 + tid + j * LD];
-      __syncthreads();
-      const T ajj = T(1) / L[j + j * LD];
-      for (int i = j + 1 + tid; i < kb; i += kThreads) {
-         T s = T(0);
-         for (int c = j + 1; c <= i; ++c) s += L[i + c * LD] * tmp[c - j - 1];
-         L[i + j * LD] = -ajj * s;
+   for (int j = nbk - 2; j >= 0; --j) {
+      // T_i = sum_{k=j+1..i} X_ik L_kj for the block rows below j
+      for (int i = j + 1 + wave; i < nbk; i += 4) {
+         acc_t acc = acc_t{0, 0, 0, 0};
+         for (int k = j + 1; k <= i; ++k)
+            acc = lds_mma16<T, false>(L + i * 16 + k * 16 * LD, LD, L + k * 16 + j * 16 * LD, LD, acc);
+#pragma unroll
+         for (int v = 0; v < 4; ++v)
+            Tmp[(i * 16 + Mfma<T>::row(lane, v)) + (lane & 15) * kNB] = acc[v];
       }
       __syncthreads();
-      if (tid == 0) L[j + j * LD] = ajj;
+      // X_ij = -T_i X_jj
+      for (int i = j + 1 + wave; i < nbk; i += 4) {
+         acc_t acc = acc_t{0, 0, 0, 0};
+         acc = lds_mma16<T, false>(Tmp + i * 16, kNB, L + j * 16 + j * 16 * LD, LD, acc);
+#pragma unroll
+         for (int v = 0; v < 4; ++v)
+            L[(i * 16 + Mfma<T>::row(lane, v)) + (j * 16 + (lane & 15)) * LD] = -acc[v];
+      }
       __syncthreads();
    }
-   for (int e = tid; e < kb * kb; e += kThreads) {
-      int i = e % kb, j = e / kb;
-      if (i < j) L[i + j * LD] = T(0);
-   }
-   __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
 // Diagonal block of the large-front path: POTRF of the kb x kb block at
 // (k0, k0), L written back, its inverse written to dinv (ld = kb) for the
-// TRSM-by-GEMM and the solves.
+// TRSM-by-GEMM and the solves. With do_potrf == 0 only the inverse of the
+// already factored block is formed (small fronts, for the solves).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ tasks,
                                                    const NodeDesc* __restrict__ nodes,
                                                    T* __restrict__ arena, T* __restrict__ dinv,
-                                                   int* __restrict__ info, int do_potrf) {
+                                                   T* __restrict__ dinvT, int* __restrict__ info,
+                                                   int do_potrf) {
    constexpr int LD = kNB + 1;
    __shared__ T L[kNB * LD];
-   __shared__ T tmp[kNB];
+   __shared__ T Tmp[kNB * 16];
+   __shared__ int sh_bad;
    const DiagTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    const int kb = t.kb;
+   const int kb16 = (kb + 15) & ~15;
    T* blk = arena + nd.off + t.k0 + (int64_t)t.k0 * nd.ld;
-   for (int e = threadIdx.x; e < kb * kb; e += kThreads) {
-      int i = e % kb, j = e / kb;
-      L[i + j * LD] = (i >= j) ? blk[i + (int64_t)j * nd.ld] : T(0);
+   for (int e = threadIdx.x; e < kb16 * kb16; e += kThreads) {
+      const int i = e % kb16, j = e / kb16;
+      T v;
+      if (i < kb && j < kb)
+         v = (i >= j) ? blk[i + (int64_t)j * nd.ld] : T(0);
+      else
+         v = (i == j) ? T(1) : T(0);
+      L[i + j * LD] = v;
    }
    __syncthreads();
    if (do_potrf) {
-      int bad = lds_potrf<T, LD>(L, kb);
+      const int bad = lds_potrf_blocked<T, LD>(L, kb16, &sh_bad);
       if (bad) {
          if (threadIdx.x == 0) info[t.node] = t.k0 + bad;
          return;
       }
       for (int e = threadIdx.x; e < kb * kb; e += kThreads) {
-         int i = e % kb, j = e / kb;
+         const int i = e % kb, j = e / kb;
          if (i >= j) blk[i + (int64_t)j * nd.ld] = L[i + j * LD];
       }
    }
-   lds_trtri<T, LD>(L, tmp, kb);
+   lds_trtri_blocked<T, LD>(L, Tmp, kb16);
    T* out = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
+   T* outT = dinvT + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
    for (int e = threadIdx.x; e < kb * kb; e += kThreads) {
-      int i = e % kb, j = e / kb;
-      out[i + j * kb] = L[i + j * LD];
+      const int i = e % kb, j = e / kb;
+      out[i + j * kb] = (i >= j) ? L[i + j * LD] : T(0);
+      outT[i + j * kb] = (j >= i) ? L[j + i * LD] : T(0);
    }
 }
 
@@ -312,6 +429,222 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nt(const GemmTask* __restrict
             T* p = c + i + (int64_t)j * t.ldc;
             T old = (beta == T(0)) ? T(0) : *p;
             *p = beta * old + alpha * acc[x][y][v];
+         }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Schur-complement update of the large-front path: C -= A * B^T on one
+// 128x128 tile (A, B: K panel columns of the front, rows of the tile's row and
+// column blocks). Accumulators start from C so the epilogue only stores; A is
+// negated on its way into the MFMA. K is streamed in chunks of 16 through a
+// double-buffered LDS with the next chunk prefetched into registers (16-byte
+// loads: front rows are 8-aligned and leading dimensions a multiple of 8), one
+// barrier per chunk. Columns below `pad0` of the tile are left untouched
+// (tiles may start a few rows above the trailing block to stay aligned; those
+// rows fall in the upper triangle).
+// The launch is padded to a multiple of 8 workgroups and workgroup b runs
+// task (b % 8) * (G / 8) + b / 8: workgroups are dealt round-robin to the 8
+// XCDs, so each XCD gets a contiguous run of tiles sharing A row blocks.
+// ---------------------------------------------------------------------------
+template <typename T> struct V16;
+template <> struct V16<double> {
+   typedef double vt __attribute__((ext_vector_type(2)));
+   static constexpr int N = 2;
+};
+template <> struct V16<float> {
+   typedef float vt __attribute__((ext_vector_type(4)));
+   static constexpr int N = 4;
+};
+
+__device__ inline int xcd_task(int b, int G) { return (b & 7) * (G >> 3) + (b >> 3); }
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict__ tasks,
+                                                     T* __restrict__ arena) {
+   constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
+   constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+   typedef typename V16<T>::vt vt;
+   constexpr int VN = V16<T>::N;
+   constexpr int AV = BM * BK / VN / kThreads;
+   constexpr int BV = BN * BK / VN / kThreads;
+   typedef typename Mfma<T>::acc_t acc_t;
+   __shared__ T As[2][BK][BM + PAD];
+   __shared__ T Bs[2][BK][BN + PAD];
+   const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
+   if (t.M <= 0) return;
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int wm = wave >> 1, wn = wave & 1;
+   const int ld = t.lda;
+   const T* a = arena + t.a_off;
+   const T* b = arena + t.b_off;
+   T* c = arena + t.c_off;
+   acc_t acc[FM][FN];
+#pragma unroll
+   for (int x = 0; x < FM; ++x)
+#pragma unroll
+      for (int y = 0; y < FN; ++y) {
+         const int j = wn * WN + y * 16 + (lane & 15);
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int i = wm * WM + x * 16 + Mfma<T>::row(lane, v);
+            acc[x][y][v] = (i < t.M && j < t.N) ? c[i + (int64_t)j * ld] : T(0);
+         }
+      }
+   vt ra[AV], rb[BV];
+   auto gload = [&](int k0) {
+#pragma unroll
+      for (int u = 0; u < AV; ++u) {
+         const int v = tid + u * kThreads;
+         const int kk = v / (BM / VN), row = (v % (BM / VN)) * VN;
+         ra[u] = (k0 + kk < t.K && row < t.M) ? *reinterpret_cast<const vt*>(a + row + (int64_t)(k0 + kk) * ld)
+                                              : vt(0);
+      }
+#pragma unroll
+      for (int u = 0; u < BV; ++u) {
+         const int v = tid + u * kThreads;
+         const int kk = v / (BN / VN), row = (v % (BN / VN)) * VN;
+         rb[u] = (k0 + kk < t.K && row < t.N) ? *reinterpret_cast<const vt*>(b + row + (int64_t)(k0 + kk) * ld)
+                                              : vt(0);
+      }
+   };
+   const int nk = (t.K + BK - 1) / BK;
+   gload(0);
+   for (int kc = 0; kc < nk; ++kc) {
+      const int buf = kc & 1;
+#pragma unroll
+      for (int u = 0; u < AV; ++u) {
+         const int v = tid + u * kThreads;
+         const int kk = v / (BM / VN), row = (v % (BM / VN)) * VN;
+#pragma unroll
+         for (int e = 0; e < VN; ++e) As[buf][kk][row + e] = ra[u][e];
+      }
+#pragma unroll
+      for (int u = 0; u < BV; ++u) {
+         const int v = tid + u * kThreads;
+         const int kk = v / (BN / VN), row = (v % (BN / VN)) * VN;
+#pragma unroll
+         for (int e = 0; e < VN; ++e) Bs[buf][kk][row + e] = rb[u][e];
+      }
+      __syncthreads();
+      if (kc + 1 < nk) gload((kc + 1) * BK);
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+         T af[FM], bf[FN];
+#pragma unroll
+         for (int x = 0; x < FM; ++x) af[x] = -As[buf][kk + (lane >> 4)][wm * WM + x * 16 + (lane & 15)];
+#pragma unroll
+         for (int y = 0; y < FN; ++y) bf[y] = Bs[buf][kk + (lane >> 4)][wn * WN + y * 16 + (lane & 15)];
+#pragma unroll
+         for (int x = 0; x < FM; ++x)
+#pragma unroll
+            for (int y = 0; y < FN; ++y) acc[x][y] = Mfma<T>::op(af[x], bf[y], acc[x][y]);
+      }
+   }
+#pragma unroll
+   for (int x = 0; x < FM; ++x)
+#pragma unroll
+      for (int y = 0; y < FN; ++y) {
+         const int j = wn * WN + y * 16 + (lane & 15);
+         if (j >= t.N || j < t.pad0) continue;
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int i = wm * WM + x * 16 + Mfma<T>::row(lane, v);
+            if (i < t.M) c[i + (int64_t)j * ld] = acc[x][y][v];
+         }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// BF16 Schur update for wide fronts in the mixed-precision tier: FP32 front
+// storage, the K chunk (32 columns) of A and B rounded to BF16 (round to
+// nearest even) as it is staged into LDS in k-contiguous rows, products on the
+// 16x16x32 BF16 MFMA accumulated in FP32 (accumulators start from C). Lane l
+// of the MFMA reads row l%16, k = 8*(l/16)..+7 of each operand (one 16-byte
+// LDS read). BF16 keeps FP32's 8-bit exponent, so a power-of-two rescaling of
+// a block would not change any rounding; none is applied.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__global__ __launch_bounds__(kThreads) void k_update_bf16(const GemmTask* __restrict__ tasks,
+                                                          float* __restrict__ arena) {
+   constexpr int BM = 128, BN = 128, BK = 32, PADK = 8;
+   constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+   typedef float f4 __attribute__((ext_vector_type(4)));
+   constexpr int AV = BM * BK / 4 / kThreads;  // float4 vectors per thread per chunk
+   __shared__ __attribute__((aligned(16))) __bf16 As[2][BM][BK + PADK];
+   __shared__ __attribute__((aligned(16))) __bf16 Bs[2][BN][BK + PADK];
+   const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
+   if (t.M <= 0) return;
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int wm = wave >> 1, wn = wave & 1;
+   const int ld = t.lda;
+   const float* a = arena + t.a_off;
+   const float* b = arena + t.b_off;
+   float* c = arena + t.c_off;
+   v4f acc[FM][FN];
+#pragma unroll
+   for (int x = 0; x < FM; ++x)
+#pragma unroll
+      for (int y = 0; y < FN; ++y) {
+         const int j = wn * WN + y * 16 + (lane & 15);
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int i = wm * WM + x * 16 + 4 * (lane >> 4) + v;
+            acc[x][y][v] = (i < t.M && j < t.N) ? c[i + (int64_t)j * ld] : 0.0f;
+         }
+      }
+   f4 ra[AV], rb[AV];
+   auto gload = [&](int k0) {
+#pragma unroll
+      for (int u = 0; u < AV; ++u) {
+         const int v = tid + u * kThreads;
+         const int kk = v / (BM / 4), row = (v % (BM / 4)) * 4;
+         ra[u] = (k0 + kk < t.K && row < t.M) ? *reinterpret_cast<const f4*>(a + row + (int64_t)(k0 + kk) * ld)
+                                              : f4(0.0f);
+         rb[u] = (k0 + kk < t.K && row < t.N) ? *reinterpret_cast<const f4*>(b + row + (int64_t)(k0 + kk) * ld)
+                                              : f4(0.0f);
+      }
+   };
+   const int nk = (t.K + BK - 1) / BK;
+   gload(0);
+   for (int kc = 0; kc < nk; ++kc) {
+      const int buf = kc & 1;
+#pragma unroll
+      for (int u = 0; u < AV; ++u) {
+         const int v = tid + u * kThreads;
+         const int kk = v / (BM / 4), row = (v % (BM / 4)) * 4;
+#pragma unroll
+         for (int e = 0; e < 4; ++e) {
+            As[buf][row + e][kk] = (__bf16)(-ra[u][e]);
+            Bs[buf][row + e][kk] = (__bf16)(rb[u][e]);
+         }
+      }
+      __syncthreads();
+      if (kc + 1 < nk) gload((kc + 1) * BK);
+      bf16x8 af[FM], bf[FN];
+#pragma unroll
+      for (int x = 0; x < FM; ++x)
+         af[x] = *reinterpret_cast<const bf16x8*>(&As[buf][wm * WM + x * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+      for (int y = 0; y < FN; ++y)
+         bf[y] = *reinterpret_cast<const bf16x8*>(&Bs[buf][wn * WN + y * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+      for (int x = 0; x < FM; ++x)
+#pragma unroll
+         for (int y = 0; y < FN; ++y)
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[x], bf[y], acc[x][y], 0, 0, 0);
+   }
+#pragma unroll
+   for (int x = 0; x < FM; ++x)
+#pragma unroll
+      for (int y = 0; y < FN; ++y) {
+         const int j = wn * WN + y * 16 + (lane & 15);
+         if (j >= t.N || j < t.pad0) continue;
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int i = wm * WM + x * 16 + 4 * (lane >> 4) + v;
+            if (i < t.M) c[i + (int64_t)j * ld] = acc[x][y][v];
          }
       }
 }
@@ -429,84 +762,165 @@ __global__ __launch_bounds__(kThreads) void k_small_front(const int* __restrict_
 
 // ---------------------------------------------------------------------------
 // Solves. Vectors are FP64 whatever the factor precision. Each node owns a
-// column-major m x nrhs work vector at solve_off (ld = m).
+// column-major m x nrhs work vector at solve_off*nrhs (ld = m). Forward:
+// gather, then per block [k0, k0+kb): y = inv(L_kk) x_k (in place and into the
+// solution), rows below -= L[r, k] y. Backward: gather, own rows -= L21^T x_cb,
+// then per block from the last: x_k = inv(L_kk)^T x_k, earlier rows -=
+// L[k, j]^T x_k. A task covers rows [r0, r1) of one node (or columns, for the
+// transposed products).
 // ---------------------------------------------------------------------------
-// Forward gather: own rows from b, CB rows zero, then children's CB parts
-// added in child-list order (deterministic).
-__global__ __launch_bounds__(kThreads) void k_fwd_gather(const int* __restrict__ node_list,
+__device__ inline int lower_bound_i(const int* a, int n, int v) {
+   int lo = 0, hi = n;
+   while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (a[mid] < v)
+         lo = mid + 1;
+      else
+         hi = mid;
+   }
+   return lo;
+}
+
+// Forward gather of rows [r0, r1): own rows from b, the others zero, then the
+// children's contribution rows that map into the range (the child map is
+// increasing, so they are a contiguous run found by binary search), children
+// in list order.
+__global__ __launch_bounds__(kThreads) void k_fwd_gather(const SolveTask* __restrict__ tasks,
                                                          const NodeDesc* __restrict__ nodes,
                                                          const int* __restrict__ rlist,
                                                          const int* __restrict__ cmap,
                                                          const double* __restrict__ b, int ldb,
                                                          double* __restrict__ xloc, int nrhs) {
-   const int node = node_list[blockIdx.x];
-   const NodeDesc nd = nodes[node];
+   const SolveTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
    double* xl = xloc + nd.solve_off * nrhs;
-   const int m = nd.m;
-   for (int e = threadIdx.x; e < m * nrhs; e += kThreads) {
-      int r = e % m, c = e / m;
+   const int m = nd.m, R = t.r1 - t.r0;
+   for (int e = threadIdx.x; e < R * nrhs; e += kThreads) {
+      const int r = t.r0 + e % R, c = e / R;
       xl[r + (int64_t)c * m] = (r < nd.n) ? b[rlist[nd.rlist_off + r] - 1 + (int64_t)c * ldb] : 0.0;
    }
    for (int ch = nd.first_child; ch != -1;) {
       __syncthreads();
       const NodeDesc cn = nodes[ch];
       const int q = cn.m - cn.n;
+      const int* cm = cmap + cn.cmap_off;
+      const int i0 = lower_bound_i(cm, q, t.r0);
+      const int i1 = lower_bound_i(cm, q, t.r1);
+      const int Q = i1 - i0;
       const double* cx = xloc + cn.solve_off * nrhs;
-      for (int e = threadIdx.x; e < q * nrhs; e += kThreads) {
-         int i = e % q, c = e / q;
-         xl[cmap[cn.cmap_off + i] + (int64_t)c * m] += cx[cn.n + i + (int64_t)c * cn.m];
+      for (int e = threadIdx.x; e < Q * nrhs; e += kThreads) {
+         const int i = i0 + e % Q, c = e / Q;
+         xl[cm[i] + (int64_t)c * m] += cx[cn.n + i + (int64_t)c * cn.m];
       }
       ch = cn.next_child;
    }
 }
 
-// Forward step on block [k0, k0+kb): y = inv(L_kk) x_k (written to xsol by the
-// task with write=1), then rows [r0, r1) of the work vector -= L[r, k] y.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fwd_step(const SolveTask* __restrict__ tasks,
-                                                       const NodeDesc* __restrict__ nodes,
-                                                       const T* __restrict__ arena,
-                                                       const T* __restrict__ dinv,
-                                                       const int* __restrict__ rlist,
-                                                       double* __restrict__ xloc,
-                                                       double* __restrict__ xsol, int ldsol,
-                                                       int nrhs) {
+// Backward gather of rows [r0, r1) from the current solution.
+__global__ __launch_bounds__(kThreads) void k_bwd_gather(const SolveTask* __restrict__ tasks,
+                                                         const NodeDesc* __restrict__ nodes,
+                                                         const int* __restrict__ rlist,
+                                                         const double* __restrict__ xsol,
+                                                         int ldsol, double* __restrict__ xloc,
+                                                         int nrhs) {
+   const SolveTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
+   double* xl = xloc + nd.solve_off * nrhs;
+   const int m = nd.m, R = t.r1 - t.r0;
+   for (int e = threadIdx.x; e < R * nrhs; e += kThreads) {
+      const int r = t.r0 + e % R, c = e / R;
+      xl[r + (int64_t)c * m] = xsol[rlist[nd.rlist_off + r] - 1 + (int64_t)c * ldsol];
+   }
+}
+
+// y = M x_k for the kb x kb triangular block M (ld kb) of dinv (forward:
+// inv(L_kk), lower; backward: its transpose, upper); y overwrites x_k in the
+// work vector and is written to the solution.
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(kThreads) void k_solve_diag(const SolveTask* __restrict__ tasks,
+                                                         const NodeDesc* __restrict__ nodes,
+                                                         const T* __restrict__ dinv,
+                                                         const int* __restrict__ rlist,
+                                                         double* __restrict__ xloc,
+                                                         double* __restrict__ xsol, int ldsol,
+                                                         int nrhs) {
    __shared__ double xs[kNB * kMaxRhsChunk];
+   const SolveTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
+   const int m = nd.m, kb = t.kb;
+   double* xl = xloc + nd.solve_off * nrhs;
+   const T* Mk = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
+   for (int c0 = 0; c0 < nrhs; c0 += kMaxRhsChunk) {
+      const int nc = min(kMaxRhsChunk, nrhs - c0);
+      __syncthreads();
+      for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
+         const int i = e % kb, c = e / kb;
+         xs[i + c * kNB] = xl[t.k0 + i + (int64_t)(c0 + c) * m];
+      }
+      __syncthreads();
+      const int i = threadIdx.x;
+      if (i < kb) {
+         double acc[kMaxRhsChunk];
+#pragma unroll
+         for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = 0.0;
+         const int l0 = UPPER ? i : 0, l1 = UPPER ? kb : i + 1;
+         // 16 independent loads in flight per batch; entries outside the
+         // triangle are stored as zeros, so whole batches can be read
+         for (int lb = l0 & ~15; lb < l1; lb += 16) {
+            double mv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) mv[u] = (lb + u < kb) ? (double)Mk[i + (lb + u) * kb] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+               if (lb + u >= kb) break;
+#pragma unroll
+               for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += mv[u] * xs[lb + u + c * kNB];
+            }
+         }
+         const int row = rlist[nd.rlist_off + t.k0 + i] - 1;
+#pragma unroll
+         for (int c = 0; c < kMaxRhsChunk; ++c)
+            if (c < nc) {
+               xl[t.k0 + i + (int64_t)(c0 + c) * m] = acc[c];
+               xsol[row + (int64_t)(c0 + c) * ldsol] = acc[c];
+            }
+      }
+   }
+}
+
+// Forward update of rows [r0, r1): x_r -= L[r, k0:k0+kb] y (one row per thread).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fwd_upd(const SolveTask* __restrict__ tasks,
+                                                      const NodeDesc* __restrict__ nodes,
+                                                      const T* __restrict__ arena,
+                                                      double* __restrict__ xloc, int nrhs) {
    __shared__ double ys[kNB * kMaxRhsChunk];
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    const int m = nd.m, kb = t.kb;
    double* xl = xloc + nd.solve_off * nrhs;
-   const T* Li = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
    const T* F = arena + nd.off;
    for (int c0 = 0; c0 < nrhs; c0 += kMaxRhsChunk) {
       const int nc = min(kMaxRhsChunk, nrhs - c0);
       __syncthreads();
-      for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
-         int i = e % kb, c = e / kb;
-         xs[i + c * kNB] = xl[t.k0 + i + (int64_t)(c0 + c) * m];
+      for (int e = threadIdx.x; e < kNB * kMaxRhsChunk; e += kThreads) {
+         const int i = e % kNB, c = e / kNB;
+         ys[i + c * kNB] = (c < nc && i < kb) ? xl[t.k0 + i + (int64_t)(c0 + c) * m] : 0.0;
       }
       __syncthreads();
-      for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
-         int i = e % kb, c = e / kb;
-         double s = 0.0;
-         for (int l = 0; l <= i; ++l) s += (double)Li[i + l * kb] * xs[l + c * kNB];
-         ys[i + c * kNB] = s;
-      }
-      __syncthreads();
-      if (t.write)
-         for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
-            int i = e % kb, c = e / kb;
-            xsol[rlist[nd.rlist_off + t.k0 + i] - 1 + (int64_t)(c0 + c) * ldsol] = ys[i + c * kNB];
-         }
       for (int r = t.r0 + threadIdx.x; r < t.r1; r += kThreads) {
          double acc[kMaxRhsChunk];
 #pragma unroll
          for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = 0.0;
-         for (int l = 0; l < kb; ++l) {
-            const double lv = (double)F[r + (int64_t)(t.k0 + l) * nd.ld];
+         const T* Fr = F + r + (int64_t)t.k0 * nd.ld;
+         for (int lb = 0; lb < kb; lb += 16) {
+            double lv[16];
 #pragma unroll
-            for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += lv * ys[l + c * kNB];
+            for (int u = 0; u < 16; ++u) lv[u] = (lb + u < kb) ? (double)Fr[(int64_t)(lb + u) * nd.ld] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+#pragma unroll
+               for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += lv[u] * ys[lb + u + c * kNB];
          }
 #pragma unroll
          for (int c = 0; c < kMaxRhsChunk; ++c)
@@ -515,129 +929,80 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const SolveTask* __restri
    }
 }
 
-// Backward gather: every row of the node's work vector from the current
-// solution (own rows hold the forward result, CB rows the final values).
-__global__ __launch_bounds__(kThreads) void k_bwd_gather(const int* __restrict__ node_list,
-                                                         const NodeDesc* __restrict__ nodes,
-                                                         const int* __restrict__ rlist,
-                                                         const double* __restrict__ xsol,
-                                                         int ldsol, double* __restrict__ xloc,
-                                                         int nrhs) {
-   const int node = node_list[blockIdx.x];
-   const NodeDesc nd = nodes[node];
-   double* xl = xloc + nd.solve_off * nrhs;
-   const int m = nd.m;
-   for (int e = threadIdx.x; e < m * nrhs; e += kThreads) {
-      int r = e % m, c = e / m;
-      xl[r + (int64_t)c * m] = xsol[rlist[nd.rlist_off + r] - 1 + (int64_t)c * ldsol];
-   }
-}
-
-// Wave-wide sum.
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-   return v;
-}
-
-// Backward CB step: x[j] -= sum_{r >= n} L[r, j] x[r] for the columns
-// [r0, r1) of the task (one wave per column).
+// Transposed update x_j -= sum_{r in [k0, kb)} L[r, j] x_r for the columns
+// j in [r0, r1) (at most 64 per task): the front rows [k0, kb) (here k0 is the
+// first row and kb one past the last) are streamed through LDS in chunks of
+// 64 rows, the next chunk loaded into registers while the current one is
+// used; each thread owns one column and four right-hand sides and keeps a
+// partial sum per chunk.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd_cb(const SolveTask* __restrict__ tasks,
-                                                     const NodeDesc* __restrict__ nodes,
-                                                     const T* __restrict__ arena,
-                                                     double* __restrict__ xloc, int nrhs) {
+__global__ __launch_bounds__(kThreads) void k_bwd_upd(const SolveTask* __restrict__ tasks,
+                                                      const NodeDesc* __restrict__ nodes,
+                                                      const T* __restrict__ arena,
+                                                      double* __restrict__ xloc, int nrhs) {
+   constexpr int RC = 64, JC = 64;
+   constexpr int AE = RC * JC / kThreads;            // A elements per thread per chunk
+   constexpr int BE = RC * kMaxRhsChunk / kThreads;  // B elements per thread per chunk
+   __shared__ T As[RC][JC + 1];
+   __shared__ double Bs[RC][kMaxRhsChunk];
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    const int m = nd.m;
    double* xl = xloc + nd.solve_off * nrhs;
    const T* F = arena + nd.off;
-   for (int j = t.r0 + wave; j < t.r1; j += 4) {
-      for (int c0 = 0; c0 < nrhs; c0 += kMaxRhsChunk) {
-         const int nc = min(kMaxRhsChunk, nrhs - c0);
-         double acc[kMaxRhsChunk];
-#pragma unroll
-         for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = 0.0;
-         for (int r = nd.n + lane; r < m; r += 64) {
-            const double lv = (double)F[r + (int64_t)j * nd.ld];
-#pragma unroll
-            for (int c = 0; c < kMaxRhsChunk; ++c)
-               if (c < nc) acc[c] += lv * xl[r + (int64_t)(c0 + c) * m];
-         }
-#pragma unroll
-         for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = wave_sum(acc[c]);
-         if (lane < nc) {
-            double s = 0.0;
-#pragma unroll
-            for (int c = 0; c < kMaxRhsChunk; ++c)
-               if (c == lane) s = acc[c];
-            xl[j + (int64_t)(c0 + lane) * m] -= s;
-         }
-      }
-   }
-}
-
-// Backward step on block [k0, k0+kb): x_k = inv(L_kk)^T x_k (written to xsol
-// by the task with write=1), then columns [r0, r1) (< k0) of the work vector
-// -= L[k0:k0+kb, j]^T x_k (one wave per column).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd_step(const SolveTask* __restrict__ tasks,
-                                                       const NodeDesc* __restrict__ nodes,
-                                                       const T* __restrict__ arena,
-                                                       const T* __restrict__ dinv,
-                                                       const int* __restrict__ rlist,
-                                                       double* __restrict__ xloc,
-                                                       double* __restrict__ xsol, int ldsol,
-                                                       int nrhs) {
-   __shared__ double xs[kNB * kMaxRhsChunk];
-   __shared__ double ys[kNB * kMaxRhsChunk];
-   const SolveTask t = tasks[blockIdx.x];
-   const NodeDesc nd = nodes[t.node];
-   const int m = nd.m, kb = t.kb;
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-   double* xl = xloc + nd.solve_off * nrhs;
-   const T* Li = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
-   const T* F = arena + nd.off;
+   const int jj = threadIdx.x & 63, cq = threadIdx.x >> 6;
+   const int rbeg = t.k0, rend = t.kb;
    for (int c0 = 0; c0 < nrhs; c0 += kMaxRhsChunk) {
       const int nc = min(kMaxRhsChunk, nrhs - c0);
-      __syncthreads();
-      for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
-         int i = e % kb, c = e / kb;
-         xs[i + c * kNB] = xl[t.k0 + i + (int64_t)(c0 + c) * m];
-      }
-      __syncthreads();
-      for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
-         int i = e % kb, c = e / kb;
-         double s = 0.0;
-         for (int l = i; l < kb; ++l) s += (double)Li[l + i * kb] * xs[l + c * kNB];
-         ys[i + c * kNB] = s;
-      }
-      __syncthreads();
-      if (t.write)
-         for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
-            int i = e % kb, c = e / kb;
-            xsol[rlist[nd.rlist_off + t.k0 + i] - 1 + (int64_t)(c0 + c) * ldsol] = ys[i + c * kNB];
-         }
-      for (int j = t.r0 + wave; j < t.r1; j += 4) {
-         double acc[kMaxRhsChunk];
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      T ra[AE];
+      double rb[BE];
+      auto load = [&](int rb0) {
 #pragma unroll
-         for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = 0.0;
-         for (int l = lane; l < kb; l += 64) {
-            const double lv = (double)F[t.k0 + l + (int64_t)j * nd.ld];
-#pragma unroll
-            for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += lv * ys[l + c * kNB];
+         for (int u = 0; u < AE; ++u) {
+            const int e = threadIdx.x + u * kThreads;
+            const int rr = e % RC, jc = e / RC;
+            const int r = rb0 + rr, j = t.r0 + jc;
+            ra[u] = (r < rend && j < t.r1) ? F[r + (int64_t)j * nd.ld] : T(0);
          }
 #pragma unroll
-         for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = wave_sum(acc[c]);
-         if (lane < nc) {
-            double s = 0.0;
-#pragma unroll
-            for (int c = 0; c < kMaxRhsChunk; ++c)
-               if (c == lane) s = acc[c];
-            xl[j + (int64_t)(c0 + lane) * m] -= s;
+         for (int u = 0; u < BE; ++u) {
+            const int e = threadIdx.x + u * kThreads;
+            const int rr = e % RC, c = e / RC;
+            const int r = rb0 + rr;
+            rb[u] = (r < rend && c < nc) ? xl[r + (int64_t)(c0 + c) * m] : 0.0;
          }
+      };
+      load(rbeg);
+      for (int rb0 = rbeg; rb0 < rend; rb0 += RC) {
+         __syncthreads();
+#pragma unroll
+         for (int u = 0; u < AE; ++u) {
+            const int e = threadIdx.x + u * kThreads;
+            As[e % RC][e / RC] = ra[u];
+         }
+#pragma unroll
+         for (int u = 0; u < BE; ++u) {
+            const int e = threadIdx.x + u * kThreads;
+            Bs[e % RC][e / RC] = rb[u];
+         }
+         __syncthreads();
+         if (rb0 + RC < rend) load(rb0 + RC);
+         double part[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 16
+         for (int rr = 0; rr < RC; ++rr) {
+            const double a = (double)As[rr][jj];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) part[q] += a * Bs[rr][4 * cq + q];
+         }
+#pragma unroll
+         for (int q = 0; q < 4; ++q) acc[q] += part[q];
       }
+      const int j = t.r0 + jj;
+      if (j < t.r1)
+#pragma unroll
+         for (int q = 0; q < 4; ++q)
+            if (4 * cq + q < nc) xl[j + (int64_t)(c0 + 4 * cq + q) * m] -= acc[q];
    }
 }
 
@@ -677,6 +1042,84 @@ __global__ __launch_bounds__(kThreads) void k_colmax(int n, const double* __rest
       m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
       atomicMax(out + c, (unsigned long long)__double_as_longlong(m));
    }
+}
+
+// y = A x (both triangles, pivot order), one row per thread.
+__global__ __launch_bounds__(kThreads) void k_spmv(int n, const int64_t* __restrict__ rowptr,
+                                                   const int* __restrict__ col,
+                                                   const double* __restrict__ val,
+                                                   const double* __restrict__ x,
+                                                   double* __restrict__ y, int nrhs) {
+   const int i = blockIdx.x * kThreads + threadIdx.x;
+   if (i >= n) return;
+   for (int c = 0; c < nrhs; ++c) {
+      double s = 0.0;
+      for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) s += val[p] * x[col[p] + (int64_t)c * n];
+      y[i + (int64_t)c * n] = s;
+   }
+}
+
+// Column dot products <x_c, y_c>, reproducible: a fixed grid of kDotBlocks
+// blocks per column writes one partial each (fixed thread-to-element map,
+// fixed tree), then k_dot_final adds the partials in index order.
+constexpr int kDotBlocks = 256;
+// grid (kDotBlocks, nrhs, nvec): <x_c, (y + z*ystride)_c> for vector z.
+__global__ __launch_bounds__(kThreads) void k_dot_partial(int n, const double* __restrict__ x,
+                                                          const double* __restrict__ ybase,
+                                                          int64_t ystride,
+                                                          double* __restrict__ part) {
+   const int c = blockIdx.y;
+   const double* y = ybase + (int64_t)blockIdx.z * ystride;
+   double s = 0.0;
+   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += kDotBlocks * kThreads)
+      s += x[i + (int64_t)c * n] * y[i + (int64_t)c * n];
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+   __shared__ double ws[4];
+   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+   __syncthreads();
+   if (threadIdx.x == 0)
+      part[((int64_t)blockIdx.z * gridDim.y + c) * kDotBlocks + blockIdx.x] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+// count = nvec * nrhs sums of kDotBlocks partials, in index order.
+__global__ void k_dot_final(const double* __restrict__ part, int count, double* __restrict__ out) {
+   const int c = blockIdx.x * blockDim.x + threadIdx.x;
+   if (c >= count) return;
+   double s = 0.0;
+   for (int b = 0; b < kDotBlocks; ++b) s += part[c * kDotBlocks + b];
+   out[c] = s;
+}
+
+// y_c += a_c x_c with per-column coefficients in device memory.
+__global__ __launch_bounds__(kThreads) void k_axpy_cols(int n, int nrhs, const double* __restrict__ a,
+                                                        const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+   const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+   if (e >= (int64_t)n * nrhs) return;
+   y[e] += a[e / n] * x[e];
+}
+
+// W_c -= sum_{z < nvec} h[z*nrhs + c] V_z,c (the Gram-Schmidt update).
+__global__ __launch_bounds__(kThreads) void k_multi_axpy(int n, int nrhs, int nvec,
+                                                         const double* __restrict__ h,
+                                                         const double* __restrict__ V,
+                                                         int64_t vstride, double* __restrict__ W) {
+   const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+   if (e >= (int64_t)n * nrhs) return;
+   const int c = (int)(e / n);
+   double s = W[e];
+   for (int z = 0; z < nvec; ++z) s -= h[z * nrhs + c] * V[z * vstride + e];
+   W[e] = s;
+}
+
+// y_c = a_c x_c.
+__global__ __launch_bounds__(kThreads) void k_scale_cols(int n, int nrhs, const double* __restrict__ a,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ y) {
+   const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+   if (e >= (int64_t)n * nrhs) return;
+   y[e] = a[e / n] * x[e];
 }
 
 __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.hpp"
@@ -68,11 +69,13 @@ enum LaunchKind {
    L_TRSM,
    L_UPDATE,
    L_DINV,
+   L_UPDATE_WIDE,
    L_FGATHER,
-   L_FSTEP,
+   L_FDIAG,
+   L_FUPD,
    L_BGATHER,
-   L_BCB,
-   L_BSTEP
+   L_BDIAG,
+   L_BUPD
 };
 
 struct Launch {
@@ -115,6 +118,7 @@ int device_count() {
 struct DevicePlan {
    int n = 0, nnodes = 0, nlevels = 0;
    int small_max = 256;
+   int wide_ncol = 1024;  // fronts with more pivots take the wide-front update launches
    std::vector<NodeDesc> hnodes;
    std::vector<std::vector<int>> levels;
    std::vector<int> hcmap;
@@ -170,6 +174,7 @@ struct DeviceFactors {
    bool posdef = true;
    void* arena = nullptr;
    void* dinv = nullptr;
+   void* dinvT = nullptr;
    double* d_aval = nullptr;
    double* d_scaling = nullptr;
    double* d_csr_val = nullptr;
@@ -186,6 +191,13 @@ struct DeviceFactors {
    double* d_r = nullptr;
    double* d_sol = nullptr;
    unsigned long long* d_norms = nullptr;
+   // GMRES-IR workspace (allocated on first use)
+   int gm_nrhs = 0, gm_m = 0;
+   double* d_V = nullptr;      // (gm_m + 1) Krylov vectors, n x nrhs each
+   double* d_W = nullptr;
+   double* d_tmp = nullptr;
+   double* d_part = nullptr;
+   double* d_coef = nullptr;
    // statistics
    double factor_ms = 0, upload_ms = 0, solve_ms = 0, bwd_err = -1;
    int refine_iters = 0;
@@ -197,9 +209,10 @@ struct DeviceFactors {
 
    ~DeviceFactors() {
       if (graph) (void)hipGraphExecDestroy(graph);
-      for (void* p : {arena, dinv, (void*)d_aval, (void*)d_scaling, (void*)d_csr_val,
+      for (void* p : {arena, dinv, dinvT, (void*)d_aval, (void*)d_scaling, (void*)d_csr_val,
                       (void*)d_info, (void*)d_xloc, (void*)d_b, (void*)d_x, (void*)d_r,
-                      (void*)d_sol, (void*)d_norms})
+                      (void*)d_sol, (void*)d_norms, (void*)d_V, (void*)d_W, (void*)d_tmp,
+                      (void*)d_part, (void*)d_coef})
          dfree(p);
       if (stream) (void)hipStreamDestroy(stream);
    }
@@ -210,11 +223,12 @@ void destroy_factors(DeviceFactors* f) { delete f; }
 
 namespace {
 
-DevicePlan* build_plan(const Analysis& a, int small_max) {
+DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    auto* P = new DevicePlan();
    P->n = a.n;
    P->nnodes = a.nnodes;
    P->small_max = small_max;
+   P->wide_ncol = wide_ncol;
    const int nn = a.nnodes;
    P->hnodes.resize(nn);
    std::vector<int> height(nn + 1, 0);
@@ -224,7 +238,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max) {
       d.m = f.nrow;
       d.n = f.ncol;
       d.parent = f.parent;
-      d.ld = f.nrow;
+      d.ld = (int)align_up(f.nrow, 8);  // 8-aligned columns for 16-byte loads
       d.rlist_off = f.rlist_off;
       d.amap_off = f.amap_off;
       d.num_a = f.num_a;
@@ -369,30 +383,61 @@ DevicePlan* build_plan(const Analysis& a, int small_max) {
             }
          }
          push(P->fprog, L_TRSM, t0, P->gemm.size() - t0);
+         // Schur update. Block columns are paired: after the first of a pair
+         // only the next panel's columns are updated (narrow update), after the
+         // second the trailing block receives both panels at once (K = 256),
+         // halving the passes over the contribution block.
+         const bool first_of_pair = ((k0 / kNB) % 2) == 0;
+         for (int wide = 0; wide < 2; ++wide) {
          int64_t u0 = P->gemm.size();
          for (int i : large) {
             const NodeDesc& d = P->hnodes[i];
             if (k0 >= d.n) continue;
-            int kb = std::min(kNB, d.n - k0);
-            int r0 = k0 + kb;
-            int R = d.m - r0;
-            int nt = (R + 127) / 128;
+            if ((d.n > wide_ncol) != (wide == 1)) continue;
+            const int kb = std::min(kNB, d.n - k0);
+            if (first_of_pair && k0 + kNB < d.n) {
+               const int k1 = k0 + kNB, kb1 = std::min(kNB, d.n - k1);
+               for (int r = k1; r < d.m; r += 128) {
+                  GemmTask g{};
+                  g.a_off = d.off + r + (int64_t)k0 * d.ld;
+                  g.b_off = d.off + k1 + (int64_t)k0 * d.ld;
+                  g.c_off = d.off + r + (int64_t)k1 * d.ld;
+                  g.lda = g.ldb = g.ldc = d.ld;
+                  g.M = std::min(128, d.m - r);
+                  g.N = kb1;
+                  g.K = kb;
+                  g.pad0 = 0;
+                  P->gemm.push_back(g);
+               }
+               continue;
+            }
+            const int kstart = first_of_pair ? k0 : k0 - kNB;
+            const int K = k0 + kb - kstart;
+            const int r0 = k0 + kb;
+            const int r0a = r0 & ~7;
+            const int R = d.m - r0a;
+            if (d.m <= r0) continue;
+            const int nt = (R + 127) / 128;
             for (int ti = 0; ti < nt; ++ti)
                for (int tj = 0; tj <= ti; ++tj) {
                   GemmTask g{};
-                  g.a_off = d.off + r0 + ti * 128 + (int64_t)k0 * d.ld;
-                  g.b_off = d.off + r0 + tj * 128 + (int64_t)k0 * d.ld;
-                  g.c_off = d.off + (r0 + ti * 128) + (int64_t)(r0 + tj * 128) * d.ld;
-                  g.lda = d.ld;
-                  g.ldb = d.ld;
-                  g.ldc = d.ld;
+                  g.a_off = d.off + r0a + ti * 128 + (int64_t)kstart * d.ld;
+                  g.b_off = d.off + r0a + tj * 128 + (int64_t)kstart * d.ld;
+                  g.c_off = d.off + (r0a + ti * 128) + (int64_t)(r0a + tj * 128) * d.ld;
+                  g.lda = g.ldb = g.ldc = d.ld;
                   g.M = std::min(128, R - ti * 128);
                   g.N = std::min(128, R - tj * 128);
-                  g.K = kb;
+                  g.K = K;
+                  g.pad0 = (tj == 0) ? r0 - r0a : 0;
                   P->gemm.push_back(g);
                }
          }
-         push(P->fprog, L_UPDATE, u0, P->gemm.size() - u0);
+         while ((P->gemm.size() - u0) % 8 != 0 && P->gemm.size() > (size_t)u0) {
+            GemmTask g{};
+            P->gemm.push_back(g);  // empty task: keeps the launch a multiple of 8
+         }
+         push(P->fprog, wide ? L_UPDATE_WIDE : L_UPDATE, u0, P->gemm.size() - u0);
+         }
       }
    }
    // inverted diagonal blocks of the small fronts (the large path writes its own)
@@ -452,12 +497,17 @@ DevicePlan* build_plan(const Analysis& a, int small_max) {
                L.words += 2.0 * g.M * g.K + 0.5 * g.K * g.K;
                break;
             }
-            case L_UPDATE: {
+            case L_UPDATE:
+            case L_UPDATE_WIDE: {
                const GemmTask& g = P->gemm[t];
-               if (g.a_off == g.b_off)
-                  L.flops += (double)g.M * (g.M + 1) * g.K;
-               else
-                  L.flops += 2.0 * g.M * g.N * g.K;
+               if (g.M <= 0) break;
+               const int rows_skip = g.pad0;  // aligned-down rows of the first tile
+               if (g.a_off == g.b_off) {
+                  const double mm = g.M - rows_skip;
+                  L.flops += mm * (mm + 1) * g.K;
+               } else {
+                  L.flops += 2.0 * g.M * (g.N - rows_skip) * g.K;
+               }
                L.words += 2.0 * g.M * g.N + (double)(g.M + g.N) * g.K;
                break;
             }
@@ -468,61 +518,85 @@ DevicePlan* build_plan(const Analysis& a, int small_max) {
    }
 
    // ---- solve programs ----
+   // Forward: per level, gather (row chunks), then per block column a
+   // diagonal multiply and the update of the rows below. Backward: per level
+   // from the top, gather, the contribution-block transposed update, then per
+   // block column from the last a diagonal multiply and the transposed update
+   // of the earlier rows.
+   constexpr int kRowChunk = 256;
+   constexpr int kColChunk = 64;
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
-      int64_t g0 = P->nlistv.size();
-      for (int i : lv) P->nlistv.push_back(i);
-      push(P->fwdprog, L_FGATHER, g0, lv.size());
+      int64_t g0 = P->stask.size();
+      for (int i : lv) {
+         const NodeDesc& d = P->hnodes[i];
+         for (int r = 0; r < d.m; r += kRowChunk)
+            P->stask.push_back(SolveTask{i, 0, 0, r, std::min(d.m, r + kRowChunk), 0});
+      }
+      push(P->fwdprog, L_FGATHER, g0, P->stask.size() - g0);
       int maxblk = 0;
       for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
-      for (int s = 0; s < maxblk; ++s) {
+      for (int sblk = 0; sblk < maxblk; ++sblk) {
          int64_t t0 = P->stask.size();
          for (int i : lv) {
             const NodeDesc& d = P->hnodes[i];
-            int k0 = s * kNB;
+            const int k0 = sblk * kNB;
             if (k0 >= d.n) continue;
-            int kb = std::min(kNB, d.n - k0);
-            int rb = k0 + kb;
-            if (rb >= d.m) {
-               P->stask.push_back(SolveTask{i, k0, kb, rb, rb, 1});
-            } else {
-               for (int r = rb; r < d.m; r += kThreads)
-                  P->stask.push_back(SolveTask{i, k0, kb, r, std::min(d.m, r + kThreads), r == rb ? 1 : 0});
-            }
+            P->stask.push_back(SolveTask{i, k0, std::min(kNB, d.n - k0), 0, 0, 1});
          }
-         push(P->fwdprog, L_FSTEP, t0, P->stask.size() - t0);
+         push(P->fwdprog, L_FDIAG, t0, P->stask.size() - t0);
+         int64_t u0 = P->stask.size();
+         for (int i : lv) {
+            const NodeDesc& d = P->hnodes[i];
+            const int k0 = sblk * kNB;
+            if (k0 >= d.n) continue;
+            const int kb = std::min(kNB, d.n - k0);
+            for (int r = k0 + kb; r < d.m; r += kRowChunk)
+               P->stask.push_back(SolveTask{i, k0, kb, r, std::min(d.m, r + kRowChunk), 0});
+         }
+         push(P->fwdprog, L_FUPD, u0, P->stask.size() - u0);
       }
    }
    for (int L = nlev - 1; L >= 0; --L) {
       const std::vector<int>& lv = P->levels[L];
-      int64_t g0 = P->nlistv.size();
-      for (int i : lv) P->nlistv.push_back(i);
-      push(P->bwdprog, L_BGATHER, g0, lv.size());
+      int64_t g0 = P->stask.size();
+      for (int i : lv) {
+         const NodeDesc& d = P->hnodes[i];
+         for (int r = 0; r < d.m; r += kRowChunk)
+            P->stask.push_back(SolveTask{i, 0, 0, r, std::min(d.m, r + kRowChunk), 0});
+      }
+      push(P->bwdprog, L_BGATHER, g0, P->stask.size() - g0);
+      // contribution-block part: columns [0, n) against rows [n, m)
       int64_t c0 = P->stask.size();
       for (int i : lv) {
          const NodeDesc& d = P->hnodes[i];
          if (d.m - d.n <= 0 || d.n == 0) continue;
-         for (int j = 0; j < d.n; j += 16)
-            P->stask.push_back(SolveTask{i, 0, 0, j, std::min(d.n, j + 16), 0});
+         for (int j = 0; j < d.n; j += kColChunk)
+            P->stask.push_back(SolveTask{i, d.n, d.m, j, std::min(d.n, j + kColChunk), 0});
       }
-      push(P->bwdprog, L_BCB, c0, P->stask.size() - c0);
+      push(P->bwdprog, L_BUPD, c0, P->stask.size() - c0);
       int maxblk = 0;
       for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
-      for (int s = maxblk - 1; s >= 0; --s) {
+      for (int sblk = maxblk - 1; sblk >= 0; --sblk) {
          int64_t t0 = P->stask.size();
          for (int i : lv) {
             const NodeDesc& d = P->hnodes[i];
-            int k0 = s * kNB;
+            const int k0 = sblk * kNB;
             if (k0 >= d.n) continue;
-            int kb = std::min(kNB, d.n - k0);
-            if (k0 == 0) {
-               P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
-            } else {
-               for (int j = 0; j < k0; j += 32)
-                  P->stask.push_back(SolveTask{i, k0, kb, j, std::min(k0, j + 32), j == 0 ? 1 : 0});
-            }
+            P->stask.push_back(SolveTask{i, k0, std::min(kNB, d.n - k0), 0, 0, 1});
          }
-         push(P->bwdprog, L_BSTEP, t0, P->stask.size() - t0);
+         push(P->bwdprog, L_BDIAG, t0, P->stask.size() - t0);
+         int64_t u0 = P->stask.size();
+         for (int i : lv) {
+            const NodeDesc& d = P->hnodes[i];
+            const int k0 = sblk * kNB;
+            if (k0 >= d.n || k0 == 0) continue;
+            const int kb = std::min(kNB, d.n - k0);
+            // rows [k0, k0+kb) against the earlier columns [0, k0)
+            for (int j = 0; j < k0; j += kColChunk)
+               P->stask.push_back(SolveTask{i, k0, k0 + kb, j, std::min(k0, j + kColChunk), 0});
+         }
+         push(P->bwdprog, L_BUPD, u0, P->stask.size() - u0);
       }
    }
 
@@ -585,8 +659,10 @@ inline unsigned grid1(int64_t n) { return (unsigned)((n + kThreads - 1) / kThrea
 template <typename T>
 void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
                         std::vector<hipEvent_t>* events = nullptr) {
+   const bool bf16_wide = std::is_same<T, float>::value && F->precision == SYLVER_AMD_PREC_MIXED;
    T* arena = static_cast<T*>(F->arena);
    T* dinv = static_cast<T*>(F->dinv);
+   T* dinvT = static_cast<T*>(F->dinvT);
    size_t li = 0;
    for (const Launch& L : P->fprog) {
       dim3 g(L.count), b(kThreads);
@@ -606,16 +682,22 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
             k_small_front<T><<<g, b, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, F->d_info);
             break;
          case L_DIAG:
-            k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, F->d_info, 1);
+            k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, dinvT, F->d_info, 1);
             break;
          case L_TRSM:
             k_gemm_nt<T, 64, 128><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, dinv, arena, T(1), T(0));
             break;
          case L_UPDATE:
-            k_gemm_nt<T, 128, 128><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, arena, arena, T(-1), T(1));
+            k_update<T><<<g, b, 0, s>>>(P->d_gemm + L.off, arena);
+            break;
+         case L_UPDATE_WIDE:
+            if (bf16_wide)
+               k_update_bf16<<<g, b, 0, s>>>(P->d_gemm + L.off, reinterpret_cast<float*>(arena));
+            else
+               k_update<T><<<g, b, 0, s>>>(P->d_gemm + L.off, arena);
             break;
          case L_DINV:
-            k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, F->d_info, 0);
+            k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, dinvT, F->d_info, 0);
             break;
          default:
             break;
@@ -631,28 +713,31 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
                        const double* b, double* xsol, int nrhs, hipStream_t s) {
    const T* arena = static_cast<const T*>(F->arena);
    const T* dinv = static_cast<const T*>(F->dinv);
+   const T* dinvT = static_cast<const T*>(F->dinvT);
    const int n = P->n;
    for (const Launch& L : prog) {
       dim3 g(L.count), bl(kThreads);
+      const SolveTask* tk = P->d_stask + L.off;
       switch (L.kind) {
          case L_FGATHER:
-            k_fwd_gather<<<g, bl, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, P->d_rlist, P->d_cmap, b,
-                                          n, F->d_xloc, nrhs);
+            k_fwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_rlist, P->d_cmap, b, n, F->d_xloc, nrhs);
             break;
-         case L_FSTEP:
-            k_fwd_step<T><<<g, bl, 0, s>>>(P->d_stask + L.off, P->d_nodes, arena, dinv, P->d_rlist,
-                                           F->d_xloc, xsol, n, nrhs);
+         case L_FDIAG:
+            k_solve_diag<T, false><<<g, bl, 0, s>>>(tk, P->d_nodes, dinv, P->d_rlist, F->d_xloc, xsol, n,
+                                                    nrhs);
+            break;
+         case L_FUPD:
+            k_fwd_upd<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, F->d_xloc, nrhs);
             break;
          case L_BGATHER:
-            k_bwd_gather<<<g, bl, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, P->d_rlist, xsol, n,
-                                          F->d_xloc, nrhs);
+            k_bwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_rlist, xsol, n, F->d_xloc, nrhs);
             break;
-         case L_BCB:
-            k_bwd_cb<T><<<g, bl, 0, s>>>(P->d_stask + L.off, P->d_nodes, arena, F->d_xloc, nrhs);
+         case L_BDIAG:
+            k_solve_diag<T, true><<<g, bl, 0, s>>>(tk, P->d_nodes, dinvT, P->d_rlist, F->d_xloc, xsol, n,
+                                                   nrhs);
             break;
-         case L_BSTEP:
-            k_bwd_step<T><<<g, bl, 0, s>>>(P->d_stask + L.off, P->d_nodes, arena, dinv, P->d_rlist,
-                                           F->d_xloc, xsol, n, nrhs);
+         case L_BUPD:
+            k_bwd_upd<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, F->d_xloc, nrhs);
             break;
          default:
             break;
@@ -698,7 +783,8 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
    (void)opts;
    try {
       int small_max = xo.small_front_max > 0 ? xo.small_front_max : 256;
-      if (*plan && (*plan)->small_max != small_max) {
+      int wide_ncol = xo.bf16_min_ncol > 0 ? xo.bf16_min_ncol : 1024;
+      if (*plan && ((*plan)->small_max != small_max || (*plan)->wide_ncol != wide_ncol)) {
          if (*fact && (*fact)->plan == *plan) {
             destroy_factors(*fact);
             *fact = nullptr;
@@ -706,7 +792,7 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
          destroy_plan(*plan);
          *plan = nullptr;
       }
-      if (!*plan) *plan = build_plan(a, small_max);
+      if (!*plan) *plan = build_plan(a, small_max, wide_ncol);
       DevicePlan* P = *plan;
       int prec = xo.precision;
       if (prec != SYLVER_AMD_PREC_FP64 && prec != SYLVER_AMD_PREC_FP32 && prec != SYLVER_AMD_PREC_MIXED)
@@ -726,6 +812,7 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
          size_t es = low ? sizeof(float) : sizeof(double);
          HIPCHECK(hipMalloc(&F->arena, std::max<int64_t>(P->arena_elems, 1) * es));
          HIPCHECK(hipMalloc(&F->dinv, std::max<int64_t>(P->dinv_elems, 1) * es));
+         HIPCHECK(hipMalloc(&F->dinvT, std::max<int64_t>(P->dinv_elems, 1) * es));
          F->d_aval = dalloc<double>(std::max<int64_t>(P->clean_nnz, 1));
          F->d_scaling = dalloc<double>(std::max(a.n, 1));
          F->d_csr_val = dalloc<double>(std::max<int64_t>(P->csr_nnz, 1));
@@ -890,6 +977,117 @@ double device_bwd_err(const DevicePlan* P, DeviceFactors* F, const double* b, co
    return worst;
 }
 
+// Per-column dot products <x, V_z> for z < nvec (reproducible two-pass
+// reduction), returned on the host as h[z * nrhs + c].
+void device_dots(DeviceFactors* F, int n, int nrhs, const double* x, const double* V, int64_t vstride,
+                 int nvec, std::vector<double>& h, hipStream_t s) {
+   dim3 g(kDotBlocks, nrhs, nvec);
+   k_dot_partial<<<g, kThreads, 0, s>>>(n, x, V, vstride, F->d_part);
+   const int cnt = nvec * nrhs;
+   k_dot_final<<<(cnt + 63) / 64, 64, 0, s>>>(F->d_part, cnt, F->d_coef);
+   h.resize(cnt);
+   HIPCHECK(hipMemcpyAsync(h.data(), F->d_coef, cnt * sizeof(double), hipMemcpyDeviceToHost, s));
+   HIPCHECK(hipStreamSynchronize(s));
+}
+
+void ensure_gmres_ws(const DevicePlan* P, DeviceFactors* F, int nrhs, int m) {
+   if (F->gm_nrhs >= nrhs && F->gm_m >= m) return;
+   for (void* p : {(void*)F->d_V, (void*)F->d_W, (void*)F->d_tmp, (void*)F->d_part, (void*)F->d_coef})
+      dfree(p);
+   const size_t nn = (size_t)P->n * nrhs;
+   F->d_V = dalloc<double>(nn * (m + 1));
+   F->d_W = dalloc<double>(nn);
+   F->d_tmp = dalloc<double>(nn);
+   F->d_part = dalloc<double>((size_t)kDotBlocks * nrhs * (m + 1));
+   F->d_coef = dalloc<double>((size_t)nrhs * (m + 1));
+   F->gm_nrhs = nrhs;
+   F->gm_m = m;
+}
+
+// One GMRES-IR correction: approximately solves A d = r by GMRES in FP64 on
+// the left-preconditioned system inv(L L^T) A d = inv(L L^T) r (the
+// preconditioner is the low-precision factorization), Gram-Schmidt applied
+// twice per step; stops when every column's preconditioned residual fell by
+// `rtol` or after m steps. d is written to `dout`.
+template <typename T>
+int gmres_correction(const DevicePlan* P, DeviceFactors* F, const double* r, double* dout, int nrhs,
+                     int m, double rtol, hipStream_t s) {
+   const int n = P->n;
+   const int64_t nn = (int64_t)n * nrhs;
+   ensure_gmres_ws(P, F, nrhs, m);
+   double* V = F->d_V;
+   solve_once<T>(P, F, 0, r, V, nrhs, s);
+   std::vector<double> h, beta(nrhs), coef(nrhs);
+   device_dots(F, n, nrhs, V, V, 0, 1, h, s);
+   for (int c = 0; c < nrhs; ++c) {
+      beta[c] = std::sqrt(h[c]);
+      coef[c] = beta[c] > 0 ? 1.0 / beta[c] : 0.0;
+   }
+   HIPCHECK(hipMemcpyAsync(F->d_coef, coef.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, s));
+   k_scale_cols<<<grid1(nn), kThreads, 0, s>>>(n, nrhs, F->d_coef, V, V);
+   // Hessenberg columns and Givens rotations per right-hand side
+   std::vector<std::vector<double>> H(nrhs, std::vector<double>((size_t)(m + 1) * m, 0.0));
+   std::vector<std::vector<double>> cs(nrhs, std::vector<double>(m)), sn(nrhs, std::vector<double>(m));
+   std::vector<std::vector<double>> g(nrhs, std::vector<double>(m + 1, 0.0));
+   for (int c = 0; c < nrhs; ++c) g[c][0] = beta[c];
+   int steps = 0;
+   for (int j = 0; j < m; ++j) {
+      double* Vj = V + (int64_t)j * nn;
+      double* Vn = V + (int64_t)(j + 1) * nn;
+      k_spmv<<<grid1(n), kThreads, 0, s>>>(n, P->d_csr_ptr, P->d_csr_col, F->d_csr_val, Vj, F->d_tmp, nrhs);
+      solve_once<T>(P, F, 0, F->d_tmp, F->d_W, nrhs, s);
+      std::vector<double> hsum((size_t)(j + 1) * nrhs, 0.0);
+      for (int pass = 0; pass < 2; ++pass) {
+         device_dots(F, n, nrhs, F->d_W, V, nn, j + 1, h, s);
+         for (size_t k = 0; k < h.size(); ++k) hsum[k] += h[k];
+         HIPCHECK(hipMemcpyAsync(F->d_coef, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+         k_multi_axpy<<<grid1(nn), kThreads, 0, s>>>(n, nrhs, j + 1, F->d_coef, V, nn, F->d_W);
+      }
+      device_dots(F, n, nrhs, F->d_W, F->d_W, 0, 1, h, s);
+      bool done = true;
+      for (int c = 0; c < nrhs; ++c) {
+         const double hn = std::sqrt(h[c]);
+         coef[c] = hn > 0 ? 1.0 / hn : 0.0;
+         std::vector<double>& Hc = H[c];
+         for (int i = 0; i <= j; ++i) Hc[i + (size_t)j * (m + 1)] = hsum[(size_t)i * nrhs + c];
+         Hc[j + 1 + (size_t)j * (m + 1)] = hn;
+         for (int i = 0; i < j; ++i) {
+            const double a = Hc[i + (size_t)j * (m + 1)], b = Hc[i + 1 + (size_t)j * (m + 1)];
+            Hc[i + (size_t)j * (m + 1)] = cs[c][i] * a + sn[c][i] * b;
+            Hc[i + 1 + (size_t)j * (m + 1)] = -sn[c][i] * a + cs[c][i] * b;
+         }
+         const double a = Hc[j + (size_t)j * (m + 1)], b = Hc[j + 1 + (size_t)j * (m + 1)];
+         const double rr = std::hypot(a, b);
+         cs[c][j] = rr > 0 ? a / rr : 1.0;
+         sn[c][j] = rr > 0 ? b / rr : 0.0;
+         Hc[j + (size_t)j * (m + 1)] = rr;
+         Hc[j + 1 + (size_t)j * (m + 1)] = 0.0;
+         g[c][j + 1] = -sn[c][j] * g[c][j];
+         g[c][j] = cs[c][j] * g[c][j];
+         if (beta[c] > 0 && std::fabs(g[c][j + 1]) > rtol * beta[c]) done = false;
+      }
+      HIPCHECK(hipMemcpyAsync(F->d_coef, coef.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, s));
+      k_scale_cols<<<grid1(nn), kThreads, 0, s>>>(n, nrhs, F->d_coef, F->d_W, Vn);
+      steps = j + 1;
+      if (done) break;
+   }
+   // y = H^{-1} g per column, d = sum_i y_i V_i
+   std::vector<double> y((size_t)steps * nrhs, 0.0);
+   for (int c = 0; c < nrhs; ++c) {
+      for (int i = steps - 1; i >= 0; --i) {
+         double v = g[c][i];
+         for (int k = i + 1; k < steps; ++k) v -= H[c][i + (size_t)k * (m + 1)] * y[(size_t)k * nrhs + c];
+         const double d = H[c][i + (size_t)i * (m + 1)];
+         y[(size_t)i * nrhs + c] = d != 0 ? v / d : 0.0;
+      }
+   }
+   for (auto& v : y) v = -v;  // k_multi_axpy subtracts
+   k_fill<<<grid1(nn), kThreads, 0, s>>>(nn, 0.0, dout);
+   HIPCHECK(hipMemcpyAsync(F->d_coef, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice, s));
+   k_multi_axpy<<<grid1(nn), kThreads, 0, s>>>(n, nrhs, steps, F->d_coef, V, nn, dout);
+   return steps;
+}
+
 }  // namespace
 
 SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int job, int nrhs,
@@ -920,16 +1118,23 @@ SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int
             solve_once<double>(P, F, job, F->d_b, F->d_sol, nrhs, s);
          res.refine_mode = SYLVER_AMD_REFINE_NONE;
       } else {
-         // Classical iterative refinement in FP64: x0 = 0, r0 = b,
-         // x_{k+1} = x_k + solve(r_k), r_{k+1} = b - A x_{k+1}.
+         // FP64 refinement: x0 = 0, r0 = b; classical steps
+         // x_{k+1} = x_k + solve(r_k), r_{k+1} = b - A x_{k+1}; in AUTO mode a
+         // step that does not halve the backward error switches the remaining
+         // steps to GMRES-IR corrections.
          k_fill<<<grid1(nn), kThreads, 0, s>>>(nn, 0.0, F->d_x);
          k_copy<<<grid1(nn), kThreads, 0, s>>>(nn, F->d_b, F->d_r);
          double err = 1e300;
          int it = 0;
+         bool gm = xo.refine == SYLVER_AMD_REFINE_GMRES;
          for (; it < xo.refine_max_iter; ++it) {
-            solve_once<float>(P, F, 0, F->d_r, F->d_sol, nrhs, s);
+            if (gm)
+               gmres_correction<float>(P, F, F->d_r, F->d_sol, nrhs, 20, 1e-10, s);
+            else
+               solve_once<float>(P, F, 0, F->d_r, F->d_sol, nrhs, s);
             k_axpy<<<grid1(nn), kThreads, 0, s>>>(nn, 1.0, F->d_sol, F->d_x);
             double e = device_bwd_err(P, F, F->d_b, F->d_x, F->d_r, nrhs, s);
+            if (!gm && xo.refine == SYLVER_AMD_REFINE_AUTO && it > 0 && !(e < 0.5 * err)) gm = true;
             err = e;
             if (!(e > xo.refine_tol)) {
                ++it;
@@ -937,9 +1142,9 @@ SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int
             }
          }
          k_copy<<<grid1(nn), kThreads, 0, s>>>(nn, F->d_x, F->d_sol);
+         res.refine_mode = gm ? SYLVER_AMD_REFINE_GMRES : SYLVER_AMD_REFINE_IR;
          res.refine_iters = it;
          res.bwd_err = err;
-         res.refine_mode = SYLVER_AMD_REFINE_IR;
       }
       HIPCHECK(hipEventRecord(e1, s));
       HIPCHECK(hipMemcpy2DAsync(x, (size_t)ldx * sizeof(double), F->d_sol, (size_t)n * sizeof(double),
@@ -1013,16 +1218,15 @@ int factors_node_factor(const DeviceFactors* F, int node, double* l) {
    if (!F || !F->plan || node < 0 || node >= F->plan->nnodes || !l) return SYLVER_ERROR_CALL_SEQUENCE;
    try {
       const NodeDesc& d = F->plan->hnodes[node];
-      size_t cnt = (size_t)d.m * d.n;
-      if (F->precision == SYLVER_AMD_PREC_FP64) {
-         HIPCHECK(hipMemcpy(l, static_cast<double*>(F->arena) + d.off, cnt * sizeof(double),
-                            hipMemcpyDeviceToHost));
-      } else {
-         std::vector<float> tmp(cnt);
-         HIPCHECK(hipMemcpy(tmp.data(), static_cast<float*>(F->arena) + d.off, cnt * sizeof(float),
-                            hipMemcpyDeviceToHost));
-         for (size_t i = 0; i < cnt; ++i) l[i] = tmp[i];
-      }
+      const bool dbl = F->precision == SYLVER_AMD_PREC_FP64;
+      const size_t es = dbl ? sizeof(double) : sizeof(float);
+      std::vector<char> tmp((size_t)d.m * d.n * es + 1);
+      const char* src = static_cast<const char*>(F->arena) + (size_t)d.off * es;
+      HIPCHECK(hipMemcpy2D(tmp.data(), (size_t)d.m * es, src, (size_t)d.ld * es, (size_t)d.m * es, d.n,
+                           hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < (size_t)d.m * d.n; ++i)
+         l[i] = dbl ? reinterpret_cast<const double*>(tmp.data())[i]
+                    : (double)reinterpret_cast<const float*>(tmp.data())[i];
       return 0;
    } catch (const HipError&) {
       return SYLVER_ERROR_CUDA_UNKNOWN;

@@ -202,3 +202,25 @@ def test_batched_dense_fronts(S, O, gpu, dtype):
         assert st == 0
         scale = np.abs(ref).max()
         assert np.abs(np.tril(got) - np.tril(ref)).max() <= tol * scale * max(1, m / 64), f
+
+
+def test_mixed_bf16_wide_fronts_refined(S, O, gpu):
+    """Mixed tier with the BF16 update forced on every large front
+    (bf16_min_ncol=64): factors stay close to the FP64 oracle (BF16 unit
+    roundoff 2^-8 on the update operands) and FP64 refinement brings the
+    scaled backward error to <= 1e-15."""
+    S.set_ext_options(precision=S.PREC_MIXED, bf16_min_ncol=64, small_front_max=48,
+                      refine=S.REFINE_AUTO, refine_tol=1e-15, refine_max_iter=60)
+    try:
+        A = M.laplacian_3d(16)
+        akeep, opts, sym = _analyse(S, A)
+        fkeep, inf = _factor(S, A, akeep, opts)
+        assert inf.flag == 0
+        _compare_factors(S, O, sym, fkeep, A, 3e-2)
+        b = M.random_rhs(A.n, 2, seed=5)
+        x = b.copy(order="F")
+        inf = S.spldlt_solve(x, akeep, fkeep, options=opts)
+        assert inf.flag == 0
+        assert O.bwd_err(A, x, b) <= BWD_TOL
+    finally:
+        S.set_ext_options(bf16_min_ncol=1024, refine_max_iter=30)
