@@ -677,17 +677,15 @@ int analyse(int n, int* order, const int64_t* ptr, const int* row, bool check,
          merge = true;  // fundamental: no fill
       } else if (nelim[ps] < a.nemin && nelim[s] < a.nemin) {
          merge = true;  // amalgamation of small nodes
-      } else if (nchild_col[p] == 1) {
-         // Relaxed merge along a chain: accept when the explicit zeros the
-         // merge adds are a small fraction of the merged front.
+      } else {
+         // Relaxed merge: accept when the explicit zeros the merge adds
+         // (counted cumulatively) are a small fraction of the merged front.
          double ns = nelim[s], np = nelim[ps];
          double ms = ns + cc[j] - 1, mp = np + cc[p] - 1;
          double merged = trap(ns + np, ns + mp);
          double zeros = merged - trap(ns, ms) - trap(np, mp) + zcount[s] + zcount[ps];
          merge = zeros <= kRelaxFill * merged;
          if (merge) zcount[ps] = zeros - zcount[s];
-      } else {
-         merge = false;
       }
       if (!merge) continue;
       // members: s's columns first, then ps's

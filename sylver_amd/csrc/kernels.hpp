@@ -144,7 +144,22 @@ __global__ __launch_bounds__(kThreads) void k_extend_add(const EaTask* __restric
       const int pj = cm[j];
       T* dst = arena + p.off + (int64_t)pj * p.ld;
       const T* src = arena + c.off + (int64_t)(c.n + j) * c.ld + c.n;
-      for (int i = j + lane; i < q; i += 64) dst[cm[i]] += src[i];
+      // 8 rows per lane per batch: all loads issued before the stores
+      for (int i0 = j; i0 < q; i0 += 8 * 64) {
+         int ci[8];
+         T sv[8], dv[8];
+#pragma unroll
+         for (int u = 0; u < 8; ++u) {
+            const int i = i0 + u * 64 + lane;
+            ci[u] = (i < q) ? cm[i] : 0;
+            sv[u] = (i < q) ? src[i] : T(0);
+         }
+#pragma unroll
+         for (int u = 0; u < 8; ++u) dv[u] = (i0 + u * 64 + lane < q) ? dst[ci[u]] : T(0);
+#pragma unroll
+         for (int u = 0; u < 8; ++u)
+            if (i0 + u * 64 + lane < q) dst[ci[u]] = dv[u] + sv[u];
+      }
    }
 }
 
@@ -155,10 +170,12 @@ __global__ __launch_bounds__(kThreads) void k_extend_add(const EaTask* __restric
 // the inverse of [A 0; 0 I] are [L 0; 0 I] and [inv(L) 0; 0 I].
 // ---------------------------------------------------------------------------
 // Cholesky of a 16x16 block at D (ld LD) by one wave: lane i < 16 holds row
-// i in registers, pivot columns are broadcast by cross-lane shuffles.
+// i in registers; after each pivot the scaled column is published through
+// LDS (one write per lane) and read back by every lane as broadcasts, so no
+// step waits on a chain of cross-lane shuffles. Col is 16 elements of LDS.
 // Returns 0 or the 1-based column of the first non-positive pivot.
 template <typename T, int LD>
-__device__ int wave_potrf16(T* D) {
+__device__ int wave_potrf16(T* D, T* Col) {
    const int lane = threadIdx.x & 63;
    const bool own = lane < 16;
    T r[16];
@@ -167,19 +184,25 @@ __device__ int wave_potrf16(T* D) {
    int bad = 0;
 #pragma unroll
    for (int j = 0; j < 16; ++j) {
-      T d = __shfl(r[j], j, 64);
+      if (own) Col[lane] = r[j];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      T col[16];
+#pragma unroll
+      for (int c = j; c < 16; ++c) col[c] = Col[c];
+      const T d = col[j];
       if (bad == 0 && !(d > T(0))) bad = j + 1;
       if (bad == 0) {
-         d = dsqrt(d);
-         const T inv = T(1) / d;
-         if (lane == j) r[j] = d;
+         const T sd = dsqrt(d);
+         const T inv = T(1) / sd;
+         if (lane == j) r[j] = sd;
          if (lane > j) r[j] *= inv;
 #pragma unroll
-         for (int c = j + 1; c < 16; ++c) {
-            const T lcj = __shfl(r[j], c, 64);
-            if (lane >= c) r[c] -= r[j] * lcj;
-         }
+         for (int c = j + 1; c < 16; ++c)
+            if (lane >= c) r[c] -= r[j] * (col[c] * inv);
       }
+      __builtin_amdgcn_wave_barrier();
    }
    if (own)
 #pragma unroll
@@ -193,13 +216,15 @@ __device__ int wave_potrf16(T* D) {
 template <typename T, int LD>
 __device__ void wave_trtri16(T* D) {
    const int lane = threadIdx.x & 63;
-   T x[16];
+   T x[16], rd[16];
+#pragma unroll
+   for (int i = 0; i < 16; ++i) rd[i] = T(1) / D[i + i * LD];
 #pragma unroll
    for (int i = 0; i < 16; ++i) {
       T s = (i == lane) ? T(1) : T(0);
 #pragma unroll
       for (int k = 0; k < i; ++k) s -= D[i + k * LD] * x[k];
-      x[i] = (i >= lane) ? s / D[i + i * LD] : T(0);
+      x[i] = (i >= lane) ? s * rd[i] : T(0);
    }
    __builtin_amdgcn_wave_barrier();
    if (lane < 16)
@@ -231,11 +256,14 @@ __device__ int lds_potrf_blocked(T* L, int kb16, int* sh_bad) {
    typedef typename Mfma<T>::acc_t acc_t;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int nbk = kb16 / 16;
+   __shared__ T Rcp[16];
+   __shared__ T Col[16];
    for (int s = 0; s < nbk; ++s) {
       const int o = s * 16;
       if (wave == 0) {
-         int bad = wave_potrf16<T, LD>(L + o + o * LD);
+         int bad = wave_potrf16<T, LD>(L + o + o * LD, Col);
          if (lane == 0) *sh_bad = bad ? o + bad : 0;
+         if (lane < 16) Rcp[lane] = T(1) / L[(o + lane) + (o + lane) * LD];
       }
       __syncthreads();
       if (*sh_bad) return *sh_bad;
@@ -250,7 +278,7 @@ __device__ int lds_potrf_blocked(T* L, int kb16, int* sh_bad) {
             T sum = x[c];
 #pragma unroll
             for (int cc = 0; cc < c; ++cc) sum -= x[cc] * L[(o + c) + (o + cc) * LD];
-            x[c] = sum / L[(o + c) + (o + c) * LD];
+            x[c] = sum * Rcp[c];
          }
 #pragma unroll
          for (int c = 0; c < 16; ++c) L[r + (o + c) * LD] = x[c];
@@ -331,14 +359,21 @@ __global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ 
    const int kb = t.kb;
    const int kb16 = (kb + 15) & ~15;
    T* blk = arena + nd.off + t.k0 + (int64_t)t.k0 * nd.ld;
-   for (int e = threadIdx.x; e < kb16 * kb16; e += kThreads) {
-      const int i = e % kb16, j = e / kb16;
-      T v;
-      if (i < kb && j < kb)
-         v = (i >= j) ? blk[i + (int64_t)j * nd.ld] : T(0);
-      else
-         v = (i == j) ? T(1) : T(0);
-      L[i + j * LD] = v;
+   // 16 loads in flight per thread per batch
+   for (int e0 = 0; e0 < kb16 * kb16; e0 += 16 * kThreads) {
+      T v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+         const int e = e0 + u * kThreads + threadIdx.x;
+         const int i = e % kb16, j = e / kb16;
+         v[u] = (e < kb16 * kb16 && i < kb && j < kb && i >= j) ? blk[i + (int64_t)j * nd.ld] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+         const int e = e0 + u * kThreads + threadIdx.x;
+         const int i = e % kb16, j = e / kb16;
+         if (e < kb16 * kb16) L[i + j * LD] = (i < kb && j < kb) ? v[u] : ((i == j) ? T(1) : T(0));
+      }
    }
    __syncthreads();
    if (do_potrf) {
@@ -436,8 +471,9 @@ __global__ __launch_bounds__(kThreads) void k_gemm_nt(const GemmTask* __restrict
 // ---------------------------------------------------------------------------
 // Schur-complement update of the large-front path: C -= A * B^T on one
 // 128x128 tile (A, B: K panel columns of the front, rows of the tile's row and
-// column blocks). Accumulators start from C so the epilogue only stores; A is
-// negated on its way into the MFMA. K is streamed in chunks of 16 through a
+// column blocks). The product is accumulated from zero and subtracted from C
+// once at the end (accumulating onto C would round every step at C's
+// magnitude). K is streamed in chunks of 16 through a
 // double-buffered LDS with the next chunk prefetched into registers (16-byte
 // loads: front rows are 8-aligned and leading dimensions a multiple of 8), one
 // barrier per chunk. Columns below `pad0` of the tile are left untouched
@@ -459,9 +495,15 @@ template <> struct V16<float> {
 
 __device__ inline int xcd_task(int b, int G) { return (b & 7) * (G >> 3) + (b >> 3); }
 
-template <typename T>
+// MODE 0: Schur update C -= A B^T (B in the arena). MODE 1: panel TRSM
+// C = A B^T with B = inv(L_kk) from `bsrc`, C == A in place (the tile spans
+// all kb columns, so each workgroup owns its rows and has read all of A
+// before it writes). Tasks whose offsets or leading dimensions are not
+// multiples of the vector width take scalar loads.
+template <typename T, int MODE>
 __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict__ tasks,
-                                                     T* __restrict__ arena) {
+                                                     T* __restrict__ arena,
+                                                     const T* __restrict__ bsrc) {
    constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
    typedef typename V16<T>::vt vt;
@@ -475,37 +517,52 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
    if (t.M <= 0) return;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int wm = wave >> 1, wn = wave & 1;
-   const int ld = t.lda;
+   const int ld = t.lda, ldb = t.ldb;
    const T* a = arena + t.a_off;
-   const T* b = arena + t.b_off;
+   const T* b = (MODE == 1 ? bsrc : arena) + t.b_off;
    T* c = arena + t.c_off;
+   const bool vec = ((t.a_off | t.b_off | (int64_t)ld | (int64_t)ldb) % VN) == 0;
    acc_t acc[FM][FN];
 #pragma unroll
    for (int x = 0; x < FM; ++x)
 #pragma unroll
-      for (int y = 0; y < FN; ++y) {
-         const int j = wn * WN + y * 16 + (lane & 15);
-#pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int i = wm * WM + x * 16 + Mfma<T>::row(lane, v);
-            acc[x][y][v] = (i < t.M && j < t.N) ? c[i + (int64_t)j * ld] : T(0);
-         }
-      }
+      for (int y = 0; y < FN; ++y) acc[x][y] = acc_t{0, 0, 0, 0};
    vt ra[AV], rb[BV];
    auto gload = [&](int k0) {
+      if (vec) {
 #pragma unroll
-      for (int u = 0; u < AV; ++u) {
-         const int v = tid + u * kThreads;
-         const int kk = v / (BM / VN), row = (v % (BM / VN)) * VN;
-         ra[u] = (k0 + kk < t.K && row < t.M) ? *reinterpret_cast<const vt*>(a + row + (int64_t)(k0 + kk) * ld)
-                                              : vt(0);
-      }
+         for (int u = 0; u < AV; ++u) {
+            const int v = tid + u * kThreads;
+            const int kk = v / (BM / VN), row = (v % (BM / VN)) * VN;
+            ra[u] = (k0 + kk < t.K && row < t.M)
+                        ? *reinterpret_cast<const vt*>(a + row + (int64_t)(k0 + kk) * ld)
+                        : vt(0);
+         }
 #pragma unroll
-      for (int u = 0; u < BV; ++u) {
-         const int v = tid + u * kThreads;
-         const int kk = v / (BN / VN), row = (v % (BN / VN)) * VN;
-         rb[u] = (k0 + kk < t.K && row < t.N) ? *reinterpret_cast<const vt*>(b + row + (int64_t)(k0 + kk) * ld)
-                                              : vt(0);
+         for (int u = 0; u < BV; ++u) {
+            const int v = tid + u * kThreads;
+            const int kk = v / (BN / VN), row = (v % (BN / VN)) * VN;
+            rb[u] = (k0 + kk < t.K && row < t.N)
+                        ? *reinterpret_cast<const vt*>(b + row + (int64_t)(k0 + kk) * ldb)
+                        : vt(0);
+         }
+      } else {
+#pragma unroll
+         for (int u = 0; u < AV; ++u) {
+            const int v = tid + u * kThreads;
+            const int kk = v / (BM / VN), row = (v % (BM / VN)) * VN;
+#pragma unroll
+            for (int e = 0; e < VN; ++e)
+               ra[u][e] = (k0 + kk < t.K && row + e < t.M) ? a[row + e + (int64_t)(k0 + kk) * ld] : T(0);
+         }
+#pragma unroll
+         for (int u = 0; u < BV; ++u) {
+            const int v = tid + u * kThreads;
+            const int kk = v / (BN / VN), row = (v % (BN / VN)) * VN;
+#pragma unroll
+            for (int e = 0; e < VN; ++e)
+               rb[u][e] = (k0 + kk < t.K && row + e < t.N) ? b[row + e + (int64_t)(k0 + kk) * ldb] : T(0);
+         }
       }
    };
    const int nk = (t.K + BK - 1) / BK;
@@ -532,7 +589,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
       for (int kk = 0; kk < BK; kk += 4) {
          T af[FM], bf[FN];
 #pragma unroll
-         for (int x = 0; x < FM; ++x) af[x] = -As[buf][kk + (lane >> 4)][wm * WM + x * 16 + (lane & 15)];
+         for (int x = 0; x < FM; ++x) af[x] = As[buf][kk + (lane >> 4)][wm * WM + x * 16 + (lane & 15)];
 #pragma unroll
          for (int y = 0; y < FN; ++y) bf[y] = Bs[buf][kk + (lane >> 4)][wn * WN + y * 16 + (lane & 15)];
 #pragma unroll
@@ -550,7 +607,12 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
 #pragma unroll
          for (int v = 0; v < 4; ++v) {
             const int i = wm * WM + x * 16 + Mfma<T>::row(lane, v);
-            if (i < t.M) c[i + (int64_t)j * ld] = acc[x][y][v];
+            if (i < t.M) {
+               if (MODE == 1)
+                  c[i + (int64_t)j * ld] = acc[x][y][v];
+               else
+                  c[i + (int64_t)j * ld] -= acc[x][y][v];
+            }
          }
       }
 }
@@ -559,7 +621,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
 // BF16 Schur update for wide fronts in the mixed-precision tier: FP32 front
 // storage, the K chunk (32 columns) of A and B rounded to BF16 (round to
 // nearest even) as it is staged into LDS in k-contiguous rows, products on the
-// 16x16x32 BF16 MFMA accumulated in FP32 (accumulators start from C). Lane l
+// 16x16x32 BF16 MFMA accumulated in FP32, subtracted from C at the end. Lane l
 // of the MFMA reads row l%16, k = 8*(l/16)..+7 of each operand (one 16-byte
 // LDS read). BF16 keeps FP32's 8-bit exponent, so a power-of-two rescaling of
 // a block would not change any rounding; none is applied.
@@ -586,14 +648,7 @@ __global__ __launch_bounds__(kThreads) void k_update_bf16(const GemmTask* __rest
 #pragma unroll
    for (int x = 0; x < FM; ++x)
 #pragma unroll
-      for (int y = 0; y < FN; ++y) {
-         const int j = wn * WN + y * 16 + (lane & 15);
-#pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int i = wm * WM + x * 16 + 4 * (lane >> 4) + v;
-            acc[x][y][v] = (i < t.M && j < t.N) ? c[i + (int64_t)j * ld] : 0.0f;
-         }
-      }
+      for (int y = 0; y < FN; ++y) acc[x][y] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
    f4 ra[AV], rb[AV];
    auto gload = [&](int k0) {
 #pragma unroll
@@ -616,7 +671,7 @@ __global__ __launch_bounds__(kThreads) void k_update_bf16(const GemmTask* __rest
          const int kk = v / (BM / 4), row = (v % (BM / 4)) * 4;
 #pragma unroll
          for (int e = 0; e < 4; ++e) {
-            As[buf][row + e][kk] = (__bf16)(-ra[u][e]);
+            As[buf][row + e][kk] = (__bf16)(ra[u][e]);
             Bs[buf][row + e][kk] = (__bf16)(rb[u][e]);
          }
       }
@@ -644,7 +699,7 @@ __global__ __launch_bounds__(kThreads) void k_update_bf16(const GemmTask* __rest
 #pragma unroll
          for (int v = 0; v < 4; ++v) {
             const int i = wm * WM + x * 16 + 4 * (lane >> 4) + v;
-            if (i < t.M) c[i + (int64_t)j * ld] = acc[x][y][v];
+            if (i < t.M) c[i + (int64_t)j * ld] -= acc[x][y][v];
          }
       }
 }
@@ -728,12 +783,15 @@ __device__ void front_potrf_block(const NodeDesc& nd, T* __restrict__ F, int* in
          acc_t acc = acc_t{0, 0, 0, 0};
          const int ia = ti * 16 + (lane & 15);
          const int ib = tj * 16 + (lane & 15);
-         for (int kk = 0; kk < kb; kk += 4) {
-            const int kc = kk + (lane >> 4);
-            T av = (ia < R && kc < kb) ? P[ia + (int64_t)kc * ld] : T(0);
-            T bv = (ib < R && kc < kb) ? P[ib + (int64_t)kc * ld] : T(0);
-            acc = Mfma<T>::op(av, bv, acc);
+         T av[B / 4], bv[B / 4];
+#pragma unroll
+         for (int u = 0; u < B / 4; ++u) {
+            const int kc = 4 * u + (lane >> 4);
+            av[u] = (ia < R && kc < kb) ? P[ia + (int64_t)kc * ld] : T(0);
+            bv[u] = (ib < R && kc < kb) ? P[ib + (int64_t)kc * ld] : T(0);
          }
+#pragma unroll
+         for (int u = 0; u < B / 4; ++u) acc = Mfma<T>::op(av[u], bv[u], acc);
          const int j = tj * 16 + (lane & 15);
          if (j < R) {
 #pragma unroll
@@ -833,9 +891,36 @@ __global__ __launch_bounds__(kThreads) void k_bwd_gather(const SolveTask* __rest
    }
 }
 
+// acc(16 x 16, FP64 MFMA layout) += A(16 rows x K) * B(K x 16): A from global
+// memory (column-major, ld lda; rows >= rvalid read as zero), converted to
+// FP64; B from LDS, row-major K x 16. Eight k-steps per batch keep eight
+// global loads in flight.
+template <typename T>
+__device__ inline v4d mma_rows_gl(const T* __restrict__ A, int64_t lda, int rvalid, int K,
+                                  const double* Bs, v4d acc) {
+   const int lane = threadIdx.x & 63;
+   const int r = lane & 15, kq = lane >> 4;
+   for (int k0 = 0; k0 < K; k0 += 32) {
+      double a[8], b[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+         const int kk = k0 + 4 * u + kq;
+         a[u] = (r < rvalid && kk < K) ? (double)A[r + (int64_t)kk * lda] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+         const int kk = k0 + 4 * u + kq;
+         b[u] = (kk < K) ? Bs[kk * 16 + r] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+   }
+   return acc;
+}
+
 // y = M x_k for the kb x kb triangular block M (ld kb) of dinv (forward:
-// inv(L_kk), lower; backward: its transpose, upper); y overwrites x_k in the
-// work vector and is written to the solution.
+// inv(L_kk); backward: its transpose); y overwrites x_k in the work vector
+// and is written to the solution. Waves take 16-row tiles of y.
 template <typename T, bool UPPER>
 __global__ __launch_bounds__(kThreads) void k_solve_diag(const SolveTask* __restrict__ tasks,
                                                          const NodeDesc* __restrict__ nodes,
@@ -844,87 +929,87 @@ __global__ __launch_bounds__(kThreads) void k_solve_diag(const SolveTask* __rest
                                                          double* __restrict__ xloc,
                                                          double* __restrict__ xsol, int ldsol,
                                                          int nrhs) {
-   __shared__ double xs[kNB * kMaxRhsChunk];
+   __shared__ double xs[kNB * 16];
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    const int m = nd.m, kb = t.kb;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    double* xl = xloc + nd.solve_off * nrhs;
    const T* Mk = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
-   for (int c0 = 0; c0 < nrhs; c0 += kMaxRhsChunk) {
-      const int nc = min(kMaxRhsChunk, nrhs - c0);
+   for (int c0 = 0; c0 < nrhs; c0 += 16) {
+      const int nc = min(16, nrhs - c0);
       __syncthreads();
-      for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
-         const int i = e % kb, c = e / kb;
-         xs[i + c * kNB] = xl[t.k0 + i + (int64_t)(c0 + c) * m];
+      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
+         const int l = e >> 4, c = e & 15;
+         xs[e] = (l < kb && c < nc) ? xl[t.k0 + l + (int64_t)(c0 + c) * m] : 0.0;
       }
       __syncthreads();
-      const int i = threadIdx.x;
-      if (i < kb) {
-         double acc[kMaxRhsChunk];
+      v4d acc[2];
+      const int ntile = (kb + 15) / 16;
 #pragma unroll
-         for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = 0.0;
-         const int l0 = UPPER ? i : 0, l1 = UPPER ? kb : i + 1;
-         // 16 independent loads in flight per batch; entries outside the
-         // triangle are stored as zeros, so whole batches can be read
-         for (int lb = l0 & ~15; lb < l1; lb += 16) {
-            double mv[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) mv[u] = (lb + u < kb) ? (double)Mk[i + (lb + u) * kb] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-               if (lb + u >= kb) break;
-#pragma unroll
-               for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += mv[u] * xs[lb + u + c * kNB];
-            }
+      for (int q = 0; q < 2; ++q) {
+         acc[q] = v4d{0, 0, 0, 0};
+         const int tile = wave + 4 * q;
+         if (tile < ntile) {
+            // lower: columns [0, tile*16+16); upper: columns [tile*16, kb)
+            const int kbeg = UPPER ? tile * 16 : 0;
+            const int kend = UPPER ? kb : min(kb, tile * 16 + 16);
+            acc[q] = mma_rows_gl<T>(Mk + tile * 16 + (int64_t)kbeg * kb, kb, kb - tile * 16, kend - kbeg,
+                                    xs + kbeg * 16, acc[q]);
          }
-         const int row = rlist[nd.rlist_off + t.k0 + i] - 1;
+      }
+      __syncthreads();  // every wave has read x_k before it is overwritten
 #pragma unroll
-         for (int c = 0; c < kMaxRhsChunk; ++c)
-            if (c < nc) {
-               xl[t.k0 + i + (int64_t)(c0 + c) * m] = acc[c];
-               xsol[row + (int64_t)(c0 + c) * ldsol] = acc[c];
-            }
+      for (int q = 0; q < 2; ++q) {
+         const int tile = wave + 4 * q;
+         if (tile >= ntile) continue;
+         const int c = lane & 15;
+         if (c >= nc) continue;
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int i = tile * 16 + Mfma<double>::row(lane, v);
+            if (i >= kb) continue;
+            xl[t.k0 + i + (int64_t)(c0 + c) * m] = acc[q][v];
+            xsol[rlist[nd.rlist_off + t.k0 + i] - 1 + (int64_t)(c0 + c) * ldsol] = acc[q][v];
+         }
       }
    }
 }
 
-// Forward update of rows [r0, r1): x_r -= L[r, k0:k0+kb] y (one row per thread).
+// Forward update of rows [r0, r1): x_r -= L[r, k0:k0+kb] y, 16-row MFMA tiles.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_fwd_upd(const SolveTask* __restrict__ tasks,
                                                       const NodeDesc* __restrict__ nodes,
                                                       const T* __restrict__ arena,
                                                       double* __restrict__ xloc, int nrhs) {
-   __shared__ double ys[kNB * kMaxRhsChunk];
+   __shared__ double ys[kNB * 16];
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    const int m = nd.m, kb = t.kb;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    double* xl = xloc + nd.solve_off * nrhs;
    const T* F = arena + nd.off;
-   for (int c0 = 0; c0 < nrhs; c0 += kMaxRhsChunk) {
-      const int nc = min(kMaxRhsChunk, nrhs - c0);
+   const int R = t.r1 - t.r0;
+   const int ntile = (R + 15) / 16;
+   for (int c0 = 0; c0 < nrhs; c0 += 16) {
+      const int nc = min(16, nrhs - c0);
       __syncthreads();
-      for (int e = threadIdx.x; e < kNB * kMaxRhsChunk; e += kThreads) {
-         const int i = e % kNB, c = e / kNB;
-         ys[i + c * kNB] = (c < nc && i < kb) ? xl[t.k0 + i + (int64_t)(c0 + c) * m] : 0.0;
+      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
+         const int l = e >> 4, c = e & 15;
+         ys[e] = (l < kb && c < nc) ? xl[t.k0 + l + (int64_t)(c0 + c) * m] : 0.0;
       }
       __syncthreads();
-      for (int r = t.r0 + threadIdx.x; r < t.r1; r += kThreads) {
-         double acc[kMaxRhsChunk];
+      for (int tile = wave; tile < ntile; tile += 4) {
+         const int rb = t.r0 + tile * 16;
+         v4d acc = v4d{0, 0, 0, 0};
+         acc = mma_rows_gl<T>(F + rb + (int64_t)t.k0 * nd.ld, nd.ld, t.r1 - rb, kb, ys, acc);
+         const int c = lane & 15;
+         if (c < nc)
 #pragma unroll
-         for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = 0.0;
-         const T* Fr = F + r + (int64_t)t.k0 * nd.ld;
-         for (int lb = 0; lb < kb; lb += 16) {
-            double lv[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) lv[u] = (lb + u < kb) ? (double)Fr[(int64_t)(lb + u) * nd.ld] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 16; ++u)
-#pragma unroll
-               for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += lv[u] * ys[lb + u + c * kNB];
-         }
-#pragma unroll
-         for (int c = 0; c < kMaxRhsChunk; ++c)
-            if (c < nc) xl[r + (int64_t)(c0 + c) * m] -= acc[c];
+            for (int v = 0; v < 4; ++v) {
+               const int r = rb + Mfma<double>::row(lane, v);
+               if (r < t.r1) xl[r + (int64_t)(c0 + c) * m] -= acc[v];
+            }
       }
    }
 }

@@ -368,7 +368,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             const NodeDesc& d = P->hnodes[i];
             if (k0 >= d.n) continue;
             int kb = std::min(kNB, d.n - k0);
-            for (int r = k0 + kb; r < d.m; r += 64) {
+            for (int r = k0 + kb; r < d.m; r += 128) {
                GemmTask g{};
                g.a_off = d.off + r + (int64_t)k0 * d.ld;
                g.b_off = d.dinv_off + (int64_t)(k0 / kNB) * kNB * kNB;
@@ -376,12 +376,13 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
                g.lda = d.ld;
                g.ldb = kb;
                g.ldc = d.ld;
-               g.M = std::min(64, d.m - r);
+               g.M = std::min(128, d.m - r);
                g.N = kb;
                g.K = kb;
                P->gemm.push_back(g);
             }
          }
+         while ((P->gemm.size() - t0) % 8 != 0 && P->gemm.size() > (size_t)t0) P->gemm.push_back(GemmTask{});
          push(P->fprog, L_TRSM, t0, P->gemm.size() - t0);
          // Schur update. Block columns are paired: after the first of a pair
          // only the next panel's columns are updated (narrow update), after the
@@ -493,6 +494,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             }
             case L_TRSM: {
                const GemmTask& g = P->gemm[t];
+               if (g.M <= 0) break;
                L.flops += (double)g.M * g.K * g.K;
                L.words += 2.0 * g.M * g.K + 0.5 * g.K * g.K;
                break;
@@ -685,16 +687,16 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
             k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, dinvT, F->d_info, 1);
             break;
          case L_TRSM:
-            k_gemm_nt<T, 64, 128><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, dinv, arena, T(1), T(0));
+            k_update<T, 1><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, dinv);
             break;
          case L_UPDATE:
-            k_update<T><<<g, b, 0, s>>>(P->d_gemm + L.off, arena);
+            k_update<T, 0><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, nullptr);
             break;
          case L_UPDATE_WIDE:
             if (bf16_wide)
                k_update_bf16<<<g, b, 0, s>>>(P->d_gemm + L.off, reinterpret_cast<float*>(arena));
             else
-               k_update<T><<<g, b, 0, s>>>(P->d_gemm + L.off, arena);
+               k_update<T, 0><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, nullptr);
             break;
          case L_DINV:
             k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, dinvT, F->d_info, 0);
