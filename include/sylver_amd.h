@@ -217,6 +217,20 @@ int sylver_amd_fkeep_node_factor(void const *fkeep, int node, double *l);
  * factorization) is returned when the factorization ran with
  * record_timing == 2 (debug mode). */
 int sylver_amd_fkeep_node_assembled(void const *fkeep, int node, double *front);
+/* Indefinite factors of node `node` (0-based): dims[4] = {m, n, nelim,
+ * ndelay_in} (front rows and fully-summed columns including delayed ones,
+ * eliminated pivots, delayed columns received); rows (m ints): 1-based
+ * pivot-order index of each front row after pivoting (the reference's
+ * NumericFront::perm followed by the row list, src/NumericTree.hxx:423-431);
+ * l (m x nelim, ld m): the columns of L (unit diagonal implied); dinv
+ * (2*nelim): D^{-1} as 1x1 [d, 0] and 2x2 [a, b, inf, c]
+ * (src/kernels/ldlt_app.hxx:166-171). Any pointer may be NULL. */
+int sylver_amd_fkeep_indef_node(void const *fkeep, int node, int *dims, int *rows, double *l,
+                                double *dinv);
+/* out[8] = {num_delay, num_neg, num_two, num_zero, maxfront,
+ * not_first_pass, not_second_pass, 0} of the last indefinite factorization
+ * (the counters of the reference's inform_c, src/sylver_ciface.hxx:73-86). */
+int sylver_amd_fkeep_indef_stats(void const *fkeep, int *out);
 
 /* Batched dense front factorization: for each front f, the nrow[f] x nrow[f]
  * column-major block at data + offset[f] (elements, ld = nrow[f], device

@@ -1,7 +1,7 @@
 """CPU oracle (test infrastructure only): ctypes access to oracle/liboracle.so.
 
-This package restates the reference's posdef numeric phase on the CPU in
-FP64. Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+This package restates the reference's numeric phase (posdef LL^T and APP
+LDL^T with delayed pivots) on the CPU in FP64. Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
 may use it; the library never does. Build it with ``python oracle/build.py``.
 
 Which function each oracle entry restates is stated in oracle/oracle.h.
@@ -39,6 +39,15 @@ def lib():
         L.oracle_front_potrf.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int]
         L.oracle_bwd_err.argtypes = [C.c_int, i64p, ip, dp, C.c_int, dp, C.c_int, dp, C.c_int]
         L.oracle_bwd_err.restype = C.c_double
+        L.oracle_factor_indef.argtypes = [C.c_int, C.c_int, ip, ip, i64p, ip, i64p, i64p, dp,
+                                          C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]
+        L.oracle_factor_indef.restype = C.c_void_p
+        L.oracle_indef_stats.argtypes = [C.c_void_p, ip]
+        L.oracle_indef_node.argtypes = [C.c_void_p, C.c_int, ip, ip, dp, dp]
+        L.oracle_solve_indef.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, C.c_int]
+        L.oracle_indef_free.argtypes = [C.c_void_p]
+        L.oracle_front_ldlt.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_double, C.c_double,
+                                        C.c_int, C.c_int, ip, dp, ip]
         _lib = L
     return _lib
 
@@ -133,3 +142,72 @@ def bwd_err(A, x, b):
     return float(lib().oracle_bwd_err(A.n, _p(A.ptr, C.c_int64), _p(A.row, C.c_int),
                                       _p(A.val, C.c_double), x.shape[1], _p(x, C.c_double), A.n,
                                       _p(b, C.c_double), A.n))
+
+
+STAT_KEYS = ("flag", "num_delay", "num_neg", "num_two", "num_zero", "maxfront",
+             "not_first_pass", "not_second_pass")
+
+
+class IndefFactor:
+    """Oracle APP LDL^T factorization of a whole tree (see oracle.h)."""
+
+    def __init__(self, sym, aval, nb=128, u=0.01, small=1e-20, action=True, failed_tpp=True):
+        self.sym = sym
+        aval = np.ascontiguousarray(aval, np.float64)
+        self.h = lib().oracle_factor_indef(
+            sym["n"], sym["nnodes"], _p(sym["sptr"], C.c_int), _p(sym["sparent"], C.c_int),
+            _p(sym["rptr"], C.c_int64), _p(sym["rlist"], C.c_int), _p(sym["nptr"], C.c_int64),
+            _p(sym["nlist"], C.c_int64), _p(aval, C.c_double), nb, u, small, int(action),
+            int(failed_tpp))
+        out = np.zeros(8, np.int32)
+        lib().oracle_indef_stats(self.h, _p(out, C.c_int))
+        self.stats = dict(zip(STAT_KEYS, out.tolist()))
+
+    def node(self, i):
+        dims = np.zeros(4, np.int32)
+        lib().oracle_indef_node(self.h, i, _p(dims, C.c_int), None, None, None)
+        m, n, ne, nd = dims.tolist()
+        rows = np.zeros(max(m, 1), np.int32)
+        L = np.zeros(max(m * ne, 1))
+        d = np.zeros(max(2 * ne, 1))
+        lib().oracle_indef_node(self.h, i, _p(dims, C.c_int), _p(rows, C.c_int),
+                                _p(L, C.c_double), _p(d, C.c_double))
+        return dict(m=m, n=n, nelim=ne, ndelay_in=nd, rows=rows[:m],
+                    L=L[:m * ne].reshape((m, ne), order="F"), dinv=d[:2 * ne])
+
+    def solve(self, b, job=0):
+        """b in pivot order, (n,) or (n, nrhs); returns x in pivot order."""
+        x = np.asfortranarray(np.array(b, dtype=np.float64, copy=True).reshape(self.sym["n"], -1))
+        lib().oracle_solve_indef(self.h, job, x.shape[1], _p(x, C.c_double), self.sym["n"])
+        return x
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_indef_free(self.h)
+            self.h = None
+
+
+def front_ldlt(a, n, nb=128, u=0.01, small=1e-20, action=True, failed_tpp=True):
+    """Dense APP + TPP factorization of a full symmetric front (copy).
+    Returns (nelim, nelim_first, perm, dinv, factored front)."""
+    a = np.asfortranarray(np.array(a, dtype=np.float64, copy=True))
+    m = a.shape[0]
+    perm = np.zeros(max(n, 1), np.int32)
+    dinv = np.zeros(max(2 * n, 2))
+    nf = C.c_int()
+    ne = lib().oracle_front_ldlt(m, n, _p(a, C.c_double), nb, u, small, int(action),
+                                 int(failed_tpp), _p(perm, C.c_int), _p(dinv, C.c_double),
+                                 C.byref(nf))
+    return ne, nf.value, perm[:n], dinv[:2 * n], a
+
+
+def dinv_blocks(dinv, nelim):
+    """Yield (start, width, 2x2-or-1x1 D^{-1} block) from the D^{-1} array."""
+    i = 0
+    while i < nelim:
+        if i + 1 < nelim and np.isinf(dinv[2 * i + 2]):
+            yield i, 2, np.array([[dinv[2 * i], dinv[2 * i + 1]], [dinv[2 * i + 1], dinv[2 * i + 3]]])
+            i += 2
+        else:
+            yield i, 1, np.array([[dinv[2 * i]]])
+            i += 1

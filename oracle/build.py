@@ -20,9 +20,21 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-march=x86-64-v3"
 
 
 def build(verbose: bool = False) -> str:
-    src = os.path.join(HERE, "oracle.cpp")
+    """oracle.cpp keeps the compiler's default FMA contraction (it is the
+    timed CPU baseline); oracle_indef.cpp is built with -ffp-contract=off
+    because its roundings are written out explicitly."""
+    objs = []
+    for src, extra in (("oracle.cpp", []), ("oracle_indef.cpp", ["-ffp-contract=off"])):
+        obj = os.path.join(HERE, "_build", src.replace(".cpp", ".o"))
+        os.makedirs(os.path.dirname(obj), exist_ok=True)
+        cmd = ["g++", *FLAGS, *extra, "-c", os.path.join(HERE, src), "-o", obj]
+        cmd.remove("-shared")
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
     tmp = OUT + ".tmp"
-    cmd = ["g++", *FLAGS, src, "-o", tmp]
+    cmd = ["g++", "-shared", "-fopenmp", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

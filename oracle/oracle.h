@@ -73,6 +73,36 @@ int oracle_front_potrf(int m, int ncol, double *a, int lda, int nb);
 double oracle_bwd_err(int n, const int64_t *ptr, const int *row, const double *val, int nrhs,
                       const double *x, int ldx, const double *b, int ldb);
 
+/* ---- indefinite path (oracle_indef.cpp) ---------------------------------
+ * APP LDL^T with delayed pivots: factorize_indef_app_notask
+ * (src/factor_indef.hxx:600-860), permute_failed (:1190-1289),
+ * factor_front_indef_failed (src/factor_failed.hxx:29-157),
+ * assemble_delays (src/kernels/assemble.hxx:926-961), the delay-aware
+ * assembly maps (src/assemble.hxx:152-190) and the inertia count
+ * (src/NumericTree.hxx:150-179). The pivot rule inside a diagonal block and
+ * the TPP second pass are this oracle's own statement of published methods
+ * (Bunch-Parlett complete pivoting; threshold partial pivoting with the
+ * Duff-Reid 2x2 test): the reference delegates both to SPRAL, which is not
+ * in its tree. Returns an opaque handle (free with oracle_indef_free). */
+void *oracle_factor_indef(int n, int nnodes, const int *sptr, const int *sparent,
+                          const int64_t *rptr, const int *rlist, const int64_t *nptr,
+                          const int64_t *nlist, const double *aval, int nb, double u,
+                          double small, int action, int failed_tpp);
+/* out[8] = {flag, num_delay, num_neg, num_two, num_zero, maxfront,
+ * not_first_pass, not_second_pass}. */
+void oracle_indef_stats(const void *h, int *out);
+/* dims[4] = {m, n, nelim, ndelay_in}; rows (m, 1-based pivot-order index of
+ * each front row), L (m x nelim, ld m), dinv (2 nelim); any may be NULL. */
+void oracle_indef_node(const void *h, int node, int *dims, int *rows, double *L, double *dinv);
+/* job 0 full, 1 fwd, 2 diag, 3 bwd, 4 diag+bwd (src/NumericTree.hxx:408-532). */
+void oracle_solve_indef(const void *h, int job, int nrhs, double *x, int ldx);
+void oracle_indef_free(void *h);
+/* One dense front (m x m full symmetric, ld m) with n fully-summed columns:
+ * returns nelim (or -5 singular with action == 0); perm[n] is the position
+ * permutation, dinv[2n] the D^{-1} entries, *nelim_first the APP count. */
+int oracle_front_ldlt(int m, int n, double *a, int nb, double u, double small, int action,
+                      int failed_tpp, int *perm, double *dinv, int *nelim_first);
+
 #ifdef __cplusplus
 }
 #endif

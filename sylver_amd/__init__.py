@@ -95,6 +95,20 @@ class Options(C.Structure):
     ]
 
 
+class OptionsC(C.Structure):
+    """sylver_options_c (include/sylver_amd.h; reference src/sylver_ciface.hxx:39-55)."""
+    _fields_ = [("print_level", C.c_int), ("action", C.c_bool), ("small", C.c_double),
+                ("u", C.c_double), ("multiplier", C.c_double),
+                ("small_subtree_threshold", C.c_long), ("nb", C.c_int), ("pivot_method", C.c_int),
+                ("failed_pivot_method", C.c_int), ("cpu_topology", C.c_int)]
+
+
+class InformC(C.Structure):
+    """sylver_inform_c (include/sylver_amd.h; reference src/sylver_ciface.hxx:73-86)."""
+    _fields_ = [(k, C.c_int) for k in ("flag", "num_delay", "num_neg", "num_two", "num_zero",
+                                        "maxfront", "not_first_pass", "not_second_pass")]
+
+
 class ExtOptions(C.Structure):
     """sylver_amd_options_t."""
     _fields_ = [
@@ -134,6 +148,7 @@ EXPORTED = [
     "sylver_amd_get_options", "sylver_amd_fkeep_stats", "sylver_amd_akeep_sizes",
     "sylver_amd_akeep_symbolic", "sylver_amd_fkeep_device_symbolic",
     "sylver_amd_fkeep_node_factor", "sylver_amd_fkeep_node_assembled",
+    "sylver_amd_fkeep_indef_node", "sylver_amd_fkeep_indef_stats",
     "sylver_amd_batched_front_potrf", "sylver_amd_device_count", "sylver_amd_dev_alloc",
     "sylver_amd_dev_free", "sylver_amd_memcpy_h2d", "sylver_amd_memcpy_d2h",
     "sylver_amd_last_batched_ms",
@@ -170,6 +185,8 @@ def lib():
     L.sylver_amd_akeep_symbolic.argtypes = [vp, ip, ip, ip, ip, lp, ip, lp, lp]
     L.sylver_amd_fkeep_device_symbolic.argtypes = [vp, ip, ip, lp, ip, lp, lp, ip, lp]
     L.sylver_amd_fkeep_node_factor.argtypes = [vp, C.c_int, dp]
+    L.sylver_amd_fkeep_indef_node.argtypes = [vp, C.c_int, ip, ip, dp, dp]
+    L.sylver_amd_fkeep_indef_stats.argtypes = [vp, ip]
     L.sylver_amd_batched_front_potrf.argtypes = [C.c_int, ip, ip, lp, vp, C.c_int, ip, vp]
     L.sylver_amd_device_count.argtypes = []
     L.sylver_amd_dev_alloc.argtypes = [pvp, C.c_size_t]
@@ -316,6 +333,33 @@ def fkeep_node_factor(fkeep, node, m, n):
     if flag != 0:
         raise RuntimeError(f"node factor read-back failed: {flag}")
     return out
+
+
+INDEF_STAT_KEYS = ("num_delay", "num_neg", "num_two", "num_zero", "maxfront",
+                   "not_first_pass", "not_second_pass")
+
+
+def fkeep_indef_stats(fkeep):
+    out = np.zeros(8, np.int32)
+    flag = lib().sylver_amd_fkeep_indef_stats(fkeep, _p(out, C.c_int))
+    if flag != 0:
+        raise RuntimeError(f"indefinite statistics read-back failed: {flag}")
+    return dict(zip(INDEF_STAT_KEYS, out[:7].tolist()))
+
+
+def fkeep_indef_node(fkeep, node):
+    """Indefinite factors of one node: dict(m, n, nelim, ndelay_in, rows, L, dinv)."""
+    dims = np.zeros(4, np.int32)
+    flag = lib().sylver_amd_fkeep_indef_node(fkeep, node, _p(dims, C.c_int), None, None, None)
+    if flag != 0:
+        raise RuntimeError(f"indefinite node read-back failed: {flag}")
+    m, n, ne, nd = dims.tolist()
+    rows = np.zeros(max(m, 1), np.int32)
+    L = np.zeros((max(m, 1), max(ne, 1)), order="F")
+    d = np.zeros(max(2 * ne, 1))
+    lib().sylver_amd_fkeep_indef_node(fkeep, node, _p(dims, C.c_int), _p(rows, C.c_int),
+                                      _p(L, C.c_double), _p(d, C.c_double))
+    return dict(m=m, n=n, nelim=ne, ndelay_in=nd, rows=rows[:m], L=L[:m, :ne], dinv=d[:2 * ne])
 
 
 def fkeep_stats(fkeep):

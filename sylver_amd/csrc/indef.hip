@@ -1,0 +1,406 @@
+// Device numeric phase of the indefinite path: APP LDL^T with delayed pivots
+// on one GPU.
+//
+// The reference factors fronts in postorder with StarPU tasks
+// (src/NumericTree.hxx:60-180, factor_front_indef_app + factor_front_indef_failed,
+// src/factor_indef.hxx:1338-1930, src/factor_failed.hxx:29-157). Delayed
+// columns change the size of every ancestor front, so the level schedule is
+// driven from the host one level at a time: the host learns the number of
+// delayed columns of a level (one small read-back) before it lays out the
+// fronts of the next level. Per level:
+//   zero -> scatter A (rows shifted by the delays) -> delayed columns of the
+//   children -> extend-add (one launch per child rank) -> panel
+//   factorization (one workgroup per front, k_app_front) -> read back nelim
+//   -> L D into a workspace -> MFMA GEMM update of the contribution blocks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "kernels.hpp"
+#include "kernels_indef.hpp"
+#include "numeric.hpp"
+#include "numeric_impl.hpp"
+
+namespace sylver_amd {
+
+struct IndefState {
+   int nnodes = 0;
+   std::vector<IndefDesc> h;
+   std::vector<std::vector<int>> levels;
+   std::vector<int> height;
+   IndefDesc* d_nodes = nullptr;
+   double* arena = nullptr;
+   int64_t arena_cap = 0, arena_used = 0;
+   int* d_vars = nullptr;
+   int64_t vars_cap = 0;
+   int* d_pres = nullptr;
+   int64_t pres_cap = 0;
+   double* d_dinv = nullptr;
+   int64_t dinv_cap = 0;
+   double* d_ws = nullptr;
+   int64_t ws_cap = 0;
+   int* d_list = nullptr;
+   int64_t list_cap = 0;
+   GemmTask* d_tasks = nullptr;
+   int64_t tasks_cap = 0;
+   int* d_status = nullptr;
+   int* d_out = nullptr;
+   int64_t out_cap = 0;
+   double* d_xw = nullptr;
+   int64_t xw_cap = 0;
+   int64_t x_rows = 0;  // sum of 2 m over nodes
+   // statistics of the last factorization
+   int num_delay = 0, num_neg = 0, num_two = 0, num_zero = 0, maxfront = 0;
+   int not_first_pass = 0, not_second_pass = 0;
+   int64_t num_factor = 0;
+   double flops = 0;
+
+   ~IndefState() {
+      for (void* p : {(void*)d_nodes, (void*)arena, (void*)d_vars, (void*)d_pres, (void*)d_dinv,
+                      (void*)d_ws, (void*)d_list, (void*)d_tasks, (void*)d_status, (void*)d_out,
+                      (void*)d_xw})
+         dfree(p);
+   }
+};
+
+void destroy_indef(IndefState* s) { delete s; }
+
+namespace {
+
+inline unsigned grid_of(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kThreads - 1) / kThreads); }
+
+// Grow a device buffer to at least `need` elements, keeping `keep` elements.
+template <typename T>
+void grow(T*& p, int64_t& cap, int64_t need, int64_t keep, hipStream_t s) {
+   if (need <= cap) return;
+   int64_t nc = std::max<int64_t>(need, cap + cap / 2);
+   T* q = dalloc<T>(nc);
+   if (p && keep > 0) HIPCHECK(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
+   if (p) {
+      HIPCHECK(hipStreamSynchronize(s));
+      dfree(p);
+   }
+   p = q;
+   cap = nc;
+}
+
+int64_t ws_elems(int m, int n) { return (int64_t)m * (kAppNB + n + 5) + 8; }
+
+}  // namespace
+
+void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, const sylver_options_t& opts,
+                  FactorResult& res) {
+   hipStream_t s = F->stream;
+   if (!F->ind) F->ind = new IndefState();
+   IndefState& S = *F->ind;
+   const int nn = a.nnodes;
+   S.nnodes = nn;
+   S.h.assign(nn, IndefDesc{});
+   S.height.assign(nn, 0);
+   for (int i = 0; i < nn; ++i) {
+      const SymbolicFront& f = a.fronts[i];
+      IndefDesc& d = S.h[i];
+      d.ncol = f.ncol;
+      d.nrow = f.nrow;
+      d.parent = f.parent;
+      d.first_child = f.first_child;
+      d.next_child = f.next_child;
+      d.rlist_off = f.rlist_off;
+      d.amap_off = f.amap_off;
+      d.num_a = f.num_a;
+      d.cmap_off = P->hnodes[i].cmap_off;
+      if (f.parent < nn) S.height[f.parent] = std::max(S.height[f.parent], S.height[i] + 1);
+   }
+   int nlev = 0;
+   for (int i = 0; i < nn; ++i) nlev = std::max(nlev, S.height[i] + 1);
+   S.levels.assign(nlev, {});
+   for (int i = 0; i < nn; ++i) S.levels[S.height[i]].push_back(i);
+   if (!S.d_nodes) S.d_nodes = dalloc<IndefDesc>(std::max(nn, 1));
+   if (!S.d_status) S.d_status = dalloc<int>(std::max(nn, 1));
+   HIPCHECK(hipMemsetAsync(S.d_status, 0, std::max(nn, 1) * sizeof(int), s));
+   // initial capacities from the symbolic sizes; grown when delays need more
+   {
+      int64_t ar = 0, vr = 0, pr = 0;
+      for (int i = 0; i < nn; ++i) {
+         const int m = S.h[i].nrow;
+         ar += (int64_t)((m + 7) / 8 * 8) * m + 8;
+         vr += m;
+         pr += S.h[i].ncol;
+      }
+      grow(S.arena, S.arena_cap, ar + ar / 8 + 64, 0, s);
+      grow(S.d_vars, S.vars_cap, vr + vr / 4 + 64, 0, s);
+      grow(S.d_pres, S.pres_cap, pr + pr / 4 + 64, 0, s);
+      grow(S.d_dinv, S.dinv_cap, 2 * (pr + pr / 4 + 64), 0, s);
+   }
+   const double u = opts.u, small = opts.small;
+   const int action = opts.action ? 1 : 0;
+   const int failed_tpp = (opts.failed_pivot_method == 1 || opts.pivot_method == 3) ? 1 : 0;
+   int64_t arena_used = 0, vars_used = 0, pres_used = 0, dinv_used = 0, x_used = 0;
+   S.num_factor = 0;
+   S.maxfront = 0;
+   S.flops = 0;
+   for (int L = 0; L < nlev; ++L) {
+      const std::vector<int>& lv = S.levels[L];
+      // sizes of this level's fronts from the children's delays
+      int64_t lv_arena = 0, lv_ws = 0;
+      const int64_t vars_before = vars_used, pres_before = pres_used, dinv_before = dinv_used;
+      for (int i : lv) {
+         IndefDesc& d = S.h[i];
+         int ndin = 0, dcol = d.ncol;
+         for (int c = d.first_child; c != -1; c = S.h[c].next_child) {
+            IndefDesc& cd = S.h[c];
+            cd.delay_col0 = dcol;
+            dcol += cd.n - cd.nelim;
+            ndin += cd.n - cd.nelim;
+         }
+         d.ndelay_in = ndin;
+         d.m = d.nrow + ndin;
+         d.n = d.ncol + ndin;
+         d.ld = (d.m + 7) / 8 * 8;
+         d.off = arena_used + lv_arena;
+         lv_arena += (int64_t)d.ld * d.m;
+         d.var_off = vars_used;
+         vars_used += d.m;
+         d.pre_off = pres_used;
+         pres_used += d.n;
+         d.d_off = dinv_used;
+         dinv_used += 2 * (int64_t)d.n;
+         d.ws_off = lv_ws;
+         lv_ws += ws_elems(d.m, d.n);
+         d.x_off = x_used;
+         x_used += 2 * (int64_t)d.m;
+         d.nelim = 0;
+         d.nelim_first = 0;
+         S.maxfront = std::max(S.maxfront, d.m);
+      }
+      grow(S.arena, S.arena_cap, arena_used + lv_arena, arena_used, s);
+      grow(S.d_vars, S.vars_cap, vars_used, vars_before, s);
+      grow(S.d_pres, S.pres_cap, pres_used, pres_before, s);
+      grow(S.d_dinv, S.dinv_cap, dinv_used, dinv_before, s);
+      grow(S.d_ws, S.ws_cap, lv_ws, 0, s);
+      // lists: level nodes, children with delays, children by rank
+      std::vector<int> lists(lv.begin(), lv.end());
+      const int64_t o_delay = (int64_t)lists.size();
+      for (int i : lv)
+         for (int c = S.h[i].first_child; c != -1; c = S.h[c].next_child)
+            if (S.h[c].n > S.h[c].nelim) lists.push_back(c);
+      const int n_delay = (int)(lists.size() - o_delay);
+      std::vector<std::pair<int64_t, int>> ranks;  // (offset, count)
+      for (int r = 0;; ++r) {
+         const int64_t o = (int64_t)lists.size();
+         for (int i : lv) {
+            int c = S.h[i].first_child;
+            for (int k = 0; k < r && c != -1; ++k) c = S.h[c].next_child;
+            if (c != -1 && S.h[c].m > S.h[c].n) lists.push_back(c);
+         }
+         const int cnt = (int)(lists.size() - o);
+         bool any = false;
+         for (int i : lv) {
+            int c = S.h[i].first_child;
+            for (int k = 0; k < r && c != -1; ++k) c = S.h[c].next_child;
+            if (c != -1) any = true;
+         }
+         if (!any) break;
+         if (cnt > 0) ranks.push_back({o, cnt});
+      }
+      grow(S.d_list, S.list_cap, (int64_t)lists.size(), 0, s);
+      grow(S.d_out, S.out_cap, 2 * (int64_t)lv.size(), 0, s);
+      HIPCHECK(hipMemcpyAsync(S.d_list, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemcpyAsync(S.d_nodes, S.h.data(), nn * sizeof(IndefDesc), hipMemcpyHostToDevice, s));
+      const int nl = (int)lv.size();
+      // zero the level's fronts (contiguous)
+      k_fill<<<grid_of(lv_arena), kThreads, 0, s>>>(lv_arena, 0.0, S.arena + arena_used);
+      k_ind_init<<<nl, kThreads, 0, s>>>(S.d_list, S.d_nodes, P->d_nlist, P->d_rlist, F->d_aval, S.arena,
+                                         S.d_vars, S.d_pres);
+      if (n_delay > 0)
+         k_ind_delays<<<n_delay, kThreads, 0, s>>>(S.d_list + o_delay, S.d_nodes, P->d_cmap, S.arena, S.d_vars);
+      for (auto& rk : ranks) {
+         dim3 g(64, rk.second);
+         k_ind_ea<<<g, kThreads, 0, s>>>(S.d_list + rk.first, S.d_nodes, P->d_cmap, S.arena);
+      }
+      k_app_front<<<nl, kThreads, 0, s>>>(S.d_list, S.d_nodes, S.arena, S.d_vars, S.d_pres, S.d_dinv,
+                                          S.d_ws, u, small, action, failed_tpp, S.d_status, S.d_out);
+      std::vector<int> out(2 * nl);
+      HIPCHECK(hipMemcpyAsync(out.data(), S.d_out, out.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      // contribution blocks: L D into the workspace, then the MFMA update
+      std::vector<int> cb_nodes;
+      std::vector<GemmTask> tasks;
+      for (int k = 0; k < nl; ++k) {
+         IndefDesc& d = S.h[lv[k]];
+         d.nelim = out[2 * k];
+         d.nelim_first = out[2 * k + 1];
+         S.num_factor += (int64_t)d.m * d.nelim;
+         for (int64_t r = d.m - d.nelim + 1; r <= d.m; ++r) S.flops += (double)r * r;
+         const int q = d.m - d.n;
+         if (q <= 0 || d.nelim <= 0) continue;
+         cb_nodes.push_back(lv[k]);
+         const int nt = (q + 127) / 128;
+         for (int tj = 0; tj < nt; ++tj)
+            for (int ti = tj; ti < nt; ++ti) {
+               GemmTask t{};
+               t.a_off = d.off + d.n + (int64_t)ti * 128;
+               t.b_off = d.ws_off + (int64_t)tj * 128;
+               t.c_off = d.off + (d.n + (int64_t)ti * 128) + (d.n + (int64_t)tj * 128) * d.ld;
+               t.lda = d.ld;
+               t.ldb = q;
+               t.ldc = d.ld;
+               t.M = std::min(128, q - ti * 128);
+               t.N = std::min(128, q - tj * 128);
+               t.K = d.nelim;
+               tasks.push_back(t);
+            }
+      }
+      if (!cb_nodes.empty()) {
+         while (tasks.size() % 8) {
+            GemmTask t{};
+            t.M = 0;
+            tasks.push_back(t);
+         }
+         grow(S.d_list, S.list_cap, (int64_t)cb_nodes.size(), 0, s);
+         grow(S.d_tasks, S.tasks_cap, (int64_t)tasks.size(), 0, s);
+         HIPCHECK(hipMemcpyAsync(S.d_list, cb_nodes.data(), cb_nodes.size() * sizeof(int), hipMemcpyHostToDevice, s));
+         HIPCHECK(hipMemcpyAsync(S.d_tasks, tasks.data(), tasks.size() * sizeof(GemmTask), hipMemcpyHostToDevice, s));
+         HIPCHECK(hipMemcpyAsync(S.d_nodes, S.h.data(), nn * sizeof(IndefDesc), hipMemcpyHostToDevice, s));
+         dim3 g(64, (unsigned)cb_nodes.size());
+         k_ind_ld<<<g, kThreads, 0, s>>>(S.d_list, S.d_nodes, S.arena, S.d_dinv, S.d_ws);
+         k_update<double, 2><<<(unsigned)tasks.size(), kThreads, 0, s>>>(S.d_tasks, S.arena, S.d_ws);
+         HIPCHECK(hipStreamSynchronize(s));  // the task and list buffers are rewritten next level
+      }
+      arena_used += lv_arena;
+   }
+   S.arena_used = arena_used;
+   S.x_rows = x_used;
+   HIPCHECK(hipMemcpyAsync(S.d_nodes, S.h.data(), nn * sizeof(IndefDesc), hipMemcpyHostToDevice, s));
+   // flags and statistics (src/NumericTree.hxx:150-179, src/factor_failed.hxx:55-150)
+   std::vector<int> status(std::max(nn, 1));
+   std::vector<double> dv(std::max<int64_t>(dinv_used, 1));
+   HIPCHECK(hipMemcpyAsync(status.data(), S.d_status, nn * sizeof(int), hipMemcpyDeviceToHost, s));
+   if (dinv_used > 0)
+      HIPCHECK(hipMemcpyAsync(dv.data(), S.d_dinv, dinv_used * sizeof(double), hipMemcpyDeviceToHost, s));
+   HIPCHECK(hipStreamSynchronize(s));
+   S.num_delay = S.num_neg = S.num_two = S.num_zero = 0;
+   S.not_first_pass = S.not_second_pass = 0;
+   bool singular = false;
+   for (int i = 0; i < nn; ++i) {
+      const IndefDesc& d = S.h[i];
+      if (status[i] != 0) singular = true;
+      S.num_delay += d.n - d.nelim;
+      if (d.nelim_first < d.n) S.not_first_pass += d.n - d.nelim_first;
+      if (failed_tpp || d.m == d.n) S.not_second_pass += d.n - d.nelim;
+      if (d.parent >= nn && d.nelim < d.n) singular = true;
+      const double* dd = dv.data() + d.d_off;
+      for (int k = 0; k < d.nelim;) {
+         const double a11 = dd[2 * k], a21 = dd[2 * k + 1];
+         if (k + 1 == d.nelim || std::isfinite(dd[2 * k + 2])) {
+            if (a11 == 0.0) ++S.num_zero;
+            if (a11 < 0.0) ++S.num_neg;
+            ++k;
+         } else {
+            const double a22 = dd[2 * k + 3];
+            ++S.num_two;
+            const double det = a11 * a22 - a21 * a21, tr = a11 + a22;
+            if (det < 0)
+               ++S.num_neg;
+            else if (tr < 0)
+               S.num_neg += 2;
+            k += 2;
+         }
+      }
+   }
+   res.num_delay = S.num_delay;
+   res.num_neg = S.num_neg;
+   res.num_two = S.num_two;
+   res.num_zero = S.num_zero;
+   res.not_first_pass = S.not_first_pass;
+   res.not_second_pass = S.not_second_pass;
+   res.matrix_rank = a.n - S.num_zero;
+   res.maxfront = S.maxfront;
+   res.num_factor = S.num_factor;
+   res.num_flops = S.flops;
+   if (singular)
+      res.flag = SYLVER_ERROR_SINGULAR;
+   else if (S.num_zero > 0)
+      res.flag = SYLVER_WARNING_FACT_SINGULAR;
+}
+
+// Solves with the indefinite factors: job 0 full, 1 fwd, 2 diag, 3 bwd,
+// 4 diag+bwd. b and sol are n x nrhs in pivot order on the device.
+void solve_indef(const DevicePlan* P, DeviceFactors* F, int job, const double* b, double* sol, int nrhs,
+                 hipStream_t s) {
+   IndefState& S = *F->ind;
+   const int n = P->n;
+   const int64_t nn = (int64_t)n * nrhs;
+   grow(S.d_xw, S.xw_cap, std::max<int64_t>(S.x_rows * nrhs, 1), 0, s);
+   int64_t lcap = 0;
+   for (auto& lv : S.levels) lcap += (int64_t)lv.size();
+   grow(S.d_list, S.list_cap, lcap, 0, s);
+   std::vector<int> all;
+   std::vector<int64_t> loff;
+   for (auto& lv : S.levels) {
+      loff.push_back((int64_t)all.size());
+      all.insert(all.end(), lv.begin(), lv.end());
+   }
+   HIPCHECK(hipMemcpyAsync(S.d_list, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice, s));
+   const bool fwd = job == 0 || job == 1, diag = job == 0 || job == 2 || job == 4,
+              bwd = job == 0 || job == 3 || job == 4;
+   if (fwd) {
+      for (size_t L = 0; L < S.levels.size(); ++L)
+         k_ind_fwd<<<(unsigned)S.levels[L].size(), kThreads, 0, s>>>(
+             S.d_list + loff[L], S.d_nodes, S.arena, S.d_vars, S.d_pres, P->d_cmap, b, n, S.d_xw, sol, nrhs);
+   } else {
+      k_copy<<<grid_of(nn), kThreads, 0, s>>>(nn, b, sol);
+   }
+   if (diag)
+      k_ind_diag<<<std::max(S.nnodes, 1), kThreads, 0, s>>>(S.d_nodes, S.nnodes, S.d_vars, S.d_dinv, sol, n,
+                                                            nrhs);
+   if (bwd)
+      for (int L = (int)S.levels.size() - 1; L >= 0; --L)
+         k_ind_bwd<<<(unsigned)S.levels[L].size(), kThreads, 0, s>>>(S.d_list + loff[L], S.d_nodes, S.arena,
+                                                                     S.d_vars, S.d_xw, sol, n, nrhs);
+}
+
+int indef_stats(const DeviceFactors* F, int* out) {
+   if (!F || !F->ind) return SYLVER_ERROR_CALL_SEQUENCE;
+   const IndefState& S = *F->ind;
+   out[0] = S.num_delay;
+   out[1] = S.num_neg;
+   out[2] = S.num_two;
+   out[3] = S.num_zero;
+   out[4] = S.maxfront;
+   out[5] = S.not_first_pass;
+   out[6] = S.not_second_pass;
+   out[7] = 0;
+   return 0;
+}
+
+// Read-back of one node of the indefinite factors: dims {m, n, nelim,
+// ndelay_in}, the front rows' variables, L (m x nelim, ld m) and D^{-1}.
+int indef_node(const DeviceFactors* F, int node, int* dims, int* rows, double* L, double* dinv) {
+   if (!F || !F->ind || node < 0 || node >= F->ind->nnodes) return SYLVER_ERROR_CALL_SEQUENCE;
+   const IndefState& S = *F->ind;
+   const IndefDesc& d = S.h[node];
+   try {
+      if (dims) {
+         dims[0] = d.m;
+         dims[1] = d.n;
+         dims[2] = d.nelim;
+         dims[3] = d.ndelay_in;
+      }
+      if (rows && d.m > 0) HIPCHECK(hipMemcpy(rows, S.d_vars + d.var_off, d.m * sizeof(int), hipMemcpyDeviceToHost));
+      if (L && d.nelim > 0)
+         HIPCHECK(hipMemcpy2D(L, (size_t)d.m * sizeof(double), S.arena + d.off, (size_t)d.ld * sizeof(double),
+                              (size_t)d.m * sizeof(double), d.nelim, hipMemcpyDeviceToHost));
+      if (dinv && d.nelim > 0)
+         HIPCHECK(hipMemcpy(dinv, S.d_dinv + d.d_off, 2 * d.nelim * sizeof(double), hipMemcpyDeviceToHost));
+      return 0;
+   } catch (const HipError&) {
+      return SYLVER_ERROR_CUDA_UNKNOWN;
+   }
+}
+
+}  // namespace sylver_amd

@@ -51,6 +51,19 @@ struct GemmTask {
 };
 struct SolveTask { int node, k0, kb, r0, r1, write; };
 
+// Zero pad for masked loads. A masked-off element is read from here instead
+// of being skipped: `ok ? *p : 0` compiles to a branch around the load with
+// a wait for it at the join, which serializes a batch of loads; selecting
+// the address keeps every load unconditional and in flight together. The
+// pad is never written (device globals start zeroed) and is not const, so
+// the compiler cannot fold the load back into a select.
+__device__ __align__(16) double g_zero_pad[4];
+
+template <typename P>
+__device__ __forceinline__ const P* live_or_zero(bool ok, const P* p) {
+   return ok ? p : reinterpret_cast<const P*>(g_zero_pad);
+}
+
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -150,12 +163,12 @@ __global__ __launch_bounds__(kThreads) void k_extend_add(const EaTask* __restric
          T sv[8], dv[8];
 #pragma unroll
          for (int u = 0; u < 8; ++u) {
-            const int i = i0 + u * 64 + lane;
-            ci[u] = (i < q) ? cm[i] : 0;
-            sv[u] = (i < q) ? src[i] : T(0);
+            const int i = min(i0 + u * 64 + lane, q - 1);
+            ci[u] = cm[i];
+            sv[u] = src[i];
          }
 #pragma unroll
-         for (int u = 0; u < 8; ++u) dv[u] = (i0 + u * 64 + lane < q) ? dst[ci[u]] : T(0);
+         for (int u = 0; u < 8; ++u) dv[u] = dst[ci[u]];
 #pragma unroll
          for (int u = 0; u < 8; ++u)
             if (i0 + u * 64 + lane < q) dst[ci[u]] = dv[u] + sv[u];
@@ -366,7 +379,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ 
       for (int u = 0; u < 16; ++u) {
          const int e = e0 + u * kThreads + threadIdx.x;
          const int i = e % kb16, j = e / kb16;
-         v[u] = (e < kb16 * kb16 && i < kb && j < kb && i >= j) ? blk[i + (int64_t)j * nd.ld] : T(0);
+         v[u] = *live_or_zero(e < kb16 * kb16 && i < kb && j < kb && i >= j, blk + i + (int64_t)j * nd.ld);
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
@@ -495,7 +508,8 @@ template <> struct V16<float> {
 
 __device__ inline int xcd_task(int b, int G) { return (b & 7) * (G >> 3) + (b >> 3); }
 
-// MODE 0: Schur update C -= A B^T (B in the arena). MODE 1: panel TRSM
+// MODE 0: Schur update C -= A B^T (B in the arena). MODE 2: the same with B
+// from `bsrc` (the L D workspace of the indefinite path). MODE 1: panel TRSM
 // C = A B^T with B = inv(L_kk) from `bsrc`, C == A in place (the tile spans
 // all kb columns, so each workgroup owns its rows and has read all of A
 // before it writes). Tasks whose offsets or leading dimensions are not
@@ -519,7 +533,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
    const int wm = wave >> 1, wn = wave & 1;
    const int ld = t.lda, ldb = t.ldb;
    const T* a = arena + t.a_off;
-   const T* b = (MODE == 1 ? bsrc : arena) + t.b_off;
+   const T* b = (MODE == 0 ? arena : bsrc) + t.b_off;
    T* c = arena + t.c_off;
    const bool vec = ((t.a_off | t.b_off | (int64_t)ld | (int64_t)ldb) % VN) == 0;
    acc_t acc[FM][FN];
@@ -534,17 +548,15 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
          for (int u = 0; u < AV; ++u) {
             const int v = tid + u * kThreads;
             const int kk = v / (BM / VN), row = (v % (BM / VN)) * VN;
-            ra[u] = (k0 + kk < t.K && row < t.M)
-                        ? *reinterpret_cast<const vt*>(a + row + (int64_t)(k0 + kk) * ld)
-                        : vt(0);
+            ra[u] = *live_or_zero(k0 + kk < t.K && row < t.M,
+                                  reinterpret_cast<const vt*>(a + row + (int64_t)(k0 + kk) * ld));
          }
 #pragma unroll
          for (int u = 0; u < BV; ++u) {
             const int v = tid + u * kThreads;
             const int kk = v / (BN / VN), row = (v % (BN / VN)) * VN;
-            rb[u] = (k0 + kk < t.K && row < t.N)
-                        ? *reinterpret_cast<const vt*>(b + row + (int64_t)(k0 + kk) * ldb)
-                        : vt(0);
+            rb[u] = *live_or_zero(k0 + kk < t.K && row < t.N,
+                                  reinterpret_cast<const vt*>(b + row + (int64_t)(k0 + kk) * ldb));
          }
       } else {
 #pragma unroll
@@ -553,7 +565,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
             const int kk = v / (BM / VN), row = (v % (BM / VN)) * VN;
 #pragma unroll
             for (int e = 0; e < VN; ++e)
-               ra[u][e] = (k0 + kk < t.K && row + e < t.M) ? a[row + e + (int64_t)(k0 + kk) * ld] : T(0);
+               ra[u][e] = *live_or_zero(k0 + kk < t.K && row + e < t.M, a + row + e + (int64_t)(k0 + kk) * ld);
          }
 #pragma unroll
          for (int u = 0; u < BV; ++u) {
@@ -561,7 +573,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
             const int kk = v / (BN / VN), row = (v % (BN / VN)) * VN;
 #pragma unroll
             for (int e = 0; e < VN; ++e)
-               rb[u][e] = (k0 + kk < t.K && row + e < t.N) ? b[row + e + (int64_t)(k0 + kk) * ldb] : T(0);
+               rb[u][e] = *live_or_zero(k0 + kk < t.K && row + e < t.N, b + row + e + (int64_t)(k0 + kk) * ldb);
          }
       }
    };
@@ -595,26 +607,36 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
 #pragma unroll
          for (int x = 0; x < FM; ++x)
 #pragma unroll
-            for (int y = 0; y < FN; ++y) acc[x][y] = Mfma<T>::op(af[x], bf[y], acc[x][y]);
+            for (int y = 0; y < FN; ++y) acc[x][y] = Mfma<T>::op(bf[y], af[x], acc[x][y]);
       }
    }
+   // acc[x][y] holds the transposed 16x16 tile: item v of lane l is
+   // C[i = l%16][j = row(l, v)], so 16 lanes cover 16 consecutive rows of a
+   // column. The C values of one y are all loaded before any is stored.
 #pragma unroll
-   for (int x = 0; x < FM; ++x)
+   for (int y = 0; y < FN; ++y) {
+      T cv[FM][4];
+      if (MODE != 1) {
 #pragma unroll
-      for (int y = 0; y < FN; ++y) {
-         const int j = wn * WN + y * 16 + (lane & 15);
-         if (j >= t.N || j < t.pad0) continue;
+         for (int x = 0; x < FM; ++x)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+               const int i = wm * WM + x * 16 + (lane & 15);
+               const int j = wn * WN + y * 16 + Mfma<T>::row(lane, v);
+               const bool ok = i < t.M && j < t.N && j >= t.pad0;
+               cv[x][v] = *live_or_zero(ok, c + i + (int64_t)j * ld);
+            }
+      }
+#pragma unroll
+      for (int x = 0; x < FM; ++x)
 #pragma unroll
          for (int v = 0; v < 4; ++v) {
-            const int i = wm * WM + x * 16 + Mfma<T>::row(lane, v);
-            if (i < t.M) {
-               if (MODE == 1)
-                  c[i + (int64_t)j * ld] = acc[x][y][v];
-               else
-                  c[i + (int64_t)j * ld] -= acc[x][y][v];
-            }
+            const int i = wm * WM + x * 16 + (lane & 15);
+            const int j = wn * WN + y * 16 + Mfma<T>::row(lane, v);
+            if (i < t.M && j < t.N && j >= t.pad0)
+               c[i + (int64_t)j * ld] = (MODE == 1) ? acc[x][y][v] : cv[x][v] - acc[x][y][v];
          }
-      }
+   }
 }
 
 // ---------------------------------------------------------------------------
@@ -628,7 +650,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-__global__ __launch_bounds__(kThreads) void k_update_bf16(const GemmTask* __restrict__ tasks,
+static __global__ __launch_bounds__(kThreads) void k_update_bf16(const GemmTask* __restrict__ tasks,
                                                           float* __restrict__ arena) {
    constexpr int BM = 128, BN = 128, BK = 32, PADK = 8;
    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
@@ -655,10 +677,9 @@ __global__ __launch_bounds__(kThreads) void k_update_bf16(const GemmTask* __rest
       for (int u = 0; u < AV; ++u) {
          const int v = tid + u * kThreads;
          const int kk = v / (BM / 4), row = (v % (BM / 4)) * 4;
-         ra[u] = (k0 + kk < t.K && row < t.M) ? *reinterpret_cast<const f4*>(a + row + (int64_t)(k0 + kk) * ld)
-                                              : f4(0.0f);
-         rb[u] = (k0 + kk < t.K && row < t.N) ? *reinterpret_cast<const f4*>(b + row + (int64_t)(k0 + kk) * ld)
-                                              : f4(0.0f);
+         const int64_t kc = (int64_t)(k0 + kk) * ld;
+         ra[u] = *live_or_zero(k0 + kk < t.K && row < t.M, reinterpret_cast<const f4*>(a + row + kc));
+         rb[u] = *live_or_zero(k0 + kk < t.K && row < t.N, reinterpret_cast<const f4*>(b + row + kc));
       }
    };
    const int nk = (t.K + BK - 1) / BK;
@@ -688,20 +709,29 @@ __global__ __launch_bounds__(kThreads) void k_update_bf16(const GemmTask* __rest
       for (int x = 0; x < FM; ++x)
 #pragma unroll
          for (int y = 0; y < FN; ++y)
-            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[x], bf[y], acc[x][y], 0, 0, 0);
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[y], af[x], acc[x][y], 0, 0, 0);
    }
+   // transposed tiles as in k_update: item v of lane l is C[l%16][4(l/16)+v]
 #pragma unroll
-   for (int x = 0; x < FM; ++x)
+   for (int y = 0; y < FN; ++y) {
+      float cv[FM][4];
 #pragma unroll
-      for (int y = 0; y < FN; ++y) {
-         const int j = wn * WN + y * 16 + (lane & 15);
-         if (j >= t.N || j < t.pad0) continue;
+      for (int x = 0; x < FM; ++x)
 #pragma unroll
          for (int v = 0; v < 4; ++v) {
-            const int i = wm * WM + x * 16 + 4 * (lane >> 4) + v;
-            if (i < t.M) c[i + (int64_t)j * ld] -= acc[x][y][v];
+            const int i = wm * WM + x * 16 + (lane & 15);
+            const int j = wn * WN + y * 16 + 4 * (lane >> 4) + v;
+            cv[x][v] = *live_or_zero(i < t.M && j < t.N && j >= t.pad0, c + i + (int64_t)j * ld);
          }
-      }
+#pragma unroll
+      for (int x = 0; x < FM; ++x)
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int i = wm * WM + x * 16 + (lane & 15);
+            const int j = wn * WN + y * 16 + 4 * (lane >> 4) + v;
+            if (i < t.M && j < t.N && j >= t.pad0) c[i + (int64_t)j * ld] = cv[x][v] - acc[x][y][v];
+         }
+   }
 }
 
 // ---------------------------------------------------------------------------
@@ -753,7 +783,7 @@ __device__ void front_potrf_block(const NodeDesc& nd, T* __restrict__ F, int* in
       for (int r = k + kb + tid; r < m; r += kThreads) {
          T x[B];
 #pragma unroll
-         for (int c = 0; c < B; ++c) x[c] = (c < kb) ? F[r + (int64_t)(k + c) * ld] : T(0);
+         for (int c = 0; c < B; ++c) x[c] = *live_or_zero(c < kb, F + r + (int64_t)(k + c) * ld);
 #pragma unroll
          for (int c = 0; c < B; ++c) {
             if (c < kb) {
@@ -787,21 +817,23 @@ __device__ void front_potrf_block(const NodeDesc& nd, T* __restrict__ F, int* in
 #pragma unroll
          for (int u = 0; u < B / 4; ++u) {
             const int kc = 4 * u + (lane >> 4);
-            av[u] = (ia < R && kc < kb) ? P[ia + (int64_t)kc * ld] : T(0);
-            bv[u] = (ib < R && kc < kb) ? P[ib + (int64_t)kc * ld] : T(0);
+            av[u] = *live_or_zero(ia < R && kc < kb, P + ia + (int64_t)kc * ld);
+            bv[u] = *live_or_zero(ib < R && kc < kb, P + ib + (int64_t)kc * ld);
          }
 #pragma unroll
-         for (int u = 0; u < B / 4; ++u) acc = Mfma<T>::op(av[u], bv[u], acc);
-         const int j = tj * 16 + (lane & 15);
-         if (j < R) {
+         for (int u = 0; u < B / 4; ++u) acc = Mfma<T>::op(bv[u], av[u], acc);
+         // transposed tile: item v of lane l is the entry (i = l%16, j = row(l, v))
+         const int i = ti * 16 + (lane & 15);
+         T cv[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-               const int i = ti * 16 + Mfma<T>::row(lane, v);
-               if (i < R && i >= j) {
-                  T* p = F + (r0 + i) + (int64_t)(r0 + j) * ld;
-                  *p -= acc[v];
-               }
-            }
+         for (int v = 0; v < 4; ++v) {
+            const int j = tj * 16 + Mfma<T>::row(lane, v);
+            cv[v] = *live_or_zero(i < R && j <= i, F + (r0 + i) + (int64_t)(r0 + j) * ld);
+         }
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int j = tj * 16 + Mfma<T>::row(lane, v);
+            if (i < R && j <= i) F[(r0 + i) + (int64_t)(r0 + j) * ld] = cv[v] - acc[v];
          }
       }
    }
@@ -843,7 +875,7 @@ __device__ inline int lower_bound_i(const int* a, int n, int v) {
 // children's contribution rows that map into the range (the child map is
 // increasing, so they are a contiguous run found by binary search), children
 // in list order.
-__global__ __launch_bounds__(kThreads) void k_fwd_gather(const SolveTask* __restrict__ tasks,
+static __global__ __launch_bounds__(kThreads) void k_fwd_gather(const SolveTask* __restrict__ tasks,
                                                          const NodeDesc* __restrict__ nodes,
                                                          const int* __restrict__ rlist,
                                                          const int* __restrict__ cmap,
@@ -875,7 +907,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_gather(const SolveTask* __rest
 }
 
 // Backward gather of rows [r0, r1) from the current solution.
-__global__ __launch_bounds__(kThreads) void k_bwd_gather(const SolveTask* __restrict__ tasks,
+static __global__ __launch_bounds__(kThreads) void k_bwd_gather(const SolveTask* __restrict__ tasks,
                                                          const NodeDesc* __restrict__ nodes,
                                                          const int* __restrict__ rlist,
                                                          const double* __restrict__ xsol,
@@ -902,16 +934,19 @@ __device__ inline v4d mma_rows_gl(const T* __restrict__ A, int64_t lda, int rval
    const int r = lane & 15, kq = lane >> 4;
    for (int k0 = 0; k0 < K; k0 += 32) {
       double a[8], b[8];
+      T raw[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
          const int kk = k0 + 4 * u + kq;
-         a[u] = (r < rvalid && kk < K) ? (double)A[r + (int64_t)kk * lda] : 0.0;
+         raw[u] = *live_or_zero(r < rvalid && kk < K, A + r + (int64_t)kk * lda);
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
          const int kk = k0 + 4 * u + kq;
          b[u] = (kk < K) ? Bs[kk * 16 + r] : 0.0;
       }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = (double)raw[u];
 #pragma unroll
       for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
    }
@@ -939,9 +974,10 @@ __global__ __launch_bounds__(kThreads) void k_solve_diag(const SolveTask* __rest
    for (int c0 = 0; c0 < nrhs; c0 += 16) {
       const int nc = min(16, nrhs - c0);
       __syncthreads();
+#pragma unroll
       for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
          const int l = e >> 4, c = e & 15;
-         xs[e] = (l < kb && c < nc) ? xl[t.k0 + l + (int64_t)(c0 + c) * m] : 0.0;
+         xs[e] = *live_or_zero(l < kb && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
       }
       __syncthreads();
       v4d acc[2];
@@ -994,9 +1030,10 @@ __global__ __launch_bounds__(kThreads) void k_fwd_upd(const SolveTask* __restric
    for (int c0 = 0; c0 < nrhs; c0 += 16) {
       const int nc = min(16, nrhs - c0);
       __syncthreads();
+#pragma unroll
       for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
          const int l = e >> 4, c = e & 15;
-         ys[e] = (l < kb && c < nc) ? xl[t.k0 + l + (int64_t)(c0 + c) * m] : 0.0;
+         ys[e] = *live_or_zero(l < kb && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
       }
       __syncthreads();
       for (int tile = wave; tile < ntile; tile += 4) {
@@ -1004,12 +1041,17 @@ __global__ __launch_bounds__(kThreads) void k_fwd_upd(const SolveTask* __restric
          v4d acc = v4d{0, 0, 0, 0};
          acc = mma_rows_gl<T>(F + rb + (int64_t)t.k0 * nd.ld, nd.ld, t.r1 - rb, kb, ys, acc);
          const int c = lane & 15;
-         if (c < nc)
+         double cv[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-               const int r = rb + Mfma<double>::row(lane, v);
-               if (r < t.r1) xl[r + (int64_t)(c0 + c) * m] -= acc[v];
-            }
+         for (int v = 0; v < 4; ++v) {
+            const int r = rb + Mfma<double>::row(lane, v);
+            cv[v] = *live_or_zero(c < nc && r < t.r1, xl + r + (int64_t)(c0 + c) * m);
+         }
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int r = rb + Mfma<double>::row(lane, v);
+            if (c < nc && r < t.r1) xl[r + (int64_t)(c0 + c) * m] = cv[v] - acc[v];
+         }
       }
    }
 }
@@ -1048,14 +1090,14 @@ __global__ __launch_bounds__(kThreads) void k_bwd_upd(const SolveTask* __restric
             const int e = threadIdx.x + u * kThreads;
             const int rr = e % RC, jc = e / RC;
             const int r = rb0 + rr, j = t.r0 + jc;
-            ra[u] = (r < rend && j < t.r1) ? F[r + (int64_t)j * nd.ld] : T(0);
+            ra[u] = *live_or_zero(r < rend && j < t.r1, F + r + (int64_t)j * nd.ld);
          }
 #pragma unroll
          for (int u = 0; u < BE; ++u) {
             const int e = threadIdx.x + u * kThreads;
             const int rr = e % RC, c = e / RC;
             const int r = rb0 + rr;
-            rb[u] = (r < rend && c < nc) ? xl[r + (int64_t)(c0 + c) * m] : 0.0;
+            rb[u] = *live_or_zero(r < rend && c < nc, xl + r + (int64_t)(c0 + c) * m);
          }
       };
       load(rbeg);
@@ -1084,10 +1126,13 @@ __global__ __launch_bounds__(kThreads) void k_bwd_upd(const SolveTask* __restric
          for (int q = 0; q < 4; ++q) acc[q] += part[q];
       }
       const int j = t.r0 + jj;
-      if (j < t.r1)
+      double cv[4];
 #pragma unroll
-         for (int q = 0; q < 4; ++q)
-            if (4 * cq + q < nc) xl[j + (int64_t)(c0 + 4 * cq + q) * m] -= acc[q];
+      for (int q = 0; q < 4; ++q)
+         cv[q] = *live_or_zero(j < t.r1 && 4 * cq + q < nc, xl + j + (int64_t)(c0 + 4 * cq + q) * m);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+         if (j < t.r1 && 4 * cq + q < nc) xl[j + (int64_t)(c0 + 4 * cq + q) * m] = cv[q] - acc[q];
    }
 }
 
@@ -1095,7 +1140,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_upd(const SolveTask* __restric
 // FP64 refinement helpers.
 // ---------------------------------------------------------------------------
 // r = b - A x with A in CSR (both triangles, pivot order); one row per thread.
-__global__ __launch_bounds__(kThreads) void k_residual(int n, const int64_t* __restrict__ rowptr,
+static __global__ __launch_bounds__(kThreads) void k_residual(int n, const int64_t* __restrict__ rowptr,
                                                        const int* __restrict__ col,
                                                        const double* __restrict__ val,
                                                        const double* __restrict__ x,
@@ -1112,7 +1157,7 @@ __global__ __launch_bounds__(kThreads) void k_residual(int n, const int64_t* __r
 }
 
 // Per-column max |v| into out[c] (as ordered bits of a non-negative double).
-__global__ __launch_bounds__(kThreads) void k_colmax(int n, const double* __restrict__ v, int nrhs,
+static __global__ __launch_bounds__(kThreads) void k_colmax(int n, const double* __restrict__ v, int nrhs,
                                                      unsigned long long* __restrict__ out) {
    const int c = blockIdx.y;
    double m = 0.0;
@@ -1130,7 +1175,7 @@ __global__ __launch_bounds__(kThreads) void k_colmax(int n, const double* __rest
 }
 
 // y = A x (both triangles, pivot order), one row per thread.
-__global__ __launch_bounds__(kThreads) void k_spmv(int n, const int64_t* __restrict__ rowptr,
+static __global__ __launch_bounds__(kThreads) void k_spmv(int n, const int64_t* __restrict__ rowptr,
                                                    const int* __restrict__ col,
                                                    const double* __restrict__ val,
                                                    const double* __restrict__ x,
@@ -1149,7 +1194,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(int n, const int64_t* __restr
 // fixed tree), then k_dot_final adds the partials in index order.
 constexpr int kDotBlocks = 256;
 // grid (kDotBlocks, nrhs, nvec): <x_c, (y + z*ystride)_c> for vector z.
-__global__ __launch_bounds__(kThreads) void k_dot_partial(int n, const double* __restrict__ x,
+static __global__ __launch_bounds__(kThreads) void k_dot_partial(int n, const double* __restrict__ x,
                                                           const double* __restrict__ ybase,
                                                           int64_t ystride,
                                                           double* __restrict__ part) {
@@ -1168,7 +1213,7 @@ __global__ __launch_bounds__(kThreads) void k_dot_partial(int n, const double* _
 }
 
 // count = nvec * nrhs sums of kDotBlocks partials, in index order.
-__global__ void k_dot_final(const double* __restrict__ part, int count, double* __restrict__ out) {
+static __global__ void k_dot_final(const double* __restrict__ part, int count, double* __restrict__ out) {
    const int c = blockIdx.x * blockDim.x + threadIdx.x;
    if (c >= count) return;
    double s = 0.0;
@@ -1177,7 +1222,7 @@ __global__ void k_dot_final(const double* __restrict__ part, int count, double* 
 }
 
 // y_c += a_c x_c with per-column coefficients in device memory.
-__global__ __launch_bounds__(kThreads) void k_axpy_cols(int n, int nrhs, const double* __restrict__ a,
+static __global__ __launch_bounds__(kThreads) void k_axpy_cols(int n, int nrhs, const double* __restrict__ a,
                                                         const double* __restrict__ x,
                                                         double* __restrict__ y) {
    const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -1186,7 +1231,7 @@ __global__ __launch_bounds__(kThreads) void k_axpy_cols(int n, int nrhs, const d
 }
 
 // W_c -= sum_{z < nvec} h[z*nrhs + c] V_z,c (the Gram-Schmidt update).
-__global__ __launch_bounds__(kThreads) void k_multi_axpy(int n, int nrhs, int nvec,
+static __global__ __launch_bounds__(kThreads) void k_multi_axpy(int n, int nrhs, int nvec,
                                                          const double* __restrict__ h,
                                                          const double* __restrict__ V,
                                                          int64_t vstride, double* __restrict__ W) {
@@ -1199,7 +1244,7 @@ __global__ __launch_bounds__(kThreads) void k_multi_axpy(int n, int nrhs, int nv
 }
 
 // y_c = a_c x_c.
-__global__ __launch_bounds__(kThreads) void k_scale_cols(int n, int nrhs, const double* __restrict__ a,
+static __global__ __launch_bounds__(kThreads) void k_scale_cols(int n, int nrhs, const double* __restrict__ a,
                                                          const double* __restrict__ x,
                                                          double* __restrict__ y) {
    const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -1207,30 +1252,30 @@ __global__ __launch_bounds__(kThreads) void k_scale_cols(int n, int nrhs, const 
    y[e] = a[e / n] * x[e];
 }
 
-__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+static __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
    int i = blockIdx.x * blockDim.x + threadIdx.x;
    if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(kThreads) void k_axpy(int64_t n, double a, const double* __restrict__ x,
+static __global__ __launch_bounds__(kThreads) void k_axpy(int64_t n, double a, const double* __restrict__ x,
                                                    double* __restrict__ y) {
    int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
    if (i < n) y[i] += a * x[i];
 }
 
-__global__ __launch_bounds__(kThreads) void k_copy(int64_t n, const double* __restrict__ x,
+static __global__ __launch_bounds__(kThreads) void k_copy(int64_t n, const double* __restrict__ x,
                                                    double* __restrict__ y) {
    int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
    if (i < n) y[i] = x[i];
 }
 
-__global__ __launch_bounds__(kThreads) void k_fill(int64_t n, double v, double* __restrict__ y) {
+static __global__ __launch_bounds__(kThreads) void k_fill(int64_t n, double v, double* __restrict__ y) {
    int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
    if (i < n) y[i] = v;
 }
 
 // CSR values gathered from the cleaned input values.
-__global__ __launch_bounds__(kThreads) void k_gather_vals(int64_t nnz, const int64_t* __restrict__ src,
+static __global__ __launch_bounds__(kThreads) void k_gather_vals(int64_t nnz, const int64_t* __restrict__ src,
                                                           const double* __restrict__ aval,
                                                           const double* __restrict__ rowscale,
                                                           const double* __restrict__ colscale_dummy,

@@ -1,0 +1,889 @@
+// Device kernels of the indefinite (APP LDL^T) numeric phase on gfx950.
+//
+// A front of the indefinite path is m x m column-major (ld) in the arena with
+// n fully-summed columns (own pivots, then the delayed columns of the
+// children, then the remaining rows of the row list). During the panel
+// factorization the fully-summed n x n block is kept symmetric in both
+// triangles, so that a symmetric exchange of two pivot candidates is a swap
+// of two columns (all m rows) and of two rows (n columns); rows >= n are kept
+// in column storage only. The contribution block (rows and columns >= n) is
+// updated afterwards by the MFMA GEMM with every eliminated pivot.
+//
+// Pivoting decisions follow the reference's APP semantics
+// (src/factor_indef.hxx:600-860, src/kernels/ldlt_app.hxx:144-255,
+// include/sylver/kernels/Column.hxx:87-101, src/factor_failed.hxx:29-157):
+// per block of kAppNB columns a complete-pivoting (Bunch-Parlett) LDL^T of
+// the diagonal block, the a-posteriori threshold test |l| <= 1/u on every
+// uneliminated row, restore of the failed columns, update of the
+// uneliminated fully-summed entries; then failed columns are moved behind the
+// eliminated ones and retried by threshold partial pivoting over all rows.
+// Every product-sum is written with explicit roundings (fnma / dot2 with
+// contraction off) in the same order as the CPU oracle, so that a front is
+// factored bit for bit like the oracle's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace sylver_amd {
+
+constexpr int kAppNB = 64;  // APP block size of the indefinite panel
+
+struct IndefDesc {
+   int m, n, ld, nelim;      // front rows, fully-summed columns, leading dim, eliminated
+   int ncol, nrow, ndelay_in, parent;
+   int first_child, next_child, delay_col0, nelim_first;
+   int64_t off;              // front offset in the arena (elements)
+   int64_t rlist_off, amap_off, num_a, cmap_off;
+   int64_t var_off;          // m ints: 1-based pivot-order index of each front row
+   int64_t pre_off;          // n ints: pre-pivot position of each fully-summed row
+   int64_t d_off;            // 2 n doubles: D^{-1} (1x1 [d, 0], 2x2 [a, b, inf, c])
+   int64_t ws_off;           // workspace (doubles) of the panel kernel / CB update
+   int64_t x_off;            // solve work vectors: at x_off * nrhs, 2 m doubles per right-hand side
+};
+
+__device__ __forceinline__ double fnma(double a, double b, double c) { return __builtin_fma(-a, b, c); }
+__device__ __forceinline__ double dot2(double a, double b, double c, double d) {
+   return __builtin_fma(a, b, c * d);
+}
+
+// Block-wide arg-max: largest value, smallest index among equal values.
+// `sv`/`si` are LDS scratch of 4 entries (one per wave).
+__device__ inline void block_argmax(double& v, int& idx, double* sv, int* si) {
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(v, o);
+      const int oi = __shfl_xor(idx, o);
+      if (ov > v || (ov == v && oi < idx)) {
+         v = ov;
+         idx = oi;
+      }
+   }
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   __syncthreads();
+   if (lane == 0) {
+      sv[wave] = v;
+      si[wave] = idx;
+   }
+   __syncthreads();
+   v = sv[0];
+   idx = si[0];
+#pragma unroll
+   for (int w = 1; w < kThreads / 64; ++w)
+      if (sv[w] > v || (sv[w] == v && si[w] < idx)) {
+         v = sv[w];
+         idx = si[w];
+      }
+   __syncthreads();
+}
+
+__device__ inline double block_max(double v, double* sv) {
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   __syncthreads();
+   if (lane == 0) sv[wave] = v;
+   __syncthreads();
+   v = sv[0];
+#pragma unroll
+   for (int w = 1; w < kThreads / 64; ++w) v = fmax(v, sv[w]);
+   __syncthreads();
+   return v;
+}
+
+// Symmetric exchange of fully-summed positions p and q of the front.
+__device__ inline void front_swap(double* F, int ld, int m, int n, int p, int q, int* var, int* pre) {
+   if (p == q) return;
+   for (int k = threadIdx.x; k < m; k += kThreads) {
+      const double a = F[k + (int64_t)p * ld], b = F[k + (int64_t)q * ld];
+      F[k + (int64_t)p * ld] = b;
+      F[k + (int64_t)q * ld] = a;
+   }
+   __syncthreads();
+   for (int k = threadIdx.x; k < n; k += kThreads) {
+      const double a = F[p + (int64_t)k * ld], b = F[q + (int64_t)k * ld];
+      F[p + (int64_t)k * ld] = b;
+      F[q + (int64_t)k * ld] = a;
+   }
+   if (threadIdx.x == 0) {
+      int t = var[p];
+      var[p] = var[q];
+      var[q] = t;
+      t = pre[p];
+      pre[p] = pre[q];
+      pre[q] = t;
+   }
+   __syncthreads();
+}
+
+// x := x * D^{-1} for one pivot with the reference's zero-pivot handling
+// (apply_pivot, src/kernels/ldlt_app.hxx:172-205).
+__device__ __forceinline__ void apply_dinv1(double d0, double small, double& x) {
+   if (d0 == 0.0)
+      x = (fabs(x) < small) ? 0.0 : __builtin_inf() * x;
+   else
+      x = x * d0;
+}
+
+// ---------------------------------------------------------------------------
+// Panel factorization of one front per workgroup: APP over blocks of kAppNB
+// columns, permute_failed, TPP on the failed columns. Writes nelim,
+// nelim_first, the D^{-1} entries, and permutes var/pre. status[node] gets
+// -5 on a zero pivot with action == 0.
+// Workspace per node (doubles): backup m*kAppNB | permute scratch m*n |
+// TPP scratch 4m | flags and lists 2m ints (stored in m doubles).
+// ---------------------------------------------------------------------------
+static __global__ __launch_bounds__(kThreads) void k_app_front(const int* __restrict__ list,
+                                                        IndefDesc* __restrict__ nodes,
+                                                        double* __restrict__ arena,
+                                                        int* __restrict__ vars, int* __restrict__ pres,
+                                                        double* __restrict__ dinv,
+                                                        double* __restrict__ wsbase, double u,
+                                                        double small, int action, int failed_tpp,
+                                                        int* __restrict__ status, int* __restrict__ out) {
+#pragma clang fp contract(off)
+   constexpr int NB = kAppNB;
+   __shared__ double T[NB][NB + 1];
+   __shared__ double VS[NB][NB + 1];
+   __shared__ double dl[2 * NB];
+   __shared__ double red_v[4];
+   __shared__ int red_i[4];
+   __shared__ int s_fail, s_npass, s_bad;
+   const int node = list[blockIdx.x];
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, n = nd.n, ld = nd.ld, tid = threadIdx.x;
+   if (n <= 0) {
+      if (tid == 0) {
+         nodes[node].nelim = 0;
+         nodes[node].nelim_first = 0;
+         out[2 * blockIdx.x] = 0;
+         out[2 * blockIdx.x + 1] = 0;
+      }
+      return;
+   }
+   double* F = arena + nd.off;
+   int* var = vars + nd.var_off;
+   int* pre = pres + nd.pre_off;
+   double* D = dinv + nd.d_off;
+   double* Bk = wsbase + nd.ws_off;
+   double* PS = Bk + (int64_t)m * NB;
+   double* TL = PS + (int64_t)m * n;           // TPP scratch: l (2m) and v (2m)
+   int* EL = reinterpret_cast<int*>(TL + 4 * (int64_t)m);  // m flags
+   int* NP = EL + m;                                       // m ints: new positions
+   if (tid == 0) s_bad = 0;
+   // full symmetric fully-summed block; nothing eliminated yet
+   for (int64_t e = tid; e < (int64_t)n * n; e += kThreads) {
+      const int i = (int)(e % n), j = (int)(e / n);
+      if (i < j) F[i + (int64_t)j * ld] = F[j + (int64_t)i * ld];
+   }
+   for (int i = tid; i < m; i += kThreads) EL[i] = 0;
+   for (int i = tid; i < 2 * n; i += kThreads) D[i] = 0.0;
+   __syncthreads();
+   int nelim = 0;
+   for (int c0 = 0; c0 < n; c0 += NB) {
+      const int kb = min(NB, n - c0);
+      for (int64_t e = tid; e < (int64_t)kb * m; e += kThreads) {
+         const int i = (int)(e % m), j = (int)(e / m);
+         Bk[i + (int64_t)j * m] = F[i + (int64_t)(c0 + j) * ld];
+      }
+      for (int e = tid; e < kb * kb; e += kThreads) {
+         const int i = e % kb, j = e / kb;
+         T[i][j] = F[c0 + i + (int64_t)(c0 + j) * ld];
+      }
+      __syncthreads();
+      // ---- Bunch-Parlett LDL^T of the diagonal block (in LDS) ----
+      auto swap_local = [&](int p, int q) {
+         if (p == q) return;
+         for (int k = tid; k < kb; k += kThreads) {
+            double a = T[k][p];
+            T[k][p] = T[k][q];
+            T[k][q] = a;
+         }
+         __syncthreads();
+         for (int k = tid; k < kb; k += kThreads) {
+            double a = T[p][k];
+            T[p][k] = T[q][k];
+            T[q][k] = a;
+         }
+         for (int k = tid; k < m; k += kThreads) {
+            double a = Bk[k + (int64_t)p * m];
+            Bk[k + (int64_t)p * m] = Bk[k + (int64_t)q * m];
+            Bk[k + (int64_t)q * m] = a;
+         }
+         __syncthreads();
+         for (int k = tid; k < kb; k += kThreads) {
+            double a = Bk[c0 + p + (int64_t)k * m];
+            Bk[c0 + p + (int64_t)k * m] = Bk[c0 + q + (int64_t)k * m];
+            Bk[c0 + q + (int64_t)k * m] = a;
+         }
+         __syncthreads();
+         front_swap(F, ld, m, n, c0 + p, c0 + q, var, pre);
+      };
+      for (int s = 0; s < kb;) {
+         double v0 = -1.0, v1 = -1.0;
+         int i0 = 0x7fffffff, i1 = 0x7fffffff;
+         const int rem = kb - s;
+         for (int e = tid; e < rem * rem; e += kThreads) {
+            const int i = s + e % rem, j = s + e / rem;
+            if (i < j) continue;
+            const double v = fabs(T[i][j]);
+            const int idx = j * kb + i;
+            if (v > v0 || (v == v0 && idx < i0)) {
+               v0 = v;
+               i0 = idx;
+            }
+            if (i == j && (v > v1 || (v == v1 && i < i1))) {
+               v1 = v;
+               i1 = i;
+            }
+         }
+         block_argmax(v0, i0, red_v, red_i);
+         block_argmax(v1, i1, red_v, red_i);
+         if (v0 < small) {
+            if (!action) {
+               if (tid == 0) s_bad = 1;
+               __syncthreads();
+               break;
+            }
+            for (int k = s + tid; k < kb; k += kThreads) {
+               dl[2 * k] = 0.0;
+               dl[2 * k + 1] = 0.0;
+            }
+            for (int e = tid; e < kb * kb; e += kThreads) {
+               const int i = e % kb, j = e / kb;
+               if (j >= s && i > j) T[i][j] = 0.0;
+            }
+            __syncthreads();
+            break;
+         }
+         if (v1 >= 0.6403882032022076 * v0) {
+            swap_local(s, i1);
+            const double dv = T[s][s];
+            const double d0 = 1.0 / dv;
+            if (tid == 0) {
+               dl[2 * s] = d0;
+               dl[2 * s + 1] = 0.0;
+            }
+            for (int e = tid; e < rem * rem; e += kThreads) {
+               const int i = s + 1 + e % rem, j = s + 1 + e / rem;
+               if (i >= kb || j > i) continue;
+               const double li = T[i][s] * d0;
+               T[i][j] = fnma(li, T[j][s], T[i][j]);
+            }
+            __syncthreads();
+            for (int i = s + 1 + tid; i < kb; i += kThreads) T[i][s] = T[i][s] * d0;
+            s += 1;
+         } else {
+            int j0 = i0 / kb, ii0 = i0 % kb;
+            swap_local(s, j0);
+            if (ii0 == s) ii0 = j0;
+            swap_local(s + 1, ii0);
+            const double a = T[s][s], b = T[s + 1][s], c = T[s + 1][s + 1];
+            const double det = __builtin_fma(a, c, -(b * b));
+            const double d0 = c / det, d1 = -b / det, d3 = a / det;
+            if (tid == 0) {
+               dl[2 * s] = d0;
+               dl[2 * s + 1] = d1;
+               dl[2 * s + 2] = __builtin_inf();
+               dl[2 * s + 3] = d3;
+            }
+            for (int e = tid; e < rem * rem; e += kThreads) {
+               const int i = s + 2 + e % rem, j = s + 2 + e / rem;
+               if (i >= kb || j > i) continue;
+               const double a1 = T[i][s], a2 = T[i][s + 1];
+               const double l0 = dot2(d0, a1, d1, a2), l1 = dot2(d1, a1, d3, a2);
+               T[i][j] = fnma(l1, T[j][s + 1], fnma(l0, T[j][s], T[i][j]));
+            }
+            __syncthreads();
+            for (int i = s + 2 + tid; i < kb; i += kThreads) {
+               const double a1 = T[i][s], a2 = T[i][s + 1];
+               T[i][s] = dot2(d0, a1, d1, a2);
+               T[i][s + 1] = dot2(d1, a1, d3, a2);
+            }
+            if (tid == 0) T[s + 1][s] = 0.0;
+            s += 2;
+         }
+         __syncthreads();
+         for (int e = tid; e < kb * kb; e += kThreads) {  // keep the trailing block symmetric
+            const int i = e % kb, j = e / kb;
+            if (j >= s && i > j) T[j][i] = T[i][j];
+         }
+         __syncthreads();
+      }
+      if (s_bad) break;
+      // ---- ApplyN / ApplyT on every other uneliminated row + threshold ----
+      if (tid == 0) s_fail = kb;
+      __syncthreads();
+      const double inv_u = 1.0 / u;
+      for (int r = tid; r < m; r += kThreads) {
+         if (EL[r] || (r >= c0 && r < c0 + kb)) continue;
+         double x[NB];
+#pragma unroll
+         for (int j = 0; j < NB; ++j) x[j] = (j < kb) ? F[r + (int64_t)(c0 + j) * ld] : 0.0;
+#pragma unroll
+         for (int j = 1; j < NB; ++j)
+#pragma unroll
+            for (int i = 0; i < j; ++i)
+               if (j < kb) x[j] = fnma(x[i], T[j][i], x[j]);
+         int fail = kb;
+         for (int j = 0; j < kb;) {
+            if (j + 1 < kb && __builtin_isinf(dl[2 * j + 2])) {
+               const double a1 = x[j], a2 = x[j + 1];
+               x[j] = dot2(dl[2 * j], a1, dl[2 * j + 1], a2);
+               x[j + 1] = dot2(dl[2 * j + 1], a1, dl[2 * j + 3], a2);
+               j += 2;
+            } else {
+               apply_dinv1(dl[2 * j], small, x[j]);
+               j += 1;
+            }
+         }
+#pragma unroll
+         for (int j = NB - 1; j >= 0; --j)
+            if (j < kb && fabs(x[j]) > inv_u) fail = j;
+         if (fail < kb) atomicMin(&s_fail, fail);
+#pragma unroll
+         for (int j = 0; j < NB; ++j)
+            if (j < kb) {
+               F[r + (int64_t)(c0 + j) * ld] = x[j];
+               if (r < n) F[c0 + j + (int64_t)r * ld] = x[j];
+            }
+      }
+      __syncthreads();
+      if (tid == 0) {
+         int np = s_fail;
+         if (np > 0 && !__builtin_isinf(dl[2 * (np - 1)]) && dl[2 * (np - 1) + 1] != 0.0) --np;
+         s_npass = np;
+      }
+      __syncthreads();
+      const int npass = s_npass;
+      // L11 for the passed columns (rows of the block below each pivot)
+      for (int e = tid; e < kb * kb; e += kThreads) {
+         const int i = e % kb, j = e / kb;
+         if (j < npass && i > j) {
+            F[c0 + i + (int64_t)(c0 + j) * ld] = T[i][j];
+            F[c0 + j + (int64_t)(c0 + i) * ld] = T[i][j];
+         }
+      }
+      // failed columns restored from the backup on every uneliminated row
+      for (int64_t e = tid; e < (int64_t)(kb - npass) * m; e += kThreads) {
+         const int i = (int)(e % m), j = npass + (int)(e / m);
+         if (EL[i] || (i >= c0 && i < c0 + npass)) continue;
+         const double b = Bk[i + (int64_t)j * m];
+         F[i + (int64_t)(c0 + j) * ld] = b;
+         if (i < n) F[c0 + j + (int64_t)i * ld] = b;
+      }
+      __syncthreads();
+      // ---- update of the uneliminated fully-summed columns ----
+      if (npass > 0) {
+         for (int s0 = 0; s0 < n; s0 += NB) {
+            const int sc = min(NB, n - s0);
+            // VS[t][p] = (D L[s, :])_p for column s = s0 + t
+            for (int e = tid; e < sc * npass; e += kThreads) {
+               const int t = e % sc, p = e / sc;
+               const int s = s0 + t;
+               double v = 0.0;
+               if (p + 1 < npass && __builtin_isinf(dl[2 * p + 2])) {
+                  const double det = __builtin_fma(dl[2 * p], dl[2 * p + 3], -(dl[2 * p + 1] * dl[2 * p + 1]));
+                  const double D0 = dl[2 * p + 3] / det, D1 = -dl[2 * p + 1] / det;
+                  v = dot2(D0, F[s + (int64_t)(c0 + p) * ld], D1, F[s + (int64_t)(c0 + p + 1) * ld]);
+               } else if (p > 0 && __builtin_isinf(dl[2 * p])) {
+                  const int q = p - 1;
+                  const double det = __builtin_fma(dl[2 * q], dl[2 * q + 3], -(dl[2 * q + 1] * dl[2 * q + 1]));
+                  const double D1 = -dl[2 * q + 1] / det, D3 = dl[2 * q] / det;
+                  v = dot2(D1, F[s + (int64_t)(c0 + q) * ld], D3, F[s + (int64_t)(c0 + p) * ld]);
+               } else {
+                  const double D0 = (dl[2 * p] == 0.0) ? 0.0 : 1.0 / dl[2 * p];
+                  v = D0 * F[s + (int64_t)(c0 + p) * ld];
+               }
+               VS[t][p] = v;
+            }
+            __syncthreads();
+            for (int r = tid; r < m; r += kThreads) {
+               if (EL[r] || (r >= c0 && r < c0 + npass)) continue;
+               double lr[NB];
+#pragma unroll
+               for (int p = 0; p < NB; ++p) lr[p] = (p < npass) ? F[r + (int64_t)(c0 + p) * ld] : 0.0;
+               for (int t = 0; t < sc; ++t) {
+                  const int s = s0 + t;
+                  if (EL[s] || (s >= c0 && s < c0 + npass)) continue;
+                  double x = F[r + (int64_t)s * ld];
+#pragma unroll
+                  for (int p = 0; p < NB; ++p)
+                     if (p < npass) x = fnma(lr[p], VS[t][p], x);
+                  F[r + (int64_t)s * ld] = x;
+               }
+            }
+            __syncthreads();
+         }
+         // symmetric copy of the updated fully-summed block
+         for (int64_t e = tid; e < (int64_t)n * n; e += kThreads) {
+            const int i = (int)(e % n), j = (int)(e / n);
+            if (i > j && !EL[i] && !EL[j] && !(i >= c0 && i < c0 + npass) && !(j >= c0 && j < c0 + npass))
+               F[j + (int64_t)i * ld] = F[i + (int64_t)j * ld];
+         }
+      }
+      __syncthreads();
+      for (int j = tid; j < npass; j += kThreads) {
+         EL[c0 + j] = 1;
+         D[2 * (c0 + j)] = dl[2 * j];
+         D[2 * (c0 + j) + 1] = dl[2 * j + 1];
+      }
+      nelim += npass;
+      __syncthreads();
+   }
+   if (s_bad) {
+      if (tid == 0) {
+         status[node] = -5;
+         out[2 * blockIdx.x] = 0;
+         out[2 * blockIdx.x + 1] = 0;
+      }
+      return;
+   }
+   const int nelim_first = nelim;
+   // ---- permute_failed: eliminated positions first, then failed ones ----
+   if (nelim < n) {
+      if (tid == 0) {
+         int k = 0;
+         for (int p = 0; p < n; ++p)
+            if (EL[p]) NP[k++] = p;
+         for (int p = 0; p < n; ++p)
+            if (!EL[p]) NP[k++] = p;
+      }
+      __syncthreads();
+      for (int64_t e = tid; e < (int64_t)m * n; e += kThreads) {
+         const int i = (int)(e % m), k = (int)(e / m);
+         const int oi = (i < n) ? NP[i] : i;
+         PS[i + (int64_t)k * m] = F[oi + (int64_t)NP[k] * ld];
+      }
+      // var / pre / D through the backup area (free now)
+      int* ivar = reinterpret_cast<int*>(Bk);
+      int* ipre = ivar + n;
+      double* dd = Bk + n;
+      for (int k = tid; k < n; k += kThreads) {
+         ivar[k] = var[NP[k]];
+         ipre[k] = pre[NP[k]];
+         dd[2 * k] = D[2 * NP[k]];
+         dd[2 * k + 1] = D[2 * NP[k] + 1];
+      }
+      __syncthreads();
+      for (int64_t e = tid; e < (int64_t)m * n; e += kThreads) {
+         const int i = (int)(e % m), k = (int)(e / m);
+         F[i + (int64_t)k * ld] = PS[i + (int64_t)k * m];
+      }
+      for (int k = tid; k < n; k += kThreads) {
+         var[k] = ivar[k];
+         pre[k] = ipre[k];
+         D[2 * k] = dd[2 * k];
+         D[2 * k + 1] = dd[2 * k + 1];
+      }
+      __syncthreads();
+   }
+   // ---- second pass: threshold partial pivoting on the failed columns ----
+   if (nelim < n && (m == n || failed_tpp)) {
+      while (nelim < n) {
+         int piv = -1, piv_t = -1, w = 0;
+         for (int p = nelim; p < n; ++p) {
+            double amax = 0.0;
+            int t = 0x7fffffff;
+            for (int i = nelim + tid; i < m; i += kThreads) {
+               if (i == p) continue;
+               const double v = fabs(F[i + (int64_t)p * ld]);
+               if (v > amax || (v == amax && v > 0.0 && i < t)) {
+                  amax = v;
+                  t = i;
+               }
+            }
+            block_argmax(amax, t, red_v, red_i);
+            if (!(amax > 0.0)) t = -1;
+            const double app = fabs(F[p + (int64_t)p * ld]);
+            if (fmax(app, amax) < small) {
+               if (!action) {
+                  if (tid == 0) {
+                     status[node] = -5;
+                     out[2 * blockIdx.x] = nelim;
+                     out[2 * blockIdx.x + 1] = nelim_first;
+                  }
+                  return;
+               }
+               piv = p;
+               w = 0;  // zero pivot
+               break;
+            }
+            if (app >= u * amax) {
+               piv = p;
+               w = 1;
+               break;
+            }
+            if (t >= 0 && t < n) {
+               const double a11 = F[p + (int64_t)p * ld], a21 = F[t + (int64_t)p * ld],
+                            a22 = F[t + (int64_t)t * ld];
+               const double det = __builtin_fma(a11, a22, -(a21 * a21));
+               if (det != 0.0) {
+                  const double d11 = a22 / det, d21 = -a21 / det, d22 = a11 / det;
+                  double pm = 0.0, tm = 0.0;
+                  for (int i = nelim + tid; i < m; i += kThreads) {
+                     if (i == p || i == t) continue;
+                     pm = fmax(pm, fabs(F[i + (int64_t)p * ld]));
+                     tm = fmax(tm, fabs(F[i + (int64_t)t * ld]));
+                  }
+                  pm = block_max(pm, red_v);
+                  tm = block_max(tm, red_v);
+                  const double g = fmax(dot2(fabs(d11), pm, fabs(d21), tm), dot2(fabs(d21), pm, fabs(d22), tm));
+                  if (u * g <= 1.0) {
+                     piv = p;
+                     piv_t = t;
+                     w = 2;
+                     break;
+                  }
+               }
+            }
+         }
+         if (piv < 0) break;
+         front_swap(F, ld, m, n, piv, nelim, var, pre);
+         if (w == 0) {
+            for (int i = nelim + 1 + tid; i < m; i += kThreads) {
+               F[i + (int64_t)nelim * ld] = 0.0;
+               if (i < n) F[nelim + (int64_t)i * ld] = 0.0;
+            }
+            if (tid == 0) {
+               D[2 * nelim] = 0.0;
+               D[2 * nelim + 1] = 0.0;
+            }
+            __syncthreads();
+            nelim += 1;
+            continue;
+         }
+         double d[4];
+         if (w == 2) {
+            int t = piv_t;
+            if (t == nelim) t = piv;
+            front_swap(F, ld, m, n, t, nelim + 1, var, pre);
+            const double b11 = F[nelim + (int64_t)nelim * ld], b21 = F[nelim + 1 + (int64_t)nelim * ld],
+                         b22 = F[nelim + 1 + (int64_t)(nelim + 1) * ld];
+            const double dt = __builtin_fma(b11, b22, -(b21 * b21));
+            d[0] = b22 / dt;
+            d[1] = -b21 / dt;
+            d[2] = __builtin_inf();
+            d[3] = b11 / dt;
+         } else {
+            d[0] = 1.0 / F[nelim + (int64_t)nelim * ld];
+            d[1] = 0.0;
+         }
+         // l = v D^{-1} for every row below; v kept for the update
+         double* lv = TL;             // 2m
+         double* vv = TL + 2 * (int64_t)m;
+         const int r0 = nelim + w;
+         for (int r = r0 + tid; r < m; r += kThreads) {
+            const double a1 = F[r + (int64_t)nelim * ld];
+            const double a2 = (w == 2) ? F[r + (int64_t)(nelim + 1) * ld] : 0.0;
+            vv[2 * r] = a1;
+            vv[2 * r + 1] = a2;
+            if (w == 2) {
+               lv[2 * r] = dot2(d[0], a1, d[1], a2);
+               lv[2 * r + 1] = dot2(d[1], a1, d[3], a2);
+            } else {
+               double x = a1;
+               apply_dinv1(d[0], small, x);
+               lv[2 * r] = x;
+               lv[2 * r + 1] = 0.0;
+            }
+         }
+         __syncthreads();
+         for (int64_t e = tid; e < (int64_t)(m - r0) * (n - r0); e += kThreads) {
+            const int r = r0 + (int)(e % (m - r0)), s = r0 + (int)(e / (m - r0));
+            double x = fnma(lv[2 * r], vv[2 * s], F[r + (int64_t)s * ld]);
+            if (w == 2) x = fnma(lv[2 * r + 1], vv[2 * s + 1], x);
+            F[r + (int64_t)s * ld] = x;
+         }
+         __syncthreads();
+         for (int64_t e = tid; e < (int64_t)(n - r0) * (n - r0); e += kThreads) {
+            const int i = r0 + (int)(e % (n - r0)), j = r0 + (int)(e / (n - r0));
+            if (i > j) F[j + (int64_t)i * ld] = F[i + (int64_t)j * ld];
+         }
+         for (int r = r0 + tid; r < m; r += kThreads) {
+            F[r + (int64_t)nelim * ld] = lv[2 * r];
+            if (r < n) F[nelim + (int64_t)r * ld] = lv[2 * r];
+            if (w == 2) {
+               F[r + (int64_t)(nelim + 1) * ld] = lv[2 * r + 1];
+               if (r < n) F[nelim + 1 + (int64_t)r * ld] = lv[2 * r + 1];
+            }
+         }
+         if (tid == 0) {
+            D[2 * nelim] = d[0];
+            D[2 * nelim + 1] = d[1];
+            if (w == 2) {
+               D[2 * nelim + 2] = d[2];
+               D[2 * nelim + 3] = d[3];
+               F[nelim + 1 + (int64_t)nelim * ld] = 0.0;
+               F[nelim + (int64_t)(nelim + 1) * ld] = 0.0;
+            }
+         }
+         __syncthreads();
+         nelim += w;
+      }
+   }
+   if (tid == 0) {
+      nodes[node].nelim = nelim;
+      nodes[node].nelim_first = nelim_first;
+      out[2 * blockIdx.x] = nelim;
+      out[2 * blockIdx.x + 1] = nelim_first;
+   }
+}
+
+// ---------------------------------------------------------------------------
+// Assembly of the indefinite fronts of one level.
+// ---------------------------------------------------------------------------
+// Scatter of A (init_a_block, src/kernels/assemble.hxx:161-214, rows at or
+// beyond ncol shifted by ndelay_in) and the row lists: own pivots and
+// remaining rows from rlist, pre-pivot positions 0..n-1.
+static __global__ __launch_bounds__(kThreads) void k_ind_init(const int* __restrict__ list,
+                                                       const IndefDesc* __restrict__ nodes,
+                                                       const int64_t* __restrict__ nlist,
+                                                       const int* __restrict__ rlist,
+                                                       const double* __restrict__ aval,
+                                                       double* __restrict__ arena,
+                                                       int* __restrict__ vars, int* __restrict__ pres) {
+   const IndefDesc nd = nodes[list[blockIdx.x]];
+   double* F = arena + nd.off;
+   for (int64_t p = threadIdx.x; p < nd.num_a; p += kThreads) {
+      const int64_t src = nlist[2 * (nd.amap_off + p)] - 1;
+      const int64_t dest = nlist[2 * (nd.amap_off + p) + 1] - 1;
+      const int c = (int)(dest / nd.nrow);
+      int r = (int)(dest % nd.nrow);
+      if (r >= nd.ncol) r += nd.ndelay_in;
+      F[r + (int64_t)c * nd.ld] = aval[src];
+   }
+   int* var = vars + nd.var_off;
+   const int* rl = rlist + nd.rlist_off;
+   for (int r = threadIdx.x; r < nd.m; r += kThreads) {
+      if (r < nd.ncol)
+         var[r] = rl[r];
+      else if (r >= nd.n)
+         var[r] = rl[nd.ncol + (r - nd.n)];
+   }
+   for (int r = threadIdx.x; r < nd.n; r += kThreads) pres[nd.pre_off + r] = r;
+}
+
+// Delayed columns of each child into its parent (assemble_delays,
+// src/kernels/assemble.hxx:926-961): the lower triangle of the delayed block
+// and the child's contribution rows of those columns, placed through the
+// child map; the parent's row list gets the delayed variables. Children of
+// one level write disjoint parent entries.
+static __global__ __launch_bounds__(kThreads) void k_ind_delays(const int* __restrict__ children,
+                                                         const IndefDesc* __restrict__ nodes,
+                                                         const int* __restrict__ cmap,
+                                                         double* __restrict__ arena,
+                                                         int* __restrict__ vars) {
+   const IndefDesc cn = nodes[children[blockIdx.x]];
+   const IndefDesc pn = nodes[cn.parent];
+   const int nd_out = cn.n - cn.nelim;
+   const double* C = arena + cn.off;
+   double* P = arena + pn.off;
+   const int* cm = cmap + cn.cmap_off;
+   const int q = cn.m - cn.n;
+   for (int i = 0; i < nd_out; ++i) {
+      const int dc = cn.delay_col0 + i;
+      const int cc = cn.nelim + i;
+      for (int j = threadIdx.x; j < nd_out - i; j += kThreads)
+         P[dc + j + (int64_t)dc * pn.ld] = C[cc + j + (int64_t)cc * cn.ld];
+      for (int k = threadIdx.x; k < q; k += kThreads) {
+         int r = cm[k];
+         if (r >= pn.ncol) r += pn.ndelay_in;
+         const double v = C[cn.n + k + (int64_t)cc * cn.ld];
+         if (r < pn.ncol)
+            P[dc + (int64_t)r * pn.ld] = v;
+         else
+            P[r + (int64_t)dc * pn.ld] = v;
+      }
+   }
+   for (int i = threadIdx.x; i < nd_out; i += kThreads)
+      vars[pn.var_off + cn.delay_col0 + i] = vars[cn.var_off + cn.nelim + i];
+}
+
+// Extend-add of the contribution blocks of the children of one rank
+// (src/kernels/assemble.hxx:230-517 with the delay shift of the parent's
+// map, src/assemble.hxx:184-190): one workgroup per (child, column chunk).
+static __global__ __launch_bounds__(kThreads) void k_ind_ea(const int* __restrict__ children,
+                                                     const IndefDesc* __restrict__ nodes,
+                                                     const int* __restrict__ cmap,
+                                                     double* __restrict__ arena) {
+   const IndefDesc cn = nodes[children[blockIdx.y]];
+   const IndefDesc pn = nodes[cn.parent];
+   const int q = cn.m - cn.n;
+   const double* C = arena + cn.off;
+   double* P = arena + pn.off;
+   const int* cm = cmap + cn.cmap_off;
+   for (int j = blockIdx.x; j < q; j += gridDim.x) {
+      int pj = cm[j];
+      if (pj >= pn.ncol) pj += pn.ndelay_in;
+      for (int i = j + threadIdx.x; i < q; i += kThreads) {
+         int pi = cm[i];
+         if (pi >= pn.ncol) pi += pn.ndelay_in;
+         P[pi + (int64_t)pj * pn.ld] += C[cn.n + i + (int64_t)(cn.n + j) * cn.ld];
+      }
+   }
+}
+
+// LD = L[n:m, 0:nelim] D for the contribution-block update (calcLD), into the
+// node's workspace (q x nelim, ld q).
+static __global__ __launch_bounds__(kThreads) void k_ind_ld(const int* __restrict__ list,
+                                                     const IndefDesc* __restrict__ nodes,
+                                                     const double* __restrict__ arena,
+                                                     const double* __restrict__ dinv,
+                                                     double* __restrict__ wsbase) {
+#pragma clang fp contract(off)
+   const IndefDesc nd = nodes[list[blockIdx.y]];
+   const int q = nd.m - nd.n, ne = nd.nelim;
+   const double* F = arena + nd.off;
+   const double* d = dinv + nd.d_off;
+   double* W = wsbase + nd.ws_off;
+   for (int j = blockIdx.x; j < ne; j += gridDim.x) {
+      const bool second = j > 0 && __builtin_isinf(d[2 * j]);
+      const bool first = j + 1 < ne && __builtin_isinf(d[2 * j + 2]);
+      for (int i = threadIdx.x; i < q; i += kThreads) {
+         double v;
+         if (first) {
+            const double det = __builtin_fma(d[2 * j], d[2 * j + 3], -(d[2 * j + 1] * d[2 * j + 1]));
+            v = dot2(d[2 * j + 3] / det, F[nd.n + i + (int64_t)j * nd.ld], -d[2 * j + 1] / det,
+                     F[nd.n + i + (int64_t)(j + 1) * nd.ld]);
+         } else if (second) {
+            const int k = j - 1;
+            const double det = __builtin_fma(d[2 * k], d[2 * k + 3], -(d[2 * k + 1] * d[2 * k + 1]));
+            v = dot2(-d[2 * k + 1] / det, F[nd.n + i + (int64_t)k * nd.ld], d[2 * k] / det,
+                     F[nd.n + i + (int64_t)j * nd.ld]);
+         } else {
+            const double D0 = (d[2 * j] == 0.0) ? 0.0 : 1.0 / d[2 * j];
+            v = D0 * F[nd.n + i + (int64_t)j * nd.ld];
+         }
+         W[i + (int64_t)j * q] = v;
+      }
+   }
+}
+
+// ---------------------------------------------------------------------------
+// Solves (NumericTree::solve_fwd / solve_diag_bwd_inner,
+// src/NumericTree.hxx:408-532; ldlt_app_solve_fwd/diag/bwd,
+// src/kernels/ldlt_app.cxx:53-115). One workgroup per node. Per right-hand
+// side the node owns two m-vectors at x_off: the work vector in front order
+// and a pre-pivot gather buffer.
+// ---------------------------------------------------------------------------
+// Forward: gather (own rows from b, delayed rows copied from the child that
+// delayed them, contribution rows of every child added in child order),
+// permute to the pivoted order, unit lower solve on [0, nelim), update of the
+// rows below; the eliminated entries go to `sol`.
+static __global__ __launch_bounds__(kThreads) void k_ind_fwd(const int* __restrict__ list,
+                                                      const IndefDesc* __restrict__ nodes,
+                                                      const double* __restrict__ arena,
+                                                      const int* __restrict__ vars,
+                                                      const int* __restrict__ pres,
+                                                      const int* __restrict__ cmap,
+                                                      const double* __restrict__ b, int ldb,
+                                                      double* __restrict__ xw,
+                                                      double* __restrict__ sol, int nrhs) {
+   const IndefDesc nd = nodes[list[blockIdx.x]];
+   const int m = nd.m, n = nd.n, ne = nd.nelim;
+   const double* F = arena + nd.off;
+   const int* var = vars + nd.var_off;
+   const int* pre = pres + nd.pre_off;
+   for (int c = 0; c < nrhs; ++c) {
+      double* xl = xw + nd.x_off * nrhs + (int64_t)c * 2 * m;
+      double* xp = xl + m;
+      for (int r = threadIdx.x; r < m; r += kThreads) xp[r] = 0.0;
+      __syncthreads();
+      // own pivots: b at the symbolic row list (pre-pivot positions < ncol)
+      for (int r = threadIdx.x; r < n; r += kThreads) {
+         const int p = pre[r];
+         if (p < nd.ncol) xp[p] = b[var[r] - 1 + (int64_t)c * ldb];
+      }
+      __syncthreads();
+      for (int ch = nd.first_child; ch != -1;) {
+         const IndefDesc cn = nodes[ch];
+         const double* cx = xw + cn.x_off * nrhs + (int64_t)c * 2 * cn.m;
+         const int nd_out = cn.n - cn.nelim;
+         for (int i = threadIdx.x; i < nd_out; i += kThreads) xp[cn.delay_col0 + i] = cx[cn.nelim + i];
+         const int* cm = cmap + cn.cmap_off;
+         for (int k = threadIdx.x; k < cn.m - cn.n; k += kThreads) {
+            int r = cm[k];
+            if (r >= nd.ncol) r += nd.ndelay_in;
+            xp[r] += cx[cn.n + k];
+         }
+         __syncthreads();
+         ch = cn.next_child;
+      }
+      for (int r = threadIdx.x; r < m; r += kThreads) xl[r] = (r < n) ? xp[pre[r]] : xp[r];
+      __syncthreads();
+      for (int j = 0; j < ne; ++j) {
+         const double y = xl[j];
+         for (int r = j + 1 + threadIdx.x; r < m; r += kThreads)
+            xl[r] = fnma(F[r + (int64_t)j * nd.ld], y, xl[r]);
+         __syncthreads();
+      }
+      for (int r = threadIdx.x; r < ne; r += kThreads) sol[var[r] - 1 + (int64_t)c * ldb] = xl[r];
+      __syncthreads();
+   }
+}
+
+// Diagonal: sol at the eliminated variables of every node times D^{-1}.
+static __global__ __launch_bounds__(kThreads) void k_ind_diag(const IndefDesc* __restrict__ nodes,
+                                                       int nnodes, const int* __restrict__ vars,
+                                                       const double* __restrict__ dinv,
+                                                       double* __restrict__ sol, int ld, int nrhs) {
+#pragma clang fp contract(off)
+   const int node = blockIdx.x;
+   if (node >= nnodes) return;
+   const IndefDesc nd = nodes[node];
+   const int* var = vars + nd.var_off;
+   const double* d = dinv + nd.d_off;
+   for (int64_t e = threadIdx.x; e < (int64_t)nd.nelim * nrhs; e += kThreads) {
+      const int i = (int)(e % nd.nelim), c = (int)(e / nd.nelim);
+      double* x = sol + (int64_t)c * ld;
+      if (i > 0 && __builtin_isinf(d[2 * i])) continue;  // second half: done by the first
+      if (i + 1 < nd.nelim && __builtin_isinf(d[2 * i + 2])) {
+         const double a = x[var[i] - 1], bb = x[var[i + 1] - 1];
+         x[var[i] - 1] = dot2(d[2 * i], a, d[2 * i + 1], bb);
+         x[var[i + 1] - 1] = dot2(d[2 * i + 1], a, d[2 * i + 3], bb);
+      } else {
+         x[var[i] - 1] = x[var[i] - 1] * d[2 * i];
+      }
+   }
+}
+
+// Backward: x_k = L11^{-T} (x_k - L21^T x_rest) with x_rest read from `sol`
+// (ancestors are done, levels run top-down).
+static __global__ __launch_bounds__(kThreads) void k_ind_bwd(const int* __restrict__ list,
+                                                      const IndefDesc* __restrict__ nodes,
+                                                      const double* __restrict__ arena,
+                                                      const int* __restrict__ vars,
+                                                      double* __restrict__ xw,
+                                                      double* __restrict__ sol, int ld, int nrhs) {
+#pragma clang fp contract(off)
+   __shared__ double red[kThreads / 64];
+   const IndefDesc nd = nodes[list[blockIdx.x]];
+   const int m = nd.m, ne = nd.nelim;
+   const double* F = arena + nd.off;
+   const int* var = vars + nd.var_off;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   for (int c = 0; c < nrhs; ++c) {
+      double* xl = xw + nd.x_off * nrhs + (int64_t)c * 2 * m;
+      const double* x = sol + (int64_t)c * ld;
+      for (int r = threadIdx.x; r < m; r += kThreads) xl[r] = x[var[r] - 1];
+      __syncthreads();
+      for (int j = ne - 1; j >= 0; --j) {
+         double s = 0.0;
+         for (int r = j + 1 + threadIdx.x; r < m; r += kThreads) s = __builtin_fma(F[r + (int64_t)j * nd.ld], xl[r], s);
+#pragma unroll
+         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+         if (lane == 0) red[wave] = s;
+         __syncthreads();
+         if (threadIdx.x == 0) xl[j] = xl[j] - (red[0] + red[1] + red[2] + red[3]);
+         __syncthreads();
+      }
+      for (int r = threadIdx.x; r < ne; r += kThreads) sol[var[r] - 1 + (int64_t)c * ld] = xl[r];
+      __syncthreads();
+   }
+}
+
+}  // namespace sylver_amd

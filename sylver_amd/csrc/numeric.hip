@@ -25,71 +25,10 @@
 
 #include "kernels.hpp"
 #include "numeric.hpp"
+#include "numeric_impl.hpp"
 
 namespace sylver_amd {
 
-namespace {
-
-struct HipError {
-   hipError_t err;
-   int line;
-};
-
-#define HIPCHECK(x)                                   \
-   do {                                              \
-      hipError_t e_ = (x);                           \
-      if (e_ != hipSuccess) throw HipError{e_, __LINE__}; \
-   } while (0)
-
-template <typename T>
-T* dalloc(size_t n) {
-   void* p = nullptr;
-   if (n == 0) n = 1;
-   HIPCHECK(hipMalloc(&p, n * sizeof(T)));
-   return static_cast<T*>(p);
-}
-
-template <typename T>
-T* dupload(const std::vector<T>& v) {
-   T* p = dalloc<T>(v.size());
-   if (!v.empty()) HIPCHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-   return p;
-}
-
-void dfree(void* p) {
-   if (p) (void)hipFree(p);
-}
-
-enum LaunchKind {
-   L_ZERO,
-   L_INIT,
-   L_EA,
-   L_SMALL,
-   L_DIAG,
-   L_TRSM,
-   L_UPDATE,
-   L_DINV,
-   L_UPDATE_WIDE,
-   L_FGATHER,
-   L_FDIAG,
-   L_FUPD,
-   L_BGATHER,
-   L_BDIAG,
-   L_BUPD
-};
-
-struct Launch {
-   int kind;
-   int count;
-   int64_t off;
-   double flops = 0;  // algorithmic flops of the launch
-   double words = 0;  // algorithmic memory traffic in front elements
-   double bytes = 0;  // algorithmic memory traffic independent of the element size
-};
-
-int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
-
-}  // namespace
 
 sylver_amd_options_t& ext_options() {
    static sylver_amd_options_t o = [] {
@@ -115,109 +54,6 @@ int device_count() {
 // ---------------------------------------------------------------------------
 // Plan: pattern-dependent device state.
 // ---------------------------------------------------------------------------
-struct DevicePlan {
-   int n = 0, nnodes = 0, nlevels = 0;
-   int small_max = 256;
-   int wide_ncol = 1024;  // fronts with more pivots take the wide-front update launches
-   std::vector<NodeDesc> hnodes;
-   std::vector<std::vector<int>> levels;
-   std::vector<int> hcmap;
-   std::vector<int64_t> hcmap_off;
-   int64_t arena_elems = 0, dinv_elems = 0, solve_rows = 0;
-   // bit-exact device copies of the analyse outputs
-   int* d_sptr = nullptr;
-   int* d_sparent = nullptr;
-   int64_t* d_rptr = nullptr;
-   int* d_rlist = nullptr;
-   int64_t* d_nptr = nullptr;
-   int64_t* d_nlist = nullptr;
-   int* d_cmap = nullptr;
-   int64_t* d_cmap_off = nullptr;
-   NodeDesc* d_nodes = nullptr;
-   // task lists
-   std::vector<ColTask> zero;
-   std::vector<InitTask> init;
-   std::vector<EaTask> ea;
-   std::vector<int> nlistv;  // node lists (small fronts, gathers)
-   std::vector<DiagTask> diag;
-   std::vector<GemmTask> gemm;
-   std::vector<SolveTask> stask;
-   ColTask* d_zero = nullptr;
-   InitTask* d_init = nullptr;
-   EaTask* d_ea = nullptr;
-   int* d_nlistv = nullptr;
-   DiagTask* d_diag = nullptr;
-   GemmTask* d_gemm = nullptr;
-   SolveTask* d_stask = nullptr;
-   std::vector<Launch> fprog, fwdprog, bwdprog;
-   int64_t num_flops = 0;
-   // CSR of the whole matrix in pivot order, values gathered per factorize
-   int64_t csr_nnz = 0;
-   int64_t* d_csr_ptr = nullptr;
-   int* d_csr_col = nullptr;
-   int64_t* d_csr_src = nullptr;
-   int64_t clean_nnz = 0;
-
-   ~DevicePlan() {
-      for (void* p : {(void*)d_sptr, (void*)d_sparent, (void*)d_rptr, (void*)d_rlist,
-                      (void*)d_nptr, (void*)d_nlist, (void*)d_cmap, (void*)d_cmap_off,
-                      (void*)d_nodes, (void*)d_zero, (void*)d_init, (void*)d_ea,
-                      (void*)d_nlistv, (void*)d_diag, (void*)d_gemm, (void*)d_stask,
-                      (void*)d_csr_ptr, (void*)d_csr_col, (void*)d_csr_src})
-         dfree(p);
-   }
-};
-
-struct DeviceFactors {
-   const DevicePlan* plan = nullptr;
-   int precision = SYLVER_AMD_PREC_FP64;
-   bool posdef = true;
-   void* arena = nullptr;
-   void* dinv = nullptr;
-   void* dinvT = nullptr;
-   double* d_aval = nullptr;
-   double* d_scaling = nullptr;
-   double* d_csr_val = nullptr;
-   int* d_info = nullptr;
-   double anorm = 0;
-   hipStream_t stream = nullptr;
-   hipGraphExec_t graph = nullptr;
-   bool has_scaling = false;
-   // solve workspaces
-   int ws_nrhs = 0;
-   double* d_xloc = nullptr;
-   double* d_b = nullptr;
-   double* d_x = nullptr;
-   double* d_r = nullptr;
-   double* d_sol = nullptr;
-   unsigned long long* d_norms = nullptr;
-   // GMRES-IR workspace (allocated on first use)
-   int gm_nrhs = 0, gm_m = 0;
-   double* d_V = nullptr;      // (gm_m + 1) Krylov vectors, n x nrhs each
-   double* d_W = nullptr;
-   double* d_tmp = nullptr;
-   double* d_part = nullptr;
-   double* d_coef = nullptr;
-   // statistics
-   double factor_ms = 0, upload_ms = 0, solve_ms = 0, bwd_err = -1;
-   int refine_iters = 0;
-   int nlaunches = 0;
-   double kind_ms[16] = {0};
-   double kind_flops[16] = {0};
-   double kind_bytes[16] = {0};
-   int kind_launches[16] = {0};
-
-   ~DeviceFactors() {
-      if (graph) (void)hipGraphExecDestroy(graph);
-      for (void* p : {arena, dinv, dinvT, (void*)d_aval, (void*)d_scaling, (void*)d_csr_val,
-                      (void*)d_info, (void*)d_xloc, (void*)d_b, (void*)d_x, (void*)d_r,
-                      (void*)d_sol, (void*)d_norms, (void*)d_V, (void*)d_W, (void*)d_tmp,
-                      (void*)d_part, (void*)d_coef})
-         dfree(p);
-      if (stream) (void)hipStreamDestroy(stream);
-   }
-};
-
 void destroy_plan(DevicePlan* p) { delete p; }
 void destroy_factors(DeviceFactors* f) { delete f; }
 
@@ -603,7 +439,14 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    }
 
    // ---- CSR of the full matrix in pivot order ----
-   {
+   // (the internal seam passes no matrix pattern: values are then addressed
+   // through nlist only and no residual can be formed)
+   if (a.ptr.empty()) {
+      int64_t mx = 0;
+      for (size_t k = 0; k + 1 < a.nlist.size(); k += 2) mx = std::max<int64_t>(mx, a.nlist[k]);
+      P->clean_nnz = mx;
+      P->csr_nnz = 0;
+   } else {
       int n = a.n;
       const int64_t* cptr = a.ptr.data();
       const int* crow = a.row.data();
@@ -776,11 +619,6 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
       res.hip_error = (int)hipErrorNoDevice;
       return res;
    }
-   if (!posdef) {
-      // The APP LDL^T device path is not part of this build yet.
-      res.flag = SYLVER_ERROR_UNIMPLEMENTED;
-      return res;
-   }
    const sylver_amd_options_t& xo = ext_options();
    (void)opts;
    try {
@@ -801,8 +639,9 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
          prec = SYLVER_AMD_PREC_FP64;
       // Mixed precision without the BF16 update kernels runs the FP32 path.
       const bool low = prec != SYLVER_AMD_PREC_FP64;
+      if (!posdef) prec = SYLVER_AMD_PREC_FP64;  // the indefinite path factors in FP64
       DeviceFactors* F = *fact;
-      if (F && (F->plan != P || F->precision != prec)) {
+      if (F && (F->plan != P || F->precision != prec || F->posdef != posdef)) {
          destroy_factors(F);
          F = nullptr;
       }
@@ -812,9 +651,11 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
          F->plan = P;
          F->precision = prec;
          size_t es = low ? sizeof(float) : sizeof(double);
-         HIPCHECK(hipMalloc(&F->arena, std::max<int64_t>(P->arena_elems, 1) * es));
-         HIPCHECK(hipMalloc(&F->dinv, std::max<int64_t>(P->dinv_elems, 1) * es));
-         HIPCHECK(hipMalloc(&F->dinvT, std::max<int64_t>(P->dinv_elems, 1) * es));
+         if (posdef) {  // the indefinite path lays out its fronts level by level
+            HIPCHECK(hipMalloc(&F->arena, std::max<int64_t>(P->arena_elems, 1) * es));
+            HIPCHECK(hipMalloc(&F->dinv, std::max<int64_t>(P->dinv_elems, 1) * es));
+            HIPCHECK(hipMalloc(&F->dinvT, std::max<int64_t>(P->dinv_elems, 1) * es));
+         }
          F->d_aval = dalloc<double>(std::max<int64_t>(P->clean_nnz, 1));
          F->d_scaling = dalloc<double>(std::max(a.n, 1));
          F->d_csr_val = dalloc<double>(std::max<int64_t>(P->csr_nnz, 1));
@@ -834,8 +675,8 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
       if (scaling) HIPCHECK(hipMemcpyAsync(F->d_scaling, scaling, a.n * sizeof(double), hipMemcpyHostToDevice, s));
       HIPCHECK(hipMemsetAsync(F->d_info, 0, std::max(P->nnodes, 1) * sizeof(int), s));
       HIPCHECK(hipEventRecord(e1, s));
-      const bool record = xo.record_timing != 0;
-      bool use_graph = xo.use_graph != 0 && !F->has_scaling && !record;
+      const bool record = xo.record_timing != 0 && posdef;
+      bool use_graph = xo.use_graph != 0 && !F->has_scaling && !record && posdef;
       std::vector<hipEvent_t> events;
       if (record) {
          events.resize(2 * P->fprog.size());
@@ -852,7 +693,9 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
          HIPCHECK(hipGraphInstantiate(&F->graph, g, nullptr, nullptr, 0));
          (void)hipGraphDestroy(g);
       }
-      if (use_graph && F->graph) {
+      if (!posdef) {
+         factor_indef(a, P, F, opts, res);
+      } else if (use_graph && F->graph) {
          HIPCHECK(hipGraphLaunch(F->graph, s));
       } else if (low) {
          run_factor_program<float>(P, F, s, record ? &events : nullptr);
@@ -892,16 +735,17 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
          }
          for (auto& e : events) (void)hipEventDestroy(e);
       }
-      F->nlaunches = (int)P->fprog.size();
+      F->nlaunches = posdef ? (int)P->fprog.size() : 0;
       res.upload_ms = t1;
       res.factor_ms = t2;
-      for (int i = 0; i < P->nnodes; ++i)
-         if (info[i] != 0) {
-            res.flag = SYLVER_ERROR_NOT_POS_DEF;
-            break;
-         }
+      if (posdef)
+         for (int i = 0; i < P->nnodes; ++i)
+            if (info[i] != 0) {
+               res.flag = SYLVER_ERROR_NOT_POS_DEF;
+               break;
+            }
       // ||A||_inf with full symmetric row sums (scaled matrix when scaling given)
-      {
+      if (!a.ptr.empty()) {
          std::vector<double> rs(a.n, 0.0);
          for (int j = 0; j < a.n; ++j)
             for (int64_t p = a.ptr[j] - 1; p < a.ptr[j + 1] - 1; ++p) {
@@ -1113,7 +957,10 @@ SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int
       const bool low = F->precision != SYLVER_AMD_PREC_FP64;
       const bool refine = low && job == 0 && xo.refine != SYLVER_AMD_REFINE_NONE && !F->has_scaling;
       const int64_t nn = (int64_t)n * nrhs;
-      if (!refine) {
+      if (F->ind && !F->posdef) {
+         solve_indef(P, F, job, F->d_b, F->d_sol, nrhs, s);
+         res.refine_mode = SYLVER_AMD_REFINE_NONE;
+      } else if (!refine) {
          if (low)
             solve_once<float>(P, F, job, F->d_b, F->d_sol, nrhs, s);
          else

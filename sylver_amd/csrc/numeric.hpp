@@ -25,6 +25,9 @@ struct FactorResult {
    int matrix_rank = 0;
    int not_first_pass = 0;
    int not_second_pass = 0;
+   int maxfront = 0;          // indefinite path: largest front including delays
+   int64_t num_factor = 0;    // indefinite path: entries of L including delays
+   double num_flops = 0;      // indefinite path: flops including delays
    double factor_ms = 0;
    double upload_ms = 0;
 };
@@ -70,6 +73,11 @@ int plan_device_symbolic(const DevicePlan* plan, int* sptr, int* sparent, int64_
                          int64_t* cmap_off);
 int factors_node_factor(const DeviceFactors* f, int node, double* l);
 int factors_node_assembled(const DeviceFactors* f, int node, double* front);
+
+// Indefinite factors: statistics {num_delay, num_neg, num_two, num_zero,
+// maxfront, not_first_pass, not_second_pass, 0} and per-node read-back.
+int indef_stats(const DeviceFactors* F, int* out);
+int indef_node(const DeviceFactors* F, int node, int* dims, int* rows, double* L, double* dinv);
 
 int dev_alloc(void** ptr, size_t bytes);
 int dev_free(void* ptr);

@@ -1,0 +1,196 @@
+// Internal to the device numeric phase: error handling, allocation helpers,
+// launch bookkeeping and the plan / factor state shared by the posdef
+// (numeric.hip) and the indefinite (indef.hip) paths.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "kernels.hpp"
+#include "numeric.hpp"
+
+namespace sylver_amd {
+
+namespace {
+
+struct HipError {
+   hipError_t err;
+   int line;
+};
+
+#define HIPCHECK(x)                                   \
+   do {                                              \
+      hipError_t e_ = (x);                           \
+      if (e_ != hipSuccess) throw HipError{e_, __LINE__}; \
+   } while (0)
+
+template <typename T>
+T* dalloc(size_t n) {
+   void* p = nullptr;
+   if (n == 0) n = 1;
+   HIPCHECK(hipMalloc(&p, n * sizeof(T)));
+   return static_cast<T*>(p);
+}
+
+template <typename T>
+T* dupload(const std::vector<T>& v) {
+   T* p = dalloc<T>(v.size());
+   if (!v.empty()) HIPCHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+   return p;
+}
+
+void dfree(void* p) {
+   if (p) (void)hipFree(p);
+}
+
+enum LaunchKind {
+   L_ZERO,
+   L_INIT,
+   L_EA,
+   L_SMALL,
+   L_DIAG,
+   L_TRSM,
+   L_UPDATE,
+   L_DINV,
+   L_UPDATE_WIDE,
+   L_FGATHER,
+   L_FDIAG,
+   L_FUPD,
+   L_BGATHER,
+   L_BDIAG,
+   L_BUPD
+};
+
+struct Launch {
+   int kind;
+   int count;
+   int64_t off;
+   double flops = 0;  // algorithmic flops of the launch
+   double words = 0;  // algorithmic memory traffic in front elements
+   double bytes = 0;  // algorithmic memory traffic independent of the element size
+};
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+struct IndefState;  // APP LDL^T factor state (indef.hip)
+void destroy_indef(IndefState* s);
+struct DevicePlan;
+struct DeviceFactors;
+// Indefinite path (indef.hip): factorization driven level by level, solves.
+void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, const sylver_options_t& opts,
+                  FactorResult& res);
+void solve_indef(const DevicePlan* P, DeviceFactors* F, int job, const double* b, double* sol, int nrhs,
+                 hipStream_t s);
+
+struct DevicePlan {
+   int n = 0, nnodes = 0, nlevels = 0;
+   int small_max = 256;
+   int wide_ncol = 1024;  // fronts with more pivots take the wide-front update launches
+   std::vector<NodeDesc> hnodes;
+   std::vector<std::vector<int>> levels;
+   std::vector<int> hcmap;
+   std::vector<int64_t> hcmap_off;
+   int64_t arena_elems = 0, dinv_elems = 0, solve_rows = 0;
+   // bit-exact device copies of the analyse outputs
+   int* d_sptr = nullptr;
+   int* d_sparent = nullptr;
+   int64_t* d_rptr = nullptr;
+   int* d_rlist = nullptr;
+   int64_t* d_nptr = nullptr;
+   int64_t* d_nlist = nullptr;
+   int* d_cmap = nullptr;
+   int64_t* d_cmap_off = nullptr;
+   NodeDesc* d_nodes = nullptr;
+   // task lists
+   std::vector<ColTask> zero;
+   std::vector<InitTask> init;
+   std::vector<EaTask> ea;
+   std::vector<int> nlistv;  // node lists (small fronts, gathers)
+   std::vector<DiagTask> diag;
+   std::vector<GemmTask> gemm;
+   std::vector<SolveTask> stask;
+   ColTask* d_zero = nullptr;
+   InitTask* d_init = nullptr;
+   EaTask* d_ea = nullptr;
+   int* d_nlistv = nullptr;
+   DiagTask* d_diag = nullptr;
+   GemmTask* d_gemm = nullptr;
+   SolveTask* d_stask = nullptr;
+   std::vector<Launch> fprog, fwdprog, bwdprog;
+   int64_t num_flops = 0;
+   // CSR of the whole matrix in pivot order, values gathered per factorize
+   int64_t csr_nnz = 0;
+   int64_t* d_csr_ptr = nullptr;
+   int* d_csr_col = nullptr;
+   int64_t* d_csr_src = nullptr;
+   int64_t clean_nnz = 0;
+
+   ~DevicePlan() {
+      for (void* p : {(void*)d_sptr, (void*)d_sparent, (void*)d_rptr, (void*)d_rlist,
+                      (void*)d_nptr, (void*)d_nlist, (void*)d_cmap, (void*)d_cmap_off,
+                      (void*)d_nodes, (void*)d_zero, (void*)d_init, (void*)d_ea,
+                      (void*)d_nlistv, (void*)d_diag, (void*)d_gemm, (void*)d_stask,
+                      (void*)d_csr_ptr, (void*)d_csr_col, (void*)d_csr_src})
+         dfree(p);
+   }
+};
+
+struct DeviceFactors {
+   const DevicePlan* plan = nullptr;
+   int precision = SYLVER_AMD_PREC_FP64;
+   bool posdef = true;
+   void* arena = nullptr;
+   void* dinv = nullptr;
+   void* dinvT = nullptr;
+   double* d_aval = nullptr;
+   double* d_scaling = nullptr;
+   double* d_csr_val = nullptr;
+   int* d_info = nullptr;
+   double anorm = 0;
+   hipStream_t stream = nullptr;
+   hipGraphExec_t graph = nullptr;
+   bool has_scaling = false;
+   // solve workspaces
+   int ws_nrhs = 0;
+   double* d_xloc = nullptr;
+   double* d_b = nullptr;
+   double* d_x = nullptr;
+   double* d_r = nullptr;
+   double* d_sol = nullptr;
+   unsigned long long* d_norms = nullptr;
+   // GMRES-IR workspace (allocated on first use)
+   int gm_nrhs = 0, gm_m = 0;
+   double* d_V = nullptr;      // (gm_m + 1) Krylov vectors, n x nrhs each
+   double* d_W = nullptr;
+   double* d_tmp = nullptr;
+   double* d_part = nullptr;
+   double* d_coef = nullptr;
+   // statistics
+   double factor_ms = 0, upload_ms = 0, solve_ms = 0, bwd_err = -1;
+   int refine_iters = 0;
+   int nlaunches = 0;
+   double kind_ms[16] = {0};
+   double kind_flops[16] = {0};
+   double kind_bytes[16] = {0};
+   int kind_launches[16] = {0};
+   // indefinite (APP LDL^T) factors; null on the posdef path
+   IndefState* ind = nullptr;
+
+   ~DeviceFactors() {
+      if (ind) destroy_indef(ind);
+      if (graph) (void)hipGraphExecDestroy(graph);
+      for (void* p : {arena, dinv, dinvT, (void*)d_aval, (void*)d_scaling, (void*)d_csr_val,
+                      (void*)d_info, (void*)d_xloc, (void*)d_b, (void*)d_x, (void*)d_r,
+                      (void*)d_sol, (void*)d_norms, (void*)d_V, (void*)d_W, (void*)d_tmp,
+                      (void*)d_part, (void*)d_coef})
+         dfree(p);
+      if (stream) (void)hipStreamDestroy(stream);
+   }
+};
+
+
+}  // namespace sylver_amd

@@ -223,6 +223,11 @@ void spldlt_factorize(bool posdef, long const* ptr, int const* row, double const
       inform->num_two = r.num_two;
       inform->num_delay = r.num_delay;
       inform->matrix_rank = r.matrix_rank;
+      if (!posdef && r.maxfront > 0) {  // actual sizes with delayed columns
+         inform->maxfront = r.maxfront;
+         inform->num_factor = r.num_factor;
+         inform->num_flops = (long)r.num_flops;
+      }
       sylver_amd_inform_tail_t t;
       std::memset(&t, 0, sizeof(t));
       t.factor_ms = r.factor_ms;
@@ -384,6 +389,19 @@ int sylver_amd_fkeep_node_factor(void const* fkeep, int node, double* l) {
    return factors_node_factor(fk->f, node, l);
 }
 
+int sylver_amd_fkeep_indef_node(void const* fkeep, int node, int* dims, int* rows, double* l,
+                                double* dinv) {
+   const Fkeep* fk = as_fkeep(const_cast<void*>(fkeep));
+   if (!fk || !fk->f || fk->posdef) return SYLVER_ERROR_CALL_SEQUENCE;
+   return indef_node(fk->f, node, dims, rows, l, dinv);
+}
+
+int sylver_amd_fkeep_indef_stats(void const* fkeep, int* out) {
+   const Fkeep* fk = as_fkeep(const_cast<void*>(fkeep));
+   if (!fk || !fk->f || fk->posdef || !out) return SYLVER_ERROR_CALL_SEQUENCE;
+   return indef_stats(fk->f, out);
+}
+
 int sylver_amd_fkeep_node_assembled(void const* fkeep, int node, double* front) {
    const Fkeep* fk = as_fkeep(const_cast<void*>(fkeep));
    if (!fk || !fk->f) return SYLVER_ERROR_CALL_SEQUENCE;
@@ -503,7 +521,7 @@ static void* seam_create(bool posdef, void* symbolic_tree, double* aval, double 
       inform->num_neg = r.num_neg;
       inform->num_two = r.num_two;
       inform->num_zero = r.num_zero;
-      inform->maxfront = s->ak.a.maxfront;
+      inform->maxfront = (!posdef && r.maxfront > 0) ? r.maxfront : s->ak.a.maxfront;
       inform->not_first_pass = r.not_first_pass;
       inform->not_second_pass = r.not_second_pass;
       if (r.flag < 0) {

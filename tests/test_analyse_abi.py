@@ -219,3 +219,35 @@ def test_seam_symbolic_tree_roundtrip(S):
     assert t
     L.spldlt_destroy_symbolic_tree.argtypes = [C.c_void_p]
     L.spldlt_destroy_symbolic_tree(t)
+
+
+def test_seam_struct_layouts(S):
+    """sylver_options_c / sylver_inform_c as ctypes mirror the C header."""
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include <stdbool.h>
+#include "sylver_amd.h"
+#define P(T, f) printf(#T " " #f " %zu\n", offsetof(T, f));
+int main(void) {
+  P(sylver_options_c, print_level) P(sylver_options_c, action) P(sylver_options_c, small)
+  P(sylver_options_c, u) P(sylver_options_c, multiplier) P(sylver_options_c, small_subtree_threshold)
+  P(sylver_options_c, nb) P(sylver_options_c, pivot_method) P(sylver_options_c, failed_pivot_method)
+  P(sylver_options_c, cpu_topology) P(sylver_inform_c, flag) P(sylver_inform_c, not_second_pass)
+  printf("sizeof %zu %zu\n", sizeof(sylver_options_c), sizeof(sylver_inform_c));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "l")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    for line in out:
+        parts = line.split()
+        if len(parts) == 3 and parts[0] != "sizeof":
+            cls = S.OptionsC if parts[0] == "sylver_options_c" else S.InformC
+            assert getattr(cls, parts[1]).offset == int(parts[2]), line
+        elif len(parts) == 3:
+            assert C.sizeof(S.OptionsC) == int(parts[1]) and C.sizeof(S.InformC) == int(parts[2])

@@ -208,3 +208,129 @@ def test_child_maps_and_tree_invariants(S, O):
         mc = cmap[off[c]:off[c + 1]]
         assert np.array_equal(rlp[mc], rlc[n[c]:])
         assert np.all(np.diff(mc) > 0)
+
+
+# ---- indefinite path (APP LDL^T with delays) ------------------------------
+def _dinv_to_D(O, dinv, ne):
+    D = np.zeros((ne, ne))
+    for i, w, blk in O.dinv_blocks(dinv, ne):
+        D[i:i + w, i:i + w] = np.linalg.inv(blk) if blk.any() else 0.0
+    return D
+
+
+@pytest.mark.parametrize("m,n,nb,scale,zero", [
+    (40, 40, 16, 1.0, 0), (60, 25, 8, 1.0, 0), (120, 50, 16, 1.0, 0), (90, 90, 128, 1.0, 0),
+    (80, 40, 16, 1e-2, 0), (50, 50, 16, 1e-3, 25), (130, 130, 64, 1.0, 60)])
+def test_front_ldlt_closed_form(O, m, n, nb, scale, zero):
+    """P A P^T restricted to the eliminated pivots equals L D L^T; the failed
+    columns and the contribution block hold the Schur complement; every
+    entry of L respects the threshold |l| <= 1/u; the eliminated pivots'
+    inertia agrees with the Schur complement reduction (Sylvester)."""
+    u = 0.01
+    rng = np.random.default_rng(m * 7 + n)
+    a = rng.uniform(-1, 1, (m, m))
+    a = a + a.T
+    a[:n, :n] *= scale
+    if zero:
+        a[:zero, :zero] = 0.0
+    ne, nf, perm, dinv, f = O.front_ldlt(a, n, nb=nb, u=u)
+    assert 0 <= nf <= ne <= n
+    assert sorted(perm.tolist()) == list(range(n))
+    P = np.concatenate([perm, np.arange(n, m)])
+    Ap = a[np.ix_(P, P)]
+    L = np.tril(f[:, :ne], -1)
+    L[np.arange(ne), np.arange(ne)] = 1.0
+    D = _dinv_to_D(O, dinv, ne)
+    amax = max(1.0, np.abs(a).max())
+    assert np.abs(Ap[:, :ne] - L @ D @ L[:ne].T).max() <= 1e-12 * amax * max(1.0, np.abs(L).max()) ** 2
+    S = Ap[ne:, ne:] - L[ne:] @ D @ L[ne:].T
+    if n > ne:
+        assert np.abs(f[ne:, ne:n] - S[:, :n - ne]).max() <= 1e-11 * amax * max(1.0, np.abs(L).max()) ** 2
+    if m > n:
+        assert np.abs(np.tril(f[n:, n:]) - np.tril(S[n - ne:, n - ne:])).max() <= 1e-11 * amax * max(1.0, np.abs(L).max()) ** 2
+    assert np.abs(L).max() <= 1.0 / u * (1 + 1e-12)
+    if m == n and ne == n:
+        ev = np.linalg.eigvalsh(a)
+        assert (np.linalg.eigvalsh(D) < 0).sum() == (ev < 0).sum()
+
+
+def test_front_ldlt_failures_and_second_pass(O):
+    """A weak fully-summed block fails the a-posteriori test (nelim_first <
+    n); threshold partial pivoting recovers part of it, the rest is delayed;
+    with m == n (a root) everything is eliminated."""
+    rng = np.random.default_rng(5)
+    a = rng.uniform(-1, 1, (80, 80))
+    a = a + a.T
+    a[:40, :40] *= 1e-2
+    ne, nf, perm, dinv, f = O.front_ldlt(a, 40, nb=16)
+    assert nf < ne < 40
+    ne_nt, nf_nt, *_ = O.front_ldlt(a, 40, nb=16, failed_tpp=False)
+    assert ne_nt == nf_nt == nf
+    b = a[:40, :40] + np.diag(np.zeros(40))
+    ne2, _, _, _, _ = O.front_ldlt(b, 40, nb=16)
+    assert ne2 == 40
+
+
+def test_front_ldlt_singular_action(O):
+    a = np.array([[-1.0, 2.0, 0.0], [2.0, 1.0, 0.0], [0.0, 0.0, 0.0]])
+    assert O.front_ldlt(a, 3, action=False)[0] == -5
+    ne, _, perm, dinv, _ = O.front_ldlt(a, 3, action=True)
+    assert ne == 3
+    zero = [i for i, w, blk in O.dinv_blocks(dinv, ne) if w == 1 and blk[0, 0] == 0.0]
+    assert len(zero) == 1
+
+
+@pytest.mark.parametrize("side,mrows,nb", [(4, 12, 64), (6, 40, 64), (8, 100, 16), (10, 250, 32)])
+def test_indef_tree_inertia_and_solve(S, O, side, mrows, nb):
+    """KKT [H A^T; A 0] (H SPD, A full row rank): inertia (nh, m, 0) by
+    Sylvester's law; the solve matches numpy.linalg.solve; delays occur."""
+    A = M.kkt(side, mrows, seed=side)
+    sym = _analyse(S, A)
+    F = O.IndefFactor(sym, A.val, nb=nb)
+    st = F.stats
+    assert st["flag"] == 0 and st["num_neg"] == mrows and st["num_zero"] == 0
+    D = A.to_dense()
+    rng = np.random.default_rng(1)
+    b = rng.uniform(-1, 1, (A.n, 2))
+    inv = sym["invp"] - 1
+    xp = F.solve(b[inv])
+    x = np.empty_like(xp)
+    x[inv] = xp
+    np.testing.assert_allclose(x, np.linalg.solve(D, b), rtol=0, atol=1e-10 * np.abs(x).max())
+    assert O.bwd_err(A, x, b) < 1e-15
+    if side >= 8:
+        assert st["num_delay"] > 0
+
+
+def test_indef_tree_partial_solves(S, O):
+    A = M.kkt(6, 40, seed=3)
+    sym = _analyse(S, A)
+    F = O.IndefFactor(sym, A.val)
+    b = np.random.default_rng(2).uniform(-1, 1, (A.n, 1))
+    full = F.solve(b, job=0)
+    y = F.solve(F.solve(F.solve(b, job=1), job=2), job=3)
+    np.testing.assert_allclose(y, full, rtol=0, atol=1e-13)
+    z = F.solve(F.solve(b, job=1), job=4)
+    np.testing.assert_allclose(z, full, rtol=0, atol=1e-13)
+
+
+def test_indef_tree_node_structure(S, O):
+    """Every front: rows are distinct variables; eliminated pivots over all
+    nodes cover every variable exactly once; delayed columns reappear as
+    fully-summed rows of the parent."""
+    A = M.kkt(8, 100, seed=3)
+    sym = _analyse(S, A)
+    F = O.IndefFactor(sym, A.val, nb=16)
+    seen = []
+    par = sym["sparent"] - 1
+    nn = sym["nnodes"]
+    for i in range(nn):
+        nd = F.node(i)
+        assert len(set(nd["rows"].tolist())) == nd["m"]
+        seen += nd["rows"][:nd["nelim"]].tolist()
+        fails = set(nd["rows"][nd["nelim"]:nd["n"]].tolist())
+        if fails:
+            assert par[i] < nn
+            pn = F.node(par[i])
+            assert fails <= set(pn["rows"][:pn["n"]].tolist())
+    assert sorted(seen) == list(range(1, A.n + 1))
