@@ -124,6 +124,33 @@ def cpu_baseline(sym, A, seconds):
                         f" {t:.1f} s)"), seconds=round(t, 2))
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain device-to-device copy of `nbytes` (torch's elementwise copy
+    kernel), run `reps` times: under rocprofv3 --pmc its FETCH_SIZE and
+    WRITE_SIZE per dispatch give the factor between the counters and bytes."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}))
+
+
+def measured_traffic():
+    """HBM bytes per launch of the Schur-update kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    copy kernel); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_update_fp32_laplacian80_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,7 +162,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--mixed", default="fp32", choices=["fp32", "mixed"])
+    ap.add_argument("--paths", default="both", choices=["both", "mixed", "fp64"],
+                    help="run only one precision path (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (scales PMC byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -160,11 +194,17 @@ def main():
     log(f"analyse {t_an:.2f} s: n={A.n} nodes={inf.num_sup} maxfront={inf.maxfront}"
         f" flops={flops:.4e}")
 
+    prec = S.PREC_FP32 if args.mixed == "fp32" else S.PREC_MIXED
+    if args.paths != "both":  # profiling runs: one path, no report
+        pr = S.PREC_FP64 if args.paths == "fp64" else prec
+        _, recs = run_path(S, A, akeep, opts, b, pr, args.steps, args.warmup, 1e-15)
+        print(json.dumps({"paths": args.paths, "factor_ms": [r["factor_ms"] for r in recs],
+                          "solve_ms": [r["solve_ms"] for r in recs]}))
+        return
     # plain FP64 path
     fk64, r64 = run_path(S, A, akeep, opts, b, S.PREC_FP64, args.steps, args.warmup, 1e-15)
     err64 = M.backward_error(A, r64[-1]["x"], b)
     # mixed path (headline)
-    prec = S.PREC_FP32 if args.mixed == "fp32" else S.PREC_MIXED
     fkmx, rmx = run_path(S, A, akeep, opts, b, prec, args.steps, args.warmup, 1e-15)
     errmx = M.backward_error(A, rmx[-1]["x"], b)
 
@@ -189,10 +229,13 @@ def main():
         if up and up["ms"] > 0:
             ach = up["gflop"] / up["ms"]  # GFLOP/ms == TFLOP/s
             pk = PEAK["fp32"]
+            name = ("k_update<float, 0> (Schur update, FP32 MFMA 16x16x4)" if args.mixed == "fp32"
+                    else "k_update<float, 0> + k_update_bf16 (Schur update of large fronts)")
             roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": pk, "unit": "TFLOP/s",
-                    "frac": round(ach / pk, 4), "traffic": None,
-                    "kernel": "k_update<float> + k_update_bf16 (Schur update of large fronts)",
-                    "launches": up["launches"]}
+                    "frac": round(ach / pk, 4), "traffic": measured_traffic(),
+                    "kernel": name, "launches": up["launches"],
+                    "avg_launch_ms": round(up["ms"] / up["launches"], 5),
+                    "gflop_per_launch": round(up["gflop"] / up["launches"], 4)}
     S.set_ext_options(precision=S.PREC_FP64)
 
     cpu = None
