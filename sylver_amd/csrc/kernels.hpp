@@ -1012,6 +1012,177 @@ __global__ __launch_bounds__(kThreads) void k_solve_diag(const SolveTask* __rest
    }
 }
 
+// ys = M xs for the kb x kb triangular block M (ld kb; lower when !UPPER,
+// upper when UPPER), xs and ys row-major kb x 16 in LDS; 16-row tiles, two
+// per wave.
+template <typename T, bool UPPER>
+__device__ inline void block_apply(const T* __restrict__ Mk, int kb, const double* xs, double* ys) {
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int ntile = (kb + 15) / 16;
+#pragma unroll
+   for (int q = 0; q < 2; ++q) {
+      const int tile = wave + 4 * q;
+      if (tile >= ntile) continue;
+      const int kbeg = UPPER ? tile * 16 : 0;
+      const int kend = UPPER ? kb : min(kb, tile * 16 + 16);
+      v4d acc = v4d{0, 0, 0, 0};
+      acc = mma_rows_gl<T>(Mk + tile * 16 + (int64_t)kbeg * kb, kb, kb - tile * 16, kend - kbeg,
+                           xs + kbeg * 16, acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int i = tile * 16 + Mfma<double>::row(lane, v);
+         if (i < kb) ys[i * 16 + (lane & 15)] = acc[v];
+      }
+   }
+}
+
+// Forward block step, one launch per block column: every task stages x_k
+// (rows [k0, k0+kb) of the node's work vector, not modified by this launch)
+// and forms y = inv(L_kk) x_k in LDS; a task with `write` stores y into the
+// solution, the others update their rows [r0, r1): x_r -= L[r, k0:k0+kb] y.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fwd_blk(const SolveTask* __restrict__ tasks,
+                                                      const NodeDesc* __restrict__ nodes,
+                                                      const T* __restrict__ arena,
+                                                      const T* __restrict__ dinv,
+                                                      const int* __restrict__ rlist,
+                                                      double* __restrict__ xloc,
+                                                      double* __restrict__ xsol, int ldsol, int nrhs) {
+   __shared__ double xs[kNB * 16];
+   __shared__ double ys[kNB * 16];
+   const SolveTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
+   const int m = nd.m, kb = t.kb;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   double* xl = xloc + nd.solve_off * nrhs;
+   const T* Mk = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
+   const T* F = arena + nd.off;
+   const int ntile = (t.r1 - t.r0 + 15) / 16;
+   for (int c0 = 0; c0 < nrhs; c0 += 16) {
+      const int nc = min(16, nrhs - c0);
+      __syncthreads();
+#pragma unroll
+      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
+         const int l = e >> 4, c = e & 15;
+         xs[e] = *live_or_zero(l < kb && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
+      }
+      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) ys[e] = 0.0;
+      __syncthreads();
+      block_apply<T, false>(Mk, kb, xs, ys);
+      __syncthreads();
+      if (t.write) {
+         for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
+            const int i = e % kb, c = e / kb;
+            xsol[rlist[nd.rlist_off + t.k0 + i] - 1 + (int64_t)(c0 + c) * ldsol] = ys[i * 16 + c];
+         }
+         continue;
+      }
+      for (int tile = wave; tile < ntile; tile += 4) {
+         const int rb = t.r0 + tile * 16;
+         v4d acc = v4d{0, 0, 0, 0};
+         acc = mma_rows_gl<T>(F + rb + (int64_t)t.k0 * nd.ld, nd.ld, t.r1 - rb, kb, ys, acc);
+         const int c = lane & 15;
+         double cv[4];
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int r = rb + Mfma<double>::row(lane, v);
+            cv[v] = *live_or_zero(c < nc && r < t.r1, xl + r + (int64_t)(c0 + c) * m);
+         }
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int r = rb + Mfma<double>::row(lane, v);
+            if (c < nc && r < t.r1) xl[r + (int64_t)(c0 + c) * m] = cv[v] - acc[v];
+         }
+      }
+   }
+}
+
+// Backward block step: every task stages x_k and forms x_k' = inv(L_kk)^T
+// x_k in LDS (dinvT holds the transposed inverse, upper triangular); a task
+// with `write` stores x_k' into the solution, the others update a chunk of at
+// most 64 earlier columns j in [r0, r1): x_j -= sum_r L[k0 + r, j] x_k'[r],
+// the kb rows streamed through LDS in chunks of 64.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_bwd_blk(const SolveTask* __restrict__ tasks,
+                                                      const NodeDesc* __restrict__ nodes,
+                                                      const T* __restrict__ arena,
+                                                      const T* __restrict__ dinvT,
+                                                      const int* __restrict__ rlist,
+                                                      double* __restrict__ xloc,
+                                                      double* __restrict__ xsol, int ldsol, int nrhs) {
+   constexpr int RC = 64, JC = 64;
+   constexpr int AE = RC * JC / kThreads;
+   __shared__ double xs[kNB * 16];
+   __shared__ double ys[kNB * 16];
+   __shared__ T As[RC][JC + 1];
+   const SolveTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
+   const int m = nd.m, kb = t.kb;
+   double* xl = xloc + nd.solve_off * nrhs;
+   const T* Mk = dinvT + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
+   const T* F = arena + nd.off;
+   const int jj = threadIdx.x & 63, cq = threadIdx.x >> 6;
+   for (int c0 = 0; c0 < nrhs; c0 += 16) {
+      const int nc = min(16, nrhs - c0);
+      __syncthreads();
+#pragma unroll
+      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
+         const int l = e >> 4, c = e & 15;
+         xs[e] = *live_or_zero(l < kb && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
+      }
+      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) ys[e] = 0.0;
+      __syncthreads();
+      block_apply<T, true>(Mk, kb, xs, ys);
+      __syncthreads();
+      if (t.write) {
+         for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
+            const int i = e % kb, c = e / kb;
+            xsol[rlist[nd.rlist_off + t.k0 + i] - 1 + (int64_t)(c0 + c) * ldsol] = ys[i * 16 + c];
+         }
+         continue;
+      }
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      T ra[AE];
+      auto load = [&](int rb0) {
+#pragma unroll
+         for (int u = 0; u < AE; ++u) {
+            const int e = threadIdx.x + u * kThreads;
+            const int rr = e % RC, jc = e / RC;
+            const int r = rb0 + rr, j = t.r0 + jc;
+            ra[u] = *live_or_zero(r < kb && j < t.r1, F + t.k0 + r + (int64_t)j * nd.ld);
+         }
+      };
+      load(0);
+      for (int rb0 = 0; rb0 < kb; rb0 += RC) {
+         __syncthreads();
+#pragma unroll
+         for (int u = 0; u < AE; ++u) {
+            const int e = threadIdx.x + u * kThreads;
+            As[e % RC][e / RC] = ra[u];
+         }
+         __syncthreads();
+         if (rb0 + RC < kb) load(rb0 + RC);
+         double part[4] = {0.0, 0.0, 0.0, 0.0};
+         const int rn = min(RC, kb - rb0);
+         for (int rr = 0; rr < rn; ++rr) {
+            const double a = (double)As[rr][jj];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) part[q] += a * ys[(rb0 + rr) * 16 + 4 * cq + q];
+         }
+#pragma unroll
+         for (int q = 0; q < 4; ++q) acc[q] += part[q];
+      }
+      const int j = t.r0 + jj;
+      double cv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+         cv[q] = *live_or_zero(j < t.r1 && 4 * cq + q < nc, xl + j + (int64_t)(c0 + 4 * cq + q) * m);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+         if (j < t.r1 && 4 * cq + q < nc) xl[j + (int64_t)(c0 + 4 * cq + q) * m] = cv[q] - acc[q];
+   }
+}
+
 // Forward update of rows [r0, r1): x_r -= L[r, k0:k0+kb] y, 16-row MFMA tiles.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_fwd_upd(const SolveTask* __restrict__ tasks,

@@ -374,25 +374,20 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       push(P->fwdprog, L_FGATHER, g0, P->stask.size() - g0);
       int maxblk = 0;
       for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
+      // one launch per block column: the diagonal multiply (write tasks) and
+      // the update of the rows below, every task forming y = inv(L_kk) x_k
       for (int sblk = 0; sblk < maxblk; ++sblk) {
          int64_t t0 = P->stask.size();
          for (int i : lv) {
             const NodeDesc& d = P->hnodes[i];
             const int k0 = sblk * kNB;
             if (k0 >= d.n) continue;
-            P->stask.push_back(SolveTask{i, k0, std::min(kNB, d.n - k0), 0, 0, 1});
-         }
-         push(P->fwdprog, L_FDIAG, t0, P->stask.size() - t0);
-         int64_t u0 = P->stask.size();
-         for (int i : lv) {
-            const NodeDesc& d = P->hnodes[i];
-            const int k0 = sblk * kNB;
-            if (k0 >= d.n) continue;
             const int kb = std::min(kNB, d.n - k0);
+            P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
             for (int r = k0 + kb; r < d.m; r += kRowChunk)
                P->stask.push_back(SolveTask{i, k0, kb, r, std::min(d.m, r + kRowChunk), 0});
          }
-         push(P->fwdprog, L_FUPD, u0, P->stask.size() - u0);
+         push(P->fwdprog, L_FDIAG, t0, P->stask.size() - t0);
       }
    }
    for (int L = nlev - 1; L >= 0; --L) {
@@ -415,26 +410,21 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       push(P->bwdprog, L_BUPD, c0, P->stask.size() - c0);
       int maxblk = 0;
       for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
+      // one launch per block column (from the last): the transposed diagonal
+      // multiply (write tasks) and the update of the earlier columns by the
+      // block's rows, every task forming x_k' = inv(L_kk)^T x_k
       for (int sblk = maxblk - 1; sblk >= 0; --sblk) {
          int64_t t0 = P->stask.size();
          for (int i : lv) {
             const NodeDesc& d = P->hnodes[i];
             const int k0 = sblk * kNB;
             if (k0 >= d.n) continue;
-            P->stask.push_back(SolveTask{i, k0, std::min(kNB, d.n - k0), 0, 0, 1});
+            const int kb = std::min(kNB, d.n - k0);
+            P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
+            for (int j = 0; j < k0; j += kColChunk)
+               P->stask.push_back(SolveTask{i, k0, kb, j, std::min(k0, j + kColChunk), 0});
          }
          push(P->bwdprog, L_BDIAG, t0, P->stask.size() - t0);
-         int64_t u0 = P->stask.size();
-         for (int i : lv) {
-            const NodeDesc& d = P->hnodes[i];
-            const int k0 = sblk * kNB;
-            if (k0 >= d.n || k0 == 0) continue;
-            const int kb = std::min(kNB, d.n - k0);
-            // rows [k0, k0+kb) against the earlier columns [0, k0)
-            for (int j = 0; j < k0; j += kColChunk)
-               P->stask.push_back(SolveTask{i, k0, k0 + kb, j, std::min(k0, j + kColChunk), 0});
-         }
-         push(P->bwdprog, L_BUPD, u0, P->stask.size() - u0);
       }
    }
 
@@ -568,8 +558,7 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
             k_fwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_rlist, P->d_cmap, b, n, F->d_xloc, nrhs);
             break;
          case L_FDIAG:
-            k_solve_diag<T, false><<<g, bl, 0, s>>>(tk, P->d_nodes, dinv, P->d_rlist, F->d_xloc, xsol, n,
-                                                    nrhs);
+            k_fwd_blk<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinv, P->d_rlist, F->d_xloc, xsol, n, nrhs);
             break;
          case L_FUPD:
             k_fwd_upd<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, F->d_xloc, nrhs);
@@ -578,8 +567,7 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
             k_bwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_rlist, xsol, n, F->d_xloc, nrhs);
             break;
          case L_BDIAG:
-            k_solve_diag<T, true><<<g, bl, 0, s>>>(tk, P->d_nodes, dinvT, P->d_rlist, F->d_xloc, xsol, n,
-                                                   nrhs);
+            k_bwd_blk<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinvT, P->d_rlist, F->d_xloc, xsol, n, nrhs);
             break;
          case L_BUPD:
             k_bwd_upd<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, F->d_xloc, nrhs);
@@ -593,6 +581,7 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
 
 void ensure_solve_ws(const DevicePlan* P, DeviceFactors* F, int nrhs) {
    if (F->ws_nrhs >= nrhs) return;
+   F->clear_solve_graphs();  // captured with the old buffers
    for (void* p : {(void*)F->d_xloc, (void*)F->d_b, (void*)F->d_x, (void*)F->d_r, (void*)F->d_sol,
                    (void*)F->d_norms})
       dfree(p);
@@ -774,8 +763,8 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
 namespace {
 
 template <typename T>
-void solve_once(const DevicePlan* P, DeviceFactors* F, int job, const double* b, double* xsol,
-                int nrhs, hipStream_t s) {
+void solve_once_eager(const DevicePlan* P, DeviceFactors* F, int job, const double* b, double* xsol,
+                      int nrhs, hipStream_t s) {
    if (job == 0 || job == 1) run_solve_program<T>(P, F, P->fwdprog, b, xsol, nrhs, s);
    if (job == 3) {
       // backward only: the input is the current solution vector
@@ -793,6 +782,32 @@ void solve_once(const DevicePlan* P, DeviceFactors* F, int job, const double* b,
          run_solve_program<T>(P, F, P->bwdprog, xsol, xsol, nrhs, s);
       }
    }
+}
+
+// The solve program replayed from a hipGraph (one capture per job, buffers
+// and right-hand-side count; refinement replays the same graph every step).
+template <typename T>
+void solve_once(const DevicePlan* P, DeviceFactors* F, int job, const double* b, double* xsol,
+                int nrhs, hipStream_t s) {
+   if (!ext_options().use_graph) {
+      solve_once_eager<T>(P, F, job, b, xsol, nrhs, s);
+      return;
+   }
+   const int es = (int)sizeof(T);
+   for (const auto& g : F->sgraphs)
+      if (g.job == job && g.nrhs == nrhs && g.es == es && g.b == b && g.x == xsol) {
+         HIPCHECK(hipGraphLaunch(g.exec, s));
+         return;
+      }
+   hipGraph_t graph;
+   HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+   solve_once_eager<T>(P, F, job, b, xsol, nrhs, s);
+   HIPCHECK(hipStreamEndCapture(s, &graph));
+   hipGraphExec_t exec;
+   HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+   (void)hipGraphDestroy(graph);
+   F->sgraphs.push_back(DeviceFactors::SolveGraph{job, nrhs, es, b, xsol, exec});
+   HIPCHECK(hipGraphLaunch(exec, s));
 }
 
 // Scaled backward error of x for the system A x = b, worst over columns.

@@ -179,8 +179,21 @@ struct DeviceFactors {
    int kind_launches[16] = {0};
    // indefinite (APP LDL^T) factors; null on the posdef path
    IndefState* ind = nullptr;
+   // captured solve programs, keyed by job, buffers and right-hand sides
+   struct SolveGraph {
+      int job, nrhs, es;
+      const double* b;
+      double* x;
+      hipGraphExec_t exec;
+   };
+   std::vector<SolveGraph> sgraphs;
+   void clear_solve_graphs() {
+      for (auto& g : sgraphs) (void)hipGraphExecDestroy(g.exec);
+      sgraphs.clear();
+   }
 
    ~DeviceFactors() {
+      clear_solve_graphs();
       if (ind) destroy_indef(ind);
       if (graph) (void)hipGraphExecDestroy(graph);
       for (void* p : {arena, dinv, dinvT, (void*)d_aval, (void*)d_scaling, (void*)d_csr_val,
