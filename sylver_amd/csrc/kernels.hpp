@@ -953,65 +953,6 @@ __device__ inline v4d mma_rows_gl(const T* __restrict__ A, int64_t lda, int rval
    return acc;
 }
 
-// y = M x_k for the kb x kb triangular block M (ld kb) of dinv (forward:
-// inv(L_kk); backward: its transpose); y overwrites x_k in the work vector
-// and is written to the solution. Waves take 16-row tiles of y.
-template <typename T, bool UPPER>
-__global__ __launch_bounds__(kThreads) void k_solve_diag(const SolveTask* __restrict__ tasks,
-                                                         const NodeDesc* __restrict__ nodes,
-                                                         const T* __restrict__ dinv,
-                                                         const int* __restrict__ rlist,
-                                                         double* __restrict__ xloc,
-                                                         double* __restrict__ xsol, int ldsol,
-                                                         int nrhs) {
-   __shared__ double xs[kNB * 16];
-   const SolveTask t = tasks[blockIdx.x];
-   const NodeDesc nd = nodes[t.node];
-   const int m = nd.m, kb = t.kb;
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-   double* xl = xloc + nd.solve_off * nrhs;
-   const T* Mk = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
-   for (int c0 = 0; c0 < nrhs; c0 += 16) {
-      const int nc = min(16, nrhs - c0);
-      __syncthreads();
-#pragma unroll
-      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
-         const int l = e >> 4, c = e & 15;
-         xs[e] = *live_or_zero(l < kb && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
-      }
-      __syncthreads();
-      v4d acc[2];
-      const int ntile = (kb + 15) / 16;
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-         acc[q] = v4d{0, 0, 0, 0};
-         const int tile = wave + 4 * q;
-         if (tile < ntile) {
-            // lower: columns [0, tile*16+16); upper: columns [tile*16, kb)
-            const int kbeg = UPPER ? tile * 16 : 0;
-            const int kend = UPPER ? kb : min(kb, tile * 16 + 16);
-            acc[q] = mma_rows_gl<T>(Mk + tile * 16 + (int64_t)kbeg * kb, kb, kb - tile * 16, kend - kbeg,
-                                    xs + kbeg * 16, acc[q]);
-         }
-      }
-      __syncthreads();  // every wave has read x_k before it is overwritten
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-         const int tile = wave + 4 * q;
-         if (tile >= ntile) continue;
-         const int c = lane & 15;
-         if (c >= nc) continue;
-#pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int i = tile * 16 + Mfma<double>::row(lane, v);
-            if (i >= kb) continue;
-            xl[t.k0 + i + (int64_t)(c0 + c) * m] = acc[q][v];
-            xsol[rlist[nd.rlist_off + t.k0 + i] - 1 + (int64_t)(c0 + c) * ldsol] = acc[q][v];
-         }
-      }
-   }
-}
-
 // ys = M xs for the kb x kb triangular block M (ld kb; lower when !UPPER,
 // upper when UPPER), xs and ys row-major kb x 16 in LDS; 16-row tiles, two
 // per wave.
@@ -1036,10 +977,65 @@ __device__ inline void block_apply(const T* __restrict__ Mk, int kb, const doubl
    }
 }
 
-// Forward block step, one launch per block column: every task stages x_k
-// (rows [k0, k0+kb) of the node's work vector, not modified by this launch)
-// and forms y = inv(L_kk) x_k in LDS; a task with `write` stores y into the
-// solution, the others update their rows [r0, r1): x_r -= L[r, k0:k0+kb] y.
+// acc(16 x 16) += A^T(16 rows x K) * B(K x 16) with A^T[i][k] = A[k + i*lda]
+// (rows >= rvalid of A^T read as zero); B from LDS, row-major K x 16.
+template <typename T>
+__device__ inline v4d mma_rowsT_gl(const T* __restrict__ A, int64_t lda, int rvalid, int K,
+                                   const double* Bs, v4d acc) {
+   const int lane = threadIdx.x & 63;
+   const int r = lane & 15, kq = lane >> 4;
+   for (int k0 = 0; k0 < K; k0 += 32) {
+      T raw[8];
+      double a[8], b[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+         const int kk = k0 + 4 * u + kq;
+         raw[u] = *live_or_zero(r < rvalid && kk < K, A + kk + (int64_t)r * lda);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+         const int kk = k0 + 4 * u + kq;
+         b[u] = (kk < K) ? Bs[kk * 16 + r] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = (double)raw[u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+   }
+   return acc;
+}
+
+// ys[0:kb] -= L21 xs for the coupling block L21 (kb x kk, column-major ld)
+// of two diagonal blocks, or its transpose (ys[0:kk] -= L21^T xs) when TR;
+// 16-row tiles, two per wave. xs/ys row-major x 16 in LDS.
+template <typename T, bool TR>
+__device__ inline void block_couple(const T* __restrict__ L21, int64_t ld, int rows, int kdim,
+                                    const double* xs, double* ys) {
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int ntile = (rows + 15) / 16;
+#pragma unroll
+   for (int q = 0; q < 2; ++q) {
+      const int tile = wave + 4 * q;
+      if (tile >= ntile) continue;
+      v4d acc = v4d{0, 0, 0, 0};
+      if (TR)
+         acc = mma_rowsT_gl<T>(L21 + (int64_t)tile * 16 * ld, ld, rows - tile * 16, kdim, xs, acc);
+      else
+         acc = mma_rows_gl<T>(L21 + tile * 16, ld, rows - tile * 16, kdim, xs, acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int i = tile * 16 + Mfma<double>::row(lane, v);
+         if (i < rows) ys[i * 16 + (lane & 15)] -= acc[v];
+      }
+   }
+}
+
+// Forward step over a pair of diagonal blocks (columns [k0, k0+kb), kb <= 256;
+// the first block is 128 wide unless it is the last): every task stages x
+// for the pair (not modified by this launch) and forms
+//    y1 = inv(L11) x1,  y2 = inv(L22) (x2 - L21 y1)
+// in LDS; a task with `write` stores y into the solution, the others update
+// their rows [r0, r1): x_r -= L[r, k0:k0+kb] y.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_fwd_blk(const SolveTask* __restrict__ tasks,
                                                       const NodeDesc* __restrict__ nodes,
@@ -1048,28 +1044,36 @@ __global__ __launch_bounds__(kThreads) void k_fwd_blk(const SolveTask* __restric
                                                       const int* __restrict__ rlist,
                                                       double* __restrict__ xloc,
                                                       double* __restrict__ xsol, int ldsol, int nrhs) {
-   __shared__ double xs[kNB * 16];
-   __shared__ double ys[kNB * 16];
+   __shared__ double xs[2 * kNB * 16];
+   __shared__ double ys[2 * kNB * 16];
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
-   const int m = nd.m, kb = t.kb;
+   const int m = nd.m, kb = t.kb, kb1 = min(kNB, kb), kb2 = kb - kb1;
    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    double* xl = xloc + nd.solve_off * nrhs;
-   const T* Mk = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
+   const T* M1 = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
+   const T* M2 = M1 + (int64_t)kNB * kNB;
    const T* F = arena + nd.off;
    const int ntile = (t.r1 - t.r0 + 15) / 16;
    for (int c0 = 0; c0 < nrhs; c0 += 16) {
       const int nc = min(16, nrhs - c0);
       __syncthreads();
 #pragma unroll
-      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
+      for (int e = threadIdx.x; e < 2 * kNB * 16; e += kThreads) {
          const int l = e >> 4, c = e & 15;
          xs[e] = *live_or_zero(l < kb && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
+         ys[e] = 0.0;
       }
-      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) ys[e] = 0.0;
       __syncthreads();
-      block_apply<T, false>(Mk, kb, xs, ys);
+      block_apply<T, false>(M1, kb1, xs, ys);
       __syncthreads();
+      if (kb2 > 0) {
+         // xs2 -= L21 y1, then y2 = inv(L22) xs2
+         block_couple<T, false>(F + t.k0 + kb1 + (int64_t)t.k0 * nd.ld, nd.ld, kb2, kb1, ys, xs + kb1 * 16);
+         __syncthreads();
+         block_apply<T, false>(M2, kb2, xs + kb1 * 16, ys + kb1 * 16);
+         __syncthreads();
+      }
       if (t.write) {
          for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
             const int i = e % kb, c = e / kb;
@@ -1097,11 +1101,13 @@ __global__ __launch_bounds__(kThreads) void k_fwd_blk(const SolveTask* __restric
    }
 }
 
-// Backward block step: every task stages x_k and forms x_k' = inv(L_kk)^T
-// x_k in LDS (dinvT holds the transposed inverse, upper triangular); a task
-// with `write` stores x_k' into the solution, the others update a chunk of at
-// most 64 earlier columns j in [r0, r1): x_j -= sum_r L[k0 + r, j] x_k'[r],
-// the kb rows streamed through LDS in chunks of 64.
+// Backward step over a pair of diagonal blocks (columns [k0, k0+kb)): every
+// task stages x for the pair and forms
+//    x2' = inv(L22)^T x2,  x1' = inv(L11)^T (x1 - L21^T x2')
+// in LDS (dinvT holds the transposed inverses); a task with `write` stores x'
+// into the solution, the others update a chunk of at most 64 earlier columns
+// j in [r0, r1): x_j -= sum_r L[k0 + r, j] x'[r], rows streamed through LDS in
+// chunks of 64.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_bwd_blk(const SolveTask* __restrict__ tasks,
                                                       const NodeDesc* __restrict__ nodes,
@@ -1112,27 +1118,35 @@ __global__ __launch_bounds__(kThreads) void k_bwd_blk(const SolveTask* __restric
                                                       double* __restrict__ xsol, int ldsol, int nrhs) {
    constexpr int RC = 64, JC = 64;
    constexpr int AE = RC * JC / kThreads;
-   __shared__ double xs[kNB * 16];
-   __shared__ double ys[kNB * 16];
+   __shared__ double xs[2 * kNB * 16];
+   __shared__ double ys[2 * kNB * 16];
    __shared__ T As[RC][JC + 1];
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
-   const int m = nd.m, kb = t.kb;
+   const int m = nd.m, kb = t.kb, kb1 = min(kNB, kb), kb2 = kb - kb1;
    double* xl = xloc + nd.solve_off * nrhs;
-   const T* Mk = dinvT + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
+   const T* M1 = dinvT + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
+   const T* M2 = M1 + (int64_t)kNB * kNB;
    const T* F = arena + nd.off;
    const int jj = threadIdx.x & 63, cq = threadIdx.x >> 6;
    for (int c0 = 0; c0 < nrhs; c0 += 16) {
       const int nc = min(16, nrhs - c0);
       __syncthreads();
 #pragma unroll
-      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
+      for (int e = threadIdx.x; e < 2 * kNB * 16; e += kThreads) {
          const int l = e >> 4, c = e & 15;
          xs[e] = *live_or_zero(l < kb && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
+         ys[e] = 0.0;
       }
-      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) ys[e] = 0.0;
       __syncthreads();
-      block_apply<T, true>(Mk, kb, xs, ys);
+      if (kb2 > 0) {
+         block_apply<T, true>(M2, kb2, xs + kb1 * 16, ys + kb1 * 16);
+         __syncthreads();
+         // xs1 -= L21^T x2'
+         block_couple<T, true>(F + t.k0 + kb1 + (int64_t)t.k0 * nd.ld, nd.ld, kb1, kb2, ys + kb1 * 16, xs);
+         __syncthreads();
+      }
+      block_apply<T, true>(M1, kb1, xs, ys);
       __syncthreads();
       if (t.write) {
          for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
@@ -1183,73 +1197,29 @@ __global__ __launch_bounds__(kThreads) void k_bwd_blk(const SolveTask* __restric
    }
 }
 
-// Forward update of rows [r0, r1): x_r -= L[r, k0:k0+kb] y, 16-row MFMA tiles.
+// Backward contribution-block update split over row chunks: task (node,
+// rows [k0, kb), columns [r0, r1) with r1 - r0 <= 64, slot `write`) stores
+// part[slot] = L[rows, cols]^T x[rows] (64 x nrhs) without touching x; the
+// reduction below subtracts the partial sums of a column chunk in slot order.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fwd_upd(const SolveTask* __restrict__ tasks,
+__global__ __launch_bounds__(kThreads) void k_bwd_cbp(const SolveTask* __restrict__ tasks,
                                                       const NodeDesc* __restrict__ nodes,
                                                       const T* __restrict__ arena,
-                                                      double* __restrict__ xloc, int nrhs) {
-   __shared__ double ys[kNB * 16];
-   const SolveTask t = tasks[blockIdx.x];
-   const NodeDesc nd = nodes[t.node];
-   const int m = nd.m, kb = t.kb;
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-   double* xl = xloc + nd.solve_off * nrhs;
-   const T* F = arena + nd.off;
-   const int R = t.r1 - t.r0;
-   const int ntile = (R + 15) / 16;
-   for (int c0 = 0; c0 < nrhs; c0 += 16) {
-      const int nc = min(16, nrhs - c0);
-      __syncthreads();
-#pragma unroll
-      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
-         const int l = e >> 4, c = e & 15;
-         ys[e] = *live_or_zero(l < kb && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
-      }
-      __syncthreads();
-      for (int tile = wave; tile < ntile; tile += 4) {
-         const int rb = t.r0 + tile * 16;
-         v4d acc = v4d{0, 0, 0, 0};
-         acc = mma_rows_gl<T>(F + rb + (int64_t)t.k0 * nd.ld, nd.ld, t.r1 - rb, kb, ys, acc);
-         const int c = lane & 15;
-         double cv[4];
-#pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int r = rb + Mfma<double>::row(lane, v);
-            cv[v] = *live_or_zero(c < nc && r < t.r1, xl + r + (int64_t)(c0 + c) * m);
-         }
-#pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int r = rb + Mfma<double>::row(lane, v);
-            if (c < nc && r < t.r1) xl[r + (int64_t)(c0 + c) * m] = cv[v] - acc[v];
-         }
-      }
-   }
-}
-
-// Transposed update x_j -= sum_{r in [k0, kb)} L[r, j] x_r for the columns
-// j in [r0, r1) (at most 64 per task): the front rows [k0, kb) (here k0 is the
-// first row and kb one past the last) are streamed through LDS in chunks of
-// 64 rows, the next chunk loaded into registers while the current one is
-// used; each thread owns one column and four right-hand sides and keeps a
-// partial sum per chunk.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd_upd(const SolveTask* __restrict__ tasks,
-                                                      const NodeDesc* __restrict__ nodes,
-                                                      const T* __restrict__ arena,
-                                                      double* __restrict__ xloc, int nrhs) {
+                                                      const double* __restrict__ xloc,
+                                                      double* __restrict__ part, int nrhs) {
    constexpr int RC = 64, JC = 64;
-   constexpr int AE = RC * JC / kThreads;            // A elements per thread per chunk
-   constexpr int BE = RC * kMaxRhsChunk / kThreads;  // B elements per thread per chunk
+   constexpr int AE = RC * JC / kThreads;
+   constexpr int BE = RC * kMaxRhsChunk / kThreads;
    __shared__ T As[RC][JC + 1];
    __shared__ double Bs[RC][kMaxRhsChunk];
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    const int m = nd.m;
-   double* xl = xloc + nd.solve_off * nrhs;
+   const double* xl = xloc + nd.solve_off * nrhs;
    const T* F = arena + nd.off;
    const int jj = threadIdx.x & 63, cq = threadIdx.x >> 6;
    const int rbeg = t.k0, rend = t.kb;
+   double* out = part + (int64_t)t.write * 64 * nrhs;
    for (int c0 = 0; c0 < nrhs; c0 += kMaxRhsChunk) {
       const int nc = min(kMaxRhsChunk, nrhs - c0);
       double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1286,24 +1256,37 @@ __global__ __launch_bounds__(kThreads) void k_bwd_upd(const SolveTask* __restric
          }
          __syncthreads();
          if (rb0 + RC < rend) load(rb0 + RC);
-         double part[4] = {0.0, 0.0, 0.0, 0.0};
+         double p4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 16
          for (int rr = 0; rr < RC; ++rr) {
             const double a = (double)As[rr][jj];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) part[q] += a * Bs[rr][4 * cq + q];
+            for (int q = 0; q < 4; ++q) p4[q] += a * Bs[rr][4 * cq + q];
          }
 #pragma unroll
-         for (int q = 0; q < 4; ++q) acc[q] += part[q];
+         for (int q = 0; q < 4; ++q) acc[q] += p4[q];
       }
-      const int j = t.r0 + jj;
-      double cv[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-         cv[q] = *live_or_zero(j < t.r1 && 4 * cq + q < nc, xl + j + (int64_t)(c0 + 4 * cq + q) * m);
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-         if (j < t.r1 && 4 * cq + q < nc) xl[j + (int64_t)(c0 + 4 * cq + q) * m] = cv[q] - acc[q];
+         if (4 * cq + q < nc) out[(int64_t)jj * nrhs + c0 + 4 * cq + q] = acc[q];
+   }
+}
+
+// x_j -= sum over slots [k0, k0 + kb) of part[slot][j - r0] for the columns
+// [r0, r1) of one node, slots summed in order.
+static __global__ __launch_bounds__(kThreads) void k_bwd_cbr(const SolveTask* __restrict__ tasks,
+                                                      const NodeDesc* __restrict__ nodes,
+                                                      const double* __restrict__ part,
+                                                      double* __restrict__ xloc, int nrhs) {
+   const SolveTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
+   double* xl = xloc + nd.solve_off * nrhs;
+   const int w = t.r1 - t.r0;
+   for (int e = threadIdx.x; e < w * nrhs; e += kThreads) {
+      const int jj = e % w, c = e / w;
+      double s = 0.0;
+      for (int k = 0; k < t.kb; ++k) s += part[((int64_t)(t.k0 + k) * 64 + jj) * nrhs + c];
+      xl[t.r0 + jj + (int64_t)c * nd.m] -= s;
    }
 }
 

@@ -363,6 +363,8 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    // of the earlier rows.
    constexpr int kRowChunk = 256;
    constexpr int kColChunk = 64;
+   constexpr int kCbRowChunk = 512;
+   constexpr int kSB2 = 2 * kNB;  // solve step: a pair of 128-column diagonal blocks
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
       int64_t g0 = P->stask.size();
@@ -373,16 +375,16 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       }
       push(P->fwdprog, L_FGATHER, g0, P->stask.size() - g0);
       int maxblk = 0;
-      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
-      // one launch per block column: the diagonal multiply (write tasks) and
-      // the update of the rows below, every task forming y = inv(L_kk) x_k
+      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kSB2 - 1) / kSB2);
+      // one launch per pair of diagonal blocks: the diagonal solves (write
+      // tasks) and the update of the rows below, every task forming y
       for (int sblk = 0; sblk < maxblk; ++sblk) {
          int64_t t0 = P->stask.size();
          for (int i : lv) {
             const NodeDesc& d = P->hnodes[i];
-            const int k0 = sblk * kNB;
+            const int k0 = sblk * kSB2;
             if (k0 >= d.n) continue;
-            const int kb = std::min(kNB, d.n - k0);
+            const int kb = std::min(kSB2, d.n - k0);
             P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
             for (int r = k0 + kb; r < d.m; r += kRowChunk)
                P->stask.push_back(SolveTask{i, k0, kb, r, std::min(d.m, r + kRowChunk), 0});
@@ -399,27 +401,39 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             P->stask.push_back(SolveTask{i, 0, 0, r, std::min(d.m, r + kRowChunk), 0});
       }
       push(P->bwdprog, L_BGATHER, g0, P->stask.size() - g0);
-      // contribution-block part: columns [0, n) against rows [n, m)
+      // contribution-block part: columns [0, n) against rows [n, m), split
+      // into row chunks (partial sums per slot) and reduced in slot order
       int64_t c0 = P->stask.size();
+      std::vector<SolveTask> red;
+      int slots = 0;
       for (int i : lv) {
          const NodeDesc& d = P->hnodes[i];
          if (d.m - d.n <= 0 || d.n == 0) continue;
-         for (int j = 0; j < d.n; j += kColChunk)
-            P->stask.push_back(SolveTask{i, d.n, d.m, j, std::min(d.n, j + kColChunk), 0});
+         for (int j = 0; j < d.n; j += kColChunk) {
+            const int s0 = slots;
+            for (int r = d.n; r < d.m; r += kCbRowChunk)
+               P->stask.push_back(SolveTask{i, r, std::min(d.m, r + kCbRowChunk), j,
+                                            std::min(d.n, j + kColChunk), slots++});
+            red.push_back(SolveTask{i, s0, slots - s0, j, std::min(d.n, j + kColChunk), 0});
+         }
       }
+      P->cb_slots = std::max<int64_t>(P->cb_slots, slots);
       push(P->bwdprog, L_BUPD, c0, P->stask.size() - c0);
+      int64_t r0 = P->stask.size();
+      P->stask.insert(P->stask.end(), red.begin(), red.end());
+      push(P->bwdprog, L_BRED, r0, P->stask.size() - r0);
       int maxblk = 0;
-      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
-      // one launch per block column (from the last): the transposed diagonal
-      // multiply (write tasks) and the update of the earlier columns by the
-      // block's rows, every task forming x_k' = inv(L_kk)^T x_k
+      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kSB2 - 1) / kSB2);
+      // one launch per pair of diagonal blocks (from the last): the transposed
+      // diagonal solves (write tasks) and the update of the earlier columns by
+      // the pair's rows, every task forming x'
       for (int sblk = maxblk - 1; sblk >= 0; --sblk) {
          int64_t t0 = P->stask.size();
          for (int i : lv) {
             const NodeDesc& d = P->hnodes[i];
-            const int k0 = sblk * kNB;
+            const int k0 = sblk * kSB2;
             if (k0 >= d.n) continue;
-            const int kb = std::min(kNB, d.n - k0);
+            const int kb = std::min(kSB2, d.n - k0);
             P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
             for (int j = 0; j < k0; j += kColChunk)
                P->stask.push_back(SolveTask{i, k0, kb, j, std::min(k0, j + kColChunk), 0});
@@ -560,9 +574,6 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
          case L_FDIAG:
             k_fwd_blk<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinv, P->d_rlist, F->d_xloc, xsol, n, nrhs);
             break;
-         case L_FUPD:
-            k_fwd_upd<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, F->d_xloc, nrhs);
-            break;
          case L_BGATHER:
             k_bwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_rlist, xsol, n, F->d_xloc, nrhs);
             break;
@@ -570,7 +581,10 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
             k_bwd_blk<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinvT, P->d_rlist, F->d_xloc, xsol, n, nrhs);
             break;
          case L_BUPD:
-            k_bwd_upd<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, F->d_xloc, nrhs);
+            if (L.count > 0) k_bwd_cbp<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, F->d_xloc, F->d_part_cb, nrhs);
+            break;
+         case L_BRED:
+            if (L.count > 0) k_bwd_cbr<<<g, bl, 0, s>>>(tk, P->d_nodes, F->d_part_cb, F->d_xloc, nrhs);
             break;
          default:
             break;
@@ -583,9 +597,10 @@ void ensure_solve_ws(const DevicePlan* P, DeviceFactors* F, int nrhs) {
    if (F->ws_nrhs >= nrhs) return;
    F->clear_solve_graphs();  // captured with the old buffers
    for (void* p : {(void*)F->d_xloc, (void*)F->d_b, (void*)F->d_x, (void*)F->d_r, (void*)F->d_sol,
-                   (void*)F->d_norms})
+                   (void*)F->d_norms, (void*)F->d_part_cb})
       dfree(p);
    F->d_xloc = dalloc<double>((size_t)P->solve_rows * nrhs);
+   F->d_part_cb = dalloc<double>((size_t)std::max<int64_t>(P->cb_slots, 1) * 64 * nrhs);
    F->d_b = dalloc<double>((size_t)P->n * nrhs);
    F->d_x = dalloc<double>((size_t)P->n * nrhs);
    F->d_r = dalloc<double>((size_t)P->n * nrhs);

@@ -60,7 +60,8 @@ enum LaunchKind {
    L_FUPD,
    L_BGATHER,
    L_BDIAG,
-   L_BUPD
+   L_BUPD,
+   L_BRED
 };
 
 struct Launch {
@@ -122,6 +123,7 @@ struct DevicePlan {
    SolveTask* d_stask = nullptr;
    std::vector<Launch> fprog, fwdprog, bwdprog;
    int64_t num_flops = 0;
+   int64_t cb_slots = 0;  // partial-sum slots of the largest backward CB launch
    // CSR of the whole matrix in pivot order, values gathered per factorize
    int64_t csr_nnz = 0;
    int64_t* d_csr_ptr = nullptr;
@@ -161,6 +163,7 @@ struct DeviceFactors {
    double* d_x = nullptr;
    double* d_r = nullptr;
    double* d_sol = nullptr;
+   double* d_part_cb = nullptr;  // backward CB partial sums: cb_slots x 64 x nrhs
    unsigned long long* d_norms = nullptr;
    // GMRES-IR workspace (allocated on first use)
    int gm_nrhs = 0, gm_m = 0;
@@ -196,6 +199,7 @@ struct DeviceFactors {
       clear_solve_graphs();
       if (ind) destroy_indef(ind);
       if (graph) (void)hipGraphExecDestroy(graph);
+      dfree(d_part_cb);
       for (void* p : {arena, dinv, dinvT, (void*)d_aval, (void*)d_scaling, (void*)d_csr_val,
                       (void*)d_info, (void*)d_xloc, (void*)d_b, (void*)d_x, (void*)d_r,
                       (void*)d_sol, (void*)d_norms, (void*)d_V, (void*)d_W, (void*)d_tmp,
