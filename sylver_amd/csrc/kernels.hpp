@@ -923,277 +923,101 @@ static __global__ __launch_bounds__(kThreads) void k_bwd_gather(const SolveTask*
    }
 }
 
-// acc(16 x 16, FP64 MFMA layout) += A(16 rows x K) * B(K x 16): A from global
-// memory (column-major, ld lda; rows >= rvalid read as zero), converted to
-// FP64; B from LDS, row-major K x 16. Eight k-steps per batch keep eight
-// global loads in flight.
-template <typename T>
-__device__ inline v4d mma_rows_gl(const T* __restrict__ A, int64_t lda, int rvalid, int K,
-                                  const double* Bs, v4d acc) {
-   const int lane = threadIdx.x & 63;
-   const int r = lane & 15, kq = lane >> 4;
-   for (int k0 = 0; k0 < K; k0 += 32) {
-      double a[8], b[8];
-      T raw[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-         const int kk = k0 + 4 * u + kq;
-         raw[u] = *live_or_zero(r < rvalid && kk < K, A + r + (int64_t)kk * lda);
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-         const int kk = k0 + 4 * u + kq;
-         b[u] = (kk < K) ? Bs[kk * 16 + r] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] = (double)raw[u];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
-   }
-   return acc;
-}
-
-// ys = M xs for the kb x kb triangular block M (ld kb; lower when !UPPER,
-// upper when UPPER), xs and ys row-major kb x 16 in LDS; 16-row tiles, two
-// per wave.
-template <typename T, bool UPPER>
-__device__ inline void block_apply(const T* __restrict__ Mk, int kb, const double* xs, double* ys) {
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-   const int ntile = (kb + 15) / 16;
-#pragma unroll
-   for (int q = 0; q < 2; ++q) {
-      const int tile = wave + 4 * q;
-      if (tile >= ntile) continue;
-      const int kbeg = UPPER ? tile * 16 : 0;
-      const int kend = UPPER ? kb : min(kb, tile * 16 + 16);
-      v4d acc = v4d{0, 0, 0, 0};
-      acc = mma_rows_gl<T>(Mk + tile * 16 + (int64_t)kbeg * kb, kb, kb - tile * 16, kend - kbeg,
-                           xs + kbeg * 16, acc);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-         const int i = tile * 16 + Mfma<double>::row(lane, v);
-         if (i < kb) ys[i * 16 + (lane & 15)] = acc[v];
-      }
-   }
-}
-
-// acc(16 x 16) += A^T(16 rows x K) * B(K x 16) with A^T[i][k] = A[k + i*lda]
-// (rows >= rvalid of A^T read as zero); B from LDS, row-major K x 16.
-template <typename T>
-__device__ inline v4d mma_rowsT_gl(const T* __restrict__ A, int64_t lda, int rvalid, int K,
-                                   const double* Bs, v4d acc) {
-   const int lane = threadIdx.x & 63;
-   const int r = lane & 15, kq = lane >> 4;
-   for (int k0 = 0; k0 < K; k0 += 32) {
-      T raw[8];
-      double a[8], b[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-         const int kk = k0 + 4 * u + kq;
-         raw[u] = *live_or_zero(r < rvalid && kk < K, A + kk + (int64_t)r * lda);
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-         const int kk = k0 + 4 * u + kq;
-         b[u] = (kk < K) ? Bs[kk * 16 + r] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] = (double)raw[u];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
-   }
-   return acc;
-}
-
-// ys[0:kb] -= L21 xs for the coupling block L21 (kb x kk, column-major ld)
-// of two diagonal blocks, or its transpose (ys[0:kk] -= L21^T xs) when TR;
-// 16-row tiles, two per wave. xs/ys row-major x 16 in LDS.
-template <typename T, bool TR>
-__device__ inline void block_couple(const T* __restrict__ L21, int64_t ld, int rows, int kdim,
-                                    const double* xs, double* ys) {
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-   const int ntile = (rows + 15) / 16;
-#pragma unroll
-   for (int q = 0; q < 2; ++q) {
-      const int tile = wave + 4 * q;
-      if (tile >= ntile) continue;
-      v4d acc = v4d{0, 0, 0, 0};
-      if (TR)
-         acc = mma_rowsT_gl<T>(L21 + (int64_t)tile * 16 * ld, ld, rows - tile * 16, kdim, xs, acc);
-      else
-         acc = mma_rows_gl<T>(L21 + tile * 16, ld, rows - tile * 16, kdim, xs, acc);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-         const int i = tile * 16 + Mfma<double>::row(lane, v);
-         if (i < rows) ys[i * 16 + (lane & 15)] -= acc[v];
-      }
-   }
-}
-
-// Forward step over a pair of diagonal blocks (columns [k0, k0+kb), kb <= 256;
-// the first block is 128 wide unless it is the last): every task stages x
-// for the pair (not modified by this launch) and forms
-//    y1 = inv(L11) x1,  y2 = inv(L22) (x2 - L21 y1)
-// in LDS; a task with `write` stores y into the solution, the others update
-// their rows [r0, r1): x_r -= L[r, k0:k0+kb] y.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fwd_blk(const SolveTask* __restrict__ tasks,
-                                                      const NodeDesc* __restrict__ nodes,
-                                                      const T* __restrict__ arena,
-                                                      const T* __restrict__ dinv,
-                                                      const int* __restrict__ rlist,
-                                                      double* __restrict__ xloc,
-                                                      double* __restrict__ xsol, int ldsol, int nrhs) {
-   __shared__ double xs[2 * kNB * 16];
-   __shared__ double ys[2 * kNB * 16];
+// One 128-column step of the forward (BWD = false) or backward (BWD = true)
+// sweep. Every task needs y = M x_k with M = inv(L_kk) (forward) or
+// inv(L_kk)^T (backward, from dinvT), both stored full kb x kb with zeros in
+// the other triangle. The operands that do not depend on x (this wave's two
+// 16-row tiles of M and its 16-row tile of the update operand) are loaded
+// into registers before x is staged, so a step costs about two memory round
+// trips. A task with `write` stores y into the solution; forward update
+// tasks handle rows [r0, r1) (at most 64: one tile per wave),
+// x_r -= L[r, k0:k0+kb] y; backward update tasks handle columns [r0, r1),
+// x_j -= L[k0:k0+kb, j]^T y.
+template <typename T, bool BWD>
+__global__ __launch_bounds__(kThreads) void k_solve_step(const SolveTask* __restrict__ tasks,
+                                                         const NodeDesc* __restrict__ nodes,
+                                                         const T* __restrict__ arena,
+                                                         const T* __restrict__ dinvs,
+                                                         const int* __restrict__ rlist,
+                                                         double* __restrict__ xloc,
+                                                         double* __restrict__ xsol, int ldsol, int nrhs) {
+   constexpr int KU = kNB / 4;  // k-steps of the 16x16x4 MFMA over 128
+   __shared__ double xs[kNB * 16];
+   __shared__ double ys[kNB * 16];
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
-   const int m = nd.m, kb = t.kb, kb1 = min(kNB, kb), kb2 = kb - kb1;
+   const int m = nd.m, kb = t.kb;
    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int r = lane & 15, kq = lane >> 4;
    double* xl = xloc + nd.solve_off * nrhs;
-   const T* M1 = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
-   const T* M2 = M1 + (int64_t)kNB * kNB;
+   const T* Mk = dinvs + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
    const T* F = arena + nd.off;
-   const int ntile = (t.r1 - t.r0 + 15) / 16;
+   T mv[2][KU];
+#pragma unroll
+   for (int q = 0; q < 2; ++q) {
+      const int i = (wave + 4 * q) * 16 + r;
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+         const int k = 4 * u + kq;
+         mv[q][u] = *live_or_zero(i < kb && k < kb, Mk + i + (int64_t)k * kb);
+      }
+   }
+   T av[KU];
+   const int ui = t.r0 + wave * 16 + r;  // update row (forward) or column (backward) of this lane
+   if (!t.write) {
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+         const int k = 4 * u + kq;
+         const T* p = BWD ? F + t.k0 + k + (int64_t)ui * nd.ld : F + ui + (int64_t)(t.k0 + k) * nd.ld;
+         av[u] = *live_or_zero(ui < t.r1 && k < kb, p);
+      }
+   }
    for (int c0 = 0; c0 < nrhs; c0 += 16) {
       const int nc = min(16, nrhs - c0);
       __syncthreads();
 #pragma unroll
-      for (int e = threadIdx.x; e < 2 * kNB * 16; e += kThreads) {
+      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
          const int l = e >> 4, c = e & 15;
          xs[e] = *live_or_zero(l < kb && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
-         ys[e] = 0.0;
       }
-      __syncthreads();
-      block_apply<T, false>(M1, kb1, xs, ys);
-      __syncthreads();
-      if (kb2 > 0) {
-         // xs2 -= L21 y1, then y2 = inv(L22) xs2
-         block_couple<T, false>(F + t.k0 + kb1 + (int64_t)t.k0 * nd.ld, nd.ld, kb2, kb1, ys, xs + kb1 * 16);
-         __syncthreads();
-         block_apply<T, false>(M2, kb2, xs + kb1 * 16, ys + kb1 * 16);
-         __syncthreads();
-      }
-      if (t.write) {
-         for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
-            const int i = e % kb, c = e / kb;
-            xsol[rlist[nd.rlist_off + t.k0 + i] - 1 + (int64_t)(c0 + c) * ldsol] = ys[i * 16 + c];
-         }
-         continue;
-      }
-      for (int tile = wave; tile < ntile; tile += 4) {
-         const int rb = t.r0 + tile * 16;
-         v4d acc = v4d{0, 0, 0, 0};
-         acc = mma_rows_gl<T>(F + rb + (int64_t)t.k0 * nd.ld, nd.ld, t.r1 - rb, kb, ys, acc);
-         const int c = lane & 15;
-         double cv[4];
-#pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int r = rb + Mfma<double>::row(lane, v);
-            cv[v] = *live_or_zero(c < nc && r < t.r1, xl + r + (int64_t)(c0 + c) * m);
-         }
-#pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int r = rb + Mfma<double>::row(lane, v);
-            if (c < nc && r < t.r1) xl[r + (int64_t)(c0 + c) * m] = cv[v] - acc[v];
-         }
-      }
-   }
-}
-
-// Backward step over a pair of diagonal blocks (columns [k0, k0+kb)): every
-// task stages x for the pair and forms
-//    x2' = inv(L22)^T x2,  x1' = inv(L11)^T (x1 - L21^T x2')
-// in LDS (dinvT holds the transposed inverses); a task with `write` stores x'
-// into the solution, the others update a chunk of at most 64 earlier columns
-// j in [r0, r1): x_j -= sum_r L[k0 + r, j] x'[r], rows streamed through LDS in
-// chunks of 64.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd_blk(const SolveTask* __restrict__ tasks,
-                                                      const NodeDesc* __restrict__ nodes,
-                                                      const T* __restrict__ arena,
-                                                      const T* __restrict__ dinvT,
-                                                      const int* __restrict__ rlist,
-                                                      double* __restrict__ xloc,
-                                                      double* __restrict__ xsol, int ldsol, int nrhs) {
-   constexpr int RC = 64, JC = 64;
-   constexpr int AE = RC * JC / kThreads;
-   __shared__ double xs[2 * kNB * 16];
-   __shared__ double ys[2 * kNB * 16];
-   __shared__ T As[RC][JC + 1];
-   const SolveTask t = tasks[blockIdx.x];
-   const NodeDesc nd = nodes[t.node];
-   const int m = nd.m, kb = t.kb, kb1 = min(kNB, kb), kb2 = kb - kb1;
-   double* xl = xloc + nd.solve_off * nrhs;
-   const T* M1 = dinvT + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
-   const T* M2 = M1 + (int64_t)kNB * kNB;
-   const T* F = arena + nd.off;
-   const int jj = threadIdx.x & 63, cq = threadIdx.x >> 6;
-   for (int c0 = 0; c0 < nrhs; c0 += 16) {
-      const int nc = min(16, nrhs - c0);
-      __syncthreads();
-#pragma unroll
-      for (int e = threadIdx.x; e < 2 * kNB * 16; e += kThreads) {
-         const int l = e >> 4, c = e & 15;
-         xs[e] = *live_or_zero(l < kb && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
-         ys[e] = 0.0;
-      }
-      __syncthreads();
-      if (kb2 > 0) {
-         block_apply<T, true>(M2, kb2, xs + kb1 * 16, ys + kb1 * 16);
-         __syncthreads();
-         // xs1 -= L21^T x2'
-         block_couple<T, true>(F + t.k0 + kb1 + (int64_t)t.k0 * nd.ld, nd.ld, kb1, kb2, ys + kb1 * 16, xs);
-         __syncthreads();
-      }
-      block_apply<T, true>(M1, kb1, xs, ys);
-      __syncthreads();
-      if (t.write) {
-         for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
-            const int i = e % kb, c = e / kb;
-            xsol[rlist[nd.rlist_off + t.k0 + i] - 1 + (int64_t)(c0 + c) * ldsol] = ys[i * 16 + c];
-         }
-         continue;
-      }
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
-      T ra[AE];
-      auto load = [&](int rb0) {
-#pragma unroll
-         for (int u = 0; u < AE; ++u) {
-            const int e = threadIdx.x + u * kThreads;
-            const int rr = e % RC, jc = e / RC;
-            const int r = rb0 + rr, j = t.r0 + jc;
-            ra[u] = *live_or_zero(r < kb && j < t.r1, F + t.k0 + r + (int64_t)j * nd.ld);
-         }
-      };
-      load(0);
-      for (int rb0 = 0; rb0 < kb; rb0 += RC) {
-         __syncthreads();
-#pragma unroll
-         for (int u = 0; u < AE; ++u) {
-            const int e = threadIdx.x + u * kThreads;
-            As[e % RC][e / RC] = ra[u];
-         }
-         __syncthreads();
-         if (rb0 + RC < kb) load(rb0 + RC);
-         double part[4] = {0.0, 0.0, 0.0, 0.0};
-         const int rn = min(RC, kb - rb0);
-         for (int rr = 0; rr < rn; ++rr) {
-            const double a = (double)As[rr][jj];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) part[q] += a * ys[(rb0 + rr) * 16 + 4 * cq + q];
-         }
-#pragma unroll
-         for (int q = 0; q < 4; ++q) acc[q] += part[q];
-      }
-      const int j = t.r0 + jj;
+      // values this task updates, read before the arithmetic
       double cv[4];
+      const int ob = t.r0 + wave * 16;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-         cv[q] = *live_or_zero(j < t.r1 && 4 * cq + q < nc, xl + j + (int64_t)(c0 + 4 * cq + q) * m);
+      for (int v = 0; v < 4; ++v) {
+         const int o = ob + Mfma<double>::row(lane, v);
+         cv[v] = *live_or_zero(!t.write && o < t.r1 && r < nc, xl + o + (int64_t)(c0 + r) * m);
+      }
+      __syncthreads();
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-         if (j < t.r1 && 4 * cq + q < nc) xl[j + (int64_t)(c0 + 4 * cq + q) * m] = cv[q] - acc[q];
+      for (int q = 0; q < 2; ++q) {
+         const int tile = wave + 4 * q;
+         if (tile * 16 >= kb) continue;
+         v4d acc = v4d{0, 0, 0, 0};
+#pragma unroll
+         for (int u = 0; u < KU; ++u)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)mv[q][u], xs[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
+#pragma unroll
+         for (int v = 0; v < 4; ++v) ys[(tile * 16 + Mfma<double>::row(lane, v)) * 16 + r] = acc[v];
+      }
+      __syncthreads();
+      if (t.write) {
+         for (int e = threadIdx.x; e < kb * nc; e += kThreads) {
+            const int i = e % kb, c = e / kb;
+            xsol[rlist[nd.rlist_off + t.k0 + i] - 1 + (int64_t)(c0 + c) * ldsol] = ys[i * 16 + c];
+         }
+         continue;
+      }
+      if (ob >= t.r1) continue;
+      v4d acc = v4d{0, 0, 0, 0};
+#pragma unroll
+      for (int u = 0; u < KU; ++u)
+         acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u], ys[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int o = ob + Mfma<double>::row(lane, v);
+         if (o < t.r1 && r < nc) xl[o + (int64_t)(c0 + r) * m] = cv[v] - acc[v];
+      }
    }
 }
 

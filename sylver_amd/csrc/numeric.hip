@@ -364,7 +364,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    constexpr int kRowChunk = 256;
    constexpr int kColChunk = 64;
    constexpr int kCbRowChunk = 512;
-   constexpr int kSB2 = 2 * kNB;  // solve step: a pair of 128-column diagonal blocks
+   constexpr int kUpdRows = 64;   // rows per forward update task: one 16-row tile per wave
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
       int64_t g0 = P->stask.size();
@@ -375,19 +375,19 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       }
       push(P->fwdprog, L_FGATHER, g0, P->stask.size() - g0);
       int maxblk = 0;
-      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kSB2 - 1) / kSB2);
-      // one launch per pair of diagonal blocks: the diagonal solves (write
-      // tasks) and the update of the rows below, every task forming y
+      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
+      // one launch per diagonal block: the diagonal solve (write task) and
+      // the update of the rows below, every task forming y = inv(L_kk) x_k
       for (int sblk = 0; sblk < maxblk; ++sblk) {
          int64_t t0 = P->stask.size();
          for (int i : lv) {
             const NodeDesc& d = P->hnodes[i];
-            const int k0 = sblk * kSB2;
+            const int k0 = sblk * kNB;
             if (k0 >= d.n) continue;
-            const int kb = std::min(kSB2, d.n - k0);
+            const int kb = std::min(kNB, d.n - k0);
             P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
-            for (int r = k0 + kb; r < d.m; r += kRowChunk)
-               P->stask.push_back(SolveTask{i, k0, kb, r, std::min(d.m, r + kRowChunk), 0});
+            for (int r = k0 + kb; r < d.m; r += kUpdRows)
+               P->stask.push_back(SolveTask{i, k0, kb, r, std::min(d.m, r + kUpdRows), 0});
          }
          push(P->fwdprog, L_FDIAG, t0, P->stask.size() - t0);
       }
@@ -423,17 +423,17 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       P->stask.insert(P->stask.end(), red.begin(), red.end());
       push(P->bwdprog, L_BRED, r0, P->stask.size() - r0);
       int maxblk = 0;
-      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kSB2 - 1) / kSB2);
-      // one launch per pair of diagonal blocks (from the last): the transposed
-      // diagonal solves (write tasks) and the update of the earlier columns by
-      // the pair's rows, every task forming x'
+      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
+      // one launch per diagonal block (from the last): the transposed
+      // diagonal solve (write task) and the update of the earlier columns by
+      // the block's rows, every task forming x_k' = inv(L_kk)^T x_k
       for (int sblk = maxblk - 1; sblk >= 0; --sblk) {
          int64_t t0 = P->stask.size();
          for (int i : lv) {
             const NodeDesc& d = P->hnodes[i];
-            const int k0 = sblk * kSB2;
+            const int k0 = sblk * kNB;
             if (k0 >= d.n) continue;
-            const int kb = std::min(kSB2, d.n - k0);
+            const int kb = std::min(kNB, d.n - k0);
             P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
             for (int j = 0; j < k0; j += kColChunk)
                P->stask.push_back(SolveTask{i, k0, kb, j, std::min(k0, j + kColChunk), 0});
@@ -572,13 +572,15 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
             k_fwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_rlist, P->d_cmap, b, n, F->d_xloc, nrhs);
             break;
          case L_FDIAG:
-            k_fwd_blk<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinv, P->d_rlist, F->d_xloc, xsol, n, nrhs);
+            k_solve_step<T, false><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinv, P->d_rlist, F->d_xloc, xsol,
+                                                    n, nrhs);
             break;
          case L_BGATHER:
             k_bwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_rlist, xsol, n, F->d_xloc, nrhs);
             break;
          case L_BDIAG:
-            k_bwd_blk<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinvT, P->d_rlist, F->d_xloc, xsol, n, nrhs);
+            k_solve_step<T, true><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinvT, P->d_rlist, F->d_xloc, xsol,
+                                                   n, nrhs);
             break;
          case L_BUPD:
             if (L.count > 0) k_bwd_cbp<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, F->d_xloc, F->d_part_cb, nrhs);
