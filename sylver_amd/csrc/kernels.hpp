@@ -1021,78 +1021,151 @@ __global__ __launch_bounds__(kThreads) void k_solve_step(const SolveTask* __rest
    }
 }
 
+// acc += A(16 rows x K) B(K x 16) for small fronts: A from global memory
+// (A[i][k] at A[i + k*lda], or A[k + i*lda] when TR; rows >= rvalid and k >=
+// K read as zero), B row-major in LDS; loads in batches of eight k-steps.
+// Few registers, so that many workgroups share a CU.
+template <typename T, bool TR>
+__device__ inline v4d mma_small(const T* __restrict__ A, int64_t lda, int rvalid, int K,
+                                const double* Bs, v4d acc) {
+   const int lane = threadIdx.x & 63;
+   const int r = lane & 15, kq = lane >> 4;
+   for (int k0 = 0; k0 < K; k0 += 32) {
+      T raw[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+         const int kk = k0 + 4 * u + kq;
+         const T* p = TR ? A + kk + (int64_t)r * lda : A + r + (int64_t)kk * lda;
+         raw[u] = *live_or_zero(r < rvalid && kk < K, p);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+         const int kk = k0 + 4 * u + kq;
+         acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)raw[u], (kk < K) ? Bs[kk * 16 + r] : 0.0, acc, 0, 0, 0);
+      }
+   }
+   return acc;
+}
+
+// One workgroup solves a whole small front (one diagonal block, n <= 128,
+// m <= 256) in a sweep. Forward: y = inv(L11) x[0:n] to the solution and
+// x[n:m] -= L21 y. Backward: x[0:n] -= L21^T x[n:m], then inv(L11)^T x[0:n]
+// to the solution. The leaf levels hold thousands of such fronts: this
+// kernel keeps its register use low to run several workgroups per CU.
+template <typename T, bool BWD>
+__global__ __launch_bounds__(kThreads) void k_solve_small(const int* __restrict__ list,
+                                                          const NodeDesc* __restrict__ nodes,
+                                                          const T* __restrict__ arena,
+                                                          const T* __restrict__ dinvs,
+                                                          const int* __restrict__ rlist,
+                                                          double* __restrict__ xloc,
+                                                          double* __restrict__ xsol, int ldsol, int nrhs) {
+   __shared__ double xs[2 * kNB * 16];
+   __shared__ double ys[kNB * 16];
+   const NodeDesc nd = nodes[list[blockIdx.x]];
+   const int m = nd.m, n = nd.n, q = m - n;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int r = lane & 15;
+   double* xl = xloc + nd.solve_off * nrhs;
+   const T* Mk = dinvs + nd.dinv_off;
+   const T* F = arena + nd.off;
+   for (int c0 = 0; c0 < nrhs; c0 += 16) {
+      const int nc = min(16, nrhs - c0);
+      __syncthreads();
+      for (int e = threadIdx.x; e < 2 * kNB * 16; e += kThreads) {
+         const int l = e >> 4, c = e & 15;
+         xs[e] = *live_or_zero(l < m && c < nc, xl + l + (int64_t)(c0 + c) * m);
+      }
+      __syncthreads();
+      if (BWD && q > 0) {
+         // x[0:n] -= L21^T x[n:m]: output tiles over the n columns
+         for (int tile = wave; tile * 16 < n; tile += 4) {
+            v4d acc = v4d{0, 0, 0, 0};
+            acc = mma_small<T, true>(F + n + (int64_t)(tile * 16) * nd.ld, nd.ld, n - tile * 16, q,
+                                     xs + n * 16, acc);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+               const int i = tile * 16 + Mfma<double>::row(lane, v);
+               if (i < n) xs[i * 16 + r] -= acc[v];
+            }
+         }
+         __syncthreads();
+      }
+      for (int tile = wave; tile * 16 < n; tile += 4) {
+         v4d acc = v4d{0, 0, 0, 0};
+         acc = mma_small<T, false>(Mk + tile * 16, n, n - tile * 16, n, xs, acc);
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int i = tile * 16 + Mfma<double>::row(lane, v);
+            if (i < n) ys[i * 16 + r] = acc[v];
+         }
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < n * nc; e += kThreads) {
+         const int i = e % n, c = e / n;
+         xsol[rlist[nd.rlist_off + i] - 1 + (int64_t)(c0 + c) * ldsol] = ys[i * 16 + c];
+      }
+      if (!BWD && q > 0) {
+         for (int tile = wave; tile * 16 < q; tile += 4) {
+            v4d acc = v4d{0, 0, 0, 0};
+            acc = mma_small<T, false>(F + n + tile * 16, nd.ld, q - tile * 16, n, ys, acc);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+               const int o = n + tile * 16 + Mfma<double>::row(lane, v);
+               if (o < m && r < nc) xl[o + (int64_t)(c0 + r) * m] = xs[o * 16 + r] - acc[v];
+            }
+         }
+      }
+   }
+}
+
 // Backward contribution-block update split over row chunks: task (node,
-// rows [k0, kb), columns [r0, r1) with r1 - r0 <= 64, slot `write`) stores
-// part[slot] = L[rows, cols]^T x[rows] (64 x nrhs) without touching x; the
-// reduction below subtracts the partial sums of a column chunk in slot order.
+// rows [k0, kb) with kb - k0 <= 128, columns [r0, r1) with r1 - r0 <= 64,
+// slot `write`) stores part[slot] = L[rows, cols]^T x[rows] (64 x nrhs)
+// without touching x; each wave computes one 16-column tile on MFMA with its
+// operand prefetched into registers. The reduction below subtracts the
+// partial sums of a column chunk in slot order.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_bwd_cbp(const SolveTask* __restrict__ tasks,
                                                       const NodeDesc* __restrict__ nodes,
                                                       const T* __restrict__ arena,
                                                       const double* __restrict__ xloc,
                                                       double* __restrict__ part, int nrhs) {
-   constexpr int RC = 64, JC = 64;
-   constexpr int AE = RC * JC / kThreads;
-   constexpr int BE = RC * kMaxRhsChunk / kThreads;
-   __shared__ T As[RC][JC + 1];
-   __shared__ double Bs[RC][kMaxRhsChunk];
+   constexpr int KU = kNB / 4;
+   __shared__ double xs[kNB * 16];
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
-   const int m = nd.m;
+   const int m = nd.m, q = t.kb - t.k0;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int r = lane & 15, kq = lane >> 4;
    const double* xl = xloc + nd.solve_off * nrhs;
    const T* F = arena + nd.off;
-   const int jj = threadIdx.x & 63, cq = threadIdx.x >> 6;
-   const int rbeg = t.k0, rend = t.kb;
    double* out = part + (int64_t)t.write * 64 * nrhs;
-   for (int c0 = 0; c0 < nrhs; c0 += kMaxRhsChunk) {
-      const int nc = min(kMaxRhsChunk, nrhs - c0);
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
-      T ra[AE];
-      double rb[BE];
-      auto load = [&](int rb0) {
+   const int j = t.r0 + wave * 16 + r;
+   T av[KU];
 #pragma unroll
-         for (int u = 0; u < AE; ++u) {
-            const int e = threadIdx.x + u * kThreads;
-            const int rr = e % RC, jc = e / RC;
-            const int r = rb0 + rr, j = t.r0 + jc;
-            ra[u] = *live_or_zero(r < rend && j < t.r1, F + r + (int64_t)j * nd.ld);
-         }
+   for (int u = 0; u < KU; ++u) {
+      const int k = 4 * u + kq;
+      av[u] = *live_or_zero(j < t.r1 && k < q, F + t.k0 + k + (int64_t)j * nd.ld);
+   }
+   for (int c0 = 0; c0 < nrhs; c0 += 16) {
+      const int nc = min(16, nrhs - c0);
+      __syncthreads();
 #pragma unroll
-         for (int u = 0; u < BE; ++u) {
-            const int e = threadIdx.x + u * kThreads;
-            const int rr = e % RC, c = e / RC;
-            const int r = rb0 + rr;
-            rb[u] = *live_or_zero(r < rend && c < nc, xl + r + (int64_t)(c0 + c) * m);
-         }
-      };
-      load(rbeg);
-      for (int rb0 = rbeg; rb0 < rend; rb0 += RC) {
-         __syncthreads();
-#pragma unroll
-         for (int u = 0; u < AE; ++u) {
-            const int e = threadIdx.x + u * kThreads;
-            As[e % RC][e / RC] = ra[u];
-         }
-#pragma unroll
-         for (int u = 0; u < BE; ++u) {
-            const int e = threadIdx.x + u * kThreads;
-            Bs[e % RC][e / RC] = rb[u];
-         }
-         __syncthreads();
-         if (rb0 + RC < rend) load(rb0 + RC);
-         double p4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 16
-         for (int rr = 0; rr < RC; ++rr) {
-            const double a = (double)As[rr][jj];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) p4[q] += a * Bs[rr][4 * cq + q];
-         }
-#pragma unroll
-         for (int q = 0; q < 4; ++q) acc[q] += p4[q];
+      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
+         const int l = e >> 4, c = e & 15;
+         xs[e] = *live_or_zero(l < q && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
       }
+      __syncthreads();
+      v4d acc = v4d{0, 0, 0, 0};
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-         if (4 * cq + q < nc) out[(int64_t)jj * nrhs + c0 + 4 * cq + q] = acc[q];
+      for (int u = 0; u < KU; ++u)
+         acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u], xs[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int jj = wave * 16 + Mfma<double>::row(lane, v);
+         if (r < nc) out[(int64_t)jj * nrhs + c0 + r] = acc[v];
+      }
    }
 }
 

@@ -363,7 +363,10 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    // of the earlier rows.
    constexpr int kRowChunk = 256;
    constexpr int kColChunk = 64;
-   constexpr int kCbRowChunk = 512;
+   constexpr int kCbRowChunk = kNB;
+   // fronts solved by one workgroup per sweep: one diagonal block, at most
+   // 256 rows
+   auto small_solve = [](const NodeDesc& d) { return d.n <= kNB && d.m <= 2 * kNB; };
    constexpr int kUpdRows = 64;   // rows per forward update task: one 16-row tile per wave
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
@@ -374,6 +377,12 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             P->stask.push_back(SolveTask{i, 0, 0, r, std::min(d.m, r + kRowChunk), 0});
       }
       push(P->fwdprog, L_FGATHER, g0, P->stask.size() - g0);
+      {
+         const int64_t s0 = P->nlistv.size();
+         for (int i : lv)
+            if (small_solve(P->hnodes[i])) P->nlistv.push_back(i);
+         push(P->fwdprog, L_FSMALL, s0, P->nlistv.size() - s0);
+      }
       int maxblk = 0;
       for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
       // one launch per diagonal block: the diagonal solve (write task) and
@@ -385,6 +394,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             const int k0 = sblk * kNB;
             if (k0 >= d.n) continue;
             const int kb = std::min(kNB, d.n - k0);
+            if (small_solve(d)) continue;  // handled by the level's small-front launch
             P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
             for (int r = k0 + kb; r < d.m; r += kUpdRows)
                P->stask.push_back(SolveTask{i, k0, kb, r, std::min(d.m, r + kUpdRows), 0});
@@ -401,6 +411,12 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             P->stask.push_back(SolveTask{i, 0, 0, r, std::min(d.m, r + kRowChunk), 0});
       }
       push(P->bwdprog, L_BGATHER, g0, P->stask.size() - g0);
+      {
+         const int64_t s0 = P->nlistv.size();
+         for (int i : lv)
+            if (small_solve(P->hnodes[i])) P->nlistv.push_back(i);
+         push(P->bwdprog, L_BSMALL, s0, P->nlistv.size() - s0);
+      }
       // contribution-block part: columns [0, n) against rows [n, m), split
       // into row chunks (partial sums per slot) and reduced in slot order
       int64_t c0 = P->stask.size();
@@ -408,7 +424,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       int slots = 0;
       for (int i : lv) {
          const NodeDesc& d = P->hnodes[i];
-         if (d.m - d.n <= 0 || d.n == 0) continue;
+         if (d.m - d.n <= 0 || d.n == 0 || small_solve(d)) continue;
          for (int j = 0; j < d.n; j += kColChunk) {
             const int s0 = slots;
             for (int r = d.n; r < d.m; r += kCbRowChunk)
@@ -434,6 +450,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             const int k0 = sblk * kNB;
             if (k0 >= d.n) continue;
             const int kb = std::min(kNB, d.n - k0);
+            if (small_solve(d)) continue;
             P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
             for (int j = 0; j < k0; j += kColChunk)
                P->stask.push_back(SolveTask{i, k0, kb, j, std::min(k0, j + kColChunk), 0});
@@ -584,6 +601,14 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
             break;
          case L_BUPD:
             if (L.count > 0) k_bwd_cbp<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, F->d_xloc, F->d_part_cb, nrhs);
+            break;
+         case L_FSMALL:
+            k_solve_small<T, false><<<g, bl, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, dinv, P->d_rlist,
+                                                     F->d_xloc, xsol, n, nrhs);
+            break;
+         case L_BSMALL:
+            k_solve_small<T, true><<<g, bl, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, dinvT, P->d_rlist,
+                                                    F->d_xloc, xsol, n, nrhs);
             break;
          case L_BRED:
             if (L.count > 0) k_bwd_cbr<<<g, bl, 0, s>>>(tk, P->d_nodes, F->d_part_cb, F->d_xloc, nrhs);

@@ -61,7 +61,9 @@ enum LaunchKind {
    L_BGATHER,
    L_BDIAG,
    L_BUPD,
-   L_BRED
+   L_BRED,
+   L_FSMALL,
+   L_BSMALL
 };
 
 struct Launch {
