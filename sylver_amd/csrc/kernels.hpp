@@ -871,12 +871,15 @@ __device__ inline int lower_bound_i(const int* a, int n, int v) {
    return lo;
 }
 
-// Forward gather of rows [r0, r1): own rows from b, the others zero, then the
-// children's contribution rows that map into the range (the child map is
-// increasing, so they are a contiguous run found by binary search), children
-// in list order.
+// Forward gather of rows [r0, r1) (at most 256: one row per thread): own
+// rows from b, the others zero, then the contribution rows of the children
+// that map into the range, children in list order. The child ranges are
+// precomputed per task (gch[k0 .. k0+kb) = {child, i0, i1}: rows [i0, i1) of
+// the child's contribution block land in [r0, r1), the child map being
+// increasing), so no thread searches or walks the child list.
 static __global__ __launch_bounds__(kThreads) void k_fwd_gather(const SolveTask* __restrict__ tasks,
                                                          const NodeDesc* __restrict__ nodes,
+                                                         const int4* __restrict__ gch,
                                                          const int* __restrict__ rlist,
                                                          const int* __restrict__ cmap,
                                                          const double* __restrict__ b, int ldb,
@@ -884,29 +887,44 @@ static __global__ __launch_bounds__(kThreads) void k_fwd_gather(const SolveTask*
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    double* xl = xloc + nd.solve_off * nrhs;
-   const int m = nd.m, R = t.r1 - t.r0;
-   for (int e = threadIdx.x; e < R * nrhs; e += kThreads) {
-      const int r = t.r0 + e % R, c = e / R;
-      xl[r + (int64_t)c * m] = (r < nd.n) ? b[rlist[nd.rlist_off + r] - 1 + (int64_t)c * ldb] : 0.0;
-   }
-   for (int ch = nd.first_child; ch != -1;) {
-      __syncthreads();
-      const NodeDesc cn = nodes[ch];
-      const int q = cn.m - cn.n;
-      const int* cm = cmap + cn.cmap_off;
-      const int i0 = lower_bound_i(cm, q, t.r0);
-      const int i1 = lower_bound_i(cm, q, t.r1);
-      const int Q = i1 - i0;
-      const double* cx = xloc + cn.solve_off * nrhs;
-      for (int e = threadIdx.x; e < Q * nrhs; e += kThreads) {
-         const int i = i0 + e % Q, c = e / Q;
-         xl[cm[i] + (int64_t)c * m] += cx[cn.n + i + (int64_t)c * cn.m];
+   const int m = nd.m;
+   const int r = t.r0 + threadIdx.x;
+   if (r < t.r1) {
+      const int v = (r < nd.n) ? rlist[nd.rlist_off + r] - 1 : 0;
+      for (int c0 = 0; c0 < nrhs; c0 += 8) {
+         double x[8];
+#pragma unroll
+         for (int u = 0; u < 8; ++u) x[u] = *live_or_zero(r < nd.n && c0 + u < nrhs, b + v + (int64_t)(c0 + u) * ldb);
+#pragma unroll
+         for (int u = 0; u < 8; ++u)
+            if (c0 + u < nrhs) xl[r + (int64_t)(c0 + u) * m] = x[u];
       }
-      ch = cn.next_child;
+   }
+   for (int k = 0; k < t.kb; ++k) {
+      const int4 g = gch[t.k0 + k];
+      const NodeDesc cn = nodes[g.x];
+      __syncthreads();
+      const int i = g.y + threadIdx.x;
+      if (i < g.z) {
+         const int* cm = cmap + cn.cmap_off;
+         const double* cx = xloc + cn.solve_off * nrhs;
+         const int pr = cm[i];
+         for (int c0 = 0; c0 < nrhs; c0 += 8) {
+            double x[8], y[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+               x[u] = *live_or_zero(c0 + u < nrhs, cx + cn.n + i + (int64_t)(c0 + u) * cn.m);
+               y[u] = *live_or_zero(c0 + u < nrhs, xl + pr + (int64_t)(c0 + u) * m);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+               if (c0 + u < nrhs) xl[pr + (int64_t)(c0 + u) * m] = y[u] + x[u];
+         }
+      }
    }
 }
 
-// Backward gather of rows [r0, r1) from the current solution.
+// Backward gather of rows [r0, r1) (at most 256) from the current solution.
 static __global__ __launch_bounds__(kThreads) void k_bwd_gather(const SolveTask* __restrict__ tasks,
                                                          const NodeDesc* __restrict__ nodes,
                                                          const int* __restrict__ rlist,
@@ -916,10 +934,17 @@ static __global__ __launch_bounds__(kThreads) void k_bwd_gather(const SolveTask*
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    double* xl = xloc + nd.solve_off * nrhs;
-   const int m = nd.m, R = t.r1 - t.r0;
-   for (int e = threadIdx.x; e < R * nrhs; e += kThreads) {
-      const int r = t.r0 + e % R, c = e / R;
-      xl[r + (int64_t)c * m] = xsol[rlist[nd.rlist_off + r] - 1 + (int64_t)c * ldsol];
+   const int m = nd.m;
+   const int r = t.r0 + threadIdx.x;
+   if (r >= t.r1) return;
+   const int v = rlist[nd.rlist_off + r] - 1;
+   for (int c0 = 0; c0 < nrhs; c0 += 8) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = *live_or_zero(c0 + u < nrhs, xsol + v + (int64_t)(c0 + u) * ldsol);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+         if (c0 + u < nrhs) xl[r + (int64_t)(c0 + u) * m] = x[u];
    }
 }
 

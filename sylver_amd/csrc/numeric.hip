@@ -373,8 +373,21 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       int64_t g0 = P->stask.size();
       for (int i : lv) {
          const NodeDesc& d = P->hnodes[i];
-         for (int r = 0; r < d.m; r += kRowChunk)
-            P->stask.push_back(SolveTask{i, 0, 0, r, std::min(d.m, r + kRowChunk), 0});
+         for (int r = 0; r < d.m; r += kRowChunk) {
+            const int r1 = std::min(d.m, r + kRowChunk);
+            // rows of each child's contribution block that land in [r, r1)
+            const int gbeg = (int)P->gch.size();
+            for (int c = d.first_child; c != -1; c = P->hnodes[c].next_child) {
+               const NodeDesc& cd = P->hnodes[c];
+               const int q = cd.m - cd.n;
+               const int* cm = P->hcmap.data() + cd.cmap_off;
+               const int i0 = (int)(std::lower_bound(cm, cm + q, r) - cm);
+               const int i1 = (int)(std::lower_bound(cm, cm + q, r1) - cm);
+               for (int a = i0; a < i1; a += kThreads)
+                  P->gch.push_back(int4{c, a, std::min(i1, a + kThreads), 0});
+            }
+            P->stask.push_back(SolveTask{i, gbeg, (int)P->gch.size() - gbeg, r, r1, 0});
+         }
       }
       push(P->fwdprog, L_FGATHER, g0, P->stask.size() - g0);
       {
@@ -514,6 +527,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    P->d_init = dupload(P->init);
    P->d_ea = dupload(P->ea);
    P->d_nlistv = dupload(P->nlistv);
+   P->d_gch = dupload(P->gch);
    P->d_diag = dupload(P->diag);
    P->d_gemm = dupload(P->gemm);
    P->d_stask = dupload(P->stask);
@@ -586,7 +600,7 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
       const SolveTask* tk = P->d_stask + L.off;
       switch (L.kind) {
          case L_FGATHER:
-            k_fwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_rlist, P->d_cmap, b, n, F->d_xloc, nrhs);
+            k_fwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_gch, P->d_rlist, P->d_cmap, b, n, F->d_xloc, nrhs);
             break;
          case L_FDIAG:
             k_solve_step<T, false><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinv, P->d_rlist, F->d_xloc, xsol,

@@ -116,6 +116,7 @@ struct DevicePlan {
    std::vector<DiagTask> diag;
    std::vector<GemmTask> gemm;
    std::vector<SolveTask> stask;
+   std::vector<int4> gch;  // forward gather: {child, i0, i1} ranges per task
    ColTask* d_zero = nullptr;
    InitTask* d_init = nullptr;
    EaTask* d_ea = nullptr;
@@ -123,6 +124,7 @@ struct DevicePlan {
    DiagTask* d_diag = nullptr;
    GemmTask* d_gemm = nullptr;
    SolveTask* d_stask = nullptr;
+   int4* d_gch = nullptr;
    std::vector<Launch> fprog, fwdprog, bwdprog;
    int64_t num_flops = 0;
    int64_t cb_slots = 0;  // partial-sum slots of the largest backward CB launch
@@ -137,7 +139,7 @@ struct DevicePlan {
       for (void* p : {(void*)d_sptr, (void*)d_sparent, (void*)d_rptr, (void*)d_rlist,
                       (void*)d_nptr, (void*)d_nlist, (void*)d_cmap, (void*)d_cmap_off,
                       (void*)d_nodes, (void*)d_zero, (void*)d_init, (void*)d_ea,
-                      (void*)d_nlistv, (void*)d_diag, (void*)d_gemm, (void*)d_stask,
+                      (void*)d_nlistv, (void*)d_diag, (void*)d_gemm, (void*)d_stask, (void*)d_gch,
                       (void*)d_csr_ptr, (void*)d_csr_col, (void*)d_csr_src})
          dfree(p);
    }
