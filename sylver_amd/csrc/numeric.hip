@@ -140,6 +140,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
          prog.push_back(l);
       }
    };
+   int side_last = -1;  // last event recorded on the side stream, not yet waited for
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
       // zero
@@ -221,60 +222,105 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
          while ((P->gemm.size() - t0) % 8 != 0 && P->gemm.size() > (size_t)t0) P->gemm.push_back(GemmTask{});
          push(P->fprog, L_TRSM, t0, P->gemm.size() - t0);
          // Schur update. Block columns are paired: after the first of a pair
-         // only the next panel's columns are updated (narrow update), after the
-         // second the trailing block receives both panels at once (K = 256),
-         // halving the passes over the contribution block.
+         // only the next panel's columns are updated (narrow update); after the
+         // second, the trailing block receives both panels at once (K = 256),
+         // halving the passes over the contribution block. That update is
+         // split for look-ahead: the columns of the next pair (needed by the
+         // next diagonal factorization) on the factor stream, the columns
+         // beyond them on the side stream, where they overlap the next pair's
+         // diagonal/TRSM chain.
          const bool first_of_pair = ((k0 / kNB) % 2) == 0;
-         for (int wide = 0; wide < 2; ++wide) {
-         int64_t u0 = P->gemm.size();
-         for (int i : large) {
-            const NodeDesc& d = P->hnodes[i];
-            if (k0 >= d.n) continue;
-            if ((d.n > wide_ncol) != (wide == 1)) continue;
-            const int kb = std::min(kNB, d.n - k0);
-            if (first_of_pair && k0 + kNB < d.n) {
-               const int k1 = k0 + kNB, kb1 = std::min(kNB, d.n - k1);
-               for (int r = k1; r < d.m; r += 128) {
+         // lower-triangular tiles of columns [c0, c1) (rows from c0 down)
+         auto tiles = [&](const NodeDesc& d, int kstart, int K, int c0, int c1) {
+            const int c0a = c0 & ~7;
+            const int R = d.m - c0a;
+            const int ntr = (R + 127) / 128, ntc = (c1 - c0a + 127) / 128;
+            for (int tj = 0; tj < ntc; ++tj)
+               for (int ti = tj; ti < ntr; ++ti) {
                   GemmTask g{};
-                  g.a_off = d.off + r + (int64_t)k0 * d.ld;
-                  g.b_off = d.off + k1 + (int64_t)k0 * d.ld;
-                  g.c_off = d.off + r + (int64_t)k1 * d.ld;
-                  g.lda = g.ldb = g.ldc = d.ld;
-                  g.M = std::min(128, d.m - r);
-                  g.N = kb1;
-                  g.K = kb;
-                  g.pad0 = 0;
-                  P->gemm.push_back(g);
-               }
-               continue;
-            }
-            const int kstart = first_of_pair ? k0 : k0 - kNB;
-            const int K = k0 + kb - kstart;
-            const int r0 = k0 + kb;
-            const int r0a = r0 & ~7;
-            const int R = d.m - r0a;
-            if (d.m <= r0) continue;
-            const int nt = (R + 127) / 128;
-            for (int ti = 0; ti < nt; ++ti)
-               for (int tj = 0; tj <= ti; ++tj) {
-                  GemmTask g{};
-                  g.a_off = d.off + r0a + ti * 128 + (int64_t)kstart * d.ld;
-                  g.b_off = d.off + r0a + tj * 128 + (int64_t)kstart * d.ld;
-                  g.c_off = d.off + (r0a + ti * 128) + (int64_t)(r0a + tj * 128) * d.ld;
+                  g.a_off = d.off + c0a + ti * 128 + (int64_t)kstart * d.ld;
+                  g.b_off = d.off + c0a + tj * 128 + (int64_t)kstart * d.ld;
+                  g.c_off = d.off + (c0a + ti * 128) + (int64_t)(c0a + tj * 128) * d.ld;
                   g.lda = g.ldb = g.ldc = d.ld;
                   g.M = std::min(128, R - ti * 128);
-                  g.N = std::min(128, R - tj * 128);
+                  g.N = std::min(128, c1 - (c0a + tj * 128));
                   g.K = K;
-                  g.pad0 = (tj == 0) ? r0 - r0a : 0;
+                  g.pad0 = (tj == 0) ? c0 - c0a : 0;
                   P->gemm.push_back(g);
                }
+         };
+         auto pad8 = [&](int64_t u0) {
+            while ((P->gemm.size() - u0) % 8 != 0 && P->gemm.size() > (size_t)u0) P->gemm.push_back(GemmTask{});
+         };
+         // recorded on the factor stream once this block's panel is final (the
+         // DIAG/TRSM launches just pushed are the last factor-stream entries)
+         int ev_trsm = P->nevents++;
+         P->fprog.back().rec_ev = ev_trsm;
+         for (int part = 0; part < 2; ++part) {  // 0: next pair (factor stream), 1: far (side)
+            for (int wide = 0; wide < 2; ++wide) {
+               int64_t u0 = P->gemm.size();
+               for (int i : large) {
+                  const NodeDesc& d = P->hnodes[i];
+                  if (k0 >= d.n) continue;
+                  if ((d.n > wide_ncol) != (wide == 1)) continue;
+                  const int kb = std::min(kNB, d.n - k0);
+                  if (first_of_pair && k0 + kNB < d.n) {
+                     if (part == 1) continue;
+                     const int k1 = k0 + kNB, kb1 = std::min(kNB, d.n - k1);
+                     for (int r = k1; r < d.m; r += 128) {
+                        GemmTask g{};
+                        g.a_off = d.off + r + (int64_t)k0 * d.ld;
+                        g.b_off = d.off + k1 + (int64_t)k0 * d.ld;
+                        g.c_off = d.off + r + (int64_t)k1 * d.ld;
+                        g.lda = g.ldb = g.ldc = d.ld;
+                        g.M = std::min(128, d.m - r);
+                        g.N = kb1;
+                        g.K = kb;
+                        g.pad0 = 0;
+                        P->gemm.push_back(g);
+                     }
+                     continue;
+                  }
+                  const int kstart = first_of_pair ? k0 : k0 - kNB;
+                  const int K = k0 + kb - kstart;
+                  const int r0 = k0 + kb;
+                  if (d.m <= r0) continue;
+                  const int nx = std::min(d.n, r0 + 2 * kNB);  // end of the next pair's columns
+                  if (part == 0 && nx > r0) tiles(d, kstart, K, r0, nx);
+                  if (part == 1 && d.m > nx) tiles(d, kstart, K, nx, d.m);
+               }
+               pad8(u0);
+               const int64_t cnt = P->gemm.size() - u0;
+               if (cnt == 0) continue;
+               push(P->fprog, wide ? L_UPDATE_WIDE : L_UPDATE, u0, cnt);
+               Launch& L = P->fprog.back();
+               if (part == 0) {
+                  // the next pair's columns were last touched by the side stream
+                  if (side_last >= 0) {
+                     L.wait_ev = side_last;
+                     side_last = -1;
+                  }
+               } else {
+                  L.stream = 1;
+                  if (ev_trsm >= 0) {
+                     L.wait_ev = ev_trsm;
+                     ev_trsm = -1;
+                  }
+                  side_last = P->nevents++;
+                  L.rec_ev = side_last;
+               }
+            }
          }
-         while ((P->gemm.size() - u0) % 8 != 0 && P->gemm.size() > (size_t)u0) {
-            GemmTask g{};
-            P->gemm.push_back(g);  // empty task: keeps the launch a multiple of 8
-         }
-         push(P->fprog, wide ? L_UPDATE_WIDE : L_UPDATE, u0, P->gemm.size() - u0);
-         }
+      }
+      // join the side stream before the next level's assembly
+      if (side_last >= 0) {
+         Launch j;
+         j.kind = L_SYNC;
+         j.count = 0;
+         j.off = 0;
+         j.wait_ev = side_last;
+         P->fprog.push_back(j);
+         side_last = -1;
       }
    }
    // inverted diagonal blocks of the small fronts (the large path writes its own)
@@ -544,8 +590,11 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
    T* dinv = static_cast<T*>(F->dinv);
    T* dinvT = static_cast<T*>(F->dinvT);
    size_t li = 0;
+   const hipStream_t s_main = s;
    for (const Launch& L : P->fprog) {
       dim3 g(L.count), b(kThreads);
+      s = L.stream ? F->stream2 : s_main;
+      if (L.wait_ev >= 0) HIPCHECK(hipStreamWaitEvent(s, F->xev[L.wait_ev], 0));
       if (events) HIPCHECK(hipEventRecord((*events)[2 * li], s));
       switch (L.kind) {
          case L_ZERO:
@@ -579,10 +628,11 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
          case L_DINV:
             k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, dinvT, F->d_info, 0);
             break;
-         default:
+         default:  // L_SYNC
             break;
       }
       if (events) HIPCHECK(hipEventRecord((*events)[2 * li + 1], s));
+      if (L.rec_ev >= 0) HIPCHECK(hipEventRecord(F->xev[L.rec_ev], s));
       ++li;
    }
    HIPCHECK(hipGetLastError());
@@ -706,6 +756,9 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
          F->d_csr_val = dalloc<double>(std::max<int64_t>(P->csr_nnz, 1));
          F->d_info = dalloc<int>(std::max(P->nnodes, 1));
          HIPCHECK(hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking));
+         HIPCHECK(hipStreamCreateWithFlags(&F->stream2, hipStreamNonBlocking));
+         F->xev.resize(P->nevents);
+         for (auto& e : F->xev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       }
       F->posdef = posdef;
       hipStream_t s = F->stream;
@@ -773,6 +826,7 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
             float t = 0;
             HIPCHECK(hipEventElapsedTime(&t, events[2 * i], events[2 * i + 1]));
             const Launch& L = P->fprog[i];
+            if (L.kind >= 16) continue;  // L_SYNC: no kernel
             F->kind_ms[L.kind] += t;
             F->kind_flops[L.kind] += L.flops;
             F->kind_bytes[L.kind] += L.words * es + L.bytes;

@@ -63,13 +63,17 @@ enum LaunchKind {
    L_BUPD,
    L_BRED,
    L_FSMALL,
-   L_BSMALL
+   L_BSMALL,
+   L_SYNC  // no kernel: only the stream wait / event record of the entry
 };
 
 struct Launch {
    int kind;
    int count;
    int64_t off;
+   int stream = 0;    // 0: the factor stream, 1: the look-ahead side stream
+   int wait_ev = -1;  // event (index) the launch's stream waits for first
+   int rec_ev = -1;   // event (index) recorded on the launch's stream after it
    double flops = 0;  // algorithmic flops of the launch
    double words = 0;  // algorithmic memory traffic in front elements
    double bytes = 0;  // algorithmic memory traffic independent of the element size
@@ -128,6 +132,7 @@ struct DevicePlan {
    std::vector<Launch> fprog, fwdprog, bwdprog;
    int64_t num_flops = 0;
    int64_t cb_slots = 0;  // partial-sum slots of the largest backward CB launch
+   int nevents = 0;       // cross-stream events of the factor program
    // CSR of the whole matrix in pivot order, values gathered per factorize
    int64_t csr_nnz = 0;
    int64_t* d_csr_ptr = nullptr;
@@ -158,6 +163,8 @@ struct DeviceFactors {
    int* d_info = nullptr;
    double anorm = 0;
    hipStream_t stream = nullptr;
+   hipStream_t stream2 = nullptr;    // look-ahead: far trailing updates
+   std::vector<hipEvent_t> xev;      // cross-stream events of the factor program
    hipGraphExec_t graph = nullptr;
    bool has_scaling = false;
    // solve workspaces
@@ -209,6 +216,8 @@ struct DeviceFactors {
                       (void*)d_sol, (void*)d_norms, (void*)d_V, (void*)d_W, (void*)d_tmp,
                       (void*)d_part, (void*)d_coef})
          dfree(p);
+      for (auto e : xev) (void)hipEventDestroy(e);
+      if (stream2) (void)hipStreamDestroy(stream2);
       if (stream) (void)hipStreamDestroy(stream);
    }
 };
