@@ -137,6 +137,132 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}))
 
 
+def bench_batched_fronts(S, M, nfronts=20000, seed=1, cpu_fronts=1500, reps=3):
+    """Batched small fronts: ncol log-uniform in [8, 256], nrow = ncol (1 + r),
+    r ~ U[0, 2]; partial Cholesky (POTRF, TRSM, SYRK into the contribution
+    block) of all fronts in one launch, FP64 and FP32. Roofline time =
+    max(total flops / MFMA peak, total bytes / HBM bandwidth) with the
+    lower triangle read and written once per front."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    nrow, ncol = M.dense_front_sizes(nfronts, seed=seed)
+    m = nrow.astype(np.float64)
+    n = ncol.astype(np.float64)
+    q = m - n
+    flops = float((n ** 3 / 3 + q * n ** 2 + q ** 2 * n).sum())
+    off = np.zeros(nfronts, np.int64)
+    off[1:] = np.cumsum(nrow.astype(np.int64) ** 2)[:-1]
+    total = int(off[-1] + int(nrow[-1]) ** 2)
+    out = {"fronts": nfronts, "seed": seed, "gflop": round(flops / 1e9, 3)}
+    for dt, name, pk in ((np.float64, "fp64", PEAK["fp64"]), (np.float32, "fp32", PEAK["fp32"])):
+        w = np.dtype(dt).itemsize
+        nbytes = float((2 * w * m * (m + 1) / 2).sum())
+        arr = S.DeviceArray.empty(total, dt)
+        best = None
+        worst_info = 0
+        for _ in range(reps):
+            if S.batched_front_fill(arr, nrow, off, dt, seed) != 0:
+                raise RuntimeError("front fill failed")
+            rc, info, ms = S.batched_front_potrf(arr, nrow, ncol, off, dt)
+            if rc != 0:
+                raise RuntimeError(f"batched factorization failed: {rc}")
+            worst_info = max(worst_info, int(np.abs(info).max()))
+            best = ms if best is None else min(best, ms)
+        t = best / 1e3
+        bound = max(flops / (pk * 1e12), nbytes / (HBM_GBS * 1e9))
+        rec = {"ms": round(best, 3), "gflops": round(flops / t / 1e9, 1), "gbytes_s": round(nbytes / t / 1e9, 1),
+               "roofline_ms": round(bound * 1e3, 3), "frac": round(bound / t, 4), "info_max": worst_info}
+        if name == "fp64":
+            # parity and CPU baseline on the first fronts: same values, copied back before factoring
+            if S.batched_front_fill(arr, nrow, off, dt, seed) != 0:
+                raise RuntimeError("front fill failed")
+            k = min(cpu_fronts, nfronts)
+            host = [arr.read(off[i], int(nrow[i]) ** 2).reshape(nrow[i], nrow[i], order="F") for i in range(k)]
+            S.batched_front_potrf(arr, nrow, ncol, off, dt)
+            cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+
+            def one(i):
+                return oracle.front_potrf(host[i], int(ncol[i]))
+
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=cores) as ex:
+                res = list(ex.map(one, range(k)))
+            tc = time.perf_counter() - t0
+            err = 0.0
+            for i, (inf_o, fo) in enumerate(res[:200]):
+                g = arr.read(off[i], int(nrow[i]) ** 2).reshape(nrow[i], nrow[i], order="F")
+                c = int(ncol[i])
+                err = max(err, float(np.abs(np.tril(g[:, :c]) - np.tril(fo[:, :c])).max() / max(1.0, np.abs(fo).max())))
+            fs = float((n[:k] ** 3 / 3 + q[:k] * n[:k] ** 2 + q[:k] ** 2 * n[:k]).sum())
+            rec["parity_max_rel_err_200_fronts"] = err
+            out["cpu_baseline"] = {"value": round(fs / tc / 1e9, 3), "unit": "GFLOP/s", "cores": cores,
+                                   "kind": "port", "sample": f"oracle front_potrf of the first {k} fronts"}
+        arr.free()
+        out[name] = rec
+    return out
+
+
+def bench_small_laplacian(S, M, oracle_mod):
+    """2D 5-point Laplacian 100 x 100 (n = 10,000), posdef, FP64, 1 RHS."""
+    A = M.laplacian_2d(100)
+    opts = S.default_options()
+    akeep, inf, _ = S.spldlt_analyse(A.n, A.ptr, A.row, A.val, options=opts)
+    flops = float(inf.num_flops)
+    b = M.random_rhs(A.n, 1, seed=0)
+    S.set_ext_options(precision=S.PREC_FP64, use_graph=1, record_timing=0)
+    fk = None
+    fms, sms = [], []
+    for it in range(6):
+        fk, finf = S.spldlt_factorize(True, A.val, akeep, fk, options=opts)
+        x = b.copy(order="F")
+        sinf = S.spldlt_solve(x, akeep, fk, options=opts)
+        if it > 0:
+            fms.append(finf.tail().factor_ms)
+            sms.append(sinf.tail().solve_ms)
+    err = M.backward_error(A, x, b)
+    sym = S.akeep_symbolic(akeep)
+    t0 = time.perf_counter()
+    st, off, lcol = oracle_mod.factor_posdef(sym, A.val)
+    tc = time.perf_counter() - t0
+    fm = float(np.median(fms))
+    return {"n": A.n, "gflop": round(flops / 1e9, 4), "factor_ms": round(fm, 3),
+            "solve_ms": round(float(np.median(sms)), 3), "factor_gflops": round(flops / fm / 1e6, 2),
+            "bwd_err": err, "cpu_factor_ms": round(tc * 1e3, 3),
+            "cpu_kind": "port (oracle, all host threads)"}
+
+
+def bench_kkt(S, M, oracle_mod, side=12, mrows=432):
+    """KKT [H A^T; A 0], H the 7-point Laplacian on side^3, A mrows x side^3
+    with 3 entries per row: APP LDL^T in FP64 (u = 0.01, small = 1e-20)."""
+    A = M.kkt(side, mrows, seed=0)
+    opts = S.default_options()
+    akeep, inf, _ = S.spldlt_analyse(A.n, A.ptr, A.row, A.val, options=opts)
+    b = M.random_rhs(A.n, 1, seed=0)
+    S.set_ext_options(precision=S.PREC_FP64, use_graph=1, record_timing=0)
+    fk = None
+    fms = []
+    for it in range(3):
+        fk, finf = S.spldlt_factorize(False, A.val, akeep, fk, options=opts)
+        if finf.flag < 0:
+            raise RuntimeError(f"indefinite factorization failed: {finf.flag}")
+        if it > 0:
+            fms.append(finf.tail().factor_ms)
+    x = b.copy(order="F")
+    sinf = S.spldlt_solve(x, akeep, fk, options=opts)
+    err = M.backward_error(A, x, b)
+    st = S.fkeep_indef_stats(fk)
+    sym = S.akeep_symbolic(akeep)
+    t0 = time.perf_counter()
+    ref = oracle_mod.IndefFactor(sym, A.val, nb=64, u=opts.u, small=opts.small)
+    tc = time.perf_counter() - t0
+    return {"n": A.n, "nh": side ** 3, "m": mrows, "flops_executed": float(finf.num_flops),
+            "factor_ms": round(float(np.median(fms)), 3), "solve_ms": round(sinf.tail().solve_ms, 3),
+            "bwd_err": err, "num_neg": int(finf.num_neg), "num_delay": int(finf.num_delay),
+            "num_two": int(finf.num_two), "maxfront": int(finf.maxfront),
+            "stats_equal_oracle": all(st[k] == ref.stats[k] for k in st),
+            "cpu_factor_ms": round(tc * 1e3, 3), "cpu_kind": "port (oracle, one thread)"}
+
+
 def measured_traffic():
     """HBM bytes per launch of the Schur-update kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated against a
@@ -166,6 +292,8 @@ def main():
                     help="run only one precision path (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (scales PMC byte counters)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the secondary workloads (2D Laplacian, batched fronts, KKT)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -238,6 +366,20 @@ def main():
                     "gflop_per_launch": round(up["gflop"] / up["launches"], 4)}
     S.set_ext_options(precision=S.PREC_FP64)
 
+    secondary = None
+    if not args.no_secondary:
+        import oracle as oracle_mod
+        secondary = {}
+        for key, fn in (("laplacian_2d_100", lambda: bench_small_laplacian(S, M, oracle_mod)),
+                        ("batched_small_fronts", lambda: bench_batched_fronts(S, M)),
+                        ("kkt_app_12cubed_432", lambda: bench_kkt(S, M, oracle_mod))):
+            try:
+                secondary[key] = fn()
+            except Exception as e:  # a secondary workload never blocks the headline
+                secondary[key] = {"error": str(e)}
+            log(f"secondary {key}: {secondary[key]}")
+        S.set_ext_options(precision=S.PREC_FP64)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -276,6 +418,7 @@ def main():
         "kernels_fp64": kinds64,
         "kernels_mixed": kindsmx,
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
 

@@ -244,6 +244,14 @@ int sylver_amd_batched_front_potrf(int nfronts, int const *nrow_host, int const 
                                    long const *offset_host, void *data_dev, int dtype,
                                    int *info_dev, void *stream);
 
+/* Synthetic input of the batched small-front workload (no counterpart in
+ * the reference): fills each front f (nrow[f] x nrow[f], ld nrow[f], at
+ * data + offset[f] elements, device memory) with a symmetric matrix whose
+ * entries are uniform in [-1, 1] from a counter-based hash of (seed, f, i, j),
+ * plus nrow[f] on the diagonal (SPD by diagonal dominance). dtype as above. */
+int sylver_amd_batched_front_fill(int nfronts, int const *nrow_host, long const *offset_host,
+                                  void *data, int dtype, unsigned long long seed);
+
 /* Number of HIP devices visible (0 when none); never fails. */
 int sylver_amd_device_count(void);
 

@@ -149,7 +149,7 @@ EXPORTED = [
     "sylver_amd_akeep_symbolic", "sylver_amd_fkeep_device_symbolic",
     "sylver_amd_fkeep_node_factor", "sylver_amd_fkeep_node_assembled",
     "sylver_amd_fkeep_indef_node", "sylver_amd_fkeep_indef_stats",
-    "sylver_amd_batched_front_potrf", "sylver_amd_device_count", "sylver_amd_dev_alloc",
+    "sylver_amd_batched_front_potrf", "sylver_amd_batched_front_fill", "sylver_amd_device_count", "sylver_amd_dev_alloc",
     "sylver_amd_dev_free", "sylver_amd_memcpy_h2d", "sylver_amd_memcpy_d2h",
     "sylver_amd_last_batched_ms",
 ]
@@ -188,6 +188,7 @@ def lib():
     L.sylver_amd_fkeep_indef_node.argtypes = [vp, C.c_int, ip, ip, dp, dp]
     L.sylver_amd_fkeep_indef_stats.argtypes = [vp, ip]
     L.sylver_amd_batched_front_potrf.argtypes = [C.c_int, ip, ip, lp, vp, C.c_int, ip, vp]
+    L.sylver_amd_batched_front_fill.argtypes = [C.c_int, ip, lp, vp, C.c_int, C.c_ulonglong]
     L.sylver_amd_device_count.argtypes = []
     L.sylver_amd_dev_alloc.argtypes = [pvp, C.c_size_t]
     L.sylver_amd_dev_free.argtypes = [vp]
@@ -425,6 +426,25 @@ class DeviceArray:
         if lib().sylver_amd_memcpy_h2d(self.ptr, h.ctypes.data_as(C.c_void_p), self.nbytes) != 0:
             raise RuntimeError("host->device copy failed")
 
+    @classmethod
+    def empty(cls, count, dtype):
+        """An uninitialized device buffer of `count` elements."""
+        self = cls.__new__(cls)
+        self.shape, self.dtype = (int(count),), np.dtype(dtype)
+        self.nbytes = int(count) * self.dtype.itemsize
+        self.ptr = C.c_void_p()
+        if lib().sylver_amd_dev_alloc(C.byref(self.ptr), max(self.nbytes, 1)) != 0:
+            raise MemoryError("device allocation failed")
+        return self
+
+    def read(self, off, count):
+        """Copy `count` elements starting at element `off` to the host."""
+        out = np.empty(int(count), self.dtype)
+        src = C.c_void_p(self.ptr.value + int(off) * self.dtype.itemsize)
+        if lib().sylver_amd_memcpy_d2h(out.ctypes.data_as(C.c_void_p), src, out.nbytes) != 0:
+            raise RuntimeError("device->host copy failed")
+        return out
+
     def numpy(self):
         out = np.empty(self.shape, self.dtype)
         if lib().sylver_amd_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes) != 0:
@@ -441,6 +461,15 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def batched_front_fill(fronts_flat: "DeviceArray", nrow, offsets, dtype, seed):
+    """Synthetic SPD fronts (see sylver_amd_batched_front_fill)."""
+    nrow = np.ascontiguousarray(nrow, np.int32)
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    return lib().sylver_amd_batched_front_fill(
+        len(nrow), _p(nrow, C.c_int), _p(offsets, C.c_long), fronts_flat.ptr,
+        0 if np.dtype(dtype) == np.float64 else 1, int(seed))
 
 
 def batched_front_potrf(fronts_flat: "DeviceArray", nrow, ncol, offsets, dtype):

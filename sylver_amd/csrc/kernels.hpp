@@ -840,6 +840,34 @@ __device__ void front_potrf_block(const NodeDesc& nd, T* __restrict__ F, int* in
    __syncthreads();
 }
 
+// Synthetic symmetric fronts for the batched small-front workload: entry
+// (i, j) of front f is uniform in [-1, 1] from a counter-based hash of
+// (seed, f, min(i, j), max(i, j)) (splitmix64 finalizer), plus nrow on the
+// diagonal; the full square is written (ld = nrow).
+__device__ inline uint64_t mix64(uint64_t z) {
+   z += 0x9e3779b97f4a7c15ull;
+   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+   z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+   return z ^ (z >> 31);
+}
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_front_fill(const int* __restrict__ nrow,
+                                                         const int64_t* __restrict__ off, T* __restrict__ data,
+                                                         uint64_t seed) {
+   const int f = blockIdx.y;
+   const int m = nrow[f];
+   T* F = data + off[f];
+   const int64_t mm = (int64_t)m * m;
+   for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < mm; e += (int64_t)gridDim.x * kThreads) {
+      const int i = (int)(e % m), j = (int)(e / m);
+      const uint64_t lo = (uint64_t)min(i, j), hi = (uint64_t)max(i, j);
+      const uint64_t h = mix64(seed ^ mix64((uint64_t)f * 0x100000001b3ull ^ (hi << 32 | lo)));
+      double v = 2.0 * ((double)(h >> 11) * 0x1.0p-53) - 1.0;
+      if (i == j) v += (double)m;
+      F[e] = (T)v;
+   }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_small_front(const int* __restrict__ node_list,
                                                           const NodeDesc* __restrict__ nodes,

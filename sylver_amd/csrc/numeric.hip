@@ -1231,6 +1231,31 @@ double last_batched_ms() { return g_last_batched_ms; }
 // Batched dense fronts (no tree): every front through the one-workgroup
 // kernel in one variable-size launch.
 // ---------------------------------------------------------------------------
+int batched_front_fill(int nfronts, const int* nrow, const int64_t* off, void* data, int dtype,
+                       unsigned long long seed) {
+   if (nfronts < 0 || (nfronts > 0 && (!nrow || !off || !data))) return SYLVER_ERROR_CALL_SEQUENCE;
+   if (nfronts == 0) return 0;
+   if (device_count() <= 0) return SYLVER_ERROR_CUDA_UNKNOWN;
+   try {
+      int* dn = dalloc<int>(nfronts);
+      int64_t* doff = dalloc<int64_t>(nfronts);
+      HIPCHECK(hipMemcpy(dn, nrow, nfronts * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(doff, off, nfronts * sizeof(int64_t), hipMemcpyHostToDevice));
+      dim3 g(8, nfronts);
+      if (dtype == 0)
+         k_front_fill<double><<<g, kThreads>>>(dn, doff, static_cast<double*>(data), seed);
+      else
+         k_front_fill<float><<<g, kThreads>>>(dn, doff, static_cast<float*>(data), seed);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipDeviceSynchronize());
+      dfree(dn);
+      dfree(doff);
+      return 0;
+   } catch (const HipError&) {
+      return SYLVER_ERROR_CUDA_UNKNOWN;
+   }
+}
+
 int batched_front_potrf(int nfronts, const int* nrow, const int* ncol, const int64_t* off, void* data,
                         int dtype, int* info, void* stream) {
    if (nfronts < 0 || (nfronts > 0 && (!nrow || !ncol || !off || !data || !info)))

@@ -86,5 +86,7 @@ double last_batched_ms();
 
 int batched_front_potrf(int nfronts, const int* nrow, const int* ncol, const int64_t* off,
                         void* data, int dtype, int* info, void* stream);
+int batched_front_fill(int nfronts, const int* nrow, const int64_t* off, void* data, int dtype,
+                       unsigned long long seed);
 
 }  // namespace sylver_amd

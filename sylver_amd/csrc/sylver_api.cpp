@@ -408,6 +408,16 @@ int sylver_amd_fkeep_node_assembled(void const* fkeep, int node, double* front) 
    return factors_node_assembled(fk->f, node, front);
 }
 
+int sylver_amd_batched_front_fill(int nfronts, int const* nrow, long const* offset, void* data,
+                                  int dtype, unsigned long long seed) {
+   try {
+      return batched_front_fill(nfronts, nrow, reinterpret_cast<const int64_t*>(offset), data, dtype,
+                                seed);
+   } catch (...) {
+      return SYLVER_ERROR_UNKNOWN;
+   }
+}
+
 int sylver_amd_batched_front_potrf(int nfronts, int const* nrow, int const* ncol,
                                    long const* offset, void* data, int dtype, int* info,
                                    void* stream) {
