@@ -292,6 +292,8 @@ def main():
                     help="run only one precision path (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (scales PMC byte counters)")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the per-launch timed (eager, one stream) factorization of the FP32 path")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the secondary workloads (2D Laplacian, batched fronts, KKT)")
     args = ap.parse_args()
@@ -323,6 +325,10 @@ def main():
         f" flops={flops:.4e}")
 
     prec = S.PREC_FP32 if args.mixed == "fp32" else S.PREC_MIXED
+    if args.profile_only:  # rocprofv3 runs of the roofline figure: the same eager launches
+        kinds, fms, _ = kernel_profile(S, A, akeep, opts, prec, None)
+        print(json.dumps({"profile_only": True, "kernels": kinds}))
+        return
     if args.paths != "both":  # profiling runs: one path, no report
         pr = S.PREC_FP64 if args.paths == "fp64" else prec
         _, recs = run_path(S, A, akeep, opts, b, pr, args.steps, args.warmup, 1e-15)

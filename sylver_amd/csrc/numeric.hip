@@ -593,7 +593,9 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
    const hipStream_t s_main = s;
    for (const Launch& L : P->fprog) {
       dim3 g(L.count), b(kThreads);
-      s = L.stream ? F->stream2 : s_main;
+      // per-launch timing runs the program on one stream, so that each
+      // kernel is timed alone
+      s = (L.stream && !events) ? F->stream2 : s_main;
       if (L.wait_ev >= 0) HIPCHECK(hipStreamWaitEvent(s, F->xev[L.wait_ev], 0));
       if (events) HIPCHECK(hipEventRecord((*events)[2 * li], s));
       switch (L.kind) {
