@@ -151,7 +151,9 @@ typedef struct {
    int tile_nb;          /* block size of the large-front path (0 = auto) */
    int small_front_max;  /* fronts with nrow <= this use the one-workgroup kernel (0 = auto) */
    int record_timing;    /* record per-level device timings (0) */
-   int reserved[7];
+   int indef_panel_min;  /* indefinite fronts with at least this many fully-summed columns
+                            use the multi-workgroup APP panel (0 = auto: 256) */
+   int reserved[6];
 } sylver_amd_options_t;
 
 void sylver_amd_default_options(sylver_amd_options_t *opt);

@@ -88,7 +88,45 @@ void grow(T*& p, int64_t& cap, int64_t need, int64_t keep, hipStream_t s) {
    cap = nc;
 }
 
-int64_t ws_elems(int m, int n) { return (int64_t)m * (kAppNB + n + 5) + 8; }
+int64_t ws_elems(int m, int n) { return app_ws_elems(m, n); }
+
+// Multi-workgroup APP panel of one large front (k_lp_* launches, see
+// kernels_indef.hpp): per block of kAppNB columns diag -> permutation ->
+// apply + threshold -> finish -> update, then permute_failed and the second
+// pass. Everything stays on the device; the host only enqueues.
+void large_panel(IndefState& S, int node, int slot, double u, double small, int action, int failed_tpp,
+                 hipStream_t s) {
+   const IndefDesc& d = S.h[node];
+   const int m = d.m, n = d.n;
+   auto grid_cap = [](int64_t items, int64_t cap) {
+      return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cap, (items + kThreads - 1) / kThreads));
+   };
+   const int nt = (n + 63) / 64;
+   k_lp_start<<<(unsigned)std::min<int64_t>((int64_t)nt * (nt + 1) / 2 + 1, 2048), kThreads, 0, s>>>(
+       node, S.d_nodes, S.arena, S.d_dinv, S.d_ws);
+   for (int c0 = 0; c0 < n; c0 += kAppNB) {
+      const int kb = std::min(kAppNB, n - c0);
+      k_lp_diag<<<1, kThreads, 0, s>>>(node, c0, S.d_nodes, S.arena, S.d_ws, small, action);
+      k_lp_perm<<<1 + grid_cap((int64_t)m + n, 1024), kThreads, 0, s>>>(node, c0, S.d_nodes, S.arena, S.d_vars,
+                                                                       S.d_pres, S.d_ws);
+      k_lp_apply<<<grid_cap((int64_t)m - kb, 1024), kThreads, 0, s>>>(node, c0, S.d_nodes, S.arena, S.d_ws, u,
+                                                                       small);
+      k_lp_finish<<<1, kThreads, 0, s>>>(node, c0, S.d_nodes, S.arena, S.d_dinv, S.d_ws);
+      // tiles of the largest possible index space (every earlier column failed)
+      const int64_t tr = (m - c0 + kLpTile - 1) / kLpTile + (c0 + kLpTile - 1) / kLpTile;
+      const int64_t tiles = tr * (tr + 1) / 2;
+      k_lp_update<<<(unsigned)std::min<int64_t>(tiles, 1024), kThreads, 0, s>>>(node, c0, S.d_nodes, S.arena,
+                                                                                 S.d_ws);
+   }
+   k_lp_np<<<1, kThreads, 0, s>>>(node, S.d_nodes, S.d_ws);
+   const int64_t mn = (int64_t)m * n;
+   k_lp_pgather<<<grid_cap(mn, 4096), kThreads, 0, s>>>(node, S.d_nodes, S.arena, S.d_vars, S.d_pres, S.d_dinv,
+                                                        S.d_ws);
+   k_lp_pscatter<<<grid_cap(mn, 4096), kThreads, 0, s>>>(node, S.d_nodes, S.arena, S.d_vars, S.d_pres,
+                                                         S.d_dinv, S.d_ws);
+   k_lp_tail<<<1, kThreads, 0, s>>>(node, slot, S.d_nodes, S.arena, S.d_vars, S.d_pres, S.d_dinv, S.d_ws, u,
+                                    small, action, failed_tpp, S.d_status, S.d_out);
+}
 
 }  // namespace
 
@@ -138,6 +176,7 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
    }
    const double u = opts.u, small = opts.small;
    const int action = opts.action ? 1 : 0;
+   const int panel_min = ext_options().indef_panel_min > 0 ? ext_options().indef_panel_min : 256;
    const int failed_tpp = (opts.failed_pivot_method == 1 || opts.pivot_method == 3) ? 1 : 0;
    int64_t arena_used = 0, vars_used = 0, pres_used = 0, dinv_used = 0, x_used = 0;
    S.num_factor = 0;
@@ -222,11 +261,40 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
          dim3 g(64, rk.second);
          k_ind_ea<<<g, kThreads, 0, s>>>(S.d_list + rk.first, S.d_nodes, P->d_cmap, S.arena);
       }
-      k_app_front<<<nl, kThreads, 0, s>>>(S.d_list, S.d_nodes, S.arena, S.d_vars, S.d_pres, S.d_dinv,
-                                          S.d_ws, u, small, action, failed_tpp, S.d_status, S.d_out);
-      std::vector<int> out(2 * nl);
-      HIPCHECK(hipMemcpyAsync(out.data(), S.d_out, out.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+      // panels: one workgroup per front for the small ones (batched), the
+      // multi-workgroup sequence for each large one; out slots small first
+      std::vector<int> slot(nl);
+      int nsmall = 0;
+      for (int k = 0; k < nl; ++k)
+         if (S.h[lv[k]].n < panel_min) slot[k] = nsmall++;
+      {
+         int j = nsmall;
+         for (int k = 0; k < nl; ++k)
+            if (S.h[lv[k]].n >= panel_min) slot[k] = j++;
+      }
+      if (nsmall > 0) {
+         std::vector<int> sl;
+         for (int k = 0; k < nl; ++k)
+            if (S.h[lv[k]].n < panel_min) sl.push_back(lv[k]);
+         const int64_t o_small = (int64_t)lists.size();
+         grow(S.d_list, S.list_cap, o_small + nsmall, o_small, s);
+         HIPCHECK(hipMemcpyAsync(S.d_list + o_small, sl.data(), sl.size() * sizeof(int), hipMemcpyHostToDevice, s));
+         k_app_front<<<nsmall, kThreads, 0, s>>>(S.d_list + o_small, S.d_nodes, S.arena, S.d_vars, S.d_pres,
+                                                 S.d_dinv, S.d_ws, u, small, action, failed_tpp, S.d_status,
+                                                 S.d_out);
+      }
+      for (int k = 0; k < nl; ++k) {
+         const IndefDesc& d = S.h[lv[k]];
+         if (d.n < panel_min) continue;
+         large_panel(S, lv[k], slot[k], u, small, action, failed_tpp, s);
+      }
+      std::vector<int> outs(2 * nl), out(2 * nl);
+      HIPCHECK(hipMemcpyAsync(outs.data(), S.d_out, outs.size() * sizeof(int), hipMemcpyDeviceToHost, s));
       HIPCHECK(hipStreamSynchronize(s));
+      for (int k = 0; k < nl; ++k) {
+         out[2 * k] = outs[2 * slot[k]];
+         out[2 * k + 1] = outs[2 * slot[k] + 1];
+      }
       // contribution blocks: L D into the workspace, then the MFMA update
       std::vector<int> cb_nodes;
       std::vector<GemmTask> tasks;

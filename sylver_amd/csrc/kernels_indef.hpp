@@ -128,6 +128,316 @@ __device__ __forceinline__ void apply_dinv1(double d0, double small, double& x) 
       x = x * d0;
 }
 
+// ApplyN / ApplyT of one uneliminated row (apply_pivot and check_threshold,
+// src/kernels/ldlt_app.hxx:144-255): x holds the row's kb entries in the
+// block columns; x := x L11^{-T} D^{-1} with the unit lower L11 of T and the
+// D^{-1} entries dl. Returns the first column with |l| > 1/u (kb if none).
+__device__ __forceinline__ int apply_row(double (&x)[kAppNB], const double (*T)[kAppNB + 1],
+                                         const double* dl, int kb, double small, double inv_u) {
+#pragma clang fp contract(off)
+#pragma unroll
+   for (int j = 1; j < kAppNB; ++j)
+#pragma unroll
+      for (int i = 0; i < j; ++i)
+         if (j < kb) x[j] = fnma(x[i], T[j][i], x[j]);
+   bool second = false;  // j is the second column of a 2x2 pivot
+#pragma unroll
+   for (int j = 0; j < kAppNB; ++j) {
+      if (j >= kb) continue;
+      if (second) {
+         second = false;
+      } else if (j + 1 < kb && __builtin_isinf(dl[2 * j + 2])) {
+         const double a1 = x[j], a2 = x[(j + 1) % kAppNB];
+         x[j] = dot2(dl[2 * j], a1, dl[2 * j + 1], a2);
+         x[(j + 1) % kAppNB] = dot2(dl[2 * j + 1], a1, dl[2 * j + 3], a2);
+         second = true;
+      } else {
+         apply_dinv1(dl[2 * j], small, x[j]);
+      }
+   }
+   int fail = kb;
+#pragma unroll
+   for (int j = kAppNB - 1; j >= 0; --j)
+      if (j < kb && fabs(x[j]) > inv_u) fail = j;
+   return fail;
+}
+
+// Passed columns of a block from the first failing one: a 2x2 pivot split by
+// the threshold test is dropped (Column::adjust, include/sylver/kernels/Column.hxx:87-101).
+__device__ __forceinline__ int npass_adjust(int np, const double* dl) {
+   if (np > 0 && !__builtin_isinf(dl[2 * (np - 1)]) && dl[2 * (np - 1) + 1] != 0.0) --np;
+   return np;
+}
+
+// (D L[s, :])_p of the update with pivot p of a block (calcLD): D is the
+// inverse of the stored D^{-1}, 1x1 or 2x2; fm, f0, fp are L[s, p - 1],
+// L[s, p], L[s, p + 1].
+__device__ __forceinline__ double vs_value(const double* dl, int p, int npass, double fm, double f0, double fp) {
+#pragma clang fp contract(off)
+   if (p + 1 < npass && __builtin_isinf(dl[2 * p + 2])) {
+      const double det = __builtin_fma(dl[2 * p], dl[2 * p + 3], -(dl[2 * p + 1] * dl[2 * p + 1]));
+      const double D0 = dl[2 * p + 3] / det, D1 = -dl[2 * p + 1] / det;
+      return dot2(D0, f0, D1, fp);
+   }
+   if (p > 0 && __builtin_isinf(dl[2 * p])) {
+      const int q = p - 1;
+      const double det = __builtin_fma(dl[2 * q], dl[2 * q + 3], -(dl[2 * q + 1] * dl[2 * q + 1]));
+      const double D1 = -dl[2 * q + 1] / det, D3 = dl[2 * q] / det;
+      return dot2(D1, fm, D3, f0);
+   }
+   const double D0 = (dl[2 * p] == 0.0) ? 0.0 : 1.0 / dl[2 * p];
+   return D0 * f0;
+}
+
+// Bunch-Parlett LDL^T of the kb x kb diagonal block T (LDS, both triangles
+// on entry) with complete pivoting inside the block (alpha = (1 + sqrt 17) / 8):
+// 1x1 pivots [d^{-1}, 0] and 2x2 pivots [a, b, inf, c] into dl, L below the
+// diagonal of T. `swap_local(p, q)` exchanges positions p and q of the block
+// (T and whatever the caller keeps in step). A pivot below `small` ends the
+// block: with action the rest is zero pivots, without it the return is 1.
+// All threads call; the result is uniform.
+template <typename Swap>
+__device__ int bp_block(double (*T)[kAppNB + 1], double* dl, int kb, double small, int action,
+                        double* red_v, int* red_i, Swap&& swap_local) {
+#pragma clang fp contract(off)
+   const int tid = threadIdx.x;
+   for (int s = 0; s < kb;) {
+      double v0 = -1.0, v1 = -1.0;
+      int i0 = 0x7fffffff, i1 = 0x7fffffff;
+      const int rem = kb - s;
+      for (int e = tid; e < rem * rem; e += kThreads) {
+         const int i = s + e % rem, j = s + e / rem;
+         if (i < j) continue;
+         const double v = fabs(T[i][j]);
+         const int idx = j * kb + i;
+         if (v > v0 || (v == v0 && idx < i0)) {
+            v0 = v;
+            i0 = idx;
+         }
+         if (i == j && (v > v1 || (v == v1 && i < i1))) {
+            v1 = v;
+            i1 = i;
+         }
+      }
+      block_argmax(v0, i0, red_v, red_i);
+      block_argmax(v1, i1, red_v, red_i);
+      if (v0 < small) {
+         if (!action) return 1;
+         for (int k = s + tid; k < kb; k += kThreads) {
+            dl[2 * k] = 0.0;
+            dl[2 * k + 1] = 0.0;
+         }
+         for (int e = tid; e < kb * kb; e += kThreads) {
+            const int i = e % kb, j = e / kb;
+            if (j >= s && i > j) T[i][j] = 0.0;
+         }
+         __syncthreads();
+         break;
+      }
+      if (v1 >= 0.6403882032022076 * v0) {
+         swap_local(s, i1);
+         const double dv = T[s][s];
+         const double d0 = 1.0 / dv;
+         if (tid == 0) {
+            dl[2 * s] = d0;
+            dl[2 * s + 1] = 0.0;
+         }
+         for (int e = tid; e < rem * rem; e += kThreads) {
+            const int i = s + 1 + e % rem, j = s + 1 + e / rem;
+            if (i >= kb || j > i) continue;
+            const double li = T[i][s] * d0;
+            T[i][j] = fnma(li, T[j][s], T[i][j]);
+         }
+         __syncthreads();
+         for (int i = s + 1 + tid; i < kb; i += kThreads) T[i][s] = T[i][s] * d0;
+         s += 1;
+      } else {
+         int j0 = i0 / kb, ii0 = i0 % kb;
+         swap_local(s, j0);
+         if (ii0 == s) ii0 = j0;
+         swap_local(s + 1, ii0);
+         const double a = T[s][s], b = T[s + 1][s], c = T[s + 1][s + 1];
+         const double det = __builtin_fma(a, c, -(b * b));
+         const double d0 = c / det, d1 = -b / det, d3 = a / det;
+         if (tid == 0) {
+            dl[2 * s] = d0;
+            dl[2 * s + 1] = d1;
+            dl[2 * s + 2] = __builtin_inf();
+            dl[2 * s + 3] = d3;
+         }
+         for (int e = tid; e < rem * rem; e += kThreads) {
+            const int i = s + 2 + e % rem, j = s + 2 + e / rem;
+            if (i >= kb || j > i) continue;
+            const double a1 = T[i][s], a2 = T[i][s + 1];
+            const double l0 = dot2(d0, a1, d1, a2), l1 = dot2(d1, a1, d3, a2);
+            T[i][j] = fnma(l1, T[j][s + 1], fnma(l0, T[j][s], T[i][j]));
+         }
+         __syncthreads();
+         for (int i = s + 2 + tid; i < kb; i += kThreads) {
+            const double a1 = T[i][s], a2 = T[i][s + 1];
+            T[i][s] = dot2(d0, a1, d1, a2);
+            T[i][s + 1] = dot2(d1, a1, d3, a2);
+         }
+         if (tid == 0) T[s + 1][s] = 0.0;
+         s += 2;
+      }
+      __syncthreads();
+      for (int e = tid; e < kb * kb; e += kThreads) {  // keep the trailing block symmetric
+         const int i = e % kb, j = e / kb;
+         if (j >= s && i > j) T[j][i] = T[i][j];
+      }
+      __syncthreads();
+   }
+   return 0;
+}
+
+// Second pass on the failed columns nelim..n-1 (ldlt_tpp_factor semantics,
+// src/factor_failed.hxx:29-157): threshold partial pivoting over all rows,
+// 1x1 when |a_pp| >= u max|a_ip|, else a 2x2 with the largest off-diagonal
+// row when the Duff-Reid growth test passes; zero pivots below `small`
+// (with action). One workgroup; nelim is advanced. Returns 1 when a zero
+// pivot stops the factorization with action == 0.
+__device__ int tpp_pass(double* F, int ld, int m, int n, int& nelim, int* var, int* pre, double* D, double* TL,
+                        double u, double small, int action, double* red_v, int* red_i) {
+#pragma clang fp contract(off)
+   const int tid = threadIdx.x;
+   while (nelim < n) {
+      int piv = -1, piv_t = -1, w = 0;
+      for (int p = nelim; p < n; ++p) {
+         double amax = 0.0;
+         int t = 0x7fffffff;
+         for (int i = nelim + tid; i < m; i += kThreads) {
+            if (i == p) continue;
+            const double v = fabs(F[i + (int64_t)p * ld]);
+            if (v > amax || (v == amax && v > 0.0 && i < t)) {
+               amax = v;
+               t = i;
+            }
+         }
+         block_argmax(amax, t, red_v, red_i);
+         if (!(amax > 0.0)) t = -1;
+         const double app = fabs(F[p + (int64_t)p * ld]);
+         if (fmax(app, amax) < small) {
+            if (!action) return 1;
+            piv = p;
+            w = 0;  // zero pivot
+            break;
+         }
+         if (app >= u * amax) {
+            piv = p;
+            w = 1;
+            break;
+         }
+         if (t >= 0 && t < n) {
+            const double a11 = F[p + (int64_t)p * ld], a21 = F[t + (int64_t)p * ld],
+                         a22 = F[t + (int64_t)t * ld];
+            const double det = __builtin_fma(a11, a22, -(a21 * a21));
+            if (det != 0.0) {
+               const double d11 = a22 / det, d21 = -a21 / det, d22 = a11 / det;
+               double pm = 0.0, tm = 0.0;
+               for (int i = nelim + tid; i < m; i += kThreads) {
+                  if (i == p || i == t) continue;
+                  pm = fmax(pm, fabs(F[i + (int64_t)p * ld]));
+                  tm = fmax(tm, fabs(F[i + (int64_t)t * ld]));
+               }
+               pm = block_max(pm, red_v);
+               tm = block_max(tm, red_v);
+               const double g = fmax(dot2(fabs(d11), pm, fabs(d21), tm), dot2(fabs(d21), pm, fabs(d22), tm));
+               if (u * g <= 1.0) {
+                  piv = p;
+                  piv_t = t;
+                  w = 2;
+                  break;
+               }
+            }
+         }
+      }
+      if (piv < 0) break;
+      front_swap(F, ld, m, n, piv, nelim, var, pre);
+      if (w == 0) {
+         for (int i = nelim + 1 + tid; i < m; i += kThreads) {
+            F[i + (int64_t)nelim * ld] = 0.0;
+            if (i < n) F[nelim + (int64_t)i * ld] = 0.0;
+         }
+         if (tid == 0) {
+            D[2 * nelim] = 0.0;
+            D[2 * nelim + 1] = 0.0;
+         }
+         __syncthreads();
+         nelim += 1;
+         continue;
+      }
+      double d[4];
+      if (w == 2) {
+         int t = piv_t;
+         if (t == nelim) t = piv;
+         front_swap(F, ld, m, n, t, nelim + 1, var, pre);
+         const double b11 = F[nelim + (int64_t)nelim * ld], b21 = F[nelim + 1 + (int64_t)nelim * ld],
+                      b22 = F[nelim + 1 + (int64_t)(nelim + 1) * ld];
+         const double dt = __builtin_fma(b11, b22, -(b21 * b21));
+         d[0] = b22 / dt;
+         d[1] = -b21 / dt;
+         d[2] = __builtin_inf();
+         d[3] = b11 / dt;
+      } else {
+         d[0] = 1.0 / F[nelim + (int64_t)nelim * ld];
+         d[1] = 0.0;
+      }
+      // l = v D^{-1} for every row below; v kept for the update
+      double* lv = TL;             // 2m
+      double* vv = TL + 2 * (int64_t)m;
+      const int r0 = nelim + w;
+      for (int r = r0 + tid; r < m; r += kThreads) {
+         const double a1 = F[r + (int64_t)nelim * ld];
+         const double a2 = (w == 2) ? F[r + (int64_t)(nelim + 1) * ld] : 0.0;
+         vv[2 * r] = a1;
+         vv[2 * r + 1] = a2;
+         if (w == 2) {
+            lv[2 * r] = dot2(d[0], a1, d[1], a2);
+            lv[2 * r + 1] = dot2(d[1], a1, d[3], a2);
+         } else {
+            double x = a1;
+            apply_dinv1(d[0], small, x);
+            lv[2 * r] = x;
+            lv[2 * r + 1] = 0.0;
+         }
+      }
+      __syncthreads();
+      for (int64_t e = tid; e < (int64_t)(m - r0) * (n - r0); e += kThreads) {
+         const int r = r0 + (int)(e % (m - r0)), s = r0 + (int)(e / (m - r0));
+         double x = fnma(lv[2 * r], vv[2 * s], F[r + (int64_t)s * ld]);
+         if (w == 2) x = fnma(lv[2 * r + 1], vv[2 * s + 1], x);
+         F[r + (int64_t)s * ld] = x;
+      }
+      __syncthreads();
+      for (int64_t e = tid; e < (int64_t)(n - r0) * (n - r0); e += kThreads) {
+         const int i = r0 + (int)(e % (n - r0)), j = r0 + (int)(e / (n - r0));
+         if (i > j) F[j + (int64_t)i * ld] = F[i + (int64_t)j * ld];
+      }
+      for (int r = r0 + tid; r < m; r += kThreads) {
+         F[r + (int64_t)nelim * ld] = lv[2 * r];
+         if (r < n) F[nelim + (int64_t)r * ld] = lv[2 * r];
+         if (w == 2) {
+            F[r + (int64_t)(nelim + 1) * ld] = lv[2 * r + 1];
+            if (r < n) F[nelim + 1 + (int64_t)r * ld] = lv[2 * r + 1];
+         }
+      }
+      if (tid == 0) {
+         D[2 * nelim] = d[0];
+         D[2 * nelim + 1] = d[1];
+         if (w == 2) {
+            D[2 * nelim + 2] = d[2];
+            D[2 * nelim + 3] = d[3];
+            F[nelim + 1 + (int64_t)nelim * ld] = 0.0;
+            F[nelim + (int64_t)(nelim + 1) * ld] = 0.0;
+         }
+      }
+      __syncthreads();
+      nelim += w;
+   }
+   return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Panel factorization of one front per workgroup: APP over blocks of kAppNB
 // columns, permute_failed, TPP on the failed columns. Writes nelim,
@@ -222,95 +532,8 @@ static __global__ __launch_bounds__(kThreads) void k_app_front(const int* __rest
          __syncthreads();
          front_swap(F, ld, m, n, c0 + p, c0 + q, var, pre);
       };
-      for (int s = 0; s < kb;) {
-         double v0 = -1.0, v1 = -1.0;
-         int i0 = 0x7fffffff, i1 = 0x7fffffff;
-         const int rem = kb - s;
-         for (int e = tid; e < rem * rem; e += kThreads) {
-            const int i = s + e % rem, j = s + e / rem;
-            if (i < j) continue;
-            const double v = fabs(T[i][j]);
-            const int idx = j * kb + i;
-            if (v > v0 || (v == v0 && idx < i0)) {
-               v0 = v;
-               i0 = idx;
-            }
-            if (i == j && (v > v1 || (v == v1 && i < i1))) {
-               v1 = v;
-               i1 = i;
-            }
-         }
-         block_argmax(v0, i0, red_v, red_i);
-         block_argmax(v1, i1, red_v, red_i);
-         if (v0 < small) {
-            if (!action) {
-               if (tid == 0) s_bad = 1;
-               __syncthreads();
-               break;
-            }
-            for (int k = s + tid; k < kb; k += kThreads) {
-               dl[2 * k] = 0.0;
-               dl[2 * k + 1] = 0.0;
-            }
-            for (int e = tid; e < kb * kb; e += kThreads) {
-               const int i = e % kb, j = e / kb;
-               if (j >= s && i > j) T[i][j] = 0.0;
-            }
-            __syncthreads();
-            break;
-         }
-         if (v1 >= 0.6403882032022076 * v0) {
-            swap_local(s, i1);
-            const double dv = T[s][s];
-            const double d0 = 1.0 / dv;
-            if (tid == 0) {
-               dl[2 * s] = d0;
-               dl[2 * s + 1] = 0.0;
-            }
-            for (int e = tid; e < rem * rem; e += kThreads) {
-               const int i = s + 1 + e % rem, j = s + 1 + e / rem;
-               if (i >= kb || j > i) continue;
-               const double li = T[i][s] * d0;
-               T[i][j] = fnma(li, T[j][s], T[i][j]);
-            }
-            __syncthreads();
-            for (int i = s + 1 + tid; i < kb; i += kThreads) T[i][s] = T[i][s] * d0;
-            s += 1;
-         } else {
-            int j0 = i0 / kb, ii0 = i0 % kb;
-            swap_local(s, j0);
-            if (ii0 == s) ii0 = j0;
-            swap_local(s + 1, ii0);
-            const double a = T[s][s], b = T[s + 1][s], c = T[s + 1][s + 1];
-            const double det = __builtin_fma(a, c, -(b * b));
-            const double d0 = c / det, d1 = -b / det, d3 = a / det;
-            if (tid == 0) {
-               dl[2 * s] = d0;
-               dl[2 * s + 1] = d1;
-               dl[2 * s + 2] = __builtin_inf();
-               dl[2 * s + 3] = d3;
-            }
-            for (int e = tid; e < rem * rem; e += kThreads) {
-               const int i = s + 2 + e % rem, j = s + 2 + e / rem;
-               if (i >= kb || j > i) continue;
-               const double a1 = T[i][s], a2 = T[i][s + 1];
-               const double l0 = dot2(d0, a1, d1, a2), l1 = dot2(d1, a1, d3, a2);
-               T[i][j] = fnma(l1, T[j][s + 1], fnma(l0, T[j][s], T[i][j]));
-            }
-            __syncthreads();
-            for (int i = s + 2 + tid; i < kb; i += kThreads) {
-               const double a1 = T[i][s], a2 = T[i][s + 1];
-               T[i][s] = dot2(d0, a1, d1, a2);
-               T[i][s + 1] = dot2(d1, a1, d3, a2);
-            }
-            if (tid == 0) T[s + 1][s] = 0.0;
-            s += 2;
-         }
-         __syncthreads();
-         for (int e = tid; e < kb * kb; e += kThreads) {  // keep the trailing block symmetric
-            const int i = e % kb, j = e / kb;
-            if (j >= s && i > j) T[j][i] = T[i][j];
-         }
+      if (bp_block(T, dl, kb, small, action, red_v, red_i, swap_local)) {
+         if (tid == 0) s_bad = 1;
          __syncthreads();
       }
       if (s_bad) break;
@@ -323,26 +546,7 @@ static __global__ __launch_bounds__(kThreads) void k_app_front(const int* __rest
          double x[NB];
 #pragma unroll
          for (int j = 0; j < NB; ++j) x[j] = (j < kb) ? F[r + (int64_t)(c0 + j) * ld] : 0.0;
-#pragma unroll
-         for (int j = 1; j < NB; ++j)
-#pragma unroll
-            for (int i = 0; i < j; ++i)
-               if (j < kb) x[j] = fnma(x[i], T[j][i], x[j]);
-         int fail = kb;
-         for (int j = 0; j < kb;) {
-            if (j + 1 < kb && __builtin_isinf(dl[2 * j + 2])) {
-               const double a1 = x[j], a2 = x[j + 1];
-               x[j] = dot2(dl[2 * j], a1, dl[2 * j + 1], a2);
-               x[j + 1] = dot2(dl[2 * j + 1], a1, dl[2 * j + 3], a2);
-               j += 2;
-            } else {
-               apply_dinv1(dl[2 * j], small, x[j]);
-               j += 1;
-            }
-         }
-#pragma unroll
-         for (int j = NB - 1; j >= 0; --j)
-            if (j < kb && fabs(x[j]) > inv_u) fail = j;
+         const int fail = apply_row(x, T, dl, kb, small, inv_u);
          if (fail < kb) atomicMin(&s_fail, fail);
 #pragma unroll
          for (int j = 0; j < NB; ++j)
@@ -353,9 +557,7 @@ static __global__ __launch_bounds__(kThreads) void k_app_front(const int* __rest
       }
       __syncthreads();
       if (tid == 0) {
-         int np = s_fail;
-         if (np > 0 && !__builtin_isinf(dl[2 * (np - 1)]) && dl[2 * (np - 1) + 1] != 0.0) --np;
-         s_npass = np;
+         s_npass = npass_adjust(s_fail, dl);
       }
       __syncthreads();
       const int npass = s_npass;
@@ -384,21 +586,10 @@ static __global__ __launch_bounds__(kThreads) void k_app_front(const int* __rest
             for (int e = tid; e < sc * npass; e += kThreads) {
                const int t = e % sc, p = e / sc;
                const int s = s0 + t;
-               double v = 0.0;
-               if (p + 1 < npass && __builtin_isinf(dl[2 * p + 2])) {
-                  const double det = __builtin_fma(dl[2 * p], dl[2 * p + 3], -(dl[2 * p + 1] * dl[2 * p + 1]));
-                  const double D0 = dl[2 * p + 3] / det, D1 = -dl[2 * p + 1] / det;
-                  v = dot2(D0, F[s + (int64_t)(c0 + p) * ld], D1, F[s + (int64_t)(c0 + p + 1) * ld]);
-               } else if (p > 0 && __builtin_isinf(dl[2 * p])) {
-                  const int q = p - 1;
-                  const double det = __builtin_fma(dl[2 * q], dl[2 * q + 3], -(dl[2 * q + 1] * dl[2 * q + 1]));
-                  const double D1 = -dl[2 * q + 1] / det, D3 = dl[2 * q] / det;
-                  v = dot2(D1, F[s + (int64_t)(c0 + q) * ld], D3, F[s + (int64_t)(c0 + p) * ld]);
-               } else {
-                  const double D0 = (dl[2 * p] == 0.0) ? 0.0 : 1.0 / dl[2 * p];
-                  v = D0 * F[s + (int64_t)(c0 + p) * ld];
-               }
-               VS[t][p] = v;
+               const double* fs = F + s + (int64_t)c0 * ld;
+               const double fm = p > 0 ? fs[(int64_t)(p - 1) * ld] : 0.0;
+               const double fp = p + 1 < npass ? fs[(int64_t)(p + 1) * ld] : 0.0;
+               VS[t][p] = vs_value(dl, p, npass, fm, fs[(int64_t)p * ld], fp);
             }
             __syncthreads();
             for (int r = tid; r < m; r += kThreads) {
@@ -483,146 +674,13 @@ static __global__ __launch_bounds__(kThreads) void k_app_front(const int* __rest
    }
    // ---- second pass: threshold partial pivoting on the failed columns ----
    if (nelim < n && (m == n || failed_tpp)) {
-      while (nelim < n) {
-         int piv = -1, piv_t = -1, w = 0;
-         for (int p = nelim; p < n; ++p) {
-            double amax = 0.0;
-            int t = 0x7fffffff;
-            for (int i = nelim + tid; i < m; i += kThreads) {
-               if (i == p) continue;
-               const double v = fabs(F[i + (int64_t)p * ld]);
-               if (v > amax || (v == amax && v > 0.0 && i < t)) {
-                  amax = v;
-                  t = i;
-               }
-            }
-            block_argmax(amax, t, red_v, red_i);
-            if (!(amax > 0.0)) t = -1;
-            const double app = fabs(F[p + (int64_t)p * ld]);
-            if (fmax(app, amax) < small) {
-               if (!action) {
-                  if (tid == 0) {
-                     status[node] = -5;
-                     out[2 * blockIdx.x] = nelim;
-                     out[2 * blockIdx.x + 1] = nelim_first;
-                  }
-                  return;
-               }
-               piv = p;
-               w = 0;  // zero pivot
-               break;
-            }
-            if (app >= u * amax) {
-               piv = p;
-               w = 1;
-               break;
-            }
-            if (t >= 0 && t < n) {
-               const double a11 = F[p + (int64_t)p * ld], a21 = F[t + (int64_t)p * ld],
-                            a22 = F[t + (int64_t)t * ld];
-               const double det = __builtin_fma(a11, a22, -(a21 * a21));
-               if (det != 0.0) {
-                  const double d11 = a22 / det, d21 = -a21 / det, d22 = a11 / det;
-                  double pm = 0.0, tm = 0.0;
-                  for (int i = nelim + tid; i < m; i += kThreads) {
-                     if (i == p || i == t) continue;
-                     pm = fmax(pm, fabs(F[i + (int64_t)p * ld]));
-                     tm = fmax(tm, fabs(F[i + (int64_t)t * ld]));
-                  }
-                  pm = block_max(pm, red_v);
-                  tm = block_max(tm, red_v);
-                  const double g = fmax(dot2(fabs(d11), pm, fabs(d21), tm), dot2(fabs(d21), pm, fabs(d22), tm));
-                  if (u * g <= 1.0) {
-                     piv = p;
-                     piv_t = t;
-                     w = 2;
-                     break;
-                  }
-               }
-            }
-         }
-         if (piv < 0) break;
-         front_swap(F, ld, m, n, piv, nelim, var, pre);
-         if (w == 0) {
-            for (int i = nelim + 1 + tid; i < m; i += kThreads) {
-               F[i + (int64_t)nelim * ld] = 0.0;
-               if (i < n) F[nelim + (int64_t)i * ld] = 0.0;
-            }
-            if (tid == 0) {
-               D[2 * nelim] = 0.0;
-               D[2 * nelim + 1] = 0.0;
-            }
-            __syncthreads();
-            nelim += 1;
-            continue;
-         }
-         double d[4];
-         if (w == 2) {
-            int t = piv_t;
-            if (t == nelim) t = piv;
-            front_swap(F, ld, m, n, t, nelim + 1, var, pre);
-            const double b11 = F[nelim + (int64_t)nelim * ld], b21 = F[nelim + 1 + (int64_t)nelim * ld],
-                         b22 = F[nelim + 1 + (int64_t)(nelim + 1) * ld];
-            const double dt = __builtin_fma(b11, b22, -(b21 * b21));
-            d[0] = b22 / dt;
-            d[1] = -b21 / dt;
-            d[2] = __builtin_inf();
-            d[3] = b11 / dt;
-         } else {
-            d[0] = 1.0 / F[nelim + (int64_t)nelim * ld];
-            d[1] = 0.0;
-         }
-         // l = v D^{-1} for every row below; v kept for the update
-         double* lv = TL;             // 2m
-         double* vv = TL + 2 * (int64_t)m;
-         const int r0 = nelim + w;
-         for (int r = r0 + tid; r < m; r += kThreads) {
-            const double a1 = F[r + (int64_t)nelim * ld];
-            const double a2 = (w == 2) ? F[r + (int64_t)(nelim + 1) * ld] : 0.0;
-            vv[2 * r] = a1;
-            vv[2 * r + 1] = a2;
-            if (w == 2) {
-               lv[2 * r] = dot2(d[0], a1, d[1], a2);
-               lv[2 * r + 1] = dot2(d[1], a1, d[3], a2);
-            } else {
-               double x = a1;
-               apply_dinv1(d[0], small, x);
-               lv[2 * r] = x;
-               lv[2 * r + 1] = 0.0;
-            }
-         }
-         __syncthreads();
-         for (int64_t e = tid; e < (int64_t)(m - r0) * (n - r0); e += kThreads) {
-            const int r = r0 + (int)(e % (m - r0)), s = r0 + (int)(e / (m - r0));
-            double x = fnma(lv[2 * r], vv[2 * s], F[r + (int64_t)s * ld]);
-            if (w == 2) x = fnma(lv[2 * r + 1], vv[2 * s + 1], x);
-            F[r + (int64_t)s * ld] = x;
-         }
-         __syncthreads();
-         for (int64_t e = tid; e < (int64_t)(n - r0) * (n - r0); e += kThreads) {
-            const int i = r0 + (int)(e % (n - r0)), j = r0 + (int)(e / (n - r0));
-            if (i > j) F[j + (int64_t)i * ld] = F[i + (int64_t)j * ld];
-         }
-         for (int r = r0 + tid; r < m; r += kThreads) {
-            F[r + (int64_t)nelim * ld] = lv[2 * r];
-            if (r < n) F[nelim + (int64_t)r * ld] = lv[2 * r];
-            if (w == 2) {
-               F[r + (int64_t)(nelim + 1) * ld] = lv[2 * r + 1];
-               if (r < n) F[nelim + 1 + (int64_t)r * ld] = lv[2 * r + 1];
-            }
-         }
+      if (tpp_pass(F, ld, m, n, nelim, var, pre, D, TL, u, small, action, red_v, red_i)) {
          if (tid == 0) {
-            D[2 * nelim] = d[0];
-            D[2 * nelim + 1] = d[1];
-            if (w == 2) {
-               D[2 * nelim + 2] = d[2];
-               D[2 * nelim + 3] = d[3];
-               F[nelim + 1 + (int64_t)nelim * ld] = 0.0;
-               F[nelim + (int64_t)(nelim + 1) * ld] = 0.0;
-            }
+            status[node] = -5;
+            out[2 * blockIdx.x] = nelim;
+            out[2 * blockIdx.x + 1] = nelim_first;
          }
-         __syncthreads();
-         nelim += w;
+         return;
       }
    }
    if (tid == 0) {
@@ -630,6 +688,534 @@ static __global__ __launch_bounds__(kThreads) void k_app_front(const int* __rest
       nodes[node].nelim_first = nelim_first;
       out[2 * blockIdx.x] = nelim;
       out[2 * blockIdx.x + 1] = nelim_first;
+   }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-workgroup panel of one large front. The same APP steps as
+// k_app_front, split into launches per block of kAppNB columns, each launch
+// spread over the whole GPU except the two small bookkeeping ones:
+//   k_lp_diag    Bunch-Parlett of the diagonal block (one workgroup), the
+//                block permutation recorded instead of applied;
+//   k_lp_perm    permutation of the block's rows and columns, backup;
+//   k_lp_apply   ApplyN/ApplyT + threshold on every uneliminated row;
+//   k_lp_finish  passed count, L11, D^{-1}, failed-column list;
+//   k_lp_update  restore of the failed columns and update of every
+//                uneliminated fully-summed column (lower triangle, mirrored),
+//                128 x 128 tiles, one FMA chain per entry in pivot order.
+// Then k_lp_np / k_lp_pgather / k_lp_pscatter (permute_failed) and k_lp_tail
+// (the second pass). Every entry goes through the same arithmetic, in the
+// same order, as in k_app_front, so the two give identical factors.
+// The panel state follows the single-workgroup workspace: block T, its
+// D^{-1}, counters, the block permutation and the list of failed columns.
+// ---------------------------------------------------------------------------
+enum { kStFail = 0, kStNpass = 1, kStBad = 2, kStNelim = 3, kStNf = 4, kStNfOld = 5 };
+
+struct AppWs {
+   double* Bk;  // m x kAppNB backup of the block columns
+   double* PS;  // m x n permutation scratch
+   double* TL;  // 4 m second-pass scratch
+   int* EL;     // m eliminated flags
+   int* NP;     // m new positions
+   double* TG;  // kAppNB x kAppNB factored diagonal block
+   double* DL;  // 2 kAppNB D^{-1} entries of the block
+   int* ST;     // 16 counters (kSt*)
+   int* PERM;   // kAppNB block permutation
+   int* FL;     // n failed columns, ascending
+};
+
+__host__ __device__ inline int64_t app_ws_elems(int m, int n) {
+   return (int64_t)m * (kAppNB + n + 5) + 8 + kAppNB * kAppNB + 2 * kAppNB + 8 + kAppNB / 2 + (n + 1) / 2 + 8;
+}
+
+__device__ inline AppWs app_ws(double* base, int m, int n) {
+   AppWs w;
+   w.Bk = base;
+   w.PS = base + (int64_t)m * kAppNB;
+   w.TL = w.PS + (int64_t)m * n;
+   w.EL = reinterpret_cast<int*>(w.TL + 4 * (int64_t)m);
+   w.NP = w.EL + m;
+   w.TG = base + (int64_t)m * (kAppNB + n + 5) + 8;
+   w.DL = w.TG + kAppNB * kAppNB;
+   w.ST = reinterpret_cast<int*>(w.DL + 2 * kAppNB);
+   w.PERM = w.ST + 16;
+   w.FL = w.PERM + kAppNB;
+   return w;
+}
+
+// Both triangles of the fully-summed block (64 x 64 tiles through LDS),
+// cleared flags, D^{-1} and counters.
+static __global__ __launch_bounds__(kThreads) void k_lp_start(int node, const IndefDesc* __restrict__ nodes,
+                                                       double* __restrict__ arena, double* __restrict__ dinv,
+                                                       double* __restrict__ wsbase) {
+   __shared__ double tile[64][65];
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, n = nd.n, ld = nd.ld, tid = threadIdx.x;
+   double* F = arena + nd.off;
+   const AppWs w = app_ws(wsbase + nd.ws_off, m, n);
+   const int64_t gid = (int64_t)blockIdx.x * kThreads + tid, G = (int64_t)gridDim.x * kThreads;
+   for (int64_t i = gid; i < m; i += G) w.EL[i] = 0;
+   for (int64_t i = gid; i < 2 * (int64_t)n; i += G) dinv[nd.d_off + i] = 0.0;
+   if (gid < 16) w.ST[gid] = 0;
+   const int nt = (n + 63) / 64;
+   const int64_t ntile = (int64_t)nt * (nt + 1) / 2;
+   for (int64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
+      int bj = 0;  // tile column of the lower tile (bi >= bj)
+      int64_t rem = t;
+      while (rem >= nt - bj) {
+         rem -= nt - bj;
+         ++bj;
+      }
+      const int bi = bj + (int)rem;
+      __syncthreads();
+      for (int e = tid; e < 64 * 64; e += kThreads) {
+         const int i = e & 63, j = e >> 6;
+         const int r = bi * 64 + i, c = bj * 64 + j;
+         tile[i][j] = (r < n && c < n) ? F[r + (int64_t)c * ld] : 0.0;
+      }
+      __syncthreads();
+      for (int e = tid; e < 64 * 64; e += kThreads) {
+         const int i = e & 63, j = e >> 6;  // entry (bj*64 + i, bi*64 + j) of the upper tile
+         const int r = bj * 64 + i, c = bi * 64 + j;
+         if (r < n && c < n && r < c) F[r + (int64_t)c * ld] = tile[j][i];
+      }
+   }
+}
+
+// Bunch-Parlett of the block at c0 (one workgroup): T, D^{-1} and the block
+// permutation into the panel state; the threshold minimum starts at kb.
+static __global__ __launch_bounds__(kThreads) void k_lp_diag(int node, int c0, const IndefDesc* __restrict__ nodes,
+                                                      const double* __restrict__ arena,
+                                                      double* __restrict__ wsbase, double small, int action) {
+#pragma clang fp contract(off)
+   constexpr int NB = kAppNB;
+   __shared__ double T[NB][NB + 1];
+   __shared__ double dl[2 * NB];
+   __shared__ double red_v[4];
+   __shared__ int red_i[4];
+   __shared__ int perm[NB];
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, n = nd.n, ld = nd.ld, tid = threadIdx.x;
+   const AppWs w = app_ws(wsbase + nd.ws_off, m, n);
+   if (w.ST[kStBad]) return;
+   const int kb = min(NB, n - c0);
+   const double* F = arena + nd.off;
+   for (int e = tid; e < kb * kb; e += kThreads) {
+      const int i = e % kb, j = e / kb;
+      T[i][j] = F[c0 + i + (int64_t)(c0 + j) * ld];
+   }
+   if (tid < NB) perm[tid] = tid;
+   __syncthreads();
+   auto swap_local = [&](int p, int q) {
+      if (p == q) return;
+      for (int k = tid; k < kb; k += kThreads) {
+         const double a = T[k][p];
+         T[k][p] = T[k][q];
+         T[k][q] = a;
+      }
+      __syncthreads();
+      for (int k = tid; k < kb; k += kThreads) {
+         const double a = T[p][k];
+         T[p][k] = T[q][k];
+         T[q][k] = a;
+      }
+      if (tid == 0) {
+         const int t = perm[p];
+         perm[p] = perm[q];
+         perm[q] = t;
+      }
+      __syncthreads();
+   };
+   if (bp_block(T, dl, kb, small, action, red_v, red_i, swap_local)) {
+      if (tid == 0) w.ST[kStBad] = 1;
+      return;
+   }
+   for (int e = tid; e < kb * kb; e += kThreads) {
+      const int i = e % kb, j = e / kb;
+      w.TG[i + j * NB] = T[i][j];
+   }
+   for (int k = tid; k < 2 * kb; k += kThreads) w.DL[k] = dl[k];
+   if (tid < kb) w.PERM[tid] = perm[tid];
+   if (tid == 0) w.ST[kStFail] = kb;
+}
+
+// The block permutation applied to the front (what the symmetric exchanges
+// of k_app_front do: columns c0.. over all rows, rows c0.. over the
+// fully-summed columns, var / pre) and the backup of the permuted block
+// columns. Workgroup 0 takes the diagonal block; the others one row (or one
+// fully-summed column) per thread.
+static __global__ __launch_bounds__(kThreads) void k_lp_perm(int node, int c0, const IndefDesc* __restrict__ nodes,
+                                                      double* __restrict__ arena, int* __restrict__ vars,
+                                                      int* __restrict__ pres, double* __restrict__ wsbase) {
+   constexpr int NB = kAppNB;
+   __shared__ int pm[NB];
+   __shared__ double Tb[NB][NB + 1];
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, n = nd.n, ld = nd.ld, tid = threadIdx.x;
+   const AppWs w = app_ws(wsbase + nd.ws_off, m, n);
+   if (w.ST[kStBad]) return;
+   const int kb = min(NB, n - c0);
+   double* F = arena + nd.off;
+   if (tid < NB) pm[tid] = tid < kb ? w.PERM[tid] : tid;
+   __syncthreads();
+   if (blockIdx.x == 0) {
+      int* var = vars + nd.var_off;
+      int* pre = pres + nd.pre_off;
+      for (int e = tid; e < kb * kb; e += kThreads) {
+         const int i = e % kb, j = e / kb;
+         Tb[i][j] = F[c0 + pm[i] + (int64_t)(c0 + pm[j]) * ld];
+      }
+      int vv = 0, pp = 0;
+      if (tid < kb) {
+         vv = var[c0 + pm[tid]];
+         pp = pre[c0 + pm[tid]];
+      }
+      __syncthreads();
+      for (int e = tid; e < kb * kb; e += kThreads) {
+         const int i = e % kb, j = e / kb;
+         F[c0 + i + (int64_t)(c0 + j) * ld] = Tb[i][j];
+         w.Bk[c0 + i + (int64_t)j * m] = Tb[i][j];
+      }
+      if (tid < kb) {
+         var[c0 + tid] = vv;
+         pre[c0 + tid] = pp;
+      }
+      return;
+   }
+   const int64_t items = (int64_t)m + n;  // the launch has at least two workgroups
+   for (int64_t it = (int64_t)(blockIdx.x - 1) * kThreads + tid; it < items;
+        it += (int64_t)max(1u, gridDim.x - 1) * kThreads) {
+      double v[NB];
+      if (it < m) {
+         const int r = (int)it;
+         if (r >= c0 && r < c0 + kb) continue;
+#pragma unroll
+         for (int j = 0; j < NB; ++j) v[j] = (j < kb) ? F[r + (int64_t)(c0 + pm[j]) * ld] : 0.0;
+#pragma unroll
+         for (int j = 0; j < NB; ++j)
+            if (j < kb) {
+               F[r + (int64_t)(c0 + j) * ld] = v[j];
+               w.Bk[r + (int64_t)j * m] = v[j];
+            }
+      } else {
+         const int k = (int)(it - m);
+         if (k >= c0 && k < c0 + kb) continue;
+#pragma unroll
+         for (int j = 0; j < NB; ++j) v[j] = (j < kb) ? F[c0 + pm[j] + (int64_t)k * ld] : 0.0;
+#pragma unroll
+         for (int j = 0; j < NB; ++j)
+            if (j < kb) F[c0 + j + (int64_t)k * ld] = v[j];
+      }
+   }
+}
+
+// ApplyN/ApplyT and the threshold test on the uneliminated rows outside the
+// block: the failed columns so far, then rows c0 + kb .. m - 1.
+static __global__ __launch_bounds__(kThreads) void k_lp_apply(int node, int c0, const IndefDesc* __restrict__ nodes,
+                                                       double* __restrict__ arena, double* __restrict__ wsbase,
+                                                       double u, double small) {
+#pragma clang fp contract(off)
+   constexpr int NB = kAppNB;
+   __shared__ double T[NB][NB + 1];
+   __shared__ double dl[2 * NB];
+   __shared__ int s_fail;
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, n = nd.n, ld = nd.ld, tid = threadIdx.x;
+   const AppWs w = app_ws(wsbase + nd.ws_off, m, n);
+   if (w.ST[kStBad]) return;
+   const int kb = min(NB, n - c0);
+   double* F = arena + nd.off;
+   for (int e = tid; e < kb * kb; e += kThreads) {
+      const int i = e % kb, j = e / kb;
+      T[i][j] = w.TG[i + j * NB];
+   }
+   for (int k = tid; k < 2 * kb; k += kThreads) dl[k] = w.DL[k];
+   if (tid == 0) s_fail = kb;
+   __syncthreads();
+   const int nf = w.ST[kStNf];
+   const double inv_u = 1.0 / u;
+   const int64_t items = (int64_t)nf + (m - c0 - kb);
+   for (int64_t it = (int64_t)blockIdx.x * kThreads + tid; it < items; it += (int64_t)gridDim.x * kThreads) {
+      const int r = it < nf ? w.FL[it] : c0 + kb + (int)(it - nf);
+      double x[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) x[j] = (j < kb) ? F[r + (int64_t)(c0 + j) * ld] : 0.0;
+      const int fail = apply_row(x, T, dl, kb, small, inv_u);
+      if (fail < kb) atomicMin(&s_fail, fail);
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+         if (j < kb) {
+            F[r + (int64_t)(c0 + j) * ld] = x[j];
+            if (r < n) F[c0 + j + (int64_t)r * ld] = x[j];
+         }
+   }
+   __syncthreads();
+   if (tid == 0 && s_fail < kb) atomicMin(&w.ST[kStFail], s_fail);
+}
+
+// Passed count of the block, L11 of the passed columns, their D^{-1} and
+// flags; the failed columns of the block join the failed list.
+static __global__ __launch_bounds__(kThreads) void k_lp_finish(int node, int c0, const IndefDesc* __restrict__ nodes,
+                                                        double* __restrict__ arena, double* __restrict__ dinv,
+                                                        double* __restrict__ wsbase) {
+   constexpr int NB = kAppNB;
+   __shared__ double dl[2 * NB];
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, n = nd.n, ld = nd.ld, tid = threadIdx.x;
+   const AppWs w = app_ws(wsbase + nd.ws_off, m, n);
+   if (w.ST[kStBad]) return;
+   const int kb = min(NB, n - c0);
+   double* F = arena + nd.off;
+   for (int k = tid; k < 2 * kb; k += kThreads) dl[k] = w.DL[k];
+   const int nf = w.ST[kStNf], fail = w.ST[kStFail];
+   __syncthreads();
+   const int npass = npass_adjust(fail, dl);
+   for (int e = tid; e < kb * kb; e += kThreads) {
+      const int i = e % kb, j = e / kb;
+      if (j < npass && i > j) {
+         const double t = w.TG[i + j * NB];
+         F[c0 + i + (int64_t)(c0 + j) * ld] = t;
+         F[c0 + j + (int64_t)(c0 + i) * ld] = t;
+      }
+   }
+   for (int j = tid; j < kb; j += kThreads) {
+      if (j < npass) {
+         w.EL[c0 + j] = 1;
+         dinv[nd.d_off + 2 * (c0 + j)] = dl[2 * j];
+         dinv[nd.d_off + 2 * (c0 + j) + 1] = dl[2 * j + 1];
+      } else {
+         w.FL[nf + (j - npass)] = c0 + j;
+      }
+   }
+   __syncthreads();
+   if (tid == 0) {
+      w.ST[kStNpass] = npass;
+      w.ST[kStNelim] += npass;
+      w.ST[kStNfOld] = nf;
+      w.ST[kStNf] = nf + kb - npass;
+   }
+}
+
+constexpr int kLpTile = 128;  // tile of the multi-workgroup fully-summed update
+
+// Restore of the failed block columns and update of the uneliminated
+// fully-summed columns with the block's passed pivots: entry (r, s), r >= s,
+// of the index space U = (failed columns, ascending; c0 + kb .. n - 1) for
+// columns and U followed by rows n .. m - 1 for rows, gets
+// x -= L[r, c0 + p] (D L[s, :])_p for p = 0 .. npass - 1 in that order; the
+// fully-summed part is mirrored. Entries of the failed block columns start
+// from the backup (the restore). 128 x 128 tiles, 8 x 8 entries per thread,
+// tiles taken in turn by a grid of any size.
+static __global__ __launch_bounds__(kThreads) void k_lp_update(int node, int c0, const IndefDesc* __restrict__ nodes,
+                                                        double* __restrict__ arena, double* __restrict__ wsbase) {
+#pragma clang fp contract(off)
+   constexpr int NB = kAppNB, TT = kLpTile;
+   __shared__ __align__(16) double As[NB][TT + 2];
+   __shared__ __align__(16) double Bs[NB][TT + 2];
+   __shared__ int ridx[TT], cidx[TT];
+   __shared__ double dl[2 * NB];
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, n = nd.n, ld = nd.ld, tid = threadIdx.x;
+   const AppWs w = app_ws(wsbase + nd.ws_off, m, n);
+   if (w.ST[kStBad]) return;
+   const int kb = min(NB, n - c0);
+   double* F = arena + nd.off;
+   const int npass = w.ST[kStNpass], nf = w.ST[kStNf], nf_old = w.ST[kStNfOld];
+   const int nu = nf + (n - c0 - kb), nr = nu + (m - n);
+   if (nu <= 0) return;
+   const int Tr = (nr + TT - 1) / TT, Tc = (nu + TT - 1) / TT;
+   const int64_t ntiles = (int64_t)Tc * Tr - (int64_t)Tc * (Tc - 1) / 2;
+   for (int k = tid; k < 2 * kb; k += kThreads) dl[k] = w.DL[k];
+   const int lo = c0 + npass, hi = c0 + kb;  // failed block columns
+   const int tr8 = tid & 15, tc8 = tid >> 4;
+   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+      int tc = 0;
+      int64_t rem = t;
+      while (rem >= Tr - tc) {
+         rem -= Tr - tc;
+         ++tc;
+      }
+      const int tr = tc + (int)rem;
+      const int R0 = tr * TT, S0 = tc * TT;
+      if (npass == 0 && !(R0 < nf && R0 + TT > nf_old) && !(S0 < nf && S0 + TT > nf_old)) continue;
+      __syncthreads();
+      {
+         const int k = (tid < TT ? R0 : S0) + (tid & (TT - 1));
+         const int lim = tid < TT ? nr : nu;
+         int v = -1;
+         if (k < lim) v = k < nf ? w.FL[k] : (k < nu ? c0 + kb + (k - nf) : n + (k - nu));
+         if (tid < TT)
+            ridx[tid] = v;
+         else
+            cidx[tid - TT] = v;
+      }
+      __syncthreads();
+      for (int e = tid; e < npass * TT; e += kThreads) {
+         const int p = e / TT, i = e % TT;
+         const int r = ridx[i], s = cidx[i];
+         As[p][i] = r >= 0 ? F[r + (int64_t)(c0 + p) * ld] : 0.0;
+         double v = 0.0;
+         if (s >= 0) {
+            const double* fs = F + s + (int64_t)c0 * ld;
+            const double fm = p > 0 ? fs[(int64_t)(p - 1) * ld] : 0.0;
+            const double fp = p + 1 < npass ? fs[(int64_t)(p + 1) * ld] : 0.0;
+            v = vs_value(dl, p, npass, fm, fs[(int64_t)p * ld], fp);
+         }
+         Bs[p][i] = v;
+      }
+      double acc[8][8];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+         const int s = cidx[tc8 * 8 + b];
+#pragma unroll
+         for (int a = 0; a < 8; ++a) {
+            const int r = ridx[tr8 * 8 + a];
+            double x = 0.0;
+            if (r >= 0 && s >= 0 && r >= s) {
+               if (s >= lo && s < hi)
+                  x = w.Bk[r + (int64_t)(s - c0) * m];
+               else if (r >= lo && r < hi)
+                  x = w.Bk[s + (int64_t)(r - c0) * m];
+               else
+                  x = F[r + (int64_t)s * ld];
+            }
+            acc[a][b] = x;
+         }
+      }
+      __syncthreads();
+      for (int p = 0; p < npass; ++p) {
+         double av[8], bv[8];
+#pragma unroll
+         for (int a = 0; a < 8; ++a) av[a] = As[p][tr8 * 8 + a];
+#pragma unroll
+         for (int b = 0; b < 8; ++b) bv[b] = Bs[p][tc8 * 8 + b];
+#pragma unroll
+         for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int b = 0; b < 8; ++b) acc[a][b] = fnma(av[a], bv[b], acc[a][b]);
+      }
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+         const int s = cidx[tc8 * 8 + b];
+#pragma unroll
+         for (int a = 0; a < 8; ++a) {
+            const int r = ridx[tr8 * 8 + a];
+            if (r >= 0 && s >= 0 && r >= s) {
+               F[r + (int64_t)s * ld] = acc[a][b];
+               if (r < n && r > s) F[s + (int64_t)r * ld] = acc[a][b];
+            }
+         }
+      }
+   }
+}
+
+// permute_failed (src/factor_indef.hxx:1190-1331): new order = eliminated
+// positions, then the failed ones, both ascending.
+static __global__ __launch_bounds__(kThreads) void k_lp_np(int node, const IndefDesc* __restrict__ nodes,
+                                                    double* __restrict__ wsbase) {
+   const IndefDesc nd = nodes[node];
+   const AppWs w = app_ws(wsbase + nd.ws_off, nd.m, nd.n);
+   if (w.ST[kStBad] || w.ST[kStNelim] >= nd.n) return;
+   if (threadIdx.x == 0) {
+      int k = 0;
+      for (int p = 0; p < nd.n; ++p)
+         if (w.EL[p]) w.NP[k++] = p;
+      const int nf = w.ST[kStNf];
+      for (int p = 0; p < nf; ++p) w.NP[k++] = w.FL[p];
+   }
+}
+
+static __global__ __launch_bounds__(kThreads) void k_lp_pgather(int node, const IndefDesc* __restrict__ nodes,
+                                                         const double* __restrict__ arena,
+                                                         const int* __restrict__ vars, const int* __restrict__ pres,
+                                                         const double* __restrict__ dinv,
+                                                         double* __restrict__ wsbase) {
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, n = nd.n, ld = nd.ld;
+   const AppWs w = app_ws(wsbase + nd.ws_off, m, n);
+   if (w.ST[kStBad] || w.ST[kStNelim] >= n) return;
+   const double* F = arena + nd.off;
+   const int64_t gid = (int64_t)blockIdx.x * kThreads + threadIdx.x, G = (int64_t)gridDim.x * kThreads;
+   for (int64_t e = gid; e < (int64_t)m * n; e += G) {
+      const int i = (int)(e % m), k = (int)(e / m);
+      const int oi = (i < n) ? w.NP[i] : i;
+      w.PS[i + (int64_t)k * m] = F[oi + (int64_t)w.NP[k] * ld];
+   }
+   int* ivar = reinterpret_cast<int*>(w.Bk);
+   int* ipre = ivar + n;
+   double* dd = w.Bk + n;
+   for (int64_t k = gid; k < n; k += G) {
+      ivar[k] = vars[nd.var_off + w.NP[k]];
+      ipre[k] = pres[nd.pre_off + w.NP[k]];
+      dd[2 * k] = dinv[nd.d_off + 2 * w.NP[k]];
+      dd[2 * k + 1] = dinv[nd.d_off + 2 * w.NP[k] + 1];
+   }
+}
+
+static __global__ __launch_bounds__(kThreads) void k_lp_pscatter(int node, const IndefDesc* __restrict__ nodes,
+                                                          double* __restrict__ arena, int* __restrict__ vars,
+                                                          int* __restrict__ pres, double* __restrict__ dinv,
+                                                          double* __restrict__ wsbase) {
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, n = nd.n, ld = nd.ld;
+   const AppWs w = app_ws(wsbase + nd.ws_off, m, n);
+   if (w.ST[kStBad] || w.ST[kStNelim] >= n) return;
+   double* F = arena + nd.off;
+   const int64_t gid = (int64_t)blockIdx.x * kThreads + threadIdx.x, G = (int64_t)gridDim.x * kThreads;
+   for (int64_t e = gid; e < (int64_t)m * n; e += G) {
+      const int i = (int)(e % m), k = (int)(e / m);
+      F[i + (int64_t)k * ld] = w.PS[i + (int64_t)k * m];
+   }
+   const int* ivar = reinterpret_cast<const int*>(w.Bk);
+   const int* ipre = ivar + n;
+   const double* dd = w.Bk + n;
+   for (int64_t k = gid; k < n; k += G) {
+      vars[nd.var_off + k] = ivar[k];
+      pres[nd.pre_off + k] = ipre[k];
+      dinv[nd.d_off + 2 * k] = dd[2 * k];
+      dinv[nd.d_off + 2 * k + 1] = dd[2 * k + 1];
+   }
+}
+
+// Second pass and results of a front of the multi-workgroup panel (one
+// workgroup); out slot `slot` as k_app_front's.
+static __global__ __launch_bounds__(kThreads) void k_lp_tail(int node, int slot, IndefDesc* __restrict__ nodes,
+                                                      double* __restrict__ arena, int* __restrict__ vars,
+                                                      int* __restrict__ pres, double* __restrict__ dinv,
+                                                      double* __restrict__ wsbase, double u, double small,
+                                                      int action, int failed_tpp, int* __restrict__ status,
+                                                      int* __restrict__ out) {
+   __shared__ double red_v[4];
+   __shared__ int red_i[4];
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, n = nd.n, ld = nd.ld, tid = threadIdx.x;
+   const AppWs w = app_ws(wsbase + nd.ws_off, m, n);
+   if (w.ST[kStBad]) {
+      if (tid == 0) {
+         status[node] = -5;
+         out[2 * slot] = 0;
+         out[2 * slot + 1] = 0;
+      }
+      return;
+   }
+   int nelim = w.ST[kStNelim];
+   const int nelim_first = nelim;
+   if (nelim < n && (m == n || failed_tpp)) {
+      if (tpp_pass(arena + nd.off, ld, m, n, nelim, vars + nd.var_off, pres + nd.pre_off, dinv + nd.d_off, w.TL,
+                   u, small, action, red_v, red_i)) {
+         if (tid == 0) {
+            status[node] = -5;
+            out[2 * slot] = nelim;
+            out[2 * slot + 1] = nelim_first;
+         }
+         return;
+      }
+   }
+   if (tid == 0) {
+      nodes[node].nelim = nelim;
+      nodes[node].nelim_first = nelim_first;
+      out[2 * slot] = nelim;
+      out[2 * slot + 1] = nelim_first;
    }
 }
 

@@ -26,6 +26,15 @@ pytestmark = pytest.mark.gpu
 BWD_TOL = 1e-14
 
 
+@pytest.fixture(params=["wg", "multi"])
+def panel(request, S):
+    """Run a test with every front on the one-workgroup panel (k_app_front)
+    or on the multi-workgroup panel (k_lp_*)."""
+    S.set_ext_options(indef_panel_min=(1 << 30) if request.param == "wg" else 1)
+    yield request.param
+    S.set_ext_options(indef_panel_min=0)
+
+
 def _analyse(S, A, natural=False):
     opts = S.default_options()
     order = None
@@ -57,8 +66,9 @@ def _solve_check(S, A, akeep, fkeep, nrhs=2, seed=3):
     return M.backward_error(A, x, b)
 
 
-@pytest.mark.parametrize("n,zero_block,seed", [(40, 0, 1), (150, 0, 2), (130, 60, 3), (200, 100, 4)])
-def test_single_front_bit_exact(S, O, gpu, n, zero_block, seed):
+@pytest.mark.parametrize("n,zero_block,seed", [(40, 0, 1), (150, 0, 2), (130, 60, 3), (200, 100, 4),
+                                                (333, 150, 5)])
+def test_single_front_bit_exact(S, O, gpu, panel, n, zero_block, seed):
     A, dense = _dense_indef(n, seed, zero_block)
     akeep, opts, sym = _analyse(S, A, natural=True)
     assert sym["nnodes"] == 1
@@ -78,7 +88,7 @@ def test_single_front_bit_exact(S, O, gpu, n, zero_block, seed):
 
 
 @pytest.mark.parametrize("side,mrows,seed", [(4, 12, 1), (6, 40, 2), (8, 100, 3), (10, 250, 4)])
-def test_kkt_tree_vs_oracle(S, O, gpu, side, mrows, seed):
+def test_kkt_tree_vs_oracle(S, O, gpu, panel, side, mrows, seed):
     A = M.kkt(side, mrows, seed=seed)
     akeep, opts, sym = _analyse(S, A)
     fkeep, inf = S.spldlt_factorize(False, A.val, akeep, options=opts)
@@ -95,6 +105,31 @@ def test_kkt_tree_vs_oracle(S, O, gpu, side, mrows, seed):
     assert inf.num_delay == ref.stats["num_delay"]
     assert _solve_check(S, A, akeep, fkeep) <= BWD_TOL
     assert nh + mrows == A.n
+
+
+def test_panels_identical_on_tree(S, gpu):
+    """The multi-workgroup panel reproduces the one-workgroup panel bit for
+    bit on every front of a tree with many delays (12^3 KKT, 432 rows; the
+    root front has 478 fully-summed columns)."""
+    A = M.kkt(12, 432, seed=0)
+    akeep, opts, sym = _analyse(S, A)
+    got = {}
+    try:
+        for name, pmin in (("wg", 1 << 30), ("multi", 1), ("auto", 0)):
+            S.set_ext_options(indef_panel_min=pmin)
+            fkeep, inf = S.spldlt_factorize(False, A.val, akeep, options=opts)
+            assert inf.flag >= 0, (name, inf.flag)
+            assert inf.num_neg == 432
+            got[name] = [S.fkeep_indef_node(fkeep, i) for i in range(sym["nnodes"])]
+            assert _solve_check(S, A, akeep, fkeep) <= BWD_TOL
+    finally:
+        S.set_ext_options(indef_panel_min=0)
+    for name in ("multi", "auto"):
+        for i, (a, b) in enumerate(zip(got["wg"], got[name])):
+            assert (a["m"], a["n"], a["nelim"]) == (b["m"], b["n"], b["nelim"]), (name, i)
+            assert np.array_equal(a["rows"], b["rows"]), (name, i)
+            assert np.array_equal(a["dinv"], b["dinv"]), (name, i)
+            assert np.array_equal(np.tril(a["L"], -1), np.tril(b["L"], -1)), (name, i)
 
 
 def test_partial_solves_compose(S, gpu):
