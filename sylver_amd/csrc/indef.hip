@@ -118,14 +118,35 @@ void large_panel(IndefState& S, int node, int slot, double u, double small, int 
       k_lp_update<<<(unsigned)std::min<int64_t>(tiles, 1024), kThreads, 0, s>>>(node, c0, S.d_nodes, S.arena,
                                                                                  S.d_ws);
    }
-   k_lp_np<<<1, kThreads, 0, s>>>(node, S.d_nodes, S.d_ws);
-   const int64_t mn = (int64_t)m * n;
-   k_lp_pgather<<<grid_cap(mn, 4096), kThreads, 0, s>>>(node, S.d_nodes, S.arena, S.d_vars, S.d_pres, S.d_dinv,
-                                                        S.d_ws);
-   k_lp_pscatter<<<grid_cap(mn, 4096), kThreads, 0, s>>>(node, S.d_nodes, S.arena, S.d_vars, S.d_pres,
-                                                         S.d_dinv, S.d_ws);
-   k_lp_tail<<<1, kThreads, 0, s>>>(node, slot, S.d_nodes, S.arena, S.d_vars, S.d_pres, S.d_dinv, S.d_ws, u,
-                                    small, action, failed_tpp, S.d_status, S.d_out);
+   // the second pass needs the first pass's outcome on the host (one read-back)
+   int st[16];
+   const int64_t st_off = d.ws_off + (int64_t)m * (kAppNB + n + 5) + 8 + kAppNB * kAppNB + 2 * kAppNB;
+   HIPCHECK(hipMemcpyAsync(st, reinterpret_cast<int*>(S.d_ws + st_off), sizeof(st), hipMemcpyDeviceToHost, s));
+   HIPCHECK(hipStreamSynchronize(s));
+   const int nelim1 = st[kStNelim];
+   const bool failed = !st[kStBad] && nelim1 < n;
+   const bool allowed = failed && (m == n || failed_tpp);
+   if (failed) {
+      k_lp_np<<<1, kThreads, 0, s>>>(node, S.d_nodes, S.d_ws);
+      const int64_t mn = (int64_t)m * n;
+      k_lp_pgather<<<grid_cap(mn, 4096), kThreads, 0, s>>>(node, S.d_nodes, S.arena, S.d_vars, S.d_pres,
+                                                           S.d_dinv, S.d_ws);
+      k_lp_pscatter<<<grid_cap(mn, 4096), kThreads, 0, s>>>(node, S.d_nodes, S.arena, S.d_vars, S.d_pres,
+                                                            S.d_dinv, S.d_ws);
+   }
+   k_tpp_start<<<1, 64, 0, s>>>(node, S.d_nodes, S.d_ws, allowed ? 1 : 0);
+   if (allowed) {
+      // statistics of the failed columns, then one pivot per step (at most
+      // one step per column; steps after the last pivot return at once)
+      const int nf = n - nelim1;
+      k_tpp_update<<<nf, kThreads, 0, s>>>(node, S.d_nodes, S.arena, S.d_ws);
+      for (int k = 0; k < nf; ++k) {
+         k_tpp_decide<<<1, kThreads, 0, s>>>(node, S.d_nodes, S.arena, S.d_vars, S.d_pres, S.d_dinv, S.d_ws, u,
+                                             small, action);
+         k_tpp_update<<<std::max(1, nf - k - 1), kThreads, 0, s>>>(node, S.d_nodes, S.arena, S.d_ws);
+      }
+   }
+   k_lp_tail<<<1, 64, 0, s>>>(node, slot, S.d_nodes, S.d_ws, S.d_status, S.d_out);
 }
 
 }  // namespace
@@ -176,7 +197,14 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
    }
    const double u = opts.u, small = opts.small;
    const int action = opts.action ? 1 : 0;
-   const int panel_min = ext_options().indef_panel_min > 0 ? ext_options().indef_panel_min : 256;
+   // fronts on the multi-workgroup panel: at least panel_min fully-summed
+   // columns; in auto mode also fronts of a million entries or more (many
+   // rows, where one workgroup's pass over all rows per pivot is slow)
+   const int panel_opt = ext_options().indef_panel_min;
+   const int panel_min = panel_opt > 0 ? panel_opt : 256;
+   auto is_large = [&](const IndefDesc& d) {
+      return d.n >= panel_min || (panel_opt <= 0 && (int64_t)d.m * d.n >= (int64_t(1) << 20));
+   };
    const int failed_tpp = (opts.failed_pivot_method == 1 || opts.pivot_method == 3) ? 1 : 0;
    int64_t arena_used = 0, vars_used = 0, pres_used = 0, dinv_used = 0, x_used = 0;
    S.num_factor = 0;
@@ -266,16 +294,16 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
       std::vector<int> slot(nl);
       int nsmall = 0;
       for (int k = 0; k < nl; ++k)
-         if (S.h[lv[k]].n < panel_min) slot[k] = nsmall++;
+         if (!is_large(S.h[lv[k]])) slot[k] = nsmall++;
       {
          int j = nsmall;
          for (int k = 0; k < nl; ++k)
-            if (S.h[lv[k]].n >= panel_min) slot[k] = j++;
+            if (is_large(S.h[lv[k]])) slot[k] = j++;
       }
       if (nsmall > 0) {
          std::vector<int> sl;
          for (int k = 0; k < nl; ++k)
-            if (S.h[lv[k]].n < panel_min) sl.push_back(lv[k]);
+            if (!is_large(S.h[lv[k]])) sl.push_back(lv[k]);
          const int64_t o_small = (int64_t)lists.size();
          grow(S.d_list, S.list_cap, o_small + nsmall, o_small, s);
          HIPCHECK(hipMemcpyAsync(S.d_list + o_small, sl.data(), sl.size() * sizeof(int), hipMemcpyHostToDevice, s));
@@ -284,8 +312,7 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
                                                  S.d_out);
       }
       for (int k = 0; k < nl; ++k) {
-         const IndefDesc& d = S.h[lv[k]];
-         if (d.n < panel_min) continue;
+         if (!is_large(S.h[lv[k]])) continue;
          large_panel(S, lv[k], slot[k], u, small, action, failed_tpp, s);
       }
       std::vector<int> outs(2 * nl), out(2 * nl);

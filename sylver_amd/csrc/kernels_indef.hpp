@@ -709,7 +709,13 @@ static __global__ __launch_bounds__(kThreads) void k_app_front(const int* __rest
 // The panel state follows the single-workgroup workspace: block T, its
 // D^{-1}, counters, the block permutation and the list of failed columns.
 // ---------------------------------------------------------------------------
-enum { kStFail = 0, kStNpass = 1, kStBad = 2, kStNelim = 3, kStNf = 4, kStNfOld = 5 };
+enum {
+   kStFail = 0, kStNpass = 1, kStBad = 2, kStNelim = 3, kStNf = 4, kStNfOld = 5,
+   kStW = 6,        // second pass: width of the current pivot (0 = zero pivot or none)
+   kStDone = 7,     // second pass: no further pivot
+   kStTppErr = 8,   // second pass: zero pivot with action == 0
+   kStFirst = 9     // eliminated columns after the first pass
+};
 
 struct AppWs {
    double* Bk;  // m x kAppNB backup of the block columns
@@ -722,10 +728,12 @@ struct AppWs {
    int* ST;     // 16 counters (kSt*)
    int* PERM;   // kAppNB block permutation
    int* FL;     // n failed columns, ascending
+   double* TS;  // 4 n second-pass column statistics
 };
 
 __host__ __device__ inline int64_t app_ws_elems(int m, int n) {
-   return (int64_t)m * (kAppNB + n + 5) + 8 + kAppNB * kAppNB + 2 * kAppNB + 8 + kAppNB / 2 + (n + 1) / 2 + 8;
+   return (int64_t)m * (kAppNB + n + 5) + 8 + kAppNB * kAppNB + 2 * kAppNB + 8 + kAppNB / 2 + (n + 1) / 2 + 8 +
+          4 * (int64_t)n + 8;
 }
 
 __device__ inline AppWs app_ws(double* base, int m, int n) {
@@ -740,6 +748,7 @@ __device__ inline AppWs app_ws(double* base, int m, int n) {
    w.ST = reinterpret_cast<int*>(w.DL + 2 * kAppNB);
    w.PERM = w.ST + 16;
    w.FL = w.PERM + kAppNB;
+   w.TS = w.TG + kAppNB * kAppNB + 2 * kAppNB + 8 + kAppNB / 2 + (n + 1) / 2 + 8;
    return w;
 }
 
@@ -1177,46 +1186,282 @@ static __global__ __launch_bounds__(kThreads) void k_lp_pscatter(int node, const
    }
 }
 
-// Second pass and results of a front of the multi-workgroup panel (one
-// workgroup); out slot `slot` as k_app_front's.
-static __global__ __launch_bounds__(kThreads) void k_lp_tail(int node, int slot, IndefDesc* __restrict__ nodes,
-                                                      double* __restrict__ arena, int* __restrict__ vars,
-                                                      int* __restrict__ pres, double* __restrict__ dinv,
-                                                      double* __restrict__ wsbase, double u, double small,
-                                                      int action, int failed_tpp, int* __restrict__ status,
-                                                      int* __restrict__ out) {
-   __shared__ double red_v[4];
-   __shared__ int red_i[4];
+// ---------------------------------------------------------------------------
+// Second pass of a large front over the whole GPU, one pivot per step:
+//   k_tpp_decide  (one workgroup) picks the pivot from per-column statistics
+//                 exactly as tpp_pass's candidate loop does, exchanges it to
+//                 the front of the remaining columns, forms D^{-1}, l = v D^{-1}
+//                 (into TL) and writes l;
+//   k_tpp_update  (one workgroup per remaining column) applies the pivot to
+//                 its column, all rows, and recomputes the column's
+//                 statistics for the next step.
+// The fully-summed part is kept symmetric bit for bit (upper = lower), so a
+// column's rows above the diagonal are updated from the column's own copies
+// with the lower entry's operands, and no workgroup writes another's column.
+// ---------------------------------------------------------------------------
+
+// Two best (value, index) pairs, larger value first, smaller index among equals.
+struct Top2 {
+   double v1, v2;
+   int i1, i2;
+};
+__device__ __forceinline__ bool t2_better(double v, int i, double w, int j) { return v > w || (v == w && i < j); }
+__device__ __forceinline__ void t2_add(Top2& a, double v, int i) {
+   if (t2_better(v, i, a.v1, a.i1)) {
+      a.v2 = a.v1;
+      a.i2 = a.i1;
+      a.v1 = v;
+      a.i1 = i;
+   } else if (t2_better(v, i, a.v2, a.i2)) {
+      a.v2 = v;
+      a.i2 = i;
+   }
+}
+
+// One column per workgroup: s = nelim + blockIdx.x (nelim after the pivot of
+// this step). With kStW = w > 0 the column gets the pivot's update
+// x -= l_r v_s (+ l2_r v2_s), rows nelim .. m - 1; then its statistics over
+// those rows without the diagonal, and |F[s, s]|.
+static __global__ __launch_bounds__(kThreads) void k_tpp_update(int node, const IndefDesc* __restrict__ nodes,
+                                                         double* __restrict__ arena, double* __restrict__ wsbase) {
+#pragma clang fp contract(off)
+   __shared__ double sv1[4], sv2[4];
+   __shared__ int si1[4], si2[4];
    const IndefDesc nd = nodes[node];
    const int m = nd.m, n = nd.n, ld = nd.ld, tid = threadIdx.x;
    const AppWs w = app_ws(wsbase + nd.ws_off, m, n);
-   if (w.ST[kStBad]) {
+   if (w.ST[kStDone]) return;
+   const int r0 = w.ST[kStNelim], wd = w.ST[kStW];
+   const int s = r0 + blockIdx.x;
+   if (s >= n) return;
+   double* col = arena + nd.off + (int64_t)s * ld;
+   const double* lv = w.TL;
+   const double* vv = w.TL + 2 * (int64_t)m;
+   Top2 t{0.0, 0.0, 0x7fffffff, 0x7fffffff};
+   for (int r = r0 + tid; r < m; r += kThreads) {
+      double x = col[r];
+      if (wd > 0) {
+         if (r >= s) {
+            x = fnma(lv[2 * r], vv[2 * s], x);
+            if (wd == 2) x = fnma(lv[2 * r + 1], vv[2 * s + 1], x);
+         } else {
+            x = fnma(lv[2 * s], vv[2 * r], x);
+            if (wd == 2) x = fnma(lv[2 * s + 1], vv[2 * r + 1], x);
+         }
+         col[r] = x;
+      }
+      if (r != s) t2_add(t, fabs(x), r);
+   }
+   // block reduction of the two best
+   const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) {
+      const double a1 = __shfl_xor(t.v1, o), a2 = __shfl_xor(t.v2, o);
+      const int j1 = __shfl_xor(t.i1, o), j2 = __shfl_xor(t.i2, o);
+      t2_add(t, a1, j1);
+      t2_add(t, a2, j2);
+   }
+   if (lane == 0) {
+      sv1[wave] = t.v1;
+      sv2[wave] = t.v2;
+      si1[wave] = t.i1;
+      si2[wave] = t.i2;
+   }
+   __syncthreads();
+   if (tid == 0) {
+      Top2 a{sv1[0], sv2[0], si1[0], si2[0]};
+      for (int k = 1; k < kThreads / 64; ++k) {
+         t2_add(a, sv1[k], si1[k]);
+         t2_add(a, sv2[k], si2[k]);
+      }
+      double* ts = w.TS + 4 * (int64_t)s;
+      ts[0] = a.v1;
+      ts[1] = a.v2;
+      ts[2] = (double)a.i1;
+      ts[3] = (double)a.i2;
+   }
+}
+
+// Outcome of candidate p of the second pass from the column statistics
+// (the candidate loop of tpp_pass): 0 none, 1 zero pivot with action == 0,
+// 2 zero pivot, 3 1x1, 4 2x2 with row *t2.
+__device__ inline int tpp_candidate(const double* F, int ld, int n, int p, const double* TS, double u, double small,
+                                    int action, int* t2) {
+#pragma clang fp contract(off)
+   const double* ts = TS + 4 * (int64_t)p;
+   const double amax = ts[0];
+   int t = (int)ts[2];
+   if (!(amax > 0.0)) t = -1;
+   const double app = fabs(F[p + (int64_t)p * ld]);
+   if (fmax(app, amax) < small) return action ? 2 : 1;
+   if (app >= u * amax) return 3;
+   if (t >= 0 && t < n) {
+      const double a11 = F[p + (int64_t)p * ld], a21 = F[t + (int64_t)p * ld], a22 = F[t + (int64_t)t * ld];
+      const double det = __builtin_fma(a11, a22, -(a21 * a21));
+      if (det != 0.0) {
+         const double d11 = a22 / det, d21 = -a21 / det, d22 = a11 / det;
+         const double pm = ts[1];
+         const double* tt = TS + 4 * (int64_t)t;
+         const double tm = ((int)tt[2] == p) ? tt[1] : tt[0];
+         const double g = fmax(dot2(fabs(d11), pm, fabs(d21), tm), dot2(fabs(d21), pm, fabs(d22), tm));
+         if (u * g <= 1.0) {
+            *t2 = t;
+            return 4;
+         }
+      }
+   }
+   return 0;
+}
+
+static __global__ __launch_bounds__(kThreads) void k_tpp_decide(int node, const IndefDesc* __restrict__ nodes,
+                                                         double* __restrict__ arena, int* __restrict__ vars,
+                                                         int* __restrict__ pres, double* __restrict__ dinv,
+                                                         double* __restrict__ wsbase, double u, double small,
+                                                         int action) {
+#pragma clang fp contract(off)
+   __shared__ int s_best;
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, n = nd.n, ld = nd.ld, tid = threadIdx.x;
+   const AppWs w = app_ws(wsbase + nd.ws_off, m, n);
+   if (w.ST[kStDone]) return;
+   const int nelim = w.ST[kStNelim];
+   double* F = arena + nd.off;
+   if (tid == 0) s_best = 0x7fffffff;
+   __syncthreads();
+   for (int p = nelim + tid; p < n; p += kThreads) {
+      int t2 = -1;
+      if (tpp_candidate(F, ld, n, p, w.TS, u, small, action, &t2)) {
+         atomicMin(&s_best, p);
+         break;
+      }
+   }
+   __syncthreads();
+   const int piv = s_best;
+   if (piv == 0x7fffffff) {  // no pivot: the rest is delayed
+      if (tid == 0) w.ST[kStDone] = 1;
+      return;
+   }
+   int t = -1;
+   const int code = tpp_candidate(F, ld, n, piv, w.TS, u, small, action, &t);
+   if (code == 1) {
       if (tid == 0) {
-         status[node] = -5;
-         out[2 * slot] = 0;
-         out[2 * slot + 1] = 0;
+         w.ST[kStTppErr] = 1;
+         w.ST[kStDone] = 1;
       }
       return;
    }
-   int nelim = w.ST[kStNelim];
-   const int nelim_first = nelim;
-   if (nelim < n && (m == n || failed_tpp)) {
-      if (tpp_pass(arena + nd.off, ld, m, n, nelim, vars + nd.var_off, pres + nd.pre_off, dinv + nd.d_off, w.TL,
-                   u, small, action, red_v, red_i)) {
-         if (tid == 0) {
-            status[node] = -5;
-            out[2 * slot] = nelim;
-            out[2 * slot + 1] = nelim_first;
-         }
-         return;
+   int* var = vars + nd.var_off;
+   int* pre = pres + nd.pre_off;
+   double* D = dinv + nd.d_off;
+   front_swap(F, ld, m, n, piv, nelim, var, pre);
+   if (code == 2) {  // zero pivot
+      for (int i = nelim + 1 + tid; i < m; i += kThreads) {
+         F[i + (int64_t)nelim * ld] = 0.0;
+         if (i < n) F[nelim + (int64_t)i * ld] = 0.0;
+      }
+      if (tid == 0) {
+         D[2 * nelim] = 0.0;
+         D[2 * nelim + 1] = 0.0;
+         w.ST[kStNelim] = nelim + 1;
+         w.ST[kStW] = 0;
+      }
+      return;
+   }
+   const int wd = code == 4 ? 2 : 1;
+   double d[4];
+   if (wd == 2) {
+      int tt = t;
+      if (tt == nelim) tt = piv;
+      front_swap(F, ld, m, n, tt, nelim + 1, var, pre);
+      const double b11 = F[nelim + (int64_t)nelim * ld], b21 = F[nelim + 1 + (int64_t)nelim * ld],
+                   b22 = F[nelim + 1 + (int64_t)(nelim + 1) * ld];
+      const double dt = __builtin_fma(b11, b22, -(b21 * b21));
+      d[0] = b22 / dt;
+      d[1] = -b21 / dt;
+      d[2] = __builtin_inf();
+      d[3] = b11 / dt;
+   } else {
+      d[0] = 1.0 / F[nelim + (int64_t)nelim * ld];
+      d[1] = 0.0;
+      d[2] = d[3] = 0.0;
+   }
+   double* lv = w.TL;
+   double* vv = w.TL + 2 * (int64_t)m;
+   const int r0 = nelim + wd;
+   for (int r = r0 + tid; r < m; r += kThreads) {
+      const double a1 = F[r + (int64_t)nelim * ld];
+      const double a2 = (wd == 2) ? F[r + (int64_t)(nelim + 1) * ld] : 0.0;
+      vv[2 * r] = a1;
+      vv[2 * r + 1] = a2;
+      double l1, l2;
+      if (wd == 2) {
+         l1 = dot2(d[0], a1, d[1], a2);
+         l2 = dot2(d[1], a1, d[3], a2);
+      } else {
+         l1 = a1;
+         apply_dinv1(d[0], small, l1);
+         l2 = 0.0;
+      }
+      lv[2 * r] = l1;
+      lv[2 * r + 1] = l2;
+      F[r + (int64_t)nelim * ld] = l1;
+      if (r < n) F[nelim + (int64_t)r * ld] = l1;
+      if (wd == 2) {
+         F[r + (int64_t)(nelim + 1) * ld] = l2;
+         if (r < n) F[nelim + 1 + (int64_t)r * ld] = l2;
       }
    }
    if (tid == 0) {
-      nodes[node].nelim = nelim;
-      nodes[node].nelim_first = nelim_first;
-      out[2 * slot] = nelim;
-      out[2 * slot + 1] = nelim_first;
+      D[2 * nelim] = d[0];
+      D[2 * nelim + 1] = d[1];
+      if (wd == 2) {
+         D[2 * nelim + 2] = d[2];
+         D[2 * nelim + 3] = d[3];
+         F[nelim + 1 + (int64_t)nelim * ld] = 0.0;
+         F[nelim + (int64_t)(nelim + 1) * ld] = 0.0;
+      }
+      w.ST[kStNelim] = r0;
+      w.ST[kStW] = wd;
    }
+}
+
+
+// Start of the second pass (kStFirst = columns eliminated by the first pass;
+// `allowed` = m == n or the failed-pivot method is TPP).
+static __global__ void k_tpp_start(int node, const IndefDesc* __restrict__ nodes, double* __restrict__ wsbase,
+                                   int allowed) {
+   const IndefDesc nd = nodes[node];
+   const AppWs w = app_ws(wsbase + nd.ws_off, nd.m, nd.n);
+   if (threadIdx.x == 0) {
+      w.ST[kStFirst] = w.ST[kStNelim];
+      w.ST[kStW] = 0;
+      w.ST[kStTppErr] = 0;
+      w.ST[kStDone] = (!allowed || w.ST[kStBad] || w.ST[kStNelim] >= nd.n) ? 1 : 0;
+   }
+}
+
+// Results of a front of the multi-workgroup panel; out slot `slot` as
+// k_app_front's.
+static __global__ void k_lp_tail(int node, int slot, IndefDesc* __restrict__ nodes, double* __restrict__ wsbase,
+                                 int* __restrict__ status, int* __restrict__ out) {
+   const IndefDesc nd = nodes[node];
+   const AppWs w = app_ws(wsbase + nd.ws_off, nd.m, nd.n);
+   if (threadIdx.x != 0) return;
+   if (w.ST[kStBad]) {
+      status[node] = -5;
+      out[2 * slot] = 0;
+      out[2 * slot + 1] = 0;
+      return;
+   }
+   const int nelim = w.ST[kStNelim], nelim_first = w.ST[kStFirst];
+   out[2 * slot] = nelim;
+   out[2 * slot + 1] = nelim_first;
+   if (w.ST[kStTppErr]) {
+      status[node] = -5;
+      return;
+   }
+   nodes[node].nelim = nelim;
+   nodes[node].nelim_first = nelim_first;
 }
 
 // ---------------------------------------------------------------------------
