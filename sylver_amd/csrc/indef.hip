@@ -198,12 +198,12 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
    const double u = opts.u, small = opts.small;
    const int action = opts.action ? 1 : 0;
    // fronts on the multi-workgroup panel: at least panel_min fully-summed
-   // columns; in auto mode also fronts of a million entries or more (many
+   // columns; in auto mode also fronts of 2^18 entries or more (many
    // rows, where one workgroup's pass over all rows per pivot is slow)
    const int panel_opt = ext_options().indef_panel_min;
    const int panel_min = panel_opt > 0 ? panel_opt : 256;
    auto is_large = [&](const IndefDesc& d) {
-      return d.n >= panel_min || (panel_opt <= 0 && (int64_t)d.m * d.n >= (int64_t(1) << 20));
+      return d.n >= panel_min || (panel_opt <= 0 && (int64_t)d.m * d.n >= (int64_t(1) << 18));
    };
    const int failed_tpp = (opts.failed_pivot_method == 1 || opts.pivot_method == 3) ? 1 : 0;
    int64_t arena_used = 0, vars_used = 0, pres_used = 0, dinv_used = 0, x_used = 0;

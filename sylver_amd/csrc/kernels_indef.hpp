@@ -80,6 +80,48 @@ __device__ inline void block_argmax(double& v, int& idx, double* sv, int* si) {
    __syncthreads();
 }
 
+// Two block-wide arg-max reductions at once (same rule as block_argmax);
+// `sv`/`si` hold 8 entries.
+__device__ inline void block_argmax2(double& v0, int& i0, double& v1, int& i1, double* sv, int* si) {
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) {
+      const double a0 = __shfl_xor(v0, o), a1 = __shfl_xor(v1, o);
+      const int j0 = __shfl_xor(i0, o), j1 = __shfl_xor(i1, o);
+      if (a0 > v0 || (a0 == v0 && j0 < i0)) {
+         v0 = a0;
+         i0 = j0;
+      }
+      if (a1 > v1 || (a1 == v1 && j1 < i1)) {
+         v1 = a1;
+         i1 = j1;
+      }
+   }
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   __syncthreads();
+   if (lane == 0) {
+      sv[wave] = v0;
+      si[wave] = i0;
+      sv[4 + wave] = v1;
+      si[4 + wave] = i1;
+   }
+   __syncthreads();
+   v0 = sv[0];
+   i0 = si[0];
+   v1 = sv[4];
+   i1 = si[4];
+#pragma unroll
+   for (int w = 1; w < kThreads / 64; ++w) {
+      if (sv[w] > v0 || (sv[w] == v0 && si[w] < i0)) {
+         v0 = sv[w];
+         i0 = si[w];
+      }
+      if (sv[4 + w] > v1 || (sv[4 + w] == v1 && si[4 + w] < i1)) {
+         v1 = sv[4 + w];
+         i1 = si[4 + w];
+      }
+   }
+}
+
 __device__ inline double block_max(double v, double* sv) {
 #pragma unroll
    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
@@ -200,40 +242,45 @@ template <typename Swap>
 __device__ int bp_block(double (*T)[kAppNB + 1], double* dl, int kb, double small, int action,
                         double* red_v, int* red_i, Swap&& swap_local) {
 #pragma clang fp contract(off)
-   const int tid = threadIdx.x;
+   // thread (ti, tj): row s + ti (or s + 1 + ti ...) of the trailing block,
+   // columns tj, tj + 4, ... of it
+   const int tid = threadIdx.x, ti = tid & 63, tj = tid >> 6;
    for (int s = 0; s < kb;) {
       double v0 = -1.0, v1 = -1.0;
       int i0 = 0x7fffffff, i1 = 0x7fffffff;
-      const int rem = kb - s;
-      for (int e = tid; e < rem * rem; e += kThreads) {
-         const int i = s + e % rem, j = s + e / rem;
-         if (i < j) continue;
-         const double v = fabs(T[i][j]);
-         const int idx = j * kb + i;
-         if (v > v0 || (v == v0 && idx < i0)) {
-            v0 = v;
-            i0 = idx;
-         }
-         if (i == j && (v > v1 || (v == v1 && i < i1))) {
-            v1 = v;
-            i1 = i;
-         }
+      {
+         const int i = s + ti;
+         if (i < kb)
+            for (int j = s + tj; j <= i; j += 4) {
+               const double v = fabs(T[i][j]);
+               const int idx = j * kb + i;
+               if (v > v0 || (v == v0 && idx < i0)) {
+                  v0 = v;
+                  i0 = idx;
+               }
+               if (i == j && (v > v1 || (v == v1 && i < i1))) {
+                  v1 = v;
+                  i1 = i;
+               }
+            }
       }
-      block_argmax(v0, i0, red_v, red_i);
-      block_argmax(v1, i1, red_v, red_i);
+      block_argmax2(v0, i0, v1, i1, red_v, red_i);
       if (v0 < small) {
          if (!action) return 1;
          for (int k = s + tid; k < kb; k += kThreads) {
             dl[2 * k] = 0.0;
             dl[2 * k + 1] = 0.0;
          }
-         for (int e = tid; e < kb * kb; e += kThreads) {
-            const int i = e % kb, j = e / kb;
-            if (j >= s && i > j) T[i][j] = 0.0;
+         {
+            const int i = s + ti;
+            if (i < kb)
+               for (int j = s + tj; j < i; j += 4) T[i][j] = 0.0;
          }
          __syncthreads();
          break;
       }
+      // the trailing update also writes the mirror entry, so that the
+      // trailing block stays symmetric for the next exchanges
       if (v1 >= 0.6403882032022076 * v0) {
          swap_local(s, i1);
          const double dv = T[s][s];
@@ -242,11 +289,15 @@ __device__ int bp_block(double (*T)[kAppNB + 1], double* dl, int kb, double smal
             dl[2 * s] = d0;
             dl[2 * s + 1] = 0.0;
          }
-         for (int e = tid; e < rem * rem; e += kThreads) {
-            const int i = s + 1 + e % rem, j = s + 1 + e / rem;
-            if (i >= kb || j > i) continue;
-            const double li = T[i][s] * d0;
-            T[i][j] = fnma(li, T[j][s], T[i][j]);
+         {
+            const int i = s + 1 + ti;
+            if (i < kb)
+               for (int j = s + 1 + tj; j <= i; j += 4) {
+                  const double li = T[i][s] * d0;
+                  const double x = fnma(li, T[j][s], T[i][j]);
+                  T[i][j] = x;
+                  if (j < i) T[j][i] = x;
+               }
          }
          __syncthreads();
          for (int i = s + 1 + tid; i < kb; i += kThreads) T[i][s] = T[i][s] * d0;
@@ -265,12 +316,16 @@ __device__ int bp_block(double (*T)[kAppNB + 1], double* dl, int kb, double smal
             dl[2 * s + 2] = __builtin_inf();
             dl[2 * s + 3] = d3;
          }
-         for (int e = tid; e < rem * rem; e += kThreads) {
-            const int i = s + 2 + e % rem, j = s + 2 + e / rem;
-            if (i >= kb || j > i) continue;
-            const double a1 = T[i][s], a2 = T[i][s + 1];
-            const double l0 = dot2(d0, a1, d1, a2), l1 = dot2(d1, a1, d3, a2);
-            T[i][j] = fnma(l1, T[j][s + 1], fnma(l0, T[j][s], T[i][j]));
+         {
+            const int i = s + 2 + ti;
+            if (i < kb)
+               for (int j = s + 2 + tj; j <= i; j += 4) {
+                  const double a1 = T[i][s], a2 = T[i][s + 1];
+                  const double l0 = dot2(d0, a1, d1, a2), l1 = dot2(d1, a1, d3, a2);
+                  const double x = fnma(l1, T[j][s + 1], fnma(l0, T[j][s], T[i][j]));
+                  T[i][j] = x;
+                  if (j < i) T[j][i] = x;
+               }
          }
          __syncthreads();
          for (int i = s + 2 + tid; i < kb; i += kThreads) {
@@ -280,11 +335,6 @@ __device__ int bp_block(double (*T)[kAppNB + 1], double* dl, int kb, double smal
          }
          if (tid == 0) T[s + 1][s] = 0.0;
          s += 2;
-      }
-      __syncthreads();
-      for (int e = tid; e < kb * kb; e += kThreads) {  // keep the trailing block symmetric
-         const int i = e % kb, j = e / kb;
-         if (j >= s && i > j) T[j][i] = T[i][j];
       }
       __syncthreads();
    }
@@ -459,8 +509,8 @@ static __global__ __launch_bounds__(kThreads) void k_app_front(const int* __rest
    __shared__ double T[NB][NB + 1];
    __shared__ double VS[NB][NB + 1];
    __shared__ double dl[2 * NB];
-   __shared__ double red_v[4];
-   __shared__ int red_i[4];
+   __shared__ double red_v[8];
+   __shared__ int red_i[8];
    __shared__ int s_fail, s_npass, s_bad;
    const int node = list[blockIdx.x];
    const IndefDesc nd = nodes[node];
@@ -800,8 +850,8 @@ static __global__ __launch_bounds__(kThreads) void k_lp_diag(int node, int c0, c
    constexpr int NB = kAppNB;
    __shared__ double T[NB][NB + 1];
    __shared__ double dl[2 * NB];
-   __shared__ double red_v[4];
-   __shared__ int red_i[4];
+   __shared__ double red_v[8];
+   __shared__ int red_i[8];
    __shared__ int perm[NB];
    const IndefDesc nd = nodes[node];
    const int m = nd.m, n = nd.n, ld = nd.ld, tid = threadIdx.x;
