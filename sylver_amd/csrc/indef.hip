@@ -52,6 +52,7 @@ struct IndefState {
    int64_t out_cap = 0;
    double* d_xw = nullptr;
    int64_t xw_cap = 0;
+   hipEvent_t ev[2] = {nullptr, nullptr};  // look-ahead of the multi-workgroup panel
    int64_t x_rows = 0;  // sum of 2 m over nodes
    // statistics of the last factorization
    int num_delay = 0, num_neg = 0, num_two = 0, num_zero = 0, maxfront = 0;
@@ -64,6 +65,8 @@ struct IndefState {
                       (void*)d_ws, (void*)d_list, (void*)d_tasks, (void*)d_status, (void*)d_out,
                       (void*)d_xw})
          dfree(p);
+      for (hipEvent_t e : ev)
+         if (e) (void)hipEventDestroy(e);
    }
 };
 
@@ -95,7 +98,7 @@ int64_t ws_elems(int m, int n) { return app_ws_elems(m, n); }
 // apply + threshold -> finish -> update, then permute_failed and the second
 // pass. Everything stays on the device; the host only enqueues.
 void large_panel(IndefState& S, int node, int slot, double u, double small, int action, int failed_tpp,
-                 hipStream_t s) {
+                 hipStream_t s, hipStream_t s2) {
    const IndefDesc& d = S.h[node];
    const int m = d.m, n = d.n;
    auto grid_cap = [](int64_t items, int64_t cap) {
@@ -104,20 +107,36 @@ void large_panel(IndefState& S, int node, int slot, double u, double small, int 
    const int nt = (n + 63) / 64;
    k_lp_start<<<(unsigned)std::min<int64_t>((int64_t)nt * (nt + 1) / 2 + 1, 2048), kThreads, 0, s>>>(
        node, S.d_nodes, S.arena, S.d_dinv, S.d_ws);
-   for (int c0 = 0; c0 < n; c0 += kAppNB) {
-      const int kb = std::min(kAppNB, n - c0);
-      k_lp_diag<<<1, kThreads, 0, s>>>(node, c0, S.d_nodes, S.arena, S.d_ws, small, action);
+   // Look-ahead: after block k's finish, the update of block k+1's diagonal
+   // block runs first and block k+1's pivoting follows on the main stream,
+   // while the rest of block k's update runs on the second stream; block
+   // k+1's permutation waits for it. D^{-1} of the block is double-buffered.
+   for (hipEvent_t& e : S.ev)
+      if (!e) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+   k_lp_diag<<<1, kThreads, 0, s>>>(node, 0, 0, S.d_nodes, S.arena, S.d_ws, small, action);
+   for (int c0 = 0, k = 0; c0 < n; c0 += kAppNB, ++k) {
+      const int kb = std::min(kAppNB, n - c0), par = k & 1;
+      const bool next = c0 + kb < n;
+      if (k > 0) HIPCHECK(hipStreamWaitEvent(s, S.ev[1], 0));
       k_lp_perm<<<1 + grid_cap((int64_t)m + n, 1024), kThreads, 0, s>>>(node, c0, S.d_nodes, S.arena, S.d_vars,
                                                                        S.d_pres, S.d_ws);
-      k_lp_apply<<<grid_cap((int64_t)m - kb, 1024), kThreads, 0, s>>>(node, c0, S.d_nodes, S.arena, S.d_ws, u,
-                                                                       small);
-      k_lp_finish<<<1, kThreads, 0, s>>>(node, c0, S.d_nodes, S.arena, S.d_dinv, S.d_ws);
+      k_lp_apply<<<grid_cap((int64_t)m - kb, 1024), kThreads, 0, s>>>(node, c0, par, S.d_nodes, S.arena, S.d_ws,
+                                                                       u, small);
+      k_lp_finish<<<1, kThreads, 0, s>>>(node, c0, par, S.d_nodes, S.arena, S.d_dinv, S.d_ws);
+      HIPCHECK(hipEventRecord(S.ev[0], s));
+      if (next) {
+         k_lp_update<<<3, kThreads, 0, s>>>(node, c0, par, 1, S.d_nodes, S.arena, S.d_ws);
+         k_lp_diag<<<1, kThreads, 0, s>>>(node, c0 + kb, par ^ 1, S.d_nodes, S.arena, S.d_ws, small, action);
+      }
       // tiles of the largest possible index space (every earlier column failed)
       const int64_t tr = (m - c0 + kLpTile - 1) / kLpTile + (c0 + kLpTile - 1) / kLpTile;
       const int64_t tiles = tr * (tr + 1) / 2;
-      k_lp_update<<<(unsigned)std::min<int64_t>(tiles, 1024), kThreads, 0, s>>>(node, c0, S.d_nodes, S.arena,
-                                                                                 S.d_ws);
+      HIPCHECK(hipStreamWaitEvent(s2, S.ev[0], 0));
+      k_lp_update<<<(unsigned)std::min<int64_t>(tiles, 1024), kThreads, 0, s2>>>(node, c0, par, next ? 2 : 0,
+                                                                                  S.d_nodes, S.arena, S.d_ws);
+      HIPCHECK(hipEventRecord(S.ev[1], s2));
    }
+   HIPCHECK(hipStreamWaitEvent(s, S.ev[1], 0));
    // the second pass needs the first pass's outcome on the host (one read-back)
    int st[16];
    const int64_t st_off = d.ws_off + (int64_t)m * (kAppNB + n + 5) + 8 + kAppNB * kAppNB + 2 * kAppNB;
@@ -283,10 +302,22 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
       k_fill<<<grid_of(lv_arena), kThreads, 0, s>>>(lv_arena, 0.0, S.arena + arena_used);
       k_ind_init<<<nl, kThreads, 0, s>>>(S.d_list, S.d_nodes, P->d_nlist, P->d_rlist, F->d_aval, S.arena,
                                          S.d_vars, S.d_pres);
-      if (n_delay > 0)
-         k_ind_delays<<<n_delay, kThreads, 0, s>>>(S.d_list + o_delay, S.d_nodes, P->d_cmap, S.arena, S.d_vars);
+      if (n_delay > 0) {
+         int maxd = 1;
+         for (int k = 0; k < n_delay; ++k) {
+            const IndefDesc& c = S.h[lists[o_delay + k]];
+            maxd = std::max(maxd, c.n - c.nelim);
+         }
+         dim3 g((unsigned)std::min(maxd, 512), (unsigned)n_delay);
+         k_ind_delays<<<g, kThreads, 0, s>>>(S.d_list + o_delay, S.d_nodes, P->d_cmap, S.arena, S.d_vars);
+      }
       for (auto& rk : ranks) {
-         dim3 g(64, rk.second);
+         int maxq = 1;
+         for (int k = 0; k < rk.second; ++k) {
+            const IndefDesc& c = S.h[lists[rk.first + k]];
+            maxq = std::max(maxq, c.m - c.n);
+         }
+         dim3 g((unsigned)std::min(std::max(maxq / 8, 64), 1024), rk.second);
          k_ind_ea<<<g, kThreads, 0, s>>>(S.d_list + rk.first, S.d_nodes, P->d_cmap, S.arena);
       }
       // panels: one workgroup per front for the small ones (batched), the
@@ -313,7 +344,7 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
       }
       for (int k = 0; k < nl; ++k) {
          if (!is_large(S.h[lv[k]])) continue;
-         large_panel(S, lv[k], slot[k], u, small, action, failed_tpp, s);
+         large_panel(S, lv[k], slot[k], u, small, action, failed_tpp, s, F->stream2);
       }
       std::vector<int> outs(2 * nl), out(2 * nl);
       HIPCHECK(hipMemcpyAsync(outs.data(), S.d_out, outs.size() * sizeof(int), hipMemcpyDeviceToHost, s));

@@ -779,11 +779,12 @@ struct AppWs {
    int* PERM;   // kAppNB block permutation
    int* FL;     // n failed columns, ascending
    double* TS;  // 4 n second-pass column statistics
+   double* DL2; // 2 kAppNB D^{-1} entries of the odd blocks (DL: even blocks)
 };
 
 __host__ __device__ inline int64_t app_ws_elems(int m, int n) {
    return (int64_t)m * (kAppNB + n + 5) + 8 + kAppNB * kAppNB + 2 * kAppNB + 8 + kAppNB / 2 + (n + 1) / 2 + 8 +
-          4 * (int64_t)n + 8;
+          4 * (int64_t)n + 8 + 2 * kAppNB;
 }
 
 __device__ inline AppWs app_ws(double* base, int m, int n) {
@@ -799,6 +800,7 @@ __device__ inline AppWs app_ws(double* base, int m, int n) {
    w.PERM = w.ST + 16;
    w.FL = w.PERM + kAppNB;
    w.TS = w.TG + kAppNB * kAppNB + 2 * kAppNB + 8 + kAppNB / 2 + (n + 1) / 2 + 8;
+   w.DL2 = w.TS + 4 * (int64_t)n + 8;
    return w;
 }
 
@@ -843,7 +845,7 @@ static __global__ __launch_bounds__(kThreads) void k_lp_start(int node, const In
 
 // Bunch-Parlett of the block at c0 (one workgroup): T, D^{-1} and the block
 // permutation into the panel state; the threshold minimum starts at kb.
-static __global__ __launch_bounds__(kThreads) void k_lp_diag(int node, int c0, const IndefDesc* __restrict__ nodes,
+static __global__ __launch_bounds__(kThreads) void k_lp_diag(int node, int c0, int par, const IndefDesc* __restrict__ nodes,
                                                       const double* __restrict__ arena,
                                                       double* __restrict__ wsbase, double small, int action) {
 #pragma clang fp contract(off)
@@ -893,7 +895,8 @@ static __global__ __launch_bounds__(kThreads) void k_lp_diag(int node, int c0, c
       const int i = e % kb, j = e / kb;
       w.TG[i + j * NB] = T[i][j];
    }
-   for (int k = tid; k < 2 * kb; k += kThreads) w.DL[k] = dl[k];
+   double* DLp = par ? w.DL2 : w.DL;
+   for (int k = tid; k < 2 * kb; k += kThreads) DLp[k] = dl[k];
    if (tid < kb) w.PERM[tid] = perm[tid];
    if (tid == 0) w.ST[kStFail] = kb;
 }
@@ -970,7 +973,7 @@ static __global__ __launch_bounds__(kThreads) void k_lp_perm(int node, int c0, c
 
 // ApplyN/ApplyT and the threshold test on the uneliminated rows outside the
 // block: the failed columns so far, then rows c0 + kb .. m - 1.
-static __global__ __launch_bounds__(kThreads) void k_lp_apply(int node, int c0, const IndefDesc* __restrict__ nodes,
+static __global__ __launch_bounds__(kThreads) void k_lp_apply(int node, int c0, int par, const IndefDesc* __restrict__ nodes,
                                                        double* __restrict__ arena, double* __restrict__ wsbase,
                                                        double u, double small) {
 #pragma clang fp contract(off)
@@ -988,7 +991,7 @@ static __global__ __launch_bounds__(kThreads) void k_lp_apply(int node, int c0, 
       const int i = e % kb, j = e / kb;
       T[i][j] = w.TG[i + j * NB];
    }
-   for (int k = tid; k < 2 * kb; k += kThreads) dl[k] = w.DL[k];
+   for (int k = tid; k < 2 * kb; k += kThreads) dl[k] = (par ? w.DL2 : w.DL)[k];
    if (tid == 0) s_fail = kb;
    __syncthreads();
    const int nf = w.ST[kStNf];
@@ -1014,7 +1017,7 @@ static __global__ __launch_bounds__(kThreads) void k_lp_apply(int node, int c0, 
 
 // Passed count of the block, L11 of the passed columns, their D^{-1} and
 // flags; the failed columns of the block join the failed list.
-static __global__ __launch_bounds__(kThreads) void k_lp_finish(int node, int c0, const IndefDesc* __restrict__ nodes,
+static __global__ __launch_bounds__(kThreads) void k_lp_finish(int node, int c0, int par, const IndefDesc* __restrict__ nodes,
                                                         double* __restrict__ arena, double* __restrict__ dinv,
                                                         double* __restrict__ wsbase) {
    constexpr int NB = kAppNB;
@@ -1025,7 +1028,7 @@ static __global__ __launch_bounds__(kThreads) void k_lp_finish(int node, int c0,
    if (w.ST[kStBad]) return;
    const int kb = min(NB, n - c0);
    double* F = arena + nd.off;
-   for (int k = tid; k < 2 * kb; k += kThreads) dl[k] = w.DL[k];
+   for (int k = tid; k < 2 * kb; k += kThreads) dl[k] = (par ? w.DL2 : w.DL)[k];
    const int nf = w.ST[kStNf], fail = w.ST[kStFail];
    __syncthreads();
    const int npass = npass_adjust(fail, dl);
@@ -1057,6 +1060,16 @@ static __global__ __launch_bounds__(kThreads) void k_lp_finish(int node, int c0,
 
 constexpr int kLpTile = 128;  // tile of the multi-workgroup fully-summed update
 
+// Entries of k_lp_update by mode: 0 all, 1 only those of the next block's
+// diagonal block (row and column index positions in [nf, nf + kb2)), 2 all
+// the others. Modes 1 and 2 split one update in two launches so that the
+// next block's pivoting can start before the rest is done.
+__device__ __forceinline__ bool lp_keep(int mode, int rpos, int spos, int nf, int kb2) {
+   if (mode == 0) return true;
+   const bool in = rpos >= nf && rpos < nf + kb2 && spos >= nf && spos < nf + kb2;
+   return (mode == 1) == in;
+}
+
 // Restore of the failed block columns and update of the uneliminated
 // fully-summed columns with the block's passed pivots: entry (r, s), r >= s,
 // of the index space U = (failed columns, ascending; c0 + kb .. n - 1) for
@@ -1065,7 +1078,8 @@ constexpr int kLpTile = 128;  // tile of the multi-workgroup fully-summed update
 // fully-summed part is mirrored. Entries of the failed block columns start
 // from the backup (the restore). 128 x 128 tiles, 8 x 8 entries per thread,
 // tiles taken in turn by a grid of any size.
-static __global__ __launch_bounds__(kThreads) void k_lp_update(int node, int c0, const IndefDesc* __restrict__ nodes,
+static __global__ __launch_bounds__(kThreads) void k_lp_update(int node, int c0, int par, int mode,
+                                                        const IndefDesc* __restrict__ nodes,
                                                         double* __restrict__ arena, double* __restrict__ wsbase) {
 #pragma clang fp contract(off)
    constexpr int NB = kAppNB, TT = kLpTile;
@@ -1084,17 +1098,27 @@ static __global__ __launch_bounds__(kThreads) void k_lp_update(int node, int c0,
    if (nu <= 0) return;
    const int Tr = (nr + TT - 1) / TT, Tc = (nu + TT - 1) / TT;
    const int64_t ntiles = (int64_t)Tc * Tr - (int64_t)Tc * (Tc - 1) / 2;
-   for (int k = tid; k < 2 * kb; k += kThreads) dl[k] = w.DL[k];
+   // the next block's diagonal block: index positions [nf, nf + kb2)
+   const int kb2 = max(0, min(NB, n - c0 - kb));
+   const int t0 = nf / TT, t1 = (nf + max(kb2, 1) - 1) / TT;
+   if (mode == 1 && (kb2 == 0 || npass == 0)) return;
+   const int64_t nt = mode == 1 ? (t1 > t0 ? 3 : 1) : ntiles;
+   for (int k = tid; k < 2 * kb; k += kThreads) dl[k] = (par ? w.DL2 : w.DL)[k];
    const int lo = c0 + npass, hi = c0 + kb;  // failed block columns
    const int tr8 = tid & 15, tc8 = tid >> 4;
-   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-      int tc = 0;
-      int64_t rem = t;
-      while (rem >= Tr - tc) {
-         rem -= Tr - tc;
-         ++tc;
+   for (int64_t t = blockIdx.x; t < nt; t += gridDim.x) {
+      int tc = 0, tr = 0;
+      if (mode == 1) {
+         tr = t == 0 ? t0 : t1;
+         tc = t == 2 ? t1 : t0;
+      } else {
+         int64_t rem = t;
+         while (rem >= Tr - tc) {
+            rem -= Tr - tc;
+            ++tc;
+         }
+         tr = tc + (int)rem;
       }
-      const int tr = tc + (int)rem;
       const int R0 = tr * TT, S0 = tc * TT;
       if (npass == 0 && !(R0 < nf && R0 + TT > nf_old) && !(S0 < nf && S0 + TT > nf_old)) continue;
       __syncthreads();
@@ -1130,7 +1154,7 @@ static __global__ __launch_bounds__(kThreads) void k_lp_update(int node, int c0,
          for (int a = 0; a < 8; ++a) {
             const int r = ridx[tr8 * 8 + a];
             double x = 0.0;
-            if (r >= 0 && s >= 0 && r >= s) {
+            if (r >= 0 && s >= 0 && r >= s && lp_keep(mode, R0 + tr8 * 8 + a, S0 + tc8 * 8 + b, nf, kb2)) {
                if (s >= lo && s < hi)
                   x = w.Bk[r + (int64_t)(s - c0) * m];
                else if (r >= lo && r < hi)
@@ -1159,7 +1183,7 @@ static __global__ __launch_bounds__(kThreads) void k_lp_update(int node, int c0,
 #pragma unroll
          for (int a = 0; a < 8; ++a) {
             const int r = ridx[tr8 * 8 + a];
-            if (r >= 0 && s >= 0 && r >= s) {
+            if (r >= 0 && s >= 0 && r >= s && lp_keep(mode, R0 + tr8 * 8 + a, S0 + tc8 * 8 + b, nf, kb2)) {
                F[r + (int64_t)s * ld] = acc[a][b];
                if (r < n && r > s) F[s + (int64_t)r * ld] = acc[a][b];
             }
@@ -1552,20 +1576,20 @@ static __global__ __launch_bounds__(kThreads) void k_ind_init(const int* __restr
 // src/kernels/assemble.hxx:926-961): the lower triangle of the delayed block
 // and the child's contribution rows of those columns, placed through the
 // child map; the parent's row list gets the delayed variables. Children of
-// one level write disjoint parent entries.
+// one level write disjoint parent entries; grid (column chunks, children).
 static __global__ __launch_bounds__(kThreads) void k_ind_delays(const int* __restrict__ children,
                                                          const IndefDesc* __restrict__ nodes,
                                                          const int* __restrict__ cmap,
                                                          double* __restrict__ arena,
                                                          int* __restrict__ vars) {
-   const IndefDesc cn = nodes[children[blockIdx.x]];
+   const IndefDesc cn = nodes[children[blockIdx.y]];
    const IndefDesc pn = nodes[cn.parent];
    const int nd_out = cn.n - cn.nelim;
    const double* C = arena + cn.off;
    double* P = arena + pn.off;
    const int* cm = cmap + cn.cmap_off;
    const int q = cn.m - cn.n;
-   for (int i = 0; i < nd_out; ++i) {
+   for (int i = blockIdx.x; i < nd_out; i += gridDim.x) {  // one delayed column per workgroup
       const int dc = cn.delay_col0 + i;
       const int cc = cn.nelim + i;
       for (int j = threadIdx.x; j < nd_out - i; j += kThreads)
@@ -1580,8 +1604,9 @@ static __global__ __launch_bounds__(kThreads) void k_ind_delays(const int* __res
             P[r + (int64_t)dc * pn.ld] = v;
       }
    }
-   for (int i = threadIdx.x; i < nd_out; i += kThreads)
-      vars[pn.var_off + cn.delay_col0 + i] = vars[cn.var_off + cn.nelim + i];
+   if (blockIdx.x == 0)
+      for (int i = threadIdx.x; i < nd_out; i += kThreads)
+         vars[pn.var_off + cn.delay_col0 + i] = vars[cn.var_off + cn.nelim + i];
 }
 
 // Extend-add of the contribution blocks of the children of one rank
