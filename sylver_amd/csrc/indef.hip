@@ -34,7 +34,7 @@ struct IndefState {
    std::vector<int> height;
    IndefDesc* d_nodes = nullptr;
    double* arena = nullptr;
-   int64_t arena_cap = 0, arena_used = 0;
+   int64_t arena_cap = 0;  // workspace of one level's fronts
    int* d_vars = nullptr;
    int64_t vars_cap = 0;
    int* d_pres = nullptr;
@@ -53,6 +53,37 @@ struct IndefState {
    double* d_xw = nullptr;
    int64_t xw_cap = 0;
    hipEvent_t ev[2] = {nullptr, nullptr};  // look-ahead of the multi-workgroup panel
+   double* lstore = nullptr;  // L of every front (m x nelim, ld m)
+   int64_t lstore_cap = 0;
+   double* cbs = nullptr;     // trailing blocks of up to kCbBig entries, until the parent assembled them
+   int64_t cbs_cap = 0;
+   double* d_spart = nullptr;  // block sums of the blocked backward solve
+   int64_t spart_cap = 0;
+   // own allocations of large trailing blocks, reused (no free and
+   // re-allocation of tens of GB per factorization): free list and the
+   // capacity of each node's block
+   std::vector<std::pair<int64_t, double*>> cb_pool;
+   std::vector<int64_t> cb_cap;
+
+   double* cb_acquire(int64_t need, int64_t& cap) {
+      int best = -1;
+      for (int i = 0; i < (int)cb_pool.size(); ++i)
+         if (cb_pool[i].first >= need && cb_pool[i].first <= 2 * need &&
+             (best < 0 || cb_pool[i].first < cb_pool[best].first))
+            best = i;
+      if (best >= 0) {
+         double* p = cb_pool[best].second;
+         cap = cb_pool[best].first;
+         cb_pool.erase(cb_pool.begin() + best);
+         return p;
+      }
+      cap = need + need / 8;
+      return dalloc<double>(cap);
+   }
+   void cb_release(double*& p, int64_t cap) {
+      if (p) cb_pool.push_back({cap, p});
+      p = nullptr;
+   }
    int64_t x_rows = 0;  // sum of 2 m over nodes
    // statistics of the last factorization
    int num_delay = 0, num_neg = 0, num_two = 0, num_zero = 0, maxfront = 0;
@@ -67,6 +98,12 @@ struct IndefState {
          dfree(p);
       for (hipEvent_t e : ev)
          if (e) (void)hipEventDestroy(e);
+      dfree(lstore);
+      dfree(cbs);
+      dfree(d_spart);
+      for (IndefDesc& d : h)
+         if (d.cb_big) dfree(d.cb_big);
+      for (auto& pc : cb_pool) dfree(pc.second);
    }
 };
 
@@ -92,6 +129,10 @@ void grow(T*& p, int64_t& cap, int64_t need, int64_t keep, hipStream_t s) {
 }
 
 int64_t ws_elems(int m, int n) { return app_ws_elems(m, n); }
+
+// Trailing blocks of at least this many entries get their own allocation,
+// freed once the parent has assembled them.
+constexpr int64_t kCbBig = int64_t(1) << 22;
 
 // Multi-workgroup APP panel of one large front (k_lp_* launches, see
 // kernels_indef.hpp): per block of kAppNB columns diag -> permutation ->
@@ -177,7 +218,10 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
    IndefState& S = *F->ind;
    const int nn = a.nnodes;
    S.nnodes = nn;
+   for (int i = 0; i < (int)S.h.size(); ++i)  // left from an earlier call that stopped early
+      if (S.h[i].cb_big) S.cb_release(S.h[i].cb_big, S.cb_cap[i]);
    S.h.assign(nn, IndefDesc{});
+   S.cb_cap.assign(nn, 0);
    S.height.assign(nn, 0);
    for (int i = 0; i < nn; ++i) {
       const SymbolicFront& f = a.fronts[i];
@@ -202,14 +246,14 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
    HIPCHECK(hipMemsetAsync(S.d_status, 0, std::max(nn, 1) * sizeof(int), s));
    // initial capacities from the symbolic sizes; grown when delays need more
    {
-      int64_t ar = 0, vr = 0, pr = 0;
+      int64_t vr = 0, pr = 0, lf = 0;
       for (int i = 0; i < nn; ++i) {
          const int m = S.h[i].nrow;
-         ar += (int64_t)((m + 7) / 8 * 8) * m + 8;
          vr += m;
          pr += S.h[i].ncol;
+         lf += (int64_t)m * S.h[i].ncol;
       }
-      grow(S.arena, S.arena_cap, ar + ar / 8 + 64, 0, s);
+      grow(S.lstore, S.lstore_cap, lf + lf / 4 + 64, 0, s);
       grow(S.d_vars, S.vars_cap, vr + vr / 4 + 64, 0, s);
       grow(S.d_pres, S.pres_cap, pr + pr / 4 + 64, 0, s);
       grow(S.d_dinv, S.dinv_cap, 2 * (pr + pr / 4 + 64), 0, s);
@@ -225,7 +269,7 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
       return d.n >= panel_min || (panel_opt <= 0 && (int64_t)d.m * d.n >= (int64_t(1) << 18));
    };
    const int failed_tpp = (opts.failed_pivot_method == 1 || opts.pivot_method == 3) ? 1 : 0;
-   int64_t arena_used = 0, vars_used = 0, pres_used = 0, dinv_used = 0, x_used = 0;
+   int64_t vars_used = 0, pres_used = 0, dinv_used = 0, x_used = 0, lf_used = 0, cb_used = 0;
    S.num_factor = 0;
    S.maxfront = 0;
    S.flops = 0;
@@ -247,7 +291,7 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
          d.m = d.nrow + ndin;
          d.n = d.ncol + ndin;
          d.ld = (d.m + 7) / 8 * 8;
-         d.off = arena_used + lv_arena;
+         d.off = lv_arena;  // the level's fronts share one workspace
          lv_arena += (int64_t)d.ld * d.m;
          d.var_off = vars_used;
          vars_used += d.m;
@@ -263,7 +307,7 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
          d.nelim_first = 0;
          S.maxfront = std::max(S.maxfront, d.m);
       }
-      grow(S.arena, S.arena_cap, arena_used + lv_arena, arena_used, s);
+      grow(S.arena, S.arena_cap, lv_arena, 0, s);
       grow(S.d_vars, S.vars_cap, vars_used, vars_before, s);
       grow(S.d_pres, S.pres_cap, pres_used, pres_before, s);
       grow(S.d_dinv, S.dinv_cap, dinv_used, dinv_before, s);
@@ -299,7 +343,7 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
       HIPCHECK(hipMemcpyAsync(S.d_nodes, S.h.data(), nn * sizeof(IndefDesc), hipMemcpyHostToDevice, s));
       const int nl = (int)lv.size();
       // zero the level's fronts (contiguous)
-      k_fill<<<grid_of(lv_arena), kThreads, 0, s>>>(lv_arena, 0.0, S.arena + arena_used);
+      k_fill<<<grid_of(lv_arena), kThreads, 0, s>>>(lv_arena, 0.0, S.arena);
       k_ind_init<<<nl, kThreads, 0, s>>>(S.d_list, S.d_nodes, P->d_nlist, P->d_rlist, F->d_aval, S.arena,
                                          S.d_vars, S.d_pres);
       if (n_delay > 0) {
@@ -309,7 +353,8 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
             maxd = std::max(maxd, c.n - c.nelim);
          }
          dim3 g((unsigned)std::min(maxd, 512), (unsigned)n_delay);
-         k_ind_delays<<<g, kThreads, 0, s>>>(S.d_list + o_delay, S.d_nodes, P->d_cmap, S.arena, S.d_vars);
+         k_ind_delays<<<g, kThreads, 0, s>>>(S.d_list + o_delay, S.d_nodes, P->d_cmap, S.arena, S.d_vars,
+                                             S.cbs);
       }
       for (auto& rk : ranks) {
          int maxq = 1;
@@ -318,7 +363,7 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
             maxq = std::max(maxq, c.m - c.n);
          }
          dim3 g((unsigned)std::min(std::max(maxq / 8, 64), 1024), rk.second);
-         k_ind_ea<<<g, kThreads, 0, s>>>(S.d_list + rk.first, S.d_nodes, P->d_cmap, S.arena);
+         k_ind_ea<<<g, kThreads, 0, s>>>(S.d_list + rk.first, S.d_nodes, P->d_cmap, S.arena, S.cbs);
       }
       // panels: one workgroup per front for the small ones (batched), the
       // multi-workgroup sequence for each large one; out slots small first
@@ -353,6 +398,10 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
          out[2 * k] = outs[2 * slot[k]];
          out[2 * k + 1] = outs[2 * slot[k] + 1];
       }
+      // the children's own trailing blocks are assembled (the read-back synchronized)
+      for (int i : lv)
+         for (int c = S.h[i].first_child; c != -1; c = S.h[c].next_child)
+            if (S.h[c].cb_big) S.cb_release(S.h[c].cb_big, S.cb_cap[c]);
       // contribution blocks: L D into the workspace, then the MFMA update
       std::vector<int> cb_nodes;
       std::vector<GemmTask> tasks;
@@ -397,9 +446,37 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
          k_update<double, 2><<<(unsigned)tasks.size(), kThreads, 0, s>>>(S.d_tasks, S.arena, S.d_ws);
          HIPCHECK(hipStreamSynchronize(s));  // the task and list buffers are rewritten next level
       }
-      arena_used += lv_arena;
+      // L into the factor store, trailing blocks into the contribution store
+      const int64_t lf_before = lf_used, cb_before = cb_used;
+      for (int i : lv) {
+         IndefDesc& d = S.h[i];
+         d.lf_off = lf_used;
+         lf_used += (int64_t)d.m * d.nelim;
+         d.cb_off = 0;
+         d.cb_big = nullptr;
+         const int64_t t = d.m - d.nelim;
+         if (d.parent < nn && t > 0) {
+            if (t * t >= kCbBig) {
+               d.cb_big = S.cb_acquire(t * t, S.cb_cap[i]);
+            } else {
+               d.cb_off = cb_used;
+               cb_used += t * t;
+            }
+         }
+      }
+      grow(S.lstore, S.lstore_cap, std::max<int64_t>(lf_used, 1), lf_before, s);
+      grow(S.cbs, S.cbs_cap, std::max<int64_t>(cb_used, 1), cb_before, s);
+      grow(S.d_list, S.list_cap, nl, 0, s);
+      HIPCHECK(hipMemcpyAsync(S.d_list, lv.data(), nl * sizeof(int), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemcpyAsync(S.d_nodes, S.h.data(), nn * sizeof(IndefDesc), hipMemcpyHostToDevice, s));
+      {
+         int64_t big = 1;
+         for (int i : lv) big = std::max<int64_t>(big, (int64_t)S.h[i].m * S.h[i].m);
+         dim3 g((unsigned)std::min<int64_t>((big + kThreads - 1) / kThreads, 2048), (unsigned)nl);
+         k_ind_store<<<g, kThreads, 0, s>>>(S.d_list, S.d_nodes, S.arena, S.lstore, S.cbs, nn);
+      }
+      HIPCHECK(hipStreamSynchronize(s));  // list and node buffers are rewritten next level
    }
-   S.arena_used = arena_used;
    S.x_rows = x_used;
    HIPCHECK(hipMemcpyAsync(S.d_nodes, S.h.data(), nn * sizeof(IndefDesc), hipMemcpyHostToDevice, s));
    // flags and statistics (src/NumericTree.hxx:150-179, src/factor_failed.hxx:55-150)
@@ -474,10 +551,40 @@ void solve_indef(const DevicePlan* P, DeviceFactors* F, int job, const double* b
    HIPCHECK(hipMemcpyAsync(S.d_list, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice, s));
    const bool fwd = job == 0 || job == 1, diag = job == 0 || job == 2 || job == 4,
               bwd = job == 0 || job == 3 || job == 4;
+   int maxm = 1;
+   for (const IndefDesc& d : S.h) maxm = std::max(maxm, d.m);
+   grow(S.d_spart, S.spart_cap, (int64_t)((maxm + kThreads - 1) / kThreads + 1) * nrhs * 64, 0, s);
+   // fronts with kIsvMin eliminated columns or more: blocked elimination
+   auto blocked = [&](int L, bool forward) {
+      for (int i : S.levels[L]) {
+         const IndefDesc& d = S.h[i];
+         if (d.nelim < kIsvMin) continue;
+         if (forward) {
+            for (int j0 = 0; j0 < d.nelim; j0 += 64) {
+               const int jb = std::min(64, d.nelim - j0);
+               k_isv_fdiag<<<1, kThreads, 0, s>>>(i, j0, S.d_nodes, S.lstore, S.d_xw, nrhs);
+               if (d.m > j0 + jb)
+                  k_isv_fupd<<<grid_of(d.m - j0 - jb), kThreads, 0, s>>>(i, j0, S.d_nodes, S.lstore, S.d_xw, nrhs);
+            }
+         } else {
+            for (int j0 = (d.nelim - 1) / 64 * 64; j0 >= 0; j0 -= 64) {
+               const int jb = std::min(64, d.nelim - j0);
+               const int rows = d.m - j0 - jb, nch = rows > 0 ? (int)grid_of(rows) : 0;
+               if (nch > 0)
+                  k_isv_bdot<<<nch, kThreads, 0, s>>>(i, j0, S.d_nodes, S.lstore, S.d_xw, S.d_spart, nrhs);
+               k_isv_bdiag<<<1, kThreads, 0, s>>>(i, j0, nch, S.d_nodes, S.lstore, S.d_xw, S.d_spart, nrhs);
+            }
+         }
+         k_isv_out<<<grid_of((int64_t)d.nelim * nrhs), kThreads, 0, s>>>(i, S.d_nodes, S.d_vars, S.d_xw, sol, n,
+                                                                        nrhs);
+      }
+   };
    if (fwd) {
-      for (size_t L = 0; L < S.levels.size(); ++L)
+      for (size_t L = 0; L < S.levels.size(); ++L) {
          k_ind_fwd<<<(unsigned)S.levels[L].size(), kThreads, 0, s>>>(
-             S.d_list + loff[L], S.d_nodes, S.arena, S.d_vars, S.d_pres, P->d_cmap, b, n, S.d_xw, sol, nrhs);
+             S.d_list + loff[L], S.d_nodes, S.lstore, S.d_vars, S.d_pres, P->d_cmap, b, n, S.d_xw, sol, nrhs);
+         blocked((int)L, true);
+      }
    } else {
       k_copy<<<grid_of(nn), kThreads, 0, s>>>(nn, b, sol);
    }
@@ -485,9 +592,11 @@ void solve_indef(const DevicePlan* P, DeviceFactors* F, int job, const double* b
       k_ind_diag<<<std::max(S.nnodes, 1), kThreads, 0, s>>>(S.d_nodes, S.nnodes, S.d_vars, S.d_dinv, sol, n,
                                                             nrhs);
    if (bwd)
-      for (int L = (int)S.levels.size() - 1; L >= 0; --L)
-         k_ind_bwd<<<(unsigned)S.levels[L].size(), kThreads, 0, s>>>(S.d_list + loff[L], S.d_nodes, S.arena,
+      for (int L = (int)S.levels.size() - 1; L >= 0; --L) {
+         k_ind_bwd<<<(unsigned)S.levels[L].size(), kThreads, 0, s>>>(S.d_list + loff[L], S.d_nodes, S.lstore,
                                                                      S.d_vars, S.d_xw, sol, n, nrhs);
+         blocked(L, false);
+      }
 }
 
 int indef_stats(const DeviceFactors* F, int* out) {
@@ -519,7 +628,7 @@ int indef_node(const DeviceFactors* F, int node, int* dims, int* rows, double* L
       }
       if (rows && d.m > 0) HIPCHECK(hipMemcpy(rows, S.d_vars + d.var_off, d.m * sizeof(int), hipMemcpyDeviceToHost));
       if (L && d.nelim > 0)
-         HIPCHECK(hipMemcpy2D(L, (size_t)d.m * sizeof(double), S.arena + d.off, (size_t)d.ld * sizeof(double),
+         HIPCHECK(hipMemcpy2D(L, (size_t)d.m * sizeof(double), S.lstore + d.lf_off, (size_t)d.m * sizeof(double),
                               (size_t)d.m * sizeof(double), d.nelim, hipMemcpyDeviceToHost));
       if (dinv && d.nelim > 0)
          HIPCHECK(hipMemcpy(dinv, S.d_dinv + d.d_off, 2 * d.nelim * sizeof(double), hipMemcpyDeviceToHost));

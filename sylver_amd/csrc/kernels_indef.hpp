@@ -43,7 +43,16 @@ struct IndefDesc {
    int64_t d_off;            // 2 n doubles: D^{-1} (1x1 [d, 0], 2x2 [a, b, inf, c])
    int64_t ws_off;           // workspace (doubles) of the panel kernel / CB update
    int64_t x_off;            // solve work vectors: at x_off * nrhs, 2 m doubles per right-hand side
+   int64_t lf_off;           // L (m x nelim, ld m) in the factor store after the level
+   int64_t cb_off;           // trailing block (rows/cols nelim..m-1, ld m - nelim) in the
+   double* cb_big;           //   contribution store, or in its own allocation when cb_big
 };
+
+// Trailing block of a factored front: its delayed columns and contribution
+// block, (m - nelim) square, lower triangle, kept until the parent assembled it.
+__device__ __forceinline__ const double* child_cb(const IndefDesc& c, const double* cbs) {
+   return c.cb_big ? c.cb_big : cbs + c.cb_off;
+}
 
 __device__ __forceinline__ double fnma(double a, double b, double c) { return __builtin_fma(-a, b, c); }
 __device__ __forceinline__ double dot2(double a, double b, double c, double d) {
@@ -1581,23 +1590,23 @@ static __global__ __launch_bounds__(kThreads) void k_ind_delays(const int* __res
                                                          const IndefDesc* __restrict__ nodes,
                                                          const int* __restrict__ cmap,
                                                          double* __restrict__ arena,
-                                                         int* __restrict__ vars) {
+                                                         int* __restrict__ vars, const double* __restrict__ cbs) {
    const IndefDesc cn = nodes[children[blockIdx.y]];
    const IndefDesc pn = nodes[cn.parent];
    const int nd_out = cn.n - cn.nelim;
-   const double* C = arena + cn.off;
+   const double* C = child_cb(cn, cbs);  // front entry (nelim + i, nelim + j) at C[i + j * ldc]
+   const int64_t ldc = cn.m - cn.nelim;
    double* P = arena + pn.off;
    const int* cm = cmap + cn.cmap_off;
    const int q = cn.m - cn.n;
    for (int i = blockIdx.x; i < nd_out; i += gridDim.x) {  // one delayed column per workgroup
       const int dc = cn.delay_col0 + i;
-      const int cc = cn.nelim + i;
       for (int j = threadIdx.x; j < nd_out - i; j += kThreads)
-         P[dc + j + (int64_t)dc * pn.ld] = C[cc + j + (int64_t)cc * cn.ld];
+         P[dc + j + (int64_t)dc * pn.ld] = C[i + j + (int64_t)i * ldc];
       for (int k = threadIdx.x; k < q; k += kThreads) {
          int r = cm[k];
          if (r >= pn.ncol) r += pn.ndelay_in;
-         const double v = C[cn.n + k + (int64_t)cc * cn.ld];
+         const double v = C[nd_out + k + (int64_t)i * ldc];
          if (r < pn.ncol)
             P[dc + (int64_t)r * pn.ld] = v;
          else
@@ -1615,11 +1624,12 @@ static __global__ __launch_bounds__(kThreads) void k_ind_delays(const int* __res
 static __global__ __launch_bounds__(kThreads) void k_ind_ea(const int* __restrict__ children,
                                                      const IndefDesc* __restrict__ nodes,
                                                      const int* __restrict__ cmap,
-                                                     double* __restrict__ arena) {
+                                                     double* __restrict__ arena, const double* __restrict__ cbs) {
    const IndefDesc cn = nodes[children[blockIdx.y]];
    const IndefDesc pn = nodes[cn.parent];
-   const int q = cn.m - cn.n;
-   const double* C = arena + cn.off;
+   const int q = cn.m - cn.n, nd_out = cn.n - cn.nelim;
+   const int64_t ldc = cn.m - cn.nelim;
+   const double* C = child_cb(cn, cbs) + nd_out + nd_out * ldc;  // contribution block, ld ldc
    double* P = arena + pn.off;
    const int* cm = cmap + cn.cmap_off;
    for (int j = blockIdx.x; j < q; j += gridDim.x) {
@@ -1628,7 +1638,7 @@ static __global__ __launch_bounds__(kThreads) void k_ind_ea(const int* __restric
       for (int i = j + threadIdx.x; i < q; i += kThreads) {
          int pi = cm[i];
          if (pi >= pn.ncol) pi += pn.ndelay_in;
-         P[pi + (int64_t)pj * pn.ld] += C[cn.n + i + (int64_t)(cn.n + j) * cn.ld];
+         P[pi + (int64_t)pj * pn.ld] += C[i + (int64_t)j * ldc];
       }
    }
 }
@@ -1669,6 +1679,34 @@ static __global__ __launch_bounds__(kThreads) void k_ind_ld(const int* __restric
    }
 }
 
+// After a level: L of every front (m x nelim) into the factor store, and the
+// trailing block (lower triangle of rows/cols nelim..m-1) into the
+// contribution store for the parent's assembly. Grid (chunks, fronts).
+static __global__ __launch_bounds__(kThreads) void k_ind_store(const int* __restrict__ list,
+                                                        const IndefDesc* __restrict__ nodes,
+                                                        const double* __restrict__ arena,
+                                                        double* __restrict__ lstore, double* __restrict__ cbs,
+                                                        int nnodes) {
+   const IndefDesc nd = nodes[list[blockIdx.y]];
+   const int m = nd.m, ne = nd.nelim;
+   const double* F = arena + nd.off;
+   const int64_t G = (int64_t)gridDim.x * kThreads, gid = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+   double* L = lstore + nd.lf_off;
+   for (int64_t e = gid; e < (int64_t)m * ne; e += G) {
+      const int i = (int)(e % m), j = (int)(e / m);
+      L[i + (int64_t)j * m] = F[i + (int64_t)j * nd.ld];
+   }
+   const int t = m - ne;
+   if (nd.parent >= nnodes || t <= 0) return;
+   double* C = nd.cb_big ? nd.cb_big : cbs + nd.cb_off;
+   for (int64_t e = gid; e < (int64_t)t * t; e += G) {
+      const int i = (int)(e % t), j = (int)(e / t);
+      if (i >= j) C[i + (int64_t)j * t] = F[ne + i + (int64_t)(ne + j) * nd.ld];
+   }
+}
+
+constexpr int kIsvMin = 256;  // fronts with this many eliminated columns get the blocked solve
+
 // ---------------------------------------------------------------------------
 // Solves (NumericTree::solve_fwd / solve_diag_bwd_inner,
 // src/NumericTree.hxx:408-532; ldlt_app_solve_fwd/diag/bwd,
@@ -1682,7 +1720,7 @@ static __global__ __launch_bounds__(kThreads) void k_ind_ld(const int* __restric
 // rows below; the eliminated entries go to `sol`.
 static __global__ __launch_bounds__(kThreads) void k_ind_fwd(const int* __restrict__ list,
                                                       const IndefDesc* __restrict__ nodes,
-                                                      const double* __restrict__ arena,
+                                                      const double* __restrict__ lstore,
                                                       const int* __restrict__ vars,
                                                       const int* __restrict__ pres,
                                                       const int* __restrict__ cmap,
@@ -1691,7 +1729,7 @@ static __global__ __launch_bounds__(kThreads) void k_ind_fwd(const int* __restri
                                                       double* __restrict__ sol, int nrhs) {
    const IndefDesc nd = nodes[list[blockIdx.x]];
    const int m = nd.m, n = nd.n, ne = nd.nelim;
-   const double* F = arena + nd.off;
+   const double* F = lstore + nd.lf_off;  // L, ld m
    const int* var = vars + nd.var_off;
    const int* pre = pres + nd.pre_off;
    for (int c = 0; c < nrhs; ++c) {
@@ -1721,10 +1759,11 @@ static __global__ __launch_bounds__(kThreads) void k_ind_fwd(const int* __restri
       }
       for (int r = threadIdx.x; r < m; r += kThreads) xl[r] = (r < n) ? xp[pre[r]] : xp[r];
       __syncthreads();
+      if (ne >= kIsvMin) continue;  // blocked elimination by k_isv_* (the loop's barriers stay uniform)
       for (int j = 0; j < ne; ++j) {
          const double y = xl[j];
          for (int r = j + 1 + threadIdx.x; r < m; r += kThreads)
-            xl[r] = fnma(F[r + (int64_t)j * nd.ld], y, xl[r]);
+            xl[r] = fnma(F[r + (int64_t)j * m], y, xl[r]);
          __syncthreads();
       }
       for (int r = threadIdx.x; r < ne; r += kThreads) sol[var[r] - 1 + (int64_t)c * ldb] = xl[r];
@@ -1761,7 +1800,7 @@ static __global__ __launch_bounds__(kThreads) void k_ind_diag(const IndefDesc* _
 // (ancestors are done, levels run top-down).
 static __global__ __launch_bounds__(kThreads) void k_ind_bwd(const int* __restrict__ list,
                                                       const IndefDesc* __restrict__ nodes,
-                                                      const double* __restrict__ arena,
+                                                      const double* __restrict__ lstore,
                                                       const int* __restrict__ vars,
                                                       double* __restrict__ xw,
                                                       double* __restrict__ sol, int ld, int nrhs) {
@@ -1769,7 +1808,7 @@ static __global__ __launch_bounds__(kThreads) void k_ind_bwd(const int* __restri
    __shared__ double red[kThreads / 64];
    const IndefDesc nd = nodes[list[blockIdx.x]];
    const int m = nd.m, ne = nd.nelim;
-   const double* F = arena + nd.off;
+   const double* F = lstore + nd.lf_off;  // L, ld m
    const int* var = vars + nd.var_off;
    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    for (int c = 0; c < nrhs; ++c) {
@@ -1777,9 +1816,10 @@ static __global__ __launch_bounds__(kThreads) void k_ind_bwd(const int* __restri
       const double* x = sol + (int64_t)c * ld;
       for (int r = threadIdx.x; r < m; r += kThreads) xl[r] = x[var[r] - 1];
       __syncthreads();
+      if (ne >= kIsvMin) continue;  // blocked elimination by k_isv_*
       for (int j = ne - 1; j >= 0; --j) {
          double s = 0.0;
-         for (int r = j + 1 + threadIdx.x; r < m; r += kThreads) s = __builtin_fma(F[r + (int64_t)j * nd.ld], xl[r], s);
+         for (int r = j + 1 + threadIdx.x; r < m; r += kThreads) s = __builtin_fma(F[r + (int64_t)j * m], xl[r], s);
 #pragma unroll
          for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
          if (lane == 0) red[wave] = s;
@@ -1789,6 +1829,144 @@ static __global__ __launch_bounds__(kThreads) void k_ind_bwd(const int* __restri
       }
       for (int r = threadIdx.x; r < ne; r += kThreads) sol[var[r] - 1 + (int64_t)c * ld] = xl[r];
       __syncthreads();
+   }
+}
+
+
+// ---------------------------------------------------------------------------
+// Blocked solves of a front with many eliminated columns (kIsvMin or more),
+// 64 columns per step: the per-node kernels above gather (forward) or load
+// (backward) the work vector xl, these do the elimination, k_isv_out writes
+// the eliminated variables. Forward: the same fnma chain per row, in column
+// order, as k_ind_fwd; backward: block sums in a fixed order.
+// ---------------------------------------------------------------------------
+// Forward, triangle of block j0 (one workgroup; one wave per right-hand side).
+static __global__ __launch_bounds__(kThreads) void k_isv_fdiag(int node, int j0, const IndefDesc* __restrict__ nodes,
+                                                        const double* __restrict__ lstore,
+                                                        double* __restrict__ xw, int nrhs) {
+#pragma clang fp contract(off)
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, jb = min(64, nd.nelim - j0);
+   const double* L = lstore + nd.lf_off;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   for (int c = wave; c < nrhs; c += kThreads / 64) {
+      double* xl = xw + nd.x_off * nrhs + (int64_t)c * 2 * m;
+      double x = lane < jb ? xl[j0 + lane] : 0.0;
+      for (int j = 0; j < jb; ++j) {
+         const double y = __shfl(x, j);
+         const double l = (lane > j && lane < jb) ? L[j0 + lane + (int64_t)(j0 + j) * m] : 0.0;
+         if (lane > j && lane < jb) x = fnma(l, y, x);
+      }
+      if (lane < jb) xl[j0 + lane] = x;
+   }
+}
+
+// Forward, rows below block j0: x_r -= L[r, j] y_j, j in the block, in order.
+static __global__ __launch_bounds__(kThreads) void k_isv_fupd(int node, int j0, const IndefDesc* __restrict__ nodes,
+                                                       const double* __restrict__ lstore,
+                                                       double* __restrict__ xw, int nrhs) {
+#pragma clang fp contract(off)
+   __shared__ double ys[64];
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, jb = min(64, nd.nelim - j0);
+   const double* L = lstore + nd.lf_off;
+   const int r = j0 + jb + blockIdx.x * kThreads + threadIdx.x;
+   for (int c = 0; c < nrhs; ++c) {
+      double* xl = xw + nd.x_off * nrhs + (int64_t)c * 2 * m;
+      __syncthreads();
+      if (threadIdx.x < jb) ys[threadIdx.x] = xl[j0 + threadIdx.x];
+      __syncthreads();
+      if (r >= m) continue;
+      double x = xl[r];
+      for (int j = 0; j < jb; ++j) x = fnma(L[r + (int64_t)(j0 + j) * m], ys[j], x);
+      xl[r] = x;
+   }
+}
+
+// Backward, block j0: partial sums t_j = sum L[r, j] x_r over rows r below
+// the block, 256 rows per workgroup; part[(chunk * nrhs + c) * 64 + j].
+static __global__ __launch_bounds__(kThreads) void k_isv_bdot(int node, int j0, const IndefDesc* __restrict__ nodes,
+                                                       const double* __restrict__ lstore,
+                                                       const double* __restrict__ xw, double* __restrict__ part,
+                                                       int nrhs) {
+   __shared__ double ws[4][64];
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, jb = min(64, nd.nelim - j0);
+   const double* L = lstore + nd.lf_off;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int r = j0 + jb + blockIdx.x * kThreads + threadIdx.x;
+   const bool ok = r < m;
+   double lr[64];
+#pragma unroll
+   for (int j = 0; j < 64; ++j) lr[j] = (ok && j < jb) ? L[r + (int64_t)(j0 + j) * m] : 0.0;
+   for (int c = 0; c < nrhs; ++c) {
+      const double* xl = xw + nd.x_off * nrhs + (int64_t)c * 2 * m;
+      const double xr = ok ? xl[r] : 0.0;
+      double v[64];
+#pragma unroll
+      for (int j = 0; j < 64; ++j) v[j] = lr[j] * xr;
+      // transpose-reduce over the wave: lane l ends with column l's sum
+#pragma unroll
+      for (int o = 32, h = 32; o > 0; o >>= 1, h >>= 1) {
+         const bool up = lane & o;
+#pragma unroll
+         for (int i = 0; i < h; ++i) {
+            const double send = up ? v[i] : v[i + h];
+            const double recv = __shfl_xor(send, o);
+            v[i] = (up ? v[i + h] : v[i]) + recv;
+         }
+      }
+      __syncthreads();
+      ws[wave][lane] = v[0];
+      __syncthreads();
+      if (threadIdx.x < 64)
+         part[((int64_t)blockIdx.x * nrhs + c) * 64 + threadIdx.x] =
+             ((ws[0][threadIdx.x] + ws[1][threadIdx.x]) + ws[2][threadIdx.x]) + ws[3][threadIdx.x];
+   }
+}
+
+// Backward, triangle of block j0 (one workgroup; one wave per right-hand
+// side): x_j -= t_j + sum over block rows r > j of L[r, j] x_r, descending j.
+static __global__ __launch_bounds__(kThreads) void k_isv_bdiag(int node, int j0, int nchunks,
+                                                        const IndefDesc* __restrict__ nodes,
+                                                        const double* __restrict__ lstore,
+                                                        double* __restrict__ xw, const double* __restrict__ part,
+                                                        int nrhs) {
+   __shared__ double Lb[64][65];
+   const IndefDesc nd = nodes[node];
+   const int m = nd.m, jb = min(64, nd.nelim - j0);
+   const double* L = lstore + nd.lf_off;
+   for (int e = threadIdx.x; e < 64 * 64; e += kThreads) {
+      const int i = e & 63, j = e >> 6;
+      Lb[i][j] = (i < jb && j < jb) ? L[j0 + i + (int64_t)(j0 + j) * m] : 0.0;
+   }
+   __syncthreads();
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   for (int c = wave; c < nrhs; c += kThreads / 64) {
+      double* xl = xw + nd.x_off * nrhs + (int64_t)c * 2 * m;
+      double t = 0.0;
+      for (int k = 0; k < nchunks; ++k) t += part[((int64_t)k * nrhs + c) * 64 + lane];
+      double x = lane < jb ? xl[j0 + lane] - t : 0.0;
+      double acc = 0.0;  // sum over finished rows r > lane of L[r, lane] x_r
+      for (int j = jb - 1; j >= 0; --j) {
+         if (lane == j) x = x - acc;
+         const double xj = __shfl(x, j);
+         if (lane < j) acc = __builtin_fma(Lb[j][lane], xj, acc);
+      }
+      if (lane < jb) xl[j0 + lane] = x;
+   }
+}
+
+// Eliminated variables of a blocked front into the solution.
+static __global__ __launch_bounds__(kThreads) void k_isv_out(int node, const IndefDesc* __restrict__ nodes,
+                                                      const int* __restrict__ vars, const double* __restrict__ xw,
+                                                      double* __restrict__ sol, int ld, int nrhs) {
+   const IndefDesc nd = nodes[node];
+   const int* var = vars + nd.var_off;
+   for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < (int64_t)nd.nelim * nrhs;
+        e += (int64_t)gridDim.x * kThreads) {
+      const int r = (int)(e % nd.nelim), c = (int)(e / nd.nelim);
+      sol[var[r] - 1 + (int64_t)c * ld] = xw[nd.x_off * nrhs + (int64_t)c * 2 * nd.m + r];
    }
 }
 
