@@ -231,36 +231,55 @@ def bench_small_laplacian(S, M, oracle_mod):
             "cpu_kind": "port (oracle, all host threads)"}
 
 
-def bench_kkt(S, M, oracle_mod, side=12, mrows=432):
+def bench_kkt(S, M, oracle_mod, side=48, mrows=27648, sample_side=16):
     """KKT [H A^T; A 0], H the 7-point Laplacian on side^3, A mrows x side^3
-    with 3 entries per row: APP LDL^T in FP64 (u = 0.01, small = 1e-20)."""
+    with 3 entries per row at random columns: APP LDL^T in FP64 (u = 0.01,
+    small = 1e-20). Inertia must be (side^3, mrows, 0). The oracle (one
+    thread) and the device are also timed on a sample_side^3 case with
+    mrows = sample_side^3 / 4, where the oracle's delay counts are compared."""
+    def run(A, reps):
+        opts = S.default_options()
+        akeep, inf, _ = S.spldlt_analyse(A.n, A.ptr, A.row, A.val, options=opts)
+        S.set_ext_options(precision=S.PREC_FP64, use_graph=1, record_timing=0)
+        fk, fms = None, []
+        for it in range(reps):
+            fk, finf = S.spldlt_factorize(False, A.val, akeep, fk, options=opts)
+            if finf.flag < 0:
+                raise RuntimeError(f"indefinite factorization failed: {finf.flag}")
+            if it > 0 or reps == 1:
+                fms.append(finf.tail().factor_ms)
+        return akeep, inf, fk, finf, float(np.median(fms)), opts
+
     A = M.kkt(side, mrows, seed=0)
-    opts = S.default_options()
-    akeep, inf, _ = S.spldlt_analyse(A.n, A.ptr, A.row, A.val, options=opts)
+    akeep, ainf, fk, finf, fms, opts = run(A, 2)
     b = M.random_rhs(A.n, 1, seed=0)
-    S.set_ext_options(precision=S.PREC_FP64, use_graph=1, record_timing=0)
-    fk = None
-    fms = []
-    for it in range(3):
-        fk, finf = S.spldlt_factorize(False, A.val, akeep, fk, options=opts)
-        if finf.flag < 0:
-            raise RuntimeError(f"indefinite factorization failed: {finf.flag}")
-        if it > 0:
-            fms.append(finf.tail().factor_ms)
     x = b.copy(order="F")
     sinf = S.spldlt_solve(x, akeep, fk, options=opts)
     err = M.backward_error(A, x, b)
-    st = S.fkeep_indef_stats(fk)
-    sym = S.akeep_symbolic(akeep)
+    nh = side ** 3
+    out = {"n": A.n, "nh": nh, "m": mrows, "flops_symbolic": float(ainf.num_flops),
+           "flops_executed": float(finf.num_flops), "factor_ms": round(fms, 3),
+           "factor_gflops": round(float(finf.num_flops) / (fms * 1e6), 2),
+           "solve_ms": round(sinf.tail().solve_ms, 3), "bwd_err": err,
+           "num_neg": int(finf.num_neg), "inertia_ok": int(finf.num_neg) == mrows and finf.matrix_rank == A.n,
+           "num_delay": int(finf.num_delay), "num_two": int(finf.num_two), "maxfront": int(finf.maxfront)}
+    S.spldlt_free_fkeep(fk)
+    # the bounded sample: device and oracle on the same smaller case
+    ms = sample_side ** 3 // 4
+    A2 = M.kkt(sample_side, ms, seed=0)
+    akeep2, _, fk2, finf2, fms2, opts2 = run(A2, 2)
+    st = S.fkeep_indef_stats(fk2)
+    sym = S.akeep_symbolic(akeep2)
     t0 = time.perf_counter()
-    ref = oracle_mod.IndefFactor(sym, A.val, nb=64, u=opts.u, small=opts.small)
+    ref = oracle_mod.IndefFactor(sym, A2.val, nb=64, u=opts2.u, small=opts2.small)
     tc = time.perf_counter() - t0
-    return {"n": A.n, "nh": side ** 3, "m": mrows, "flops_executed": float(finf.num_flops),
-            "factor_ms": round(float(np.median(fms)), 3), "solve_ms": round(sinf.tail().solve_ms, 3),
-            "bwd_err": err, "num_neg": int(finf.num_neg), "num_delay": int(finf.num_delay),
-            "num_two": int(finf.num_two), "maxfront": int(finf.maxfront),
-            "stats_equal_oracle": all(st[k] == ref.stats[k] for k in st),
-            "cpu_factor_ms": round(tc * 1e3, 3), "cpu_kind": "port (oracle, one thread)"}
+    out["sample"] = {"side": sample_side, "m": ms, "n": A2.n, "flops_executed": float(finf2.num_flops),
+                     "device_factor_ms": round(fms2, 3), "cpu_factor_ms": round(tc * 1e3, 1),
+                     "cpu_gflops": round(float(finf2.num_flops) / (tc * 1e9), 3),
+                     "cpu_kind": "port (oracle, one thread)",
+                     "stats_equal_oracle": all(st[k] == ref.stats[k] for k in st)}
+    S.spldlt_free_fkeep(fk2)
+    return out
 
 
 def measured_traffic():
@@ -378,7 +397,7 @@ def main():
         secondary = {}
         for key, fn in (("laplacian_2d_100", lambda: bench_small_laplacian(S, M, oracle_mod)),
                         ("batched_small_fronts", lambda: bench_batched_fronts(S, M)),
-                        ("kkt_app_12cubed_432", lambda: bench_kkt(S, M, oracle_mod))):
+                        ("kkt_app_48cubed_27648", lambda: bench_kkt(S, M, oracle_mod))):
             try:
                 secondary[key] = fn()
             except Exception as e:  # a secondary workload never blocks the headline
@@ -413,7 +432,7 @@ def main():
                    "maxfront": int(inf.maxfront), "flops": flops},
         "mixed": {"factor_s": round(fmx_ms / 1e3, 5), "solve_ir_s": round(smx_ms / 1e3, 5),
                   "factor_gflops": round(flops / (fmx_ms * 1e6), 2),
-                  "ir_iters": int(rmx[-1]["refine_iters"]), "bwd_err": err64 if False else errmx},
+                  "ir_iters": int(rmx[-1]["refine_iters"]), "bwd_err": errmx},
         "fp64": {"factor_s": round(f64_ms / 1e3, 5), "solve_s": round(s64_ms / 1e3, 5),
                  "factor_gflops": round(flops / (f64_ms * 1e6), 2), "bwd_err": err64,
                  "frac_fp64_peak": round(flops / (f64_ms * 1e6) / 1e3 / PEAK["fp64"], 4)},

@@ -132,6 +132,23 @@ def test_panels_identical_on_tree(S, gpu):
             assert np.array_equal(np.tril(a["L"], -1), np.tril(b["L"], -1)), (name, i)
 
 
+def test_kkt_16_large_panels_vs_oracle(S, O, gpu):
+    """16^3 KKT with 1024 constraint rows: fronts up to 1813 rows with 1105
+    delayed pivots in all, through the multi-workgroup panel, the parallel
+    second pass and the blocked solves (auto mode). Inertia is the closed
+    form (4096, 1024, 0); delay statistics equal the oracle's."""
+    A = M.kkt(16, 1024, seed=0)
+    akeep, opts, sym = _analyse(S, A)
+    fkeep, inf = S.spldlt_factorize(False, A.val, akeep, options=opts)
+    assert inf.flag >= 0, inf.flag
+    assert inf.num_neg == 1024 and inf.matrix_rank == A.n
+    ref = O.IndefFactor(sym, A.val, nb=64, u=opts.u, small=opts.small)
+    st = S.fkeep_indef_stats(fkeep)
+    for k in ("num_delay", "num_neg", "num_two", "num_zero", "maxfront", "not_first_pass", "not_second_pass"):
+        assert st[k] == ref.stats[k], (k, st[k], ref.stats[k])
+    assert _solve_check(S, A, akeep, fkeep, nrhs=3) <= BWD_TOL
+
+
 def test_partial_solves_compose(S, gpu):
     """job 1 (L), 2 (D), 3 (L^T) and 4 (D then L^T) compose to job 0
     (spldlt_solve job semantics, src/spldlt_factorize_mod.F90:1169-1286)."""
