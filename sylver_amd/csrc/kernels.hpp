@@ -1252,11 +1252,24 @@ static __global__ __launch_bounds__(kThreads) void k_residual(int n, const int64
                                                        double* __restrict__ r, int nrhs) {
    const int i = blockIdx.x * kThreads + threadIdx.x;
    if (i >= n) return;
-   for (int c = 0; c < nrhs; ++c) {
-      double s = b[i + (int64_t)c * n];
-      for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p)
-         s -= val[p] * x[col[p] + (int64_t)c * n];
-      r[i + (int64_t)c * n] = s;
+   const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
+   // 16 right-hand sides per pass: the row's entries are read once and the
+   // 16 gathers of x per entry are independent loads (columns past nrhs
+   // re-read the last one and are not stored)
+   for (int c0 = 0; c0 < nrhs; c0 += 16) {
+      const int nc = min(16, nrhs - c0);
+      double s[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) s[c] = b[i + (int64_t)(c0 + min(c, nc - 1)) * n];
+      for (int64_t p = p0; p < p1; ++p) {
+         const double v = val[p];
+         const double* xj = x + col[p] + (int64_t)c0 * n;
+#pragma unroll
+         for (int c = 0; c < 16; ++c) s[c] -= v * xj[(int64_t)min(c, nc - 1) * n];
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c)
+         if (c < nc) r[i + (int64_t)(c0 + c) * n] = s[c];
    }
 }
 
@@ -1286,10 +1299,21 @@ static __global__ __launch_bounds__(kThreads) void k_spmv(int n, const int64_t* 
                                                    double* __restrict__ y, int nrhs) {
    const int i = blockIdx.x * kThreads + threadIdx.x;
    if (i >= n) return;
-   for (int c = 0; c < nrhs; ++c) {
-      double s = 0.0;
-      for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) s += val[p] * x[col[p] + (int64_t)c * n];
-      y[i + (int64_t)c * n] = s;
+   const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
+   for (int c0 = 0; c0 < nrhs; c0 += 16) {  // as k_residual
+      const int nc = min(16, nrhs - c0);
+      double s[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) s[c] = 0.0;
+      for (int64_t p = p0; p < p1; ++p) {
+         const double v = val[p];
+         const double* xj = x + col[p] + (int64_t)c0 * n;
+#pragma unroll
+         for (int c = 0; c < 16; ++c) s[c] += v * xj[(int64_t)min(c, nc - 1) * n];
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c)
+         if (c < nc) y[i + (int64_t)(c0 + c) * n] = s[c];
    }
 }
 
