@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -282,6 +283,89 @@ def bench_kkt(S, M, oracle_mod, side=48, mrows=27648, sample_side=16):
     return out
 
 
+def bench_costmodel(S, M, n_train=40, n_held=10, full=False, seed=5):
+    """Learned cost model (per-front kernel variant, one-workgroup vs tiled):
+    a seeded stream of SPD grid matrices (2D 5-point, 3D 7-point with aspect
+    ratios up to 4:1), the autotuner's exhaustive per-shape timings (FP64 and
+    FP32) as targets, the device-trained MLP's choices on the held-out
+    matrices: per-front regret against the autotuner, and end-to-end factor
+    time with the default rule, the autotuner's and the model's variants.
+    full=True uses the configured sizes (2D sides 200-1000, 3D 30-90) and
+    400 + 100 matrices; the default is a reduced stream."""
+    from sylver_amd import costmodel as CM
+    rng = np.random.default_rng(seed)
+    s2, s3 = ((200, 1000), (30, 90)) if full else ((150, 400), (20, 45))
+
+    def gen():
+        if rng.uniform() < 0.4:
+            nx = int(rng.integers(s2[0], s2[1] + 1))
+            ny = int(min(s2[1], max(s2[0], nx * rng.uniform(0.5, 1.0))))
+            return f"2d {nx}x{ny}", M.laplacian_2d(nx, ny)
+        nx = int(rng.integers(s3[0], s3[1] + 1))
+        ar = rng.uniform(1.0, 4.0)
+        ny = max(4, int(nx / math.sqrt(ar)))
+        nz = max(4, int(nx / ar))
+        return f"3d {nx}x{ny}x{nz}", M.laplacian_3d(nx, ny, nz)
+
+    t0 = time.perf_counter()
+    tuners = {S.PREC_FP64: CM.Autotuner(S, S.PREC_FP64, reps=2), S.PREC_FP32: CM.Autotuner(S, S.PREC_FP32, reps=2)}
+    xs, ts, held = [], [], []
+    for k in range(n_train + n_held):
+        name, A = gen()
+        akeep, _, _ = S.spldlt_analyse(A.n, A.ptr, A.row, A.val, options=S.default_options())
+        m, n, nc = CM.tree_fronts(S, akeep)
+        band = CM.in_band(m)
+        if k < n_train:
+            for prec, tier in ((S.PREC_FP64, 0), (S.PREC_FP32, 1)):
+                xs.append(CM.features(m[band], n[band], nc[band], tier))
+                ts.append(tuners[prec].measure(m[band], n[band]))
+            S.spldlt_free_akeep(akeep)
+        else:
+            held.append((name, A, akeep, m, n, nc))
+    t_data = time.perf_counter() - t0
+    x, t = np.concatenate(xs), np.concatenate(ts)
+    model = CM.CostModel(S, seed=seed).fit(x, t, epochs=300)
+    out = {"train_matrices": n_train, "held_out_matrices": n_held, "records": int(len(x)),
+           "shape_buckets_timed": len(tuners[S.PREC_FP64].cache) + len(tuners[S.PREC_FP32].cache),
+           "final_train_loss_log2ms": round(float(model.loss[-1]), 4), "data_and_tuning_s": round(t_data, 1),
+           "sizes": "configured (2D 200-1000, 3D 30-90)" if full else "reduced (2D 150-400, 3D 20-45)"}
+    opts = S.default_options()
+    for prec, tier, label in ((S.PREC_FP64, 0, "fp64"), (S.PREC_FP32, 1, "fp32")):
+        reg_m = reg_d = best = 0.0
+        e2e = {"default": 0.0, "autotuner": 0.0, "model": 0.0}
+        S.set_ext_options(precision=prec, use_graph=1, record_timing=0)
+        for name, A, akeep, m, n, nc in held:
+            band = CM.in_band(m)
+            tm = tuners[prec].measure(m, n)
+            pick = np.zeros(len(m), bool)
+            pick[band] = model.pick_one_wg(CM.features(m[band], n[band], nc[band], tier))
+            tb = tm[band]
+            best += tb.min(axis=1).sum()
+            reg_m += (np.where(pick[band], tb[:, 0], tb[:, 1]) - tb.min(axis=1)).sum()
+            reg_d += (np.where(m[band] <= 256, tb[:, 0], tb[:, 1]) - tb.min(axis=1)).sum()
+            variants = {"default": None, "autotuner": CM.choose_variants(m, tm[:, 0] < tm[:, 1]),
+                        "model": CM.choose_variants(m, pick)}
+            for key, v in variants.items():
+                S.set_front_variants(akeep, v)
+                fk, fms = None, []
+                for it in range(3):
+                    fk, finf = S.spldlt_factorize(True, A.val, akeep, fk, options=opts)
+                    if finf.flag != 0:
+                        raise RuntimeError(f"factorization failed: {finf.flag}")
+                    if it > 0:
+                        fms.append(finf.tail().factor_ms)
+                S.spldlt_free_fkeep(fk)
+                e2e[key] += float(np.median(fms))
+            S.set_front_variants(akeep, None)
+        out[label] = {"front_time_best_ms": round(best, 3), "regret_model_ms": round(reg_m, 3),
+                      "regret_default_rule_ms": round(reg_d, 3),
+                      "end_to_end_factor_ms": {k: round(v, 3) for k, v in e2e.items()}}
+    S.set_ext_options(precision=S.PREC_FP64)
+    for _, _, akeep, _, _, _ in held:
+        S.spldlt_free_akeep(akeep)
+    return out
+
+
 def measured_traffic():
     """HBM bytes per launch of the Schur-update kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated against a
@@ -313,6 +397,8 @@ def main():
                     help="only run a plain device copy of known size (scales PMC byte counters)")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the per-launch timed (eager, one stream) factorization of the FP32 path")
+    ap.add_argument("--costmodel-full", action="store_true",
+                    help="learned cost model on the configured stream (400 + 100 matrices, full sizes)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the secondary workloads (2D Laplacian, batched fronts, KKT)")
     args = ap.parse_args()
@@ -397,7 +483,8 @@ def main():
         secondary = {}
         for key, fn in (("laplacian_2d_100", lambda: bench_small_laplacian(S, M, oracle_mod)),
                         ("batched_small_fronts", lambda: bench_batched_fronts(S, M)),
-                        ("kkt_app_48cubed_27648", lambda: bench_kkt(S, M, oracle_mod))):
+                        ("kkt_app_48cubed_27648", lambda: bench_kkt(S, M, oracle_mod)),
+                        ("learned_cost_model", lambda: bench_costmodel(S, M, full=args.costmodel_full))):
             try:
                 secondary[key] = fn()
             except Exception as e:  # a secondary workload never blocks the headline

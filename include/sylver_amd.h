@@ -254,6 +254,36 @@ int sylver_amd_batched_front_potrf(int nfronts, int const *nrow_host, int const 
 int sylver_amd_batched_front_fill(int nfronts, int const *nrow_host, long const *offset_host,
                                   void *data, int dtype, unsigned long long seed);
 
+/* ---- Autotuner and learned cost model (no counterpart in the reference:
+ * its scheduler decides by static rules). ---- */
+
+/* Per-front kernel variant of later posdef factorizations with this akeep:
+ * variant[i] = 1 one-workgroup kernel, 0 tiled large-front path, -1 the
+ * default rule (nrow <= small_front_max); variant == NULL clears. nnodes
+ * must equal the analysis's node count. Returns 0 or
+ * SYLVER_ERROR_CALL_SEQUENCE. */
+int sylver_amd_set_front_variants(void *akeep, signed char const *variant, int nnodes);
+
+/* Device time (ms, median of reps after one warm-up) of factoring k
+ * independent synthetic SPD fronts (m[i] rows, n[i] pivots, values from the
+ * batched front generator) with the given variants (NULL = default rule) in
+ * precision SYLVER_AMD_PREC_FP64 or _FP32: the autotuner's measurement. */
+int sylver_amd_time_fronts(int k, int const *m, int const *n, signed char const *variant, int precision,
+                           int reps, double *ms);
+
+/* The cost model's 8 -> 64 -> 64 -> 2 MLP (ReLU, FP32, 4866 parameters:
+ * W1[8][64], b1[64], W2[64][64], b2[64], W3[64][2], b3[2]); x is n x 8,
+ * y and out n x 2, row-major, host memory. init: Glorot-uniform weights
+ * from a seeded stream; grad: gradient of sum (y_pred - y)^2 / (2 b) at
+ * params over the batch (device forward/backward kernels); train: Adam,
+ * seeded shuffles, loss_hist[epoch] (epochs <= 1024); predict: forward. */
+int sylver_amd_mlp_init(float *params, unsigned long long seed);
+int sylver_amd_mlp_grad(float const *params, float const *x, float const *y, int b, float *grads,
+                        float *loss);
+int sylver_amd_mlp_train(float *params, float const *x, float const *y, int n, int epochs, int batch,
+                         float lr, unsigned long long seed, float *loss_hist);
+int sylver_amd_mlp_predict(float const *params, float const *x, int n, float *out);
+
 /* Number of HIP devices visible (0 when none); never fails. */
 int sylver_amd_device_count(void);
 

@@ -151,7 +151,8 @@ EXPORTED = [
     "sylver_amd_fkeep_indef_node", "sylver_amd_fkeep_indef_stats",
     "sylver_amd_batched_front_potrf", "sylver_amd_batched_front_fill", "sylver_amd_device_count", "sylver_amd_dev_alloc",
     "sylver_amd_dev_free", "sylver_amd_memcpy_h2d", "sylver_amd_memcpy_d2h",
-    "sylver_amd_last_batched_ms",
+    "sylver_amd_last_batched_ms", "sylver_amd_set_front_variants", "sylver_amd_time_fronts",
+    "sylver_amd_mlp_init", "sylver_amd_mlp_grad", "sylver_amd_mlp_train", "sylver_amd_mlp_predict",
 ]
 
 
@@ -196,6 +197,13 @@ def lib():
     L.sylver_amd_memcpy_d2h.argtypes = [vp, vp, C.c_size_t]
     L.sylver_amd_last_batched_ms.argtypes = []
     L.sylver_amd_last_batched_ms.restype = C.c_double
+    fp = C.POINTER(C.c_float)
+    L.sylver_amd_set_front_variants.argtypes = [vp, C.POINTER(C.c_byte), C.c_int]
+    L.sylver_amd_time_fronts.argtypes = [C.c_int, ip, ip, C.POINTER(C.c_byte), C.c_int, C.c_int, dp]
+    L.sylver_amd_mlp_init.argtypes = [fp, C.c_ulonglong]
+    L.sylver_amd_mlp_grad.argtypes = [fp, fp, fp, C.c_int, fp, fp]
+    L.sylver_amd_mlp_train.argtypes = [fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp]
+    L.sylver_amd_mlp_predict.argtypes = [fp, fp, C.c_int, fp]
     _lib = L
     return L
 
@@ -485,3 +493,78 @@ def batched_front_potrf(fronts_flat: "DeviceArray", nrow, ncol, offsets, dtype):
     out = info.numpy()
     info.free()
     return rc, out, float(lib().sylver_amd_last_batched_ms())
+
+
+# ---- autotuner and learned cost model (see sylver_amd/costmodel.py) ----
+MLP_PARAMS = 8 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2
+
+
+def set_front_variants(akeep, variant):
+    """Per-front kernel variant of later posdef factorizations (1 one-workgroup
+    kernel, 0 tiled path, -1 default rule); None clears."""
+    if variant is None:
+        rc = lib().sylver_amd_set_front_variants(akeep, None, 0)
+    else:
+        v = np.ascontiguousarray(variant, np.int8)
+        rc = lib().sylver_amd_set_front_variants(akeep, _p(v, C.c_byte), len(v))
+    if rc != 0:
+        raise RuntimeError(f"sylver_amd_set_front_variants failed: {rc}")
+
+
+def time_fronts(m, n, variant=None, precision=PREC_FP64, reps=3):
+    """Device ms of factoring the synthetic fronts (m[i], n[i]) together."""
+    m = np.ascontiguousarray(m, np.int32)
+    n = np.ascontiguousarray(n, np.int32)
+    vp = None
+    if variant is not None:
+        v = np.ascontiguousarray(variant, np.int8)
+        vp = _p(v, C.c_byte)
+    ms = C.c_double(0.0)
+    rc = lib().sylver_amd_time_fronts(len(m), _p(m, C.c_int), _p(n, C.c_int), vp, int(precision), int(reps),
+                                      C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"sylver_amd_time_fronts failed: {rc}")
+    return ms.value
+
+
+def mlp_init(seed):
+    p = np.zeros(MLP_PARAMS, np.float32)
+    lib().sylver_amd_mlp_init(_p(p, C.c_float), int(seed))
+    return p
+
+
+def mlp_grad(params, x, y):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = np.zeros(1, np.float32)
+    rc = lib().sylver_amd_mlp_grad(_p(params, C.c_float), _p(x, C.c_float), _p(y, C.c_float), len(x),
+                                   _p(g, C.c_float), _p(loss, C.c_float))
+    if rc != 0:
+        raise RuntimeError(f"sylver_amd_mlp_grad failed: {rc}")
+    return g, float(loss[0])
+
+
+def mlp_train(params, x, y, epochs, batch, lr, seed):
+    """Returns (trained params, per-epoch loss)."""
+    p = np.array(params, np.float32, copy=True)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    hist = np.zeros(epochs, np.float32)
+    rc = lib().sylver_amd_mlp_train(_p(p, C.c_float), _p(x, C.c_float), _p(y, C.c_float), len(x), int(epochs),
+                                    int(batch), float(lr), int(seed), _p(hist, C.c_float))
+    if rc != 0:
+        raise RuntimeError(f"sylver_amd_mlp_train failed: {rc}")
+    return p, hist
+
+
+def mlp_predict(params, x):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros((len(x), 2), np.float32)
+    rc = lib().sylver_amd_mlp_predict(_p(params, C.c_float), _p(x, C.c_float), len(x), _p(out, C.c_float))
+    if rc != 0:
+        raise RuntimeError(f"sylver_amd_mlp_predict failed: {rc}")
+    return out
+

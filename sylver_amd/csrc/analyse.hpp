@@ -55,6 +55,11 @@ struct Analysis {
    // Symbolic tree (src/SymbolicTree.cxx:7-102); fronts[nnodes] is the root.
    std::vector<SymbolicFront> fronts;
    int64_t nfactor = 0;     // sum nrow*ncol
+   // Per-front kernel variant of the posdef factorization chosen by the cost
+   // model or the autotuner: -1 the default rule, 0 the tiled large-front
+   // path, 1 the one-workgroup kernel; variant_gen changes with every update.
+   std::vector<signed char> variant;
+   int variant_gen = 0;
    // Statistics reported through sylver_inform_t
    int64_t num_factor = 0;  // entries in L (lower)
    int64_t num_flops = 0;   // sum over nodes of sum_{r=m-n+1}^{m} r^2

@@ -852,10 +852,12 @@ __device__ inline uint64_t mix64(uint64_t z) {
 }
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_front_fill(const int* __restrict__ nrow,
+                                                         const int* __restrict__ ldv,
                                                          const int64_t* __restrict__ off, T* __restrict__ data,
                                                          uint64_t seed) {
    const int f = blockIdx.y;
    const int m = nrow[f];
+   const int64_t ld = ldv ? ldv[f] : m;  // ld >= m; rows m..ld-1 untouched
    T* F = data + off[f];
    const int64_t mm = (int64_t)m * m;
    for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < mm; e += (int64_t)gridDim.x * kThreads) {
@@ -864,7 +866,7 @@ __global__ __launch_bounds__(kThreads) void k_front_fill(const int* __restrict__
       const uint64_t h = mix64(seed ^ mix64((uint64_t)f * 0x100000001b3ull ^ (hi << 32 | lo)));
       double v = 2.0 * ((double)(h >> 11) * 0x1.0p-53) - 1.0;
       if (i == j) v += (double)m;
-      F[e] = (T)v;
+      F[i + j * ld] = (T)v;
    }
 }
 

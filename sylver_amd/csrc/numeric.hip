@@ -65,7 +65,14 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    P->nnodes = a.nnodes;
    P->small_max = small_max;
    P->wide_ncol = wide_ncol;
+   P->variant_gen = a.variant_gen;
    const int nn = a.nnodes;
+   // one-workgroup kernel or tiled path for front i
+   const bool have_variant = (int)a.variant.size() == nn;
+   auto one_wg = [&](int i) {
+      if (have_variant && a.variant[i] >= 0) return a.variant[i] == 1;
+      return a.fronts[i].nrow <= small_max;
+   };
    P->hnodes.resize(nn);
    std::vector<int> height(nn + 1, 0);
    for (int i = 0; i < nn; ++i) {
@@ -183,7 +190,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       for (int i : lv) {
          const NodeDesc& d = P->hnodes[i];
          if (d.n == 0) continue;
-         if (d.m <= small_max)
+         if (one_wg(i))
             P->nlistv.push_back(i);
          else
             large.push_back(i);
@@ -329,7 +336,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       for (int L = 0; L < nlev; ++L)
          for (int i : P->levels[L]) {
             const NodeDesc& d = P->hnodes[i];
-            if (d.n == 0 || d.m > small_max) continue;
+            if (d.n == 0 || !one_wg(i)) continue;
             for (int k0 = 0; k0 < d.n; k0 += kNB)
                P->diag.push_back(DiagTask{i, k0, std::min(kNB, d.n - k0), 0});
          }
@@ -584,7 +591,7 @@ inline unsigned grid1(int64_t n) { return (unsigned)((n + kThreads - 1) / kThrea
 
 template <typename T>
 void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
-                        std::vector<hipEvent_t>* events = nullptr) {
+                        std::vector<hipEvent_t>* events = nullptr, bool compute_only = false) {
    const bool bf16_wide = std::is_same<T, float>::value && F->precision == SYLVER_AMD_PREC_MIXED;
    T* arena = static_cast<T*>(F->arena);
    T* dinv = static_cast<T*>(F->dinv);
@@ -598,7 +605,9 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
       s = (L.stream && !events) ? F->stream2 : s_main;
       if (L.wait_ev >= 0) HIPCHECK(hipStreamWaitEvent(s, F->xev[L.wait_ev], 0));
       if (events) HIPCHECK(hipEventRecord((*events)[2 * li], s));
-      switch (L.kind) {
+      // compute_only: fronts already hold their values (front timing)
+      const int kind = (compute_only && (L.kind == L_ZERO || L.kind == L_INIT || L.kind == L_EA)) ? L_SYNC : L.kind;
+      switch (kind) {
          case L_ZERO:
             k_zero<T><<<g, b, 0, s>>>(P->d_zero + L.off, P->d_nodes, arena);
             break;
@@ -721,7 +730,8 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
    try {
       int small_max = xo.small_front_max > 0 ? xo.small_front_max : 256;
       int wide_ncol = xo.bf16_min_ncol > 0 ? xo.bf16_min_ncol : 1024;
-      if (*plan && ((*plan)->small_max != small_max || (*plan)->wide_ncol != wide_ncol)) {
+      if (*plan && ((*plan)->small_max != small_max || (*plan)->wide_ncol != wide_ncol ||
+                    (*plan)->variant_gen != a.variant_gen)) {
          if (*fact && (*fact)->plan == *plan) {
             destroy_factors(*fact);
             *fact = nullptr;
@@ -1245,15 +1255,115 @@ int batched_front_fill(int nfronts, const int* nrow, const int64_t* off, void* d
       HIPCHECK(hipMemcpy(doff, off, nfronts * sizeof(int64_t), hipMemcpyHostToDevice));
       dim3 g(8, nfronts);
       if (dtype == 0)
-         k_front_fill<double><<<g, kThreads>>>(dn, doff, static_cast<double*>(data), seed);
+         k_front_fill<double><<<g, kThreads>>>(dn, nullptr, doff, static_cast<double*>(data), seed);
       else
-         k_front_fill<float><<<g, kThreads>>>(dn, doff, static_cast<float*>(data), seed);
+         k_front_fill<float><<<g, kThreads>>>(dn, nullptr, doff, static_cast<float*>(data), seed);
       HIPCHECK(hipGetLastError());
       HIPCHECK(hipDeviceSynchronize());
       dfree(dn);
       dfree(doff);
       return 0;
    } catch (const HipError&) {
+      return SYLVER_ERROR_CUDA_UNKNOWN;
+   }
+}
+
+// Device time of the posdef factorization of K independent synthetic SPD
+// fronts (m_i rows, n_i pivots, values from k_front_fill) with the given
+// per-front variants (1 one-workgroup kernel, 0 tiled path, -1 default):
+// the plan's compute launches only, median of `reps` runs, values refilled
+// before each run. The autotuner's measurement.
+template <typename T>
+double time_fronts_t(DevicePlan* P, DeviceFactors* F, const std::vector<int>& nrow, const std::vector<int>& ld,
+                     const std::vector<int64_t>& off, int reps) {
+   const int K = (int)nrow.size();
+   int* dn = dalloc<int>(K);
+   int* dl = dalloc<int>(K);
+   int64_t* doff = dalloc<int64_t>(K);
+   HIPCHECK(hipMemcpy(dn, nrow.data(), K * sizeof(int), hipMemcpyHostToDevice));
+   HIPCHECK(hipMemcpy(dl, ld.data(), K * sizeof(int), hipMemcpyHostToDevice));
+   HIPCHECK(hipMemcpy(doff, off.data(), K * sizeof(int64_t), hipMemcpyHostToDevice));
+   hipEvent_t e0, e1;
+   HIPCHECK(hipEventCreate(&e0));
+   HIPCHECK(hipEventCreate(&e1));
+   std::vector<double> t;
+   for (int r = 0; r < reps + 1; ++r) {
+      k_front_fill<T><<<dim3(8, K), kThreads, 0, F->stream>>>(dn, dl, doff, static_cast<T*>(F->arena), 12345);
+      HIPCHECK(hipEventRecord(e0, F->stream));
+      run_factor_program<T>(P, F, F->stream, nullptr, true);
+      HIPCHECK(hipEventRecord(e1, F->stream));
+      HIPCHECK(hipStreamSynchronize(F->stream));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (r > 0) t.push_back(ms);  // the first run warms up
+   }
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   dfree(dn);
+   dfree(dl);
+   dfree(doff);
+   std::sort(t.begin(), t.end());
+   return t[t.size() / 2];
+}
+
+int time_fronts(int K, const int* m, const int* n, const signed char* variant, int precision, int reps,
+                double* ms) {
+   if (K <= 0 || !m || !n || !ms || reps < 1) return SYLVER_ERROR_CALL_SEQUENCE;
+   for (int i = 0; i < K; ++i)
+      if (n[i] < 1 || m[i] < n[i]) return SYLVER_ERROR_CALL_SEQUENCE;
+   if (device_count() <= 0) return SYLVER_ERROR_CUDA_UNKNOWN;
+   DevicePlan* P = nullptr;
+   DeviceFactors* F = nullptr;
+   try {
+      // a forest of K roots: rows 1..m_i of a matrix of order max m_i
+      Analysis a;
+      a.nnodes = K;
+      a.fronts.assign(K + 1, SymbolicFront{});
+      int64_t roff = 0;
+      for (int i = 0; i < K; ++i) {
+         a.n = std::max(a.n, m[i]);
+         SymbolicFront& f = a.fronts[i];
+         f.nrow = m[i];
+         f.ncol = n[i];
+         f.parent = K;
+         f.rlist_off = roff;
+         roff += m[i];
+         for (int r = 1; r <= m[i]; ++r) a.rlist.push_back(r);
+      }
+      if (variant) a.variant.assign(variant, variant + K);
+      P = build_plan(a, 256, 1 << 30);
+      const bool low = precision != SYLVER_AMD_PREC_FP64;
+      const size_t es = low ? sizeof(float) : sizeof(double);
+      F = new DeviceFactors();
+      F->plan = P;
+      F->precision = low ? SYLVER_AMD_PREC_FP32 : SYLVER_AMD_PREC_FP64;
+      F->posdef = true;
+      HIPCHECK(hipMalloc(&F->arena, std::max<int64_t>(P->arena_elems, 1) * es));
+      HIPCHECK(hipMalloc(&F->dinv, std::max<int64_t>(P->dinv_elems, 1) * es));
+      HIPCHECK(hipMalloc(&F->dinvT, std::max<int64_t>(P->dinv_elems, 1) * es));
+      F->d_info = dalloc<int>(K);
+      HIPCHECK(hipMemset(F->d_info, 0, K * sizeof(int)));
+      HIPCHECK(hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking));
+      HIPCHECK(hipStreamCreateWithFlags(&F->stream2, hipStreamNonBlocking));
+      F->xev.resize(P->nevents);
+      for (auto& e : F->xev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      std::vector<int> nrow(m, m + K), ld(K);
+      std::vector<int64_t> off(K);
+      for (int i = 0; i < K; ++i) {
+         ld[i] = P->hnodes[i].ld;
+         off[i] = P->hnodes[i].off;
+      }
+      *ms = low ? time_fronts_t<float>(P, F, nrow, ld, off, reps) : time_fronts_t<double>(P, F, nrow, ld, off, reps);
+      std::vector<int> info(K);
+      HIPCHECK(hipMemcpy(info.data(), F->d_info, K * sizeof(int), hipMemcpyDeviceToHost));
+      destroy_factors(F);
+      destroy_plan(P);
+      for (int v : info)
+         if (v != 0) return SYLVER_ERROR_NOT_POS_DEF;
+      return 0;
+   } catch (const HipError&) {
+      destroy_factors(F);
+      destroy_plan(P);
       return SYLVER_ERROR_CUDA_UNKNOWN;
    }
 }

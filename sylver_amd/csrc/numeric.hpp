@@ -86,6 +86,16 @@ double last_batched_ms();
 
 int batched_front_potrf(int nfronts, const int* nrow, const int* ncol, const int64_t* off,
                         void* data, int dtype, int* info, void* stream);
+int time_fronts(int K, const int* m, const int* n, const signed char* variant, int precision, int reps,
+                double* ms);
+// Cost model (costmodel.hip): an 8 -> 64 -> 64 -> 2 MLP in FP32.
+constexpr int kMlpIn = 8, kMlpHid = 64, kMlpOut = 2;
+constexpr int kMlpParams = kMlpIn * kMlpHid + kMlpHid + kMlpHid * kMlpHid + kMlpHid + kMlpHid * kMlpOut + kMlpOut;
+int mlp_init(float* params, unsigned long long seed);
+int mlp_grad(const float* params, const float* X, const float* Y, int B, float* grads, float* loss);
+int mlp_train(float* params, const float* X, const float* Y, int N, int epochs, int batch, float lr,
+              unsigned long long seed, float* loss_hist);
+int mlp_predict(const float* params, const float* X, int N, float* out);
 int batched_front_fill(int nfronts, const int* nrow, const int64_t* off, void* data, int dtype,
                        unsigned long long seed);
 

@@ -94,6 +94,7 @@ void solve_indef(const DevicePlan* P, DeviceFactors* F, int job, const double* b
                  hipStream_t s);
 
 struct DevicePlan {
+   int variant_gen = -1;  // Analysis::variant_gen the plan was built with
    int n = 0, nnodes = 0, nlevels = 0;
    int small_max = 256;
    int wide_ncol = 1024;  // fronts with more pivots take the wide-front update launches

@@ -409,6 +409,54 @@ int sylver_amd_fkeep_node_assembled(void const* fkeep, int node, double* front) 
    return factors_node_assembled(fk->f, node, front);
 }
 
+int sylver_amd_set_front_variants(void* akeep, signed char const* variant, int nnodes) {
+   Akeep* ak = as_akeep(akeep);
+   if (!ak) return SYLVER_ERROR_CALL_SEQUENCE;
+   if (variant && nnodes != ak->a.nnodes) return SYLVER_ERROR_CALL_SEQUENCE;
+   if (variant)
+      ak->a.variant.assign(variant, variant + nnodes);
+   else
+      ak->a.variant.clear();
+   ++ak->a.variant_gen;
+   return 0;
+}
+
+int sylver_amd_time_fronts(int k, int const* m, int const* n, signed char const* variant, int precision,
+                           int reps, double* ms) {
+   try {
+      return time_fronts(k, m, n, variant, precision, reps, ms);
+   } catch (...) {
+      return SYLVER_ERROR_UNKNOWN;
+   }
+}
+
+int sylver_amd_mlp_init(float* params, unsigned long long seed) { return mlp_init(params, seed); }
+
+int sylver_amd_mlp_grad(float const* params, float const* x, float const* y, int b, float* grads, float* loss) {
+   try {
+      return mlp_grad(params, x, y, b, grads, loss);
+   } catch (...) {
+      return SYLVER_ERROR_UNKNOWN;
+   }
+}
+
+int sylver_amd_mlp_train(float* params, float const* x, float const* y, int n, int epochs, int batch, float lr,
+                         unsigned long long seed, float* loss_hist) {
+   try {
+      return mlp_train(params, x, y, n, epochs, batch, lr, seed, loss_hist);
+   } catch (...) {
+      return SYLVER_ERROR_UNKNOWN;
+   }
+}
+
+int sylver_amd_mlp_predict(float const* params, float const* x, int n, float* out) {
+   try {
+      return mlp_predict(params, x, n, out);
+   } catch (...) {
+      return SYLVER_ERROR_UNKNOWN;
+   }
+}
+
 int sylver_amd_batched_front_fill(int nfronts, int const* nrow, long const* offset, void* data,
                                   int dtype, unsigned long long seed) {
    try {

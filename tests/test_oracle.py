@@ -334,3 +334,21 @@ def test_indef_tree_node_structure(S, O):
             pn = F.node(par[i])
             assert fails <= set(pn["rows"][:pn["n"]].tolist())
     assert sorted(seen) == list(range(1, A.n + 1))
+
+
+def test_mlp_oracle_gradient_finite_differences():
+    """The cost model's MLP oracle: analytic gradient equals central finite
+    differences (FP64) on a random batch."""
+    from oracle import mlp
+    rng = np.random.default_rng(3)
+    p = rng.normal(0, 0.3, mlp.NPARAMS)
+    x = rng.normal(0, 1, (13, 8))
+    y = rng.normal(0, 1, (13, 2))
+    g = mlp.grad(p, x, y)
+    idx = rng.choice(mlp.NPARAMS, 60, replace=False)
+    h = 1e-6
+    for k in idx:
+        e = np.zeros_like(p)
+        e[k] = h
+        fd = (mlp.loss(p + e, x, y) - mlp.loss(p - e, x, y)) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(g[k])), (k, fd, g[k])
