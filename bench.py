@@ -322,6 +322,8 @@ def bench_costmodel(S, M, n_train=40, n_held=10, full=False, seed=5):
             S.spldlt_free_akeep(akeep)
         else:
             held.append((name, A, akeep, m, n, nc))
+        if (k + 1) % 25 == 0:
+            log(f"  cost model: {k + 1} matrices analysed and tuned ({time.perf_counter() - t0:.0f} s)")
     t_data = time.perf_counter() - t0
     x, t = np.concatenate(xs), np.concatenate(ts)
     model = CM.CostModel(S, seed=seed).fit(x, t, epochs=300)
@@ -357,6 +359,7 @@ def bench_costmodel(S, M, n_train=40, n_held=10, full=False, seed=5):
                 S.spldlt_free_fkeep(fk)
                 e2e[key] += float(np.median(fms))
             S.set_front_variants(akeep, None)
+            log(f"  cost model {label}: held-out {name} done")
         out[label] = {"front_time_best_ms": round(best, 3), "regret_model_ms": round(reg_m, 3),
                       "regret_default_rule_ms": round(reg_d, 3),
                       "end_to_end_factor_ms": {k: round(v, 3) for k, v in e2e.items()}}

@@ -518,7 +518,7 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict__ tasks,
                                                      T* __restrict__ arena,
                                                      const T* __restrict__ bsrc) {
-   constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
+   constexpr int BM = 128, BN = 128, BK = sizeof(T) == 8 ? 32 : 16, PAD = 4;  // FP64: longer MFMA chains per stage
    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
    typedef typename V16<T>::vt vt;
    constexpr int VN = V16<T>::N;
@@ -989,18 +989,13 @@ static __global__ __launch_bounds__(kThreads) void k_bwd_gather(const SolveTask*
 // x_r -= L[r, k0:k0+kb] y; backward update tasks handle columns [r0, r1),
 // x_j -= L[k0:k0+kb, j]^T y.
 template <typename T, bool BWD>
-__global__ __launch_bounds__(kThreads) void k_solve_step(const SolveTask* __restrict__ tasks,
-                                                         const NodeDesc* __restrict__ nodes,
-                                                         const T* __restrict__ arena,
-                                                         const T* __restrict__ dinvs,
-                                                         const int* __restrict__ rlist,
-                                                         double* __restrict__ xloc,
-                                                         double* __restrict__ xsol, int ldsol, int nrhs) {
+__device__ __forceinline__ void solve_step_dev(const SolveTask& t, const NodeDesc& nd, const T* __restrict__ arena,
+                                               const T* __restrict__ dinvs, const int* __restrict__ rlist,
+                                               double* __restrict__ xloc, double* __restrict__ xsol, int ldsol,
+                                               int nrhs) {
    constexpr int KU = kNB / 4;  // k-steps of the 16x16x4 MFMA over 128
    __shared__ double xs[kNB * 16];
    __shared__ double ys[kNB * 16];
-   const SolveTask t = tasks[blockIdx.x];
-   const NodeDesc nd = nodes[t.node];
    const int m = nd.m, kb = t.kb;
    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    const int r = lane & 15, kq = lane >> 4;
@@ -1074,6 +1069,40 @@ __global__ __launch_bounds__(kThreads) void k_solve_step(const SolveTask* __rest
          if (o < t.r1 && r < nc) xl[o + (int64_t)(c0 + r) * m] = cv[v] - acc[v];
       }
    }
+}
+
+
+template <typename T, bool BWD>
+__global__ __launch_bounds__(kThreads) void k_solve_step(const SolveTask* __restrict__ tasks,
+                                                         const NodeDesc* __restrict__ nodes,
+                                                         const T* __restrict__ arena,
+                                                         const T* __restrict__ dinvs,
+                                                         const int* __restrict__ rlist,
+                                                         double* __restrict__ xloc,
+                                                         double* __restrict__ xsol, int ldsol, int nrhs) {
+   const SolveTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
+   solve_step_dev<T, BWD>(t, nd, arena, dinvs, rlist, xloc, xsol, ldsol, nrhs);
+}
+
+// The same step for a launch whose tasks all belong to one front: the front
+// and the step come as arguments, so the first loads are the operands
+// (workgroup 0 writes the block's solution; workgroup u > 0 updates rows
+// (forward) or columns (backward) [r0, r0 + chunk) with r0 = base + (u-1) chunk).
+template <typename T, bool BWD>
+__global__ __launch_bounds__(kThreads) void k_solve_step1(const NodeDesc nd, int node, int k0, int kb, int base,
+                                                          int limit, int chunk, const T* __restrict__ arena,
+                                                          const T* __restrict__ dinvs,
+                                                          const int* __restrict__ rlist,
+                                                          double* __restrict__ xloc, double* __restrict__ xsol,
+                                                          int ldsol, int nrhs) {
+   SolveTask t{node, k0, kb, 0, 0, 1};
+   if (blockIdx.x > 0) {
+      t.write = 0;
+      t.r0 = base + (int)(blockIdx.x - 1) * chunk;
+      t.r1 = min(limit, t.r0 + chunk);
+   }
+   solve_step_dev<T, BWD>(t, nd, arena, dinvs, rlist, xloc, xsol, ldsol, nrhs);
 }
 
 // acc += A(16 rows x K) B(K x 16) for small fronts: A from global memory

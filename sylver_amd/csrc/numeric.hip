@@ -421,6 +421,15 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    // 256 rows
    auto small_solve = [](const NodeDesc& d) { return d.n <= kNB && d.m <= 2 * kNB; };
    constexpr int kUpdRows = 64;   // rows per forward update task: one 16-row tile per wave
+   // a step launch whose tasks all belong to one front gets it by value
+   auto mark_single = [&](std::vector<Launch>& prog, int64_t t0) {
+      if (prog.empty() || prog.back().off != t0) return;
+      Launch& l = prog.back();
+      const int node = P->stask[t0].node;
+      for (int64_t t = t0; t < t0 + l.count; ++t)
+         if (P->stask[t].node != node) return;
+      l.node1 = node;
+   };
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
       int64_t g0 = P->stask.size();
@@ -466,6 +475,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
                P->stask.push_back(SolveTask{i, k0, kb, r, std::min(d.m, r + kUpdRows), 0});
          }
          push(P->fwdprog, L_FDIAG, t0, P->stask.size() - t0);
+         mark_single(P->fwdprog, t0);
       }
    }
    for (int L = nlev - 1; L >= 0; --L) {
@@ -522,6 +532,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
                P->stask.push_back(SolveTask{i, k0, kb, j, std::min(k0, j + kColChunk), 0});
          }
          push(P->bwdprog, L_BDIAG, t0, P->stask.size() - t0);
+         mark_single(P->bwdprog, t0);
       }
    }
 
@@ -664,15 +675,29 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
             k_fwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_gch, P->d_rlist, P->d_cmap, b, n, F->d_xloc, nrhs);
             break;
          case L_FDIAG:
-            k_solve_step<T, false><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinv, P->d_rlist, F->d_xloc, xsol,
-                                                    n, nrhs);
+            if (L.node1 >= 0) {
+               const SolveTask& t = P->stask[L.off];
+               const NodeDesc& nd = P->hnodes[L.node1];
+               k_solve_step1<T, false><<<g, bl, 0, s>>>(nd, L.node1, t.k0, t.kb, t.k0 + t.kb, nd.m, 64, arena, dinv,
+                                                        P->d_rlist, F->d_xloc, xsol, n, nrhs);
+            } else {
+               k_solve_step<T, false><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinv, P->d_rlist, F->d_xloc, xsol,
+                                                       n, nrhs);
+            }
             break;
          case L_BGATHER:
             k_bwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_rlist, xsol, n, F->d_xloc, nrhs);
             break;
          case L_BDIAG:
-            k_solve_step<T, true><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinvT, P->d_rlist, F->d_xloc, xsol,
-                                                   n, nrhs);
+            if (L.node1 >= 0) {
+               const SolveTask& t = P->stask[L.off];
+               const NodeDesc& nd = P->hnodes[L.node1];
+               k_solve_step1<T, true><<<g, bl, 0, s>>>(nd, L.node1, t.k0, t.kb, 0, t.k0, 64, arena, dinvT,
+                                                       P->d_rlist, F->d_xloc, xsol, n, nrhs);
+            } else {
+               k_solve_step<T, true><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, dinvT, P->d_rlist, F->d_xloc, xsol,
+                                                      n, nrhs);
+            }
             break;
          case L_BUPD:
             if (L.count > 0) k_bwd_cbp<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, F->d_xloc, F->d_part_cb, nrhs);

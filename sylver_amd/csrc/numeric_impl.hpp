@@ -77,6 +77,7 @@ struct Launch {
    double flops = 0;  // algorithmic flops of the launch
    double words = 0;  // algorithmic memory traffic in front elements
    double bytes = 0;  // algorithmic memory traffic independent of the element size
+   int node1 = -1;    // solve steps: the one front all tasks belong to (or -1)
 };
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
