@@ -283,7 +283,7 @@ def bench_kkt(S, M, oracle_mod, side=48, mrows=27648, sample_side=16):
     return out
 
 
-def bench_costmodel(S, M, n_train=40, n_held=10, full=False, seed=5):
+def bench_costmodel(S, M, n_train=40, n_held=10, full=False, seed=5, return_model=False):
     """Learned cost model (per-front kernel variant, one-workgroup vs tiled):
     a seeded stream of SPD grid matrices (2D 5-point, 3D 7-point with aspect
     ratios up to 4:1), the autotuner's exhaustive per-shape timings (FP64 and
@@ -366,7 +366,7 @@ def bench_costmodel(S, M, n_train=40, n_held=10, full=False, seed=5):
     S.set_ext_options(precision=S.PREC_FP64)
     for _, _, akeep, _, _, _ in held:
         S.spldlt_free_akeep(akeep)
-    return out
+    return (out, model) if return_model else out
 
 
 def measured_traffic():
@@ -400,6 +400,8 @@ def main():
                     help="only run a plain device copy of known size (scales PMC byte counters)")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the per-launch timed (eager, one stream) factorization of the FP32 path")
+    ap.add_argument("--variants", default="model", choices=["model", "default"],
+                    help="per-front kernel variants: the learned cost model's (trained first) or the default rule")
     ap.add_argument("--costmodel-full", action="store_true",
                     help="learned cost model on the configured stream (400 + 100 matrices, full sizes)")
     ap.add_argument("--no-secondary", action="store_true",
@@ -433,20 +435,41 @@ def main():
         f" flops={flops:.4e}")
 
     prec = S.PREC_FP32 if args.mixed == "fp32" else S.PREC_MIXED
+    # per-front kernel variants proposed by the learned cost model, trained
+    # first on its own matrix stream (also the config's report below)
+    cm_report, model = None, None
+    if args.variants == "model":
+        cm_report, model = bench_costmodel(S, M, full=args.costmodel_full, return_model=True)
+    from sylver_amd import costmodel as CM
+    fm, fn, fc = CM.tree_fronts(S, akeep)
+
+    def use_variants(tier):
+        if model is None:
+            S.set_front_variants(akeep, None)
+            return
+        band = CM.in_band(fm)
+        pick = np.zeros(len(fm), bool)
+        pick[band] = model.pick_one_wg(CM.features(fm[band], fn[band], fc[band], tier))
+        S.set_front_variants(akeep, CM.choose_variants(fm, pick))
+
     if args.profile_only:  # rocprofv3 runs of the roofline figure: the same eager launches
+        use_variants(1)
         kinds, fms, _ = kernel_profile(S, A, akeep, opts, prec, None)
         print(json.dumps({"profile_only": True, "kernels": kinds}))
         return
     if args.paths != "both":  # profiling runs: one path, no report
         pr = S.PREC_FP64 if args.paths == "fp64" else prec
+        use_variants(0 if pr == S.PREC_FP64 else 1)
         _, recs = run_path(S, A, akeep, opts, b, pr, args.steps, args.warmup, 1e-15)
         print(json.dumps({"paths": args.paths, "factor_ms": [r["factor_ms"] for r in recs],
                           "solve_ms": [r["solve_ms"] for r in recs]}))
         return
     # plain FP64 path
+    use_variants(0)
     fk64, r64 = run_path(S, A, akeep, opts, b, S.PREC_FP64, args.steps, args.warmup, 1e-15)
     err64 = M.backward_error(A, r64[-1]["x"], b)
     # mixed path (headline)
+    use_variants(1)
     fkmx, rmx = run_path(S, A, akeep, opts, b, prec, args.steps, args.warmup, 1e-15)
     errmx = M.backward_error(A, rmx[-1]["x"], b)
 
@@ -459,7 +482,9 @@ def main():
     roof = None
     kinds64 = kindsmx = None
     if not args.no_profile:
+        use_variants(0)
         kinds64, _, fk64 = kernel_profile(S, A, akeep, opts, S.PREC_FP64, fk64)
+        use_variants(1)
         kindsmx, _, fkmx = kernel_profile(S, A, akeep, opts, prec, fkmx)
         up = kindsmx.get("schur_update")
         wide = kindsmx.get("schur_update_wide")
@@ -487,7 +512,8 @@ def main():
         for key, fn in (("laplacian_2d_100", lambda: bench_small_laplacian(S, M, oracle_mod)),
                         ("batched_small_fronts", lambda: bench_batched_fronts(S, M)),
                         ("kkt_app_48cubed_27648", lambda: bench_kkt(S, M, oracle_mod)),
-                        ("learned_cost_model", lambda: bench_costmodel(S, M, full=args.costmodel_full))):
+                        ("learned_cost_model",
+                         lambda: cm_report if cm_report is not None else bench_costmodel(S, M, full=args.costmodel_full))):
             try:
                 secondary[key] = fn()
             except Exception as e:  # a secondary workload never blocks the headline
@@ -519,7 +545,9 @@ def main():
         "config": {"workload": f"3D 7-point Laplacian {g}^3 (n={A.n}), nested dissection,"
                                f" {args.nrhs} RHS U[-1,1] seed 0, posdef LL^T",
                    "n": A.n, "nrhs": args.nrhs, "nodes": int(inf.num_sup),
-                   "maxfront": int(inf.maxfront), "flops": flops},
+                   "maxfront": int(inf.maxfront), "flops": flops,
+                   "front_variants": ("learned cost model (per front, trained in this run)" if model is not None
+                                      else "default rule (one workgroup for nrow <= 256)")},
         "mixed": {"factor_s": round(fmx_ms / 1e3, 5), "solve_ir_s": round(smx_ms / 1e3, 5),
                   "factor_gflops": round(flops / (fmx_ms * 1e6), 2),
                   "ir_iters": int(rmx[-1]["refine_iters"]), "bwd_err": errmx},

@@ -90,6 +90,26 @@ template <> struct Mfma<float> {
 __device__ inline double dsqrt(double x) { return __builtin_sqrt(x); }
 __device__ inline float dsqrt(float x) { return __builtin_sqrtf(x); }
 
+// 1/sqrt(x) for x > 0 from the hardware estimate (v_rsq) refined by Newton
+// steps y <- y + y (1 - x y^2) / 2 (two in FP64, one in FP32): a few
+// dependent operations instead of a correctly rounded square root followed
+// by a division, on the critical path of the diagonal-block Cholesky;
+// within a few ulps. Values in the normal range only (pivots of an SPD front).
+__device__ inline double rsqrt_nr(double x) {
+   double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+   for (int k = 0; k < 2; ++k) {
+      const double r = __builtin_fma(-x * y, y, 1.0);
+      y = __builtin_fma(0.5 * y, r, y);
+   }
+   return y;
+}
+__device__ inline float rsqrt_nr(float x) {
+   const float y = __builtin_amdgcn_rsqf(x);
+   const float r = __builtin_fmaf(-x * y, y, 1.0f);
+   return __builtin_fmaf(0.5f * y, r, y);
+}
+
 // ---------------------------------------------------------------------------
 // Zero the lower triangle of the columns [c0, c1) of a front.
 // ---------------------------------------------------------------------------
@@ -182,13 +202,24 @@ __global__ __launch_bounds__(kThreads) void k_extend_add(const EaTask* __restric
 // every loop below works on whole 16x16 sub-blocks: the Cholesky factor and
 // the inverse of [A 0; 0 I] are [L 0; 0 I] and [inv(L) 0; 0 I].
 // ---------------------------------------------------------------------------
+// Value of v in lane `src` (a compile-time constant in the callers) for the
+// whole wave, through v_readlane (a scalar operand afterwards).
+__device__ __forceinline__ float lane_bcast(float v, int src) {
+   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+   const long long u = __double_as_longlong(v);
+   const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffff), src);
+   const int hi = __builtin_amdgcn_readlane((int)(u >> 32), src);
+   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Cholesky of a 16x16 block at D (ld LD) by one wave: lane i < 16 holds row
-// i in registers; after each pivot the scaled column is published through
-// LDS (one write per lane) and read back by every lane as broadcasts, so no
-// step waits on a chain of cross-lane shuffles. Col is 16 elements of LDS.
+// i in registers; the current column j is read from lanes j..15 with
+// v_readlane, so a step waits on no memory round trip.
 // Returns 0 or the 1-based column of the first non-positive pivot.
 template <typename T, int LD>
-__device__ int wave_potrf16(T* D, T* Col) {
+__device__ int wave_potrf16(T* D) {
    const int lane = threadIdx.x & 63;
    const bool own = lane < 16;
    T r[16];
@@ -197,25 +228,20 @@ __device__ int wave_potrf16(T* D, T* Col) {
    int bad = 0;
 #pragma unroll
    for (int j = 0; j < 16; ++j) {
-      if (own) Col[lane] = r[j];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       T col[16];
 #pragma unroll
-      for (int c = j; c < 16; ++c) col[c] = Col[c];
+      for (int c = j; c < 16; ++c) col[c] = lane_bcast(r[j], c);
       const T d = col[j];
       if (bad == 0 && !(d > T(0))) bad = j + 1;
       if (bad == 0) {
-         const T sd = dsqrt(d);
-         const T inv = T(1) / sd;
+         const T inv = rsqrt_nr(d);
+         const T sd = d * inv;
          if (lane == j) r[j] = sd;
          if (lane > j) r[j] *= inv;
 #pragma unroll
          for (int c = j + 1; c < 16; ++c)
             if (lane >= c) r[c] -= r[j] * (col[c] * inv);
       }
-      __builtin_amdgcn_wave_barrier();
    }
    if (own)
 #pragma unroll
@@ -264,23 +290,42 @@ __device__ inline typename Mfma<T>::acc_t lds_mma16(const T* A, int lda, const T
 }
 
 // Blocked Cholesky of the padded kb16 x kb16 block in LDS. All threads call.
+// Per 16-column step s: the panel rows below the diagonal block (one row per
+// thread, right-looking so each row's dependency chain is 16 steps), then
+// wave 0 updates the next diagonal block and factors it while the other
+// waves apply the rest of the trailing update with the same panel
+// (look-ahead); tiles are handed out through an LDS counter, so wave 0 joins
+// in when its factorization is done.
 template <typename T, int LD>
 __device__ int lds_potrf_blocked(T* L, int kb16, int* sh_bad) {
    typedef typename Mfma<T>::acc_t acc_t;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int nbk = kb16 / 16;
    __shared__ T Rcp[16];
-   __shared__ T Col[16];
+   __shared__ int next_tile;
+   auto factor_diag = [&](int o) {  // wave 0: Cholesky of the 16x16 block at (o, o)
+      const int bad = wave_potrf16<T, LD>(L + o + o * LD);
+      if (lane == 0) *sh_bad = bad ? o + bad : 0;
+      if (lane < 16) Rcp[lane] = T(1) / L[(o + lane) + (o + lane) * LD];
+   };
+   // lower 16x16 tile (ti, tj) of the trailing block at t0 minus panel o's product
+   auto update_tile = [&](int o, int t0, int ti, int tj) {
+      acc_t acc = acc_t{0, 0, 0, 0};
+      acc = lds_mma16<T, true>(L + t0 + ti * 16 + o * LD, LD, L + t0 + tj * 16 + o * LD, LD, acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int i = t0 + ti * 16 + Mfma<T>::row(lane, v);
+         const int j = t0 + tj * 16 + (lane & 15);
+         if (i >= j) L[i + j * LD] -= acc[v];
+      }
+   };
+   if (wave == 0) factor_diag(0);
+   __syncthreads();
    for (int s = 0; s < nbk; ++s) {
       const int o = s * 16;
-      if (wave == 0) {
-         int bad = wave_potrf16<T, LD>(L + o + o * LD, Col);
-         if (lane == 0) *sh_bad = bad ? o + bad : 0;
-         if (lane < 16) Rcp[lane] = T(1) / L[(o + lane) + (o + lane) * LD];
-      }
-      __syncthreads();
       if (*sh_bad) return *sh_bad;
-      // panel rows below the diagonal sub-block: X = P * inv(D)^T, one row per thread
+      // panel rows below the diagonal block: X = P * inv(D)^T, one row per thread
+      if (tid == 0) next_tile = 1;
       const int r = o + 16 + tid;
       if (r < kb16) {
          T x[16];
@@ -288,31 +333,35 @@ __device__ int lds_potrf_blocked(T* L, int kb16, int* sh_bad) {
          for (int c = 0; c < 16; ++c) x[c] = L[r + (o + c) * LD];
 #pragma unroll
          for (int c = 0; c < 16; ++c) {
-            T sum = x[c];
+            x[c] *= Rcp[c];
 #pragma unroll
-            for (int cc = 0; cc < c; ++cc) sum -= x[cc] * L[(o + c) + (o + cc) * LD];
-            x[c] = sum * Rcp[c];
+            for (int cc = c + 1; cc < 16; ++cc) x[cc] -= x[c] * L[(o + cc) + (o + c) * LD];
          }
 #pragma unroll
          for (int c = 0; c < 16; ++c) L[r + (o + c) * LD] = x[c];
       }
       __syncthreads();
-      // trailing update on 16x16 tiles (lower), MFMA with operands in LDS
+      if (s + 1 == nbk) break;
       const int t0 = o + 16;
       const int nt = (kb16 - t0) / 16;
       const int ntiles = nt * (nt + 1) / 2;
-      for (int tile = wave; tile < ntiles; tile += 4) {
+      if (wave == 0) {
+         // the next diagonal block first, then its factorization
+         update_tile(o, t0, 0, 0);
+         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+         __builtin_amdgcn_wave_barrier();
+         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+         factor_diag(t0);
+      }
+      // the other tiles (1..ntiles-1) go to whichever wave is free
+      for (;;) {
+         int tile = 0;
+         if (lane == 0) tile = atomicAdd(&next_tile, 1);
+         tile = __shfl(tile, 0);
+         if (tile >= ntiles) break;
          int ti = 0;
          while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
-         const int tj = tile - ti * (ti + 1) / 2;
-         acc_t acc = acc_t{0, 0, 0, 0};
-         acc = lds_mma16<T, true>(L + t0 + ti * 16 + o * LD, LD, L + t0 + tj * 16 + o * LD, LD, acc);
-#pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int i = t0 + ti * 16 + Mfma<T>::row(lane, v);
-            const int j = t0 + tj * 16 + (lane & 15);
-            if (i >= j) L[i + j * LD] -= acc[v];
-         }
+         update_tile(o, t0, ti, tile - ti * (ti + 1) / 2);
       }
       __syncthreads();
    }
