@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 #include <cstdint>
+#include <type_traits>
 
 namespace sylver_amd {
 
@@ -1079,25 +1080,37 @@ __device__ __forceinline__ void solve_step_dev(const SolveTask& t, const NodeDes
          const int l = e >> 4, c = e & 15;
          xs[e] = *live_or_zero(l < kb && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
       }
-      // values this task updates, read before the arithmetic
+      // values this task updates, read before the arithmetic (in the row
+      // layout of the MFMA that produces the update)
       double cv[4];
       const int ob = t.r0 + wave * 16;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-         const int o = ob + Mfma<double>::row(lane, v);
+         const int o = ob + (std::is_same<T, float>::value ? Mfma<float>::row(lane, v) : Mfma<double>::row(lane, v));
          cv[v] = *live_or_zero(!t.write && o < t.r1 && r < nc, xl + o + (int64_t)(c0 + r) * m);
       }
       __syncthreads();
+      // y = inv(L_kk) x_k: FP64 MFMA for FP64 factors; FP32 MFMA (twice the
+      // rate) for FP32 factors, whose solves feed FP64 refinement
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
          const int tile = wave + 4 * q;
          if (tile * 16 >= kb) continue;
-         v4d acc = v4d{0, 0, 0, 0};
+         if constexpr (std::is_same<T, float>::value) {
+            v4f acc = v4f{0, 0, 0, 0};
 #pragma unroll
-         for (int u = 0; u < KU; ++u)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)mv[q][u], xs[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
+            for (int u = 0; u < KU; ++u)
+               acc = __builtin_amdgcn_mfma_f32_16x16x4f32(mv[q][u], (float)xs[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
 #pragma unroll
-         for (int v = 0; v < 4; ++v) ys[(tile * 16 + Mfma<double>::row(lane, v)) * 16 + r] = acc[v];
+            for (int v = 0; v < 4; ++v) ys[(tile * 16 + Mfma<float>::row(lane, v)) * 16 + r] = (double)acc[v];
+         } else {
+            v4d acc = v4d{0, 0, 0, 0};
+#pragma unroll
+            for (int u = 0; u < KU; ++u)
+               acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)mv[q][u], xs[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) ys[(tile * 16 + Mfma<double>::row(lane, v)) * 16 + r] = acc[v];
+         }
       }
       __syncthreads();
       if (t.write) {
@@ -1108,14 +1121,26 @@ __device__ __forceinline__ void solve_step_dev(const SolveTask& t, const NodeDes
          continue;
       }
       if (ob >= t.r1) continue;
-      v4d acc = v4d{0, 0, 0, 0};
+      if constexpr (std::is_same<T, float>::value) {
+         v4f acc = v4f{0, 0, 0, 0};
 #pragma unroll
-      for (int u = 0; u < KU; ++u)
-         acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u], ys[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
+         for (int u = 0; u < KU; ++u)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], (float)ys[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-         const int o = ob + Mfma<double>::row(lane, v);
-         if (o < t.r1 && r < nc) xl[o + (int64_t)(c0 + r) * m] = cv[v] - acc[v];
+         for (int v = 0; v < 4; ++v) {
+            const int o = ob + Mfma<float>::row(lane, v);
+            if (o < t.r1 && r < nc) xl[o + (int64_t)(c0 + r) * m] = cv[v] - (double)acc[v];
+         }
+      } else {
+         v4d acc = v4d{0, 0, 0, 0};
+#pragma unroll
+         for (int u = 0; u < KU; ++u)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u], ys[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int o = ob + Mfma<double>::row(lane, v);
+            if (o < t.r1 && r < nc) xl[o + (int64_t)(c0 + r) * m] = cv[v] - acc[v];
+         }
       }
    }
 }
@@ -1159,8 +1184,8 @@ __global__ __launch_bounds__(kThreads) void k_solve_step1(const NodeDesc nd, int
 // K read as zero), B row-major in LDS; loads in batches of eight k-steps.
 // Few registers, so that many workgroups share a CU.
 template <typename T, bool TR>
-__device__ inline v4d mma_small(const T* __restrict__ A, int64_t lda, int rvalid, int K,
-                                const double* Bs, v4d acc) {
+__device__ inline typename Mfma<T>::acc_t mma_small(const T* __restrict__ A, int64_t lda, int rvalid, int K,
+                                                    const double* Bs, typename Mfma<T>::acc_t acc) {
    const int lane = threadIdx.x & 63;
    const int r = lane & 15, kq = lane >> 4;
    for (int k0 = 0; k0 < K; k0 += 32) {
@@ -1174,7 +1199,7 @@ __device__ inline v4d mma_small(const T* __restrict__ A, int64_t lda, int rvalid
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
          const int kk = k0 + 4 * u + kq;
-         acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)raw[u], (kk < K) ? Bs[kk * 16 + r] : 0.0, acc, 0, 0, 0);
+         acc = Mfma<T>::op(raw[u], (T)((kk < K) ? Bs[kk * 16 + r] : 0.0), acc);
       }
    }
    return acc;
@@ -1213,24 +1238,24 @@ __global__ __launch_bounds__(kThreads) void k_solve_small(const int* __restrict_
       if (BWD && q > 0) {
          // x[0:n] -= L21^T x[n:m]: output tiles over the n columns
          for (int tile = wave; tile * 16 < n; tile += 4) {
-            v4d acc = v4d{0, 0, 0, 0};
+            typename Mfma<T>::acc_t acc = {0, 0, 0, 0};
             acc = mma_small<T, true>(F + n + (int64_t)(tile * 16) * nd.ld, nd.ld, n - tile * 16, q,
                                      xs + n * 16, acc);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-               const int i = tile * 16 + Mfma<double>::row(lane, v);
-               if (i < n) xs[i * 16 + r] -= acc[v];
+               const int i = tile * 16 + Mfma<T>::row(lane, v);
+               if (i < n) xs[i * 16 + r] -= (double)acc[v];
             }
          }
          __syncthreads();
       }
       for (int tile = wave; tile * 16 < n; tile += 4) {
-         v4d acc = v4d{0, 0, 0, 0};
+         typename Mfma<T>::acc_t acc = {0, 0, 0, 0};
          acc = mma_small<T, false>(Mk + tile * 16, n, n - tile * 16, n, xs, acc);
 #pragma unroll
          for (int v = 0; v < 4; ++v) {
-            const int i = tile * 16 + Mfma<double>::row(lane, v);
-            if (i < n) ys[i * 16 + r] = acc[v];
+            const int i = tile * 16 + Mfma<T>::row(lane, v);
+            if (i < n) ys[i * 16 + r] = (double)acc[v];
          }
       }
       __syncthreads();
@@ -1240,12 +1265,12 @@ __global__ __launch_bounds__(kThreads) void k_solve_small(const int* __restrict_
       }
       if (!BWD && q > 0) {
          for (int tile = wave; tile * 16 < q; tile += 4) {
-            v4d acc = v4d{0, 0, 0, 0};
+            typename Mfma<T>::acc_t acc = {0, 0, 0, 0};
             acc = mma_small<T, false>(F + n + tile * 16, nd.ld, q - tile * 16, n, ys, acc);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-               const int o = n + tile * 16 + Mfma<double>::row(lane, v);
-               if (o < m && r < nc) xl[o + (int64_t)(c0 + r) * m] = xs[o * 16 + r] - acc[v];
+               const int o = n + tile * 16 + Mfma<T>::row(lane, v);
+               if (o < m && r < nc) xl[o + (int64_t)(c0 + r) * m] = xs[o * 16 + r] - (double)acc[v];
             }
          }
       }
@@ -1290,14 +1315,13 @@ __global__ __launch_bounds__(kThreads) void k_bwd_cbp(const SolveTask* __restric
          xs[e] = *live_or_zero(l < q && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
       }
       __syncthreads();
-      v4d acc = v4d{0, 0, 0, 0};
+      typename Mfma<T>::acc_t acc = {0, 0, 0, 0};
 #pragma unroll
-      for (int u = 0; u < KU; ++u)
-         acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u], xs[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
+      for (int u = 0; u < KU; ++u) acc = Mfma<T>::op(av[u], (T)xs[(4 * u + kq) * 16 + r], acc);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-         const int jj = wave * 16 + Mfma<double>::row(lane, v);
-         if (r < nc) out[(int64_t)jj * nrhs + c0 + r] = acc[v];
+         const int jj = wave * 16 + Mfma<T>::row(lane, v);
+         if (r < nc) out[(int64_t)jj * nrhs + c0 + r] = (double)acc[v];
       }
    }
 }
