@@ -283,7 +283,7 @@ def bench_kkt(S, M, oracle_mod, side=48, mrows=27648, sample_side=16):
     return out
 
 
-def bench_costmodel(S, M, n_train=40, n_held=10, full=False, seed=5, return_model=False):
+def bench_costmodel(S, M, n_train=None, n_held=None, full=False, seed=5, return_model=False, e2e_max=None):
     """Learned cost model (per-front kernel variant, one-workgroup vs tiled):
     a seeded stream of SPD grid matrices (2D 5-point, 3D 7-point with aspect
     ratios up to 4:1), the autotuner's exhaustive per-shape timings (FP64 and
@@ -293,6 +293,12 @@ def bench_costmodel(S, M, n_train=40, n_held=10, full=False, seed=5, return_mode
     full=True uses the configured sizes (2D sides 200-1000, 3D 30-90) and
     400 + 100 matrices; the default is a reduced stream."""
     from sylver_amd import costmodel as CM
+    if n_train is None:
+        n_train = 400 if full else 40
+    if n_held is None:
+        n_held = 100 if full else 10
+    if e2e_max is None:
+        e2e_max = 30 if full else n_held
     rng = np.random.default_rng(seed)
     s2, s3 = ((200, 1000), (30, 90)) if full else ((150, 400), (20, 45))
 
@@ -336,7 +342,7 @@ def bench_costmodel(S, M, n_train=40, n_held=10, full=False, seed=5, return_mode
         reg_m = reg_d = best = 0.0
         e2e = {"default": 0.0, "autotuner": 0.0, "model": 0.0}
         S.set_ext_options(precision=prec, use_graph=1, record_timing=0)
-        for name, A, akeep, m, n, nc in held:
+        for h, (name, A, akeep, m, n, nc) in enumerate(held):
             band = CM.in_band(m)
             tm = tuners[prec].measure(m, n)
             pick = np.zeros(len(m), bool)
@@ -347,6 +353,8 @@ def bench_costmodel(S, M, n_train=40, n_held=10, full=False, seed=5, return_mode
             reg_d += (np.where(m[band] <= 256, tb[:, 0], tb[:, 1]) - tb.min(axis=1)).sum()
             variants = {"default": None, "autotuner": CM.choose_variants(m, tm[:, 0] < tm[:, 1]),
                         "model": CM.choose_variants(m, pick)}
+            if h >= e2e_max:
+                continue
             for key, v in variants.items():
                 S.set_front_variants(akeep, v)
                 fk, fms = None, []
@@ -360,7 +368,8 @@ def bench_costmodel(S, M, n_train=40, n_held=10, full=False, seed=5, return_mode
                 e2e[key] += float(np.median(fms))
             S.set_front_variants(akeep, None)
             log(f"  cost model {label}: held-out {name} done")
-        out[label] = {"front_time_best_ms": round(best, 3), "regret_model_ms": round(reg_m, 3),
+        out[label] = {"end_to_end_matrices": min(e2e_max, len(held)),
+                      "front_time_best_ms": round(best, 3), "regret_model_ms": round(reg_m, 3),
                       "regret_default_rule_ms": round(reg_d, 3),
                       "end_to_end_factor_ms": {k: round(v, 3) for k, v in e2e.items()}}
     S.set_ext_options(precision=S.PREC_FP64)
