@@ -332,7 +332,9 @@ def bench_costmodel(S, M, n_train=None, n_held=None, full=False, seed=5, return_
             log(f"  cost model: {k + 1} matrices analysed and tuned ({time.perf_counter() - t0:.0f} s)")
     t_data = time.perf_counter() - t0
     x, t = np.concatenate(xs), np.concatenate(ts)
-    model = CM.CostModel(S, seed=seed).fit(x, t, epochs=300)
+    log(f"  cost model: training on {len(x)} records")
+    model = CM.CostModel(S, seed=seed).fit(x, t, epochs=300, batch=256 if len(x) < 100_000 else 1024)
+    log(f"  cost model: trained, final loss {float(model.loss[-1]):.4f}")
     out = {"train_matrices": n_train, "held_out_matrices": n_held, "records": int(len(x)),
            "shape_buckets_timed": len(tuners[S.PREC_FP64].cache) + len(tuners[S.PREC_FP32].cache),
            "final_train_loss_log2ms": round(float(model.loss[-1]), 4), "data_and_tuning_s": round(t_data, 1),
@@ -354,6 +356,7 @@ def bench_costmodel(S, M, n_train=None, n_held=None, full=False, seed=5, return_
             variants = {"default": None, "autotuner": CM.choose_variants(m, tm[:, 0] < tm[:, 1]),
                         "model": CM.choose_variants(m, pick)}
             if h >= e2e_max:
+                log(f"  cost model {label}: held-out {name} (regret only)")
                 continue
             for key, v in variants.items():
                 S.set_front_variants(akeep, v)

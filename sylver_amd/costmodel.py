@@ -109,8 +109,16 @@ class CostModel:
         self.params = S.mlp_init(seed)
         self.loss = None
 
-    def fit(self, x, t, epochs=300, batch=256, lr=3e-3, seed=1):
+    def fit(self, x, t, epochs=300, batch=256, lr=3e-3, seed=1, max_records=200_000, max_steps=60_000):
+        """Train on (features, times); at most max_records records (a seeded
+        subsample) and about max_steps optimizer steps."""
+        x = np.asarray(x, np.float32)
         y = np.log2(np.maximum(np.asarray(t, np.float64), 1e-4)).astype(np.float32)
+        if len(x) > max_records:
+            keep = np.random.default_rng(seed).choice(len(x), max_records, replace=False)
+            x, y = x[keep], y[keep]
+        steps_per_epoch = max(1, -(-len(x) // batch))
+        epochs = int(max(5, min(epochs, max_steps // steps_per_epoch)))
         self.params, self.loss = self.S.mlp_train(self.params, x, y, epochs, batch, lr, seed)
         return self
 
