@@ -138,13 +138,16 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}))
 
 
-def bench_batched_fronts(S, M, nfronts=20000, seed=1, cpu_fronts=1500, reps=3):
+def bench_batched_fronts(S, M, nfronts=20000, seed=1, cpu_fronts=1500, reps=3, model=None):
     """Batched small fronts: ncol log-uniform in [8, 256], nrow = ncol (1 + r),
     r ~ U[0, 2]; partial Cholesky (POTRF, TRSM, SYRK into the contribution
     block) of all fronts in one launch, FP64 and FP32. Roofline time =
     max(total flops / MFMA peak, total bytes / HBM bandwidth) with the
-    lower triangle read and written once per front."""
+    lower triangle read and written once per front. The fronts go through
+    the launch plan with the learned cost model's per-front variant (or the
+    default rule); the single one-workgroup-per-front launch is timed beside."""
     import oracle
+    from sylver_amd import costmodel as CM
     from concurrent.futures import ThreadPoolExecutor
     nrow, ncol = M.dense_front_sizes(nfronts, seed=seed)
     m = nrow.astype(np.float64)
@@ -155,31 +158,45 @@ def bench_batched_fronts(S, M, nfronts=20000, seed=1, cpu_fronts=1500, reps=3):
     off[1:] = np.cumsum(nrow.astype(np.int64) ** 2)[:-1]
     total = int(off[-1] + int(nrow[-1]) ** 2)
     out = {"fronts": nfronts, "seed": seed, "gflop": round(flops / 1e9, 3)}
-    for dt, name, pk in ((np.float64, "fp64", PEAK["fp64"]), (np.float32, "fp32", PEAK["fp32"])):
+    for dt, name, pk, tier in ((np.float64, "fp64", PEAK["fp64"], 0), (np.float32, "fp32", PEAK["fp32"], 1)):
         w = np.dtype(dt).itemsize
         nbytes = float((2 * w * m * (m + 1) / 2).sum())
         arr = S.DeviceArray.empty(total, dt)
-        best = None
-        worst_info = 0
-        for _ in range(reps):
-            if S.batched_front_fill(arr, nrow, off, dt, seed) != 0:
-                raise RuntimeError("front fill failed")
-            rc, info, ms = S.batched_front_potrf(arr, nrow, ncol, off, dt)
-            if rc != 0:
-                raise RuntimeError(f"batched factorization failed: {rc}")
-            worst_info = max(worst_info, int(np.abs(info).max()))
-            best = ms if best is None else min(best, ms)
+        variant = None
+        if model is not None:
+            band = CM.in_band(nrow)
+            pick = np.zeros(nfronts, bool)
+            pick[band] = model.pick_one_wg(CM.features(nrow[band], ncol[band], np.zeros(int(band.sum())), tier))
+            variant = CM.choose_variants(nrow, pick)
+
+        def timed(run):
+            best, worst = None, 0
+            for _ in range(reps):
+                if S.batched_front_fill(arr, nrow, off, dt, seed) != 0:
+                    raise RuntimeError("front fill failed")
+                rc, info, ms = run()
+                if rc != 0:
+                    raise RuntimeError(f"batched factorization failed: {rc}")
+                worst = max(worst, int(np.abs(info).max()))
+                best = ms if best is None else min(best, ms)
+            return best, worst
+
+        one_ms, _ = timed(lambda: S.batched_front_potrf(arr, nrow, ncol, off, dt))
+        best, worst_info = timed(lambda: S.batched_front_potrf_v(arr, nrow, ncol, off, dt, variant))
         t = best / 1e3
         bound = max(flops / (pk * 1e12), nbytes / (HBM_GBS * 1e9))
         rec = {"ms": round(best, 3), "gflops": round(flops / t / 1e9, 1), "gbytes_s": round(nbytes / t / 1e9, 1),
-               "roofline_ms": round(bound * 1e3, 3), "frac": round(bound / t, 4), "info_max": worst_info}
+               "roofline_ms": round(bound * 1e3, 3), "frac": round(bound / t, 4), "info_max": worst_info,
+               "variants": "learned cost model" if model is not None else "default rule (nrow <= 256)",
+               "tiled_fronts": int((variant == 0).sum()) if variant is not None else int((nrow > 256).sum()),
+               "one_workgroup_launch_ms": round(one_ms, 3)}
         if name == "fp64":
             # parity and CPU baseline on the first fronts: same values, copied back before factoring
             if S.batched_front_fill(arr, nrow, off, dt, seed) != 0:
                 raise RuntimeError("front fill failed")
             k = min(cpu_fronts, nfronts)
             host = [arr.read(off[i], int(nrow[i]) ** 2).reshape(nrow[i], nrow[i], order="F") for i in range(k)]
-            S.batched_front_potrf(arr, nrow, ncol, off, dt)
+            S.batched_front_potrf_v(arr, nrow, ncol, off, dt, variant)
             cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
 
             def one(i):
@@ -193,7 +210,7 @@ def bench_batched_fronts(S, M, nfronts=20000, seed=1, cpu_fronts=1500, reps=3):
             for i, (inf_o, fo) in enumerate(res[:200]):
                 g = arr.read(off[i], int(nrow[i]) ** 2).reshape(nrow[i], nrow[i], order="F")
                 c = int(ncol[i])
-                err = max(err, float(np.abs(np.tril(g[:, :c]) - np.tril(fo[:, :c])).max() / max(1.0, np.abs(fo).max())))
+                err = max(err, float(np.abs(np.tril(g) - np.tril(fo)).max() / max(1.0, np.abs(fo).max())))
             fs = float((n[:k] ** 3 / 3 + q[:k] * n[:k] ** 2 + q[:k] ** 2 * n[:k]).sum())
             rec["parity_max_rel_err_200_fronts"] = err
             out["cpu_baseline"] = {"value": round(fs / tc / 1e9, 3), "unit": "GFLOP/s", "cores": cores,
@@ -522,7 +539,7 @@ def main():
         import oracle as oracle_mod
         secondary = {}
         for key, fn in (("laplacian_2d_100", lambda: bench_small_laplacian(S, M, oracle_mod)),
-                        ("batched_small_fronts", lambda: bench_batched_fronts(S, M)),
+                        ("batched_small_fronts", lambda: bench_batched_fronts(S, M, model=model)),
                         ("kkt_app_48cubed_27648", lambda: bench_kkt(S, M, oracle_mod)),
                         ("learned_cost_model",
                          lambda: cm_report if cm_report is not None else bench_costmodel(S, M, full=args.costmodel_full))):

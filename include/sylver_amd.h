@@ -264,6 +264,16 @@ int sylver_amd_batched_front_fill(int nfronts, int const *nrow_host, long const 
  * SYLVER_ERROR_CALL_SEQUENCE. */
 int sylver_amd_set_front_variants(void *akeep, signed char const *variant, int nnodes);
 
+/* The batched partial factorization of sylver_amd_batched_front_potrf
+ * through the launch plan: per front the one-workgroup kernel (variant[f] =
+ * 1) or the tiled large-front path (0); -1 or variant == NULL the default
+ * rule (nrow <= 256: one workgroup). Fronts of one variant share launches.
+ * Synchronous; info[f] (device) as above; the device time is returned by
+ * sylver_amd_last_batched_ms. */
+int sylver_amd_batched_front_potrf_v(int nfronts, int const *nrow_host, int const *ncol_host,
+                                     long const *offset_host, void *data, int dtype,
+                                     signed char const *variant_host, int *info);
+
 /* Device time (ms, median of reps after one warm-up) of factoring k
  * independent synthetic SPD fronts (m[i] rows, n[i] pivots, values from the
  * batched front generator) with the given variants (NULL = default rule) in

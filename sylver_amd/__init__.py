@@ -152,6 +152,7 @@ EXPORTED = [
     "sylver_amd_batched_front_potrf", "sylver_amd_batched_front_fill", "sylver_amd_device_count", "sylver_amd_dev_alloc",
     "sylver_amd_dev_free", "sylver_amd_memcpy_h2d", "sylver_amd_memcpy_d2h",
     "sylver_amd_last_batched_ms", "sylver_amd_set_front_variants", "sylver_amd_time_fronts",
+    "sylver_amd_batched_front_potrf_v",
     "sylver_amd_mlp_init", "sylver_amd_mlp_grad", "sylver_amd_mlp_train", "sylver_amd_mlp_predict",
 ]
 
@@ -199,6 +200,7 @@ def lib():
     L.sylver_amd_last_batched_ms.restype = C.c_double
     fp = C.POINTER(C.c_float)
     L.sylver_amd_set_front_variants.argtypes = [vp, C.POINTER(C.c_byte), C.c_int]
+    L.sylver_amd_batched_front_potrf_v.argtypes = [C.c_int, ip, ip, lp, vp, C.c_int, C.POINTER(C.c_byte), ip]
     L.sylver_amd_time_fronts.argtypes = [C.c_int, ip, ip, C.POINTER(C.c_byte), C.c_int, C.c_int, dp]
     L.sylver_amd_mlp_init.argtypes = [fp, C.c_ulonglong]
     L.sylver_amd_mlp_grad.argtypes = [fp, fp, fp, C.c_int, fp, fp]
@@ -490,6 +492,25 @@ def batched_front_potrf(fronts_flat: "DeviceArray", nrow, ncol, offsets, dtype):
     rc = lib().sylver_amd_batched_front_potrf(
         len(nrow), _p(nrow, C.c_int), _p(ncol, C.c_int), _p(offsets, C.c_long), fronts_flat.ptr,
         0 if np.dtype(dtype) == np.float64 else 1, C.cast(info.ptr, C.POINTER(C.c_int)), None)
+    out = info.numpy()
+    info.free()
+    return rc, out, float(lib().sylver_amd_last_batched_ms())
+
+
+def batched_front_potrf_v(fronts_flat: "DeviceArray", nrow, ncol, offsets, dtype, variant=None):
+    """The same factorization through the launch plan with a per-front
+    variant (1 one workgroup, 0 tiled path, -1 / None default rule)."""
+    nrow = np.ascontiguousarray(nrow, np.int32)
+    ncol = np.ascontiguousarray(ncol, np.int32)
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    vp = None
+    if variant is not None:
+        v = np.ascontiguousarray(variant, np.int8)
+        vp = _p(v, C.c_byte)
+    info = DeviceArray(np.zeros(len(nrow), np.int32))
+    rc = lib().sylver_amd_batched_front_potrf_v(
+        len(nrow), _p(nrow, C.c_int), _p(ncol, C.c_int), _p(offsets, C.c_long), fronts_flat.ptr,
+        0 if np.dtype(dtype) == np.float64 else 1, vp, C.cast(info.ptr, C.POINTER(C.c_int)))
     out = info.numpy()
     info.free()
     return rc, out, float(lib().sylver_amd_last_batched_ms())

@@ -60,6 +60,10 @@ struct Analysis {
    // path, 1 the one-workgroup kernel; variant_gen changes with every update.
    std::vector<signed char> variant;
    int variant_gen = 0;
+   // Caller-given front layout (batched fronts in the caller's memory):
+   // offset and leading dimension per front; empty = the plan's own arena.
+   std::vector<int64_t> fixed_off;
+   std::vector<int> fixed_ld;
    // Statistics reported through sylver_inform_t
    int64_t num_factor = 0;  // entries in L (lower)
    int64_t num_flops = 0;   // sum over nodes of sum_{r=m-n+1}^{m} r^2

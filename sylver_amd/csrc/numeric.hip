@@ -112,6 +112,16 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
          soff += d.m;
       }
    P->arena_elems = off;
+   if ((int)a.fixed_off.size() == nn && (int)a.fixed_ld.size() == nn) {
+      int64_t end = 0;
+      for (int i = 0; i < nn; ++i) {
+         NodeDesc& d = P->hnodes[i];
+         d.off = a.fixed_off[i];
+         d.ld = a.fixed_ld[i];
+         end = std::max(end, d.off + (int64_t)d.ld * d.m);
+      }
+      P->arena_elems = end;
+   }
    P->dinv_elems = doff;
    P->solve_rows = soff;
    // child maps: position of each CB row in the parent front
@@ -1387,6 +1397,83 @@ int time_fronts(int K, const int* m, const int* n, const signed char* variant, i
          if (v != 0) return SYLVER_ERROR_NOT_POS_DEF;
       return 0;
    } catch (const HipError&) {
+      destroy_factors(F);
+      destroy_plan(P);
+      return SYLVER_ERROR_CUDA_UNKNOWN;
+   }
+}
+
+// Batched partial factorization of independent fronts in the caller's device
+// memory (front f at data + off[f], ld = nrow[f]) through the launch plan:
+// per front the one-workgroup kernel or the tiled path (variant[f] = 1 / 0,
+// -1 or NULL the default rule), all fronts of a variant batched in the same
+// launches. info[f] (device) as batched_front_potrf.
+int batched_front_potrf_v(int nfronts, const int* nrow, const int* ncol, const int64_t* off, void* data,
+                          int dtype, const signed char* variant, int* info) {
+   if (nfronts < 0 || (nfronts > 0 && (!nrow || !ncol || !off || !data || !info)))
+      return SYLVER_ERROR_CALL_SEQUENCE;
+   if (nfronts == 0) return 0;
+   for (int f = 0; f < nfronts; ++f)
+      if (nrow[f] < ncol[f] || ncol[f] < 1) return SYLVER_ERROR_CALL_SEQUENCE;
+   if (device_count() <= 0) return SYLVER_ERROR_CUDA_UNKNOWN;
+   DevicePlan* P = nullptr;
+   DeviceFactors* F = nullptr;
+   try {
+      Analysis a;
+      a.nnodes = nfronts;
+      a.fronts.assign(nfronts + 1, SymbolicFront{});
+      a.fixed_off.assign(off, off + nfronts);
+      a.fixed_ld.assign(nrow, nrow + nfronts);
+      int64_t roff = 0;
+      for (int i = 0; i < nfronts; ++i) {
+         a.n = std::max(a.n, nrow[i]);
+         SymbolicFront& f = a.fronts[i];
+         f.nrow = nrow[i];
+         f.ncol = ncol[i];
+         f.parent = nfronts;
+         f.rlist_off = roff;
+         roff += nrow[i];
+         for (int r = 1; r <= nrow[i]; ++r) a.rlist.push_back(r);
+      }
+      if (variant) a.variant.assign(variant, variant + nfronts);
+      P = build_plan(a, 256, 1 << 30);
+      const bool low = dtype != 0;
+      const size_t es = low ? sizeof(float) : sizeof(double);
+      F = new DeviceFactors();
+      F->plan = P;
+      F->precision = low ? SYLVER_AMD_PREC_FP32 : SYLVER_AMD_PREC_FP64;
+      F->posdef = true;
+      F->arena = data;  // the caller's memory: detached before the factors are freed
+      HIPCHECK(hipMalloc(&F->dinv, std::max<int64_t>(P->dinv_elems, 1) * es));
+      HIPCHECK(hipMalloc(&F->dinvT, std::max<int64_t>(P->dinv_elems, 1) * es));
+      F->d_info = dalloc<int>(nfronts);
+      HIPCHECK(hipMemset(F->d_info, 0, nfronts * sizeof(int)));
+      HIPCHECK(hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking));
+      HIPCHECK(hipStreamCreateWithFlags(&F->stream2, hipStreamNonBlocking));
+      F->xev.resize(P->nevents);
+      for (auto& e : F->xev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      hipEvent_t e0, e1;
+      HIPCHECK(hipEventCreate(&e0));
+      HIPCHECK(hipEventCreate(&e1));
+      HIPCHECK(hipEventRecord(e0, F->stream));
+      if (low)
+         run_factor_program<float>(P, F, F->stream, nullptr, true);
+      else
+         run_factor_program<double>(P, F, F->stream, nullptr, true);
+      HIPCHECK(hipEventRecord(e1, F->stream));
+      HIPCHECK(hipStreamSynchronize(F->stream));
+      float t = 0;
+      HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+      g_last_batched_ms = t;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      HIPCHECK(hipMemcpy(info, F->d_info, nfronts * sizeof(int), hipMemcpyDeviceToDevice));
+      F->arena = nullptr;
+      destroy_factors(F);
+      destroy_plan(P);
+      return 0;
+   } catch (const HipError&) {
+      if (F) F->arena = nullptr;
       destroy_factors(F);
       destroy_plan(P);
       return SYLVER_ERROR_CUDA_UNKNOWN;

@@ -409,6 +409,16 @@ int sylver_amd_fkeep_node_assembled(void const* fkeep, int node, double* front) 
    return factors_node_assembled(fk->f, node, front);
 }
 
+int sylver_amd_batched_front_potrf_v(int nfronts, int const* nrow, int const* ncol, long const* offset,
+                                     void* data, int dtype, signed char const* variant, int* info) {
+   try {
+      return batched_front_potrf_v(nfronts, nrow, ncol, reinterpret_cast<const int64_t*>(offset), data, dtype,
+                                   variant, info);
+   } catch (...) {
+      return SYLVER_ERROR_UNKNOWN;
+   }
+}
+
 int sylver_amd_set_front_variants(void* akeep, signed char const* variant, int nnodes) {
    Akeep* ak = as_akeep(akeep);
    if (!ak) return SYLVER_ERROR_CALL_SEQUENCE;

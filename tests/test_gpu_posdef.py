@@ -173,10 +173,13 @@ def test_refactorize_deterministic(S, gpu):
 
 
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
-def test_batched_dense_fronts(S, O, gpu, dtype):
+@pytest.mark.parametrize("path", ["one_launch", "plan_default", "plan_tiled", "plan_one_wg"])
+def test_batched_dense_fronts(S, O, gpu, dtype, path):
     """Slice of the batched small-front workload: fronts with ncol
     log-uniform in [8,256], nrow = ncol*(1+r), r in [0,2]; one
-    variable-size launch; every front checked against the oracle."""
+    variable-size launch, or the launch plan with per-front variants (the
+    default rule, every front tiled, every front on one workgroup); every
+    front (pivot columns and contribution block) checked against the oracle."""
     nf = 48
     nrow, ncol = M.dense_front_sizes(nf, seed=3)
     offs = np.zeros(nf, np.int64)
@@ -189,7 +192,11 @@ def test_batched_dense_fronts(S, O, gpu, dtype):
         tot += int(nrow[f]) ** 2
     host = np.concatenate([a.ravel(order="F") for a in fronts]).astype(dtype)
     dev = S.DeviceArray(host)
-    rc, info, _ = S.batched_front_potrf(dev, nrow, ncol, offs, dtype)
+    if path == "one_launch":
+        rc, info, _ = S.batched_front_potrf(dev, nrow, ncol, offs, dtype)
+    else:
+        v = {"plan_default": None, "plan_tiled": np.zeros(nf, np.int8), "plan_one_wg": np.ones(nf, np.int8)}[path]
+        rc, info, _ = S.batched_front_potrf_v(dev, nrow, ncol, offs, dtype, v)
     assert rc == 0
     assert int(np.abs(info).sum()) == 0
     out = dev.numpy().astype(np.float64)
