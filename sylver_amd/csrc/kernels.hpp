@@ -1028,6 +1028,16 @@ static __global__ __launch_bounds__(kThreads) void k_bwd_gather(const SolveTask*
    }
 }
 
+// k index of MFMA step u for lane group kq in the backward solve updates:
+// groups of VK = 16 / sizeof(T) consecutive k per lane, so a lane's operand
+// for VK steps is one contiguous 16-byte piece of a column of L (the
+// forward updates keep k = 4u + kq, contiguous across the rows instead).
+template <typename T>
+__device__ __forceinline__ int bwd_k(int u, int kq) {
+   constexpr int VK = 16 / (int)sizeof(T);
+   return (u / VK) * 4 * VK + kq * VK + (u % VK);
+}
+
 // One 128-column step of the forward (BWD = false) or backward (BWD = true)
 // sweep. Every task needs y = M x_k with M = inv(L_kk) (forward) or
 // inv(L_kk)^T (backward, from dinvT), both stored full kb x kb with zeros in
@@ -1067,7 +1077,7 @@ __device__ __forceinline__ void solve_step_dev(const SolveTask& t, const NodeDes
    if (!t.write) {
 #pragma unroll
       for (int u = 0; u < KU; ++u) {
-         const int k = 4 * u + kq;
+         const int k = BWD ? bwd_k<T>(u, kq) : 4 * u + kq;
          const T* p = BWD ? F + t.k0 + k + (int64_t)ui * nd.ld : F + ui + (int64_t)(t.k0 + k) * nd.ld;
          av[u] = *live_or_zero(ui < t.r1 && k < kb, p);
       }
@@ -1125,7 +1135,8 @@ __device__ __forceinline__ void solve_step_dev(const SolveTask& t, const NodeDes
          v4f acc = v4f{0, 0, 0, 0};
 #pragma unroll
          for (int u = 0; u < KU; ++u)
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], (float)ys[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                av[u], (float)ys[(BWD ? bwd_k<T>(u, kq) : 4 * u + kq) * 16 + r], acc, 0, 0, 0);
 #pragma unroll
          for (int v = 0; v < 4; ++v) {
             const int o = ob + Mfma<float>::row(lane, v);
@@ -1135,7 +1146,8 @@ __device__ __forceinline__ void solve_step_dev(const SolveTask& t, const NodeDes
          v4d acc = v4d{0, 0, 0, 0};
 #pragma unroll
          for (int u = 0; u < KU; ++u)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u], ys[(4 * u + kq) * 16 + r], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                (double)av[u], ys[(BWD ? bwd_k<T>(u, kq) : 4 * u + kq) * 16 + r], acc, 0, 0, 0);
 #pragma unroll
          for (int v = 0; v < 4; ++v) {
             const int o = ob + Mfma<double>::row(lane, v);
@@ -1177,6 +1189,223 @@ __global__ __launch_bounds__(kThreads) void k_solve_step1(const NodeDesc nd, int
       t.r1 = min(limit, t.r0 + chunk);
    }
    solve_step_dev<T, BWD>(t, nd, arena, dinvs, rlist, xloc, xsol, ldsol, nrhs);
+}
+
+// ---------------------------------------------------------------------------
+// Level-wide solves: one launch per level and sweep for the fronts whose
+// steps would otherwise take one launch per 128-column block. Workgroup =
+// one block of at most 128 rows of one front; it keeps its rows of x in
+// registers (FP64, in the MFMA output layout), applies the finished blocks
+// of y as their flags appear and, on a diagonal block, multiplies by the
+// stored inverse and publishes its block of y with a flag.
+//   Forward, diagonal block j (rows [128j, min(n, 128j + 128))):
+//     x_j -= L_jk y_k for k = 0 .. j-1 (ascending), y_j = inv(L_jj) x_j.
+//   Forward, contribution block (rows at or below n): x -= L_rk y_k for every
+//     k, stored back (the parent's gather reads it).
+//   Backward, diagonal block j (after the contribution-block product):
+//     x_j -= L_ij^T x_i for i = nb-1 .. j+1 (descending), x_j = inv(L_jj)^T x_j.
+// Tasks of a launch are ordered so that every flag a workgroup waits for is
+// set by a workgroup of lower index in the same launch (the host checks
+// this when it builds the plan): workgroups are dispatched in index order,
+// so the lowest-index unfinished workgroup is always resident or next to be
+// dispatched, and it waits for nothing unfinished -- the launch cannot
+// deadlock. Waits are bounded all the same: a wait that exceeds
+// kChainMaxPolls polls (or sees another wait fail) counts an error in `err`
+// and goes on, so every workgroup ends; the host reports the error.
+// Publication: y is written with device-scope (write-through) atomic
+// stores, every thread waits for its stores to complete, barrier, then
+// thread 0 stores the flag; a waiter polls the flag with device-scope
+// atomic loads and reads y with device-scope atomic loads (served from the
+// coherence point, never from a stale cache line), so no cache write-back
+// or invalidate is needed. Flags hold the number of the 16-column
+// right-hand-side chunk they are final for, and are cleared by a fill
+// kernel at the start of each sweep. Launched with no flags, the kernel
+// waits for nothing: the forward contribution blocks run that way, in a
+// launch after their level's chained launch.
+// ---------------------------------------------------------------------------
+struct ChainTask {
+   int node;
+   int blk;    // diagonal block index (< nb) or nb for a contribution block
+   int nb;     // diagonal blocks of the front
+   int r0, r1; // rows of the front this workgroup owns (at most kNB)
+   int flag0;  // index of the front's first flag
+};
+
+constexpr int kChainMaxPolls = 1 << 20;
+
+// Thread 0: waits until *flag >= target.
+__device__ inline void chain_wait(const int* flag, int target, int* err) {
+   for (int s = 0;; ++s) {
+      if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+      if (s >= kChainMaxPolls ||
+          ((s & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+         __hip_atomic_fetch_add(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+         break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+   }
+}
+
+constexpr int kChainThreads = 512;  // 8 waves: one 16-row tile of the block each
+
+// Raw buffer loads: byte offset from `base`; an offset at or beyond `bytes`
+// reads zero, which serves as the mask of the operand loads below (one
+// 32-bit offset per lane instead of a 64-bit address per element).
+template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const T* base, uint32_t bytes) {
+   return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float*) {
+   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ inline double buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double*) {
+   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
+template <typename T, bool BWD>
+__global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __restrict__ tasks,
+                                                         const NodeDesc* __restrict__ nodes,
+                                                         const T* __restrict__ arena, const T* __restrict__ dinvs,
+                                                         const int* __restrict__ rlist, double* __restrict__ xloc,
+                                                         double* __restrict__ xsol, int ldsol, int nrhs,
+                                                         int* __restrict__ flags, int* __restrict__ err) {
+   constexpr int KU = kNB / 4;                         // k-steps of the 16x16x4 MFMA over 128
+   constexpr int YU = kNB * 16 / kChainThreads;        // staged y values per thread
+   constexpr uint32_t ES = sizeof(T);
+   typedef typename Mfma<T>::acc_t acc_t;
+   __shared__ T ys[kNB * 16];  // staged y (rounded to T, as the products use it)
+   const ChainTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
+   const int m = nd.m, n = nd.n, ld = nd.ld, rows = t.r1 - t.r0;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int r = lane & 15, kq = lane >> 4;
+   const int i = wave * 16 + r;  // block row of this lane's operand loads
+   double* xl = xloc + nd.solve_off * nrhs;
+   int* fl = flags ? flags + t.flag0 : nullptr;
+   const bool diag = t.blk < t.nb;
+   const int ntile = BWD ? t.nb - 1 - t.blk : min(t.blk, t.nb);
+   // the factor columns of the front, [0, n) (the host keeps n * ld * 8 below
+   // 2^31); lanes past the block's rows read from the end (zero)
+   const uint32_t fbytes = (uint32_t)n * (uint32_t)ld * ES;
+   const __amdgpu_buffer_rsrc_t frs = buf_rsrc(arena + nd.off, fbytes);
+   // rows of inv(L_jj) for the diagonal step: loaded up front in FP32; in
+   // FP64 (twice the registers) after the tile loop
+   T mv[KU];
+   auto load_inv = [&]() {
+      const uint32_t mbytes = (uint32_t)(rows * rows) * ES;
+      const __amdgpu_buffer_rsrc_t mrs = buf_rsrc(dinvs + nd.dinv_off + (int64_t)t.blk * kNB * kNB, mbytes);
+      const uint32_t lo = i < rows ? (uint32_t)(i + kq * rows) * ES : mbytes;
+#pragma unroll
+      for (int u = 0; u < KU; ++u) mv[u] = buf_load(mrs, lo + (uint32_t)(4 * u * rows) * ES, (T*)nullptr);
+   };
+   constexpr bool kInvEarly = sizeof(T) == 4;
+   if (kInvEarly && diag) load_inv();
+   int sidx[4] = {0, 0, 0, 0};  // solution rows of this lane's outputs (diagonal blocks)
+   if (diag)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int o = wave * 16 + Mfma<T>::row(lane, v);
+         if (o < rows) sidx[v] = rlist[nd.rlist_off + t.r0 + o] - 1;
+      }
+   // tile s: forward A[i][k] = L[r0 + i][kb0 + k], backward A[i][k] = L[kb0 + k][r0 + i];
+   // columns at or beyond n read zero (forward); backward rows at or beyond n
+   // read finite entries of L that meet zero rows of y
+   T av[KU];
+   auto load_tile = [&](int s) {
+      const int kb0 = (BWD ? t.nb - 1 - s : s) * kNB;
+      if constexpr (BWD) {
+         // k = bwd_k(u, kq): 16 contiguous bytes of a column of L per load
+         constexpr int VK = 16 / (int)sizeof(T);
+         uint32_t lo = (uint32_t)(kb0 + kq * VK + (t.r0 + i) * ld) * ES;
+         if (i >= rows) lo = fbytes;
+#pragma unroll
+         for (int g = 0; g < KU / VK; ++g) {
+            const auto raw = __builtin_amdgcn_raw_buffer_load_b128(frs, (int)(lo + (uint32_t)(g * 4 * VK) * ES), 0, 0);
+            T piece[VK];
+            __builtin_memcpy(piece, &raw, 16);
+#pragma unroll
+            for (int e = 0; e < VK; ++e) av[g * VK + e] = piece[e];
+         }
+      } else {
+         uint32_t lo = (uint32_t)(t.r0 + i + (kb0 + kq) * ld) * ES;
+         const uint32_t step = (uint32_t)(4 * ld) * ES;
+         if (i >= rows) lo = fbytes;
+#pragma unroll
+         for (int u = 0; u < KU; ++u) av[u] = buf_load(frs, lo + (uint32_t)u * step, (T*)nullptr);
+      }
+   };
+   for (int c0 = 0, ci = 1; c0 < nrhs; c0 += 16, ++ci) {
+      const int nc = min(16, nrhs - c0);
+      double xv[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int o = wave * 16 + Mfma<T>::row(lane, v);
+         xv[v] = *live_or_zero(o < rows && r < nc, xl + t.r0 + o + (int64_t)(c0 + r) * m);
+      }
+      if (ntile > 0) load_tile(0);
+      for (int s = 0; s < ntile; ++s) {
+         const int kblk = BWD ? t.nb - 1 - s : s, kb0 = kblk * kNB, kbk = min(kNB, n - kb0);
+         if (flags) {
+            if (threadIdx.x == 0) chain_wait(fl + kblk, ci, err);
+            __syncthreads();
+         }
+         double yv[YU];  // all loads in flight together
+#pragma unroll
+         for (int u = 0; u < YU; ++u) {
+            const int e = threadIdx.x + u * kChainThreads, l = e >> 4, c = e & 15;
+            double* p = xl + kb0 + l + (int64_t)(c0 + c) * m;
+            yv[u] = 0.0;
+            if (l < kbk && c < nc) yv[u] = flags ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+         }
+#pragma unroll
+         for (int u = 0; u < YU; ++u) ys[threadIdx.x + u * kChainThreads] = (T)yv[u];
+         __syncthreads();
+         acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+         for (int u = 0; u < KU; ++u)
+            acc = Mfma<T>::op(av[u], ys[(BWD ? bwd_k<T>(u, kq) : 4 * u + kq) * 16 + r], acc);
+#pragma unroll
+         for (int v = 0; v < 4; ++v) xv[v] -= (double)acc[v];
+         if (s + 1 < ntile) load_tile(s + 1);
+      }
+      if (!diag) {
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int o = wave * 16 + Mfma<T>::row(lane, v);
+            if (o < rows && r < nc) xl[t.r0 + o + (int64_t)(c0 + r) * m] = xv[v];
+         }
+         continue;
+      }
+      if (!kInvEarly) load_inv();
+      __syncthreads();  // every wave is done with ys
+#pragma unroll
+      for (int v = 0; v < 4; ++v) ys[(wave * 16 + Mfma<T>::row(lane, v)) * 16 + r] = (T)xv[v];
+      __syncthreads();
+      acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int u = 0; u < KU; ++u) acc = Mfma<T>::op(mv[u], ys[(4 * u + kq) * 16 + r], acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int o = wave * 16 + Mfma<T>::row(lane, v);
+         if (o < rows && r < nc)
+            __hip_atomic_store(xl + t.r0 + o + (int64_t)(c0 + r) * m, (double)acc[v], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __builtin_amdgcn_s_waitcnt(0);  // this thread's stores of y are complete
+      __syncthreads();
+      if (threadIdx.x == 0) __hip_atomic_store(fl + t.blk, ci, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // the solution vector (read by the next level's gathers, after this launch)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int o = wave * 16 + Mfma<T>::row(lane, v);
+         if (o < rows && r < nc) xsol[sidx[v] + (int64_t)(c0 + r) * ldsol] = (double)acc[v];
+      }
+   }
+}
+
+static __global__ void k_fill_i32(int* p, int n, int v) {
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) p[i] = v;
 }
 
 // acc += A(16 rows x K) B(K x 16) for small fronts: A from global memory

@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -440,6 +441,26 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
          if (P->stask[t].node != node) return;
       l.node1 = node;
    };
+   // Level-wide chained solves (k_chain) unless SYLVER_AMD_SOLVE_STEPS=1
+   // asks for one launch per 128-column step. flag0[i]: the first flag of
+   // front i in the chained sweeps (-1: the front takes the step launches;
+   // so do fronts whose factor columns span 2^31 bytes or more in FP64, as
+   // the chained kernel addresses them with 32-bit offsets).
+   const char* steps_env = std::getenv("SYLVER_AMD_SOLVE_STEPS");
+   const bool chain = !(steps_env && steps_env[0] == '1');
+   std::vector<int> flag0(P->hnodes.size(), -1);
+   P->chain_flags = 0;
+   if (chain)
+      for (size_t i = 0; i < P->hnodes.size(); ++i) {
+         const NodeDesc& d = P->hnodes[i];
+         if (d.n == 0 || small_solve(d) || (int64_t)d.n * d.ld * 8 >= (int64_t(1) << 31)) continue;
+         flag0[i] = (int)P->chain_flags;
+         P->chain_flags += (d.n + kNB - 1) / kNB;
+      }
+   if (chain) {
+      push(P->fwdprog, L_CLEAR, 0, P->chain_flags);
+      push(P->bwdprog, L_CLEAR, P->chain_flags, P->chain_flags);
+   }
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
       int64_t g0 = P->stask.size();
@@ -468,8 +489,33 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             if (small_solve(P->hnodes[i])) P->nlistv.push_back(i);
          push(P->fwdprog, L_FSMALL, s0, P->nlistv.size() - s0);
       }
+      if (chain) {
+         // one launch for the level's diagonal blocks (a chain per front,
+         // block order), then one for the contribution blocks, which need
+         // the whole of y and so wait for nothing in their own launch
+         const int64_t q0 = P->ctask.size();
+         for (int i : lv) {
+            const NodeDesc& d = P->hnodes[i];
+            if (flag0[i] < 0) continue;
+            const int nb = (d.n + kNB - 1) / kNB;
+            for (int j = 0; j < nb; ++j)
+               P->ctask.push_back(ChainTask{i, j, nb, j * kNB, std::min(d.n, (j + 1) * kNB), flag0[i]});
+         }
+         push(P->fwdprog, L_FCHAIN, q0, P->ctask.size() - q0);
+         const int64_t q1 = P->ctask.size();
+         for (int i : lv) {
+            const NodeDesc& d = P->hnodes[i];
+            if (flag0[i] < 0) continue;
+            const int nb = (d.n + kNB - 1) / kNB;
+            for (int r = d.n; r < d.m; r += kNB)
+               P->ctask.push_back(ChainTask{i, nb, nb, r, std::min(d.m, r + kNB), flag0[i]});
+         }
+         push(P->fwdprog, L_FCB, q1, P->ctask.size() - q1);
+      }
+      // fronts outside the chained launch: one launch per diagonal block
       int maxblk = 0;
-      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
+      for (int i : lv)
+         if (flag0[i] < 0) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
       // one launch per diagonal block: the diagonal solve (write task) and
       // the update of the rows below, every task forming y = inv(L_kk) x_k
       for (int sblk = 0; sblk < maxblk; ++sblk) {
@@ -479,7 +525,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             const int k0 = sblk * kNB;
             if (k0 >= d.n) continue;
             const int kb = std::min(kNB, d.n - k0);
-            if (small_solve(d)) continue;  // handled by the level's small-front launch
+            if (small_solve(d) || flag0[i] >= 0) continue;  // small-front or chained launch
             P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
             for (int r = k0 + kb; r < d.m; r += kUpdRows)
                P->stask.push_back(SolveTask{i, k0, kb, r, std::min(d.m, r + kUpdRows), 0});
@@ -524,8 +570,21 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       int64_t r0 = P->stask.size();
       P->stask.insert(P->stask.end(), red.begin(), red.end());
       push(P->bwdprog, L_BRED, r0, P->stask.size() - r0);
+      if (chain) {
+         // one launch for the level: per front its diagonal blocks from the last
+         const int64_t q0 = P->ctask.size();
+         for (int i : lv) {
+            const NodeDesc& d = P->hnodes[i];
+            if (flag0[i] < 0) continue;
+            const int nb = (d.n + kNB - 1) / kNB;
+            for (int j = nb - 1; j >= 0; --j)
+               P->ctask.push_back(ChainTask{i, j, nb, j * kNB, std::min(d.n, (j + 1) * kNB), flag0[i]});
+         }
+         push(P->bwdprog, L_BCHAIN, q0, P->ctask.size() - q0);
+      }
       int maxblk = 0;
-      for (int i : lv) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
+      for (int i : lv)
+         if (flag0[i] < 0) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
       // one launch per diagonal block (from the last): the transposed
       // diagonal solve (write task) and the update of the earlier columns by
       // the block's rows, every task forming x_k' = inv(L_kk)^T x_k
@@ -536,7 +595,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             const int k0 = sblk * kNB;
             if (k0 >= d.n) continue;
             const int kb = std::min(kNB, d.n - k0);
-            if (small_solve(d)) continue;
+            if (small_solve(d) || flag0[i] >= 0) continue;
             P->stask.push_back(SolveTask{i, k0, kb, 0, 0, 1});
             for (int j = 0; j < k0; j += kColChunk)
                P->stask.push_back(SolveTask{i, k0, kb, j, std::min(k0, j + kColChunk), 0});
@@ -545,6 +604,27 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
          mark_single(P->bwdprog, t0);
       }
    }
+
+   // Every flag a chained launch waits for must be set by a workgroup of
+   // lower index in the same launch (the launch's freedom from deadlock).
+   for (const std::vector<Launch>* prog : {&P->fwdprog, &P->bwdprog})
+      for (const Launch& L : *prog) {
+         if (L.kind != L_FCHAIN && L.kind != L_BCHAIN) continue;
+         const bool bwd = L.kind == L_BCHAIN;
+         std::vector<int64_t> setter((size_t)P->chain_flags, -1);
+         for (int64_t t = L.off; t < L.off + L.count; ++t) {
+            const ChainTask& c = P->ctask[t];
+            const int lo = bwd ? c.blk + 1 : 0, hi = bwd ? c.nb : std::min(c.blk, c.nb);
+            for (int k = lo; k < hi; ++k)
+               if (setter[(size_t)(c.flag0 + k)] < 0)
+                  throw std::runtime_error("chained solve plan: a wait precedes its flag");
+            if (c.blk < c.nb) {
+               if (setter[(size_t)(c.flag0 + c.blk)] >= 0)
+                  throw std::runtime_error("chained solve plan: a flag is set twice");
+               setter[(size_t)(c.flag0 + c.blk)] = t;
+            }
+         }
+      }
 
    // ---- CSR of the full matrix in pivot order ----
    // (the internal seam passes no matrix pattern: values are then addressed
@@ -605,6 +685,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    P->d_diag = dupload(P->diag);
    P->d_gemm = dupload(P->gemm);
    P->d_stask = dupload(P->stask);
+   P->d_ctask = dupload(P->ctask);
    return P;
 }
 
@@ -723,6 +804,22 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
          case L_BRED:
             if (L.count > 0) k_bwd_cbr<<<g, bl, 0, s>>>(tk, P->d_nodes, F->d_part_cb, F->d_xloc, nrhs);
             break;
+         case L_CLEAR:
+            k_fill_i32<<<dim3((L.count + kThreads - 1) / kThreads), bl, 0, s>>>(F->d_chain + 1 + L.off, L.count, 0);
+            break;
+         case L_FCHAIN:
+            k_chain<T, false><<<g, dim3(kChainThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena, dinv, P->d_rlist, F->d_xloc,
+                                               xsol, n, nrhs, F->d_chain + 1, F->d_chain);
+            break;
+         case L_FCB:
+            k_chain<T, false><<<g, dim3(kChainThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena, dinv,
+                                                                P->d_rlist, F->d_xloc, xsol, n, nrhs, nullptr,
+                                                                F->d_chain);
+            break;
+         case L_BCHAIN:
+            k_chain<T, true><<<g, dim3(kChainThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena, dinvT, P->d_rlist, F->d_xloc,
+                                              xsol, n, nrhs, F->d_chain + 1 + P->chain_flags, F->d_chain);
+            break;
          default:
             break;
       }
@@ -738,6 +835,10 @@ void ensure_solve_ws(const DevicePlan* P, DeviceFactors* F, int nrhs) {
       dfree(p);
    F->d_xloc = dalloc<double>((size_t)P->solve_rows * nrhs);
    F->d_part_cb = dalloc<double>((size_t)std::max<int64_t>(P->cb_slots, 1) * 64 * nrhs);
+   if (!F->d_chain) {
+      F->d_chain = dalloc<int>((size_t)(1 + 2 * P->chain_flags));
+      HIPCHECK(hipMemset(F->d_chain, 0, (size_t)(1 + 2 * P->chain_flags) * sizeof(int)));
+   }
    F->d_b = dalloc<double>((size_t)P->n * nrhs);
    F->d_x = dalloc<double>((size_t)P->n * nrhs);
    F->d_r = dalloc<double>((size_t)P->n * nrhs);
@@ -1175,6 +1276,17 @@ SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int
       HIPCHECK(hipEventElapsedTime(&t, e0, e1));
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
+      if (F->d_chain) {
+         // waits of the chained solves that gave up (never expected: the
+         // launches cannot deadlock); the result is then not to be used
+         int werr = 0;
+         HIPCHECK(hipMemcpy(&werr, F->d_chain, sizeof(int), hipMemcpyDeviceToHost));
+         if (werr != 0) {
+            HIPCHECK(hipMemset(F->d_chain, 0, sizeof(int)));
+            res.flag = SYLVER_ERROR_CUDA_UNKNOWN;
+            res.hip_error = (int)hipErrorLaunchTimeOut;
+         }
+      }
       res.solve_ms = t;
       F->solve_ms = t;
       F->refine_iters = res.refine_iters;

@@ -64,6 +64,10 @@ enum LaunchKind {
    L_BRED,
    L_FSMALL,
    L_BSMALL,
+   L_FCHAIN,  // level-wide forward solve of the level's larger fronts (k_chain)
+   L_BCHAIN,
+   L_FCB,     // forward contribution blocks of the chained fronts (k_chain, no waits)
+   L_CLEAR,   // flag clear at the start of a sweep
    L_SYNC  // no kernel: only the stream wait / event record of the entry
 };
 
@@ -123,6 +127,8 @@ struct DevicePlan {
    std::vector<GemmTask> gemm;
    std::vector<SolveTask> stask;
    std::vector<int4> gch;  // forward gather: {child, i0, i1} ranges per task
+   std::vector<ChainTask> ctask;
+   int64_t chain_flags = 0;  // diagonal blocks of the fronts solved by k_chain
    ColTask* d_zero = nullptr;
    InitTask* d_init = nullptr;
    EaTask* d_ea = nullptr;
@@ -131,6 +137,7 @@ struct DevicePlan {
    GemmTask* d_gemm = nullptr;
    SolveTask* d_stask = nullptr;
    int4* d_gch = nullptr;
+   ChainTask* d_ctask = nullptr;
    std::vector<Launch> fprog, fwdprog, bwdprog;
    int64_t num_flops = 0;
    int64_t cb_slots = 0;  // partial-sum slots of the largest backward CB launch
@@ -146,7 +153,7 @@ struct DevicePlan {
       for (void* p : {(void*)d_sptr, (void*)d_sparent, (void*)d_rptr, (void*)d_rlist,
                       (void*)d_nptr, (void*)d_nlist, (void*)d_cmap, (void*)d_cmap_off,
                       (void*)d_nodes, (void*)d_zero, (void*)d_init, (void*)d_ea,
-                      (void*)d_nlistv, (void*)d_diag, (void*)d_gemm, (void*)d_stask, (void*)d_gch,
+                      (void*)d_nlistv, (void*)d_diag, (void*)d_gemm, (void*)d_stask, (void*)d_gch, (void*)d_ctask,
                       (void*)d_csr_ptr, (void*)d_csr_col, (void*)d_csr_src})
          dfree(p);
    }
@@ -177,6 +184,7 @@ struct DeviceFactors {
    double* d_r = nullptr;
    double* d_sol = nullptr;
    double* d_part_cb = nullptr;  // backward CB partial sums: cb_slots x 64 x nrhs
+   int* d_chain = nullptr;       // k_chain: [0] wait errors, then forward and backward flags
    unsigned long long* d_norms = nullptr;
    // GMRES-IR workspace (allocated on first use)
    int gm_nrhs = 0, gm_m = 0;
@@ -216,7 +224,7 @@ struct DeviceFactors {
       for (void* p : {arena, dinv, dinvT, (void*)d_aval, (void*)d_scaling, (void*)d_csr_val,
                       (void*)d_info, (void*)d_xloc, (void*)d_b, (void*)d_x, (void*)d_r,
                       (void*)d_sol, (void*)d_norms, (void*)d_V, (void*)d_W, (void*)d_tmp,
-                      (void*)d_part, (void*)d_coef})
+                      (void*)d_part, (void*)d_coef, (void*)d_chain})
          dfree(p);
       for (auto e : xev) (void)hipEventDestroy(e);
       if (stream2) (void)hipStreamDestroy(stream2);

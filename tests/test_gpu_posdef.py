@@ -47,9 +47,9 @@ def _compare_factors(S, O, sym, fkeep, A, tol):
 
 @pytest.fixture(autouse=True)
 def _fp64_defaults(S):
-    S.set_ext_options(precision=S.PREC_FP64, small_front_max=0, use_graph=1)
+    S.set_ext_options(precision=S.PREC_FP64, small_front_max=0, use_graph=1, refine=S.REFINE_AUTO)
     yield
-    S.set_ext_options(precision=S.PREC_FP64, small_front_max=0, use_graph=1)
+    S.set_ext_options(precision=S.PREC_FP64, small_front_max=0, use_graph=1, refine=S.REFINE_AUTO)
 
 
 def test_c_example_tridiagonal(S, gpu):
@@ -122,6 +122,39 @@ def test_laplacian_3d_large_front_path(S, O, gpu, small_max):
     x = b.copy(order="F")
     assert S.spldlt_solve(x, akeep, fkeep, options=opts).flag == 0
     assert O.bwd_err(A, x, b) <= BWD_TOL
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_chained_solve_matches_steps(S, O, gpu, monkeypatch, prec):
+    """The level-wide chained solve (one launch per level, flags between
+    the workgroups of a front) performs the per-step solve's arithmetic in
+    the same order: solutions bitwise equal, for two right-hand-side chunks
+    (20 columns) and the forward-only / backward-only jobs."""
+    precision = S.PREC_FP64 if prec == "fp64" else S.PREC_FP32
+    S.set_ext_options(precision=precision, small_front_max=48, refine=S.REFINE_NONE)
+    A = M.laplacian_3d(24)
+    b = M.random_rhs(A.n, 20, seed=3)
+    out = {}
+    for mode in ("steps", "chain"):
+        if mode == "steps":
+            monkeypatch.setenv("SYLVER_AMD_SOLVE_STEPS", "1")
+        else:
+            monkeypatch.delenv("SYLVER_AMD_SOLVE_STEPS", raising=False)
+        akeep, opts, sym = _analyse(S, A)
+        fkeep, inf = _factor(S, A, akeep, opts)
+        assert inf.flag == 0
+        res = []
+        for job in (0, 1, 3):
+            x = b.copy(order="F")
+            assert S.spldlt_solve(x, akeep, fkeep, job=job, options=opts).flag == 0
+            res.append(x)
+        out[mode] = res
+        S.spldlt_free_fkeep(fkeep)
+        S.spldlt_free_akeep(akeep)
+    for a, c in zip(out["steps"], out["chain"]):
+        assert np.array_equal(a, c), np.abs(a - c).max()
+    if prec == "fp64":
+        assert O.bwd_err(A, out["chain"][0], b) <= BWD_TOL
 
 
 def test_geometric_nd_order(S, O, gpu):
