@@ -1196,56 +1196,47 @@ __global__ __launch_bounds__(kThreads) void k_solve_step1(const NodeDesc nd, int
 // steps would otherwise take one launch per 128-column block. Workgroup =
 // one block of at most 128 rows of one front; it keeps its rows of x in
 // registers (FP64, in the MFMA output layout), applies the finished blocks
-// of y as their flags appear and, on a diagonal block, multiplies by the
-// stored inverse and publishes its block of y with a flag.
+// of y as they appear and, on a diagonal block, multiplies by the stored
+// inverse and publishes its block of y.
 //   Forward, diagonal block j (rows [128j, min(n, 128j + 128))):
 //     x_j -= L_jk y_k for k = 0 .. j-1 (ascending), y_j = inv(L_jj) x_j.
-//   Forward, contribution block (rows at or below n): x -= L_rk y_k for every
-//     k, stored back (the parent's gather reads it).
-//   Backward, diagonal block j (after the contribution-block product):
-//     x_j -= L_ij^T x_i for i = nb-1 .. j+1 (descending), x_j = inv(L_jj)^T x_j.
-// Tasks of a launch are ordered so that every flag a workgroup waits for is
-// set by a workgroup of lower index in the same launch (the host checks
-// this when it builds the plan): workgroups are dispatched in index order,
-// so the lowest-index unfinished workgroup is always resident or next to be
-// dispatched, and it waits for nothing unfinished -- the launch cannot
-// deadlock. Waits are bounded all the same: a wait that exceeds
+//   Forward, contribution block (rows at or below n): x -= L_rk y_k for
+//     every k, in a launch of its own after the level's chained launch
+//     (k_fcb).
+//   Backward, diagonal block j: x_j -= the partial sums of L21^T x2 (the
+//     contribution-block product, computed by the launch before, summed in
+//     slot order), x_j -= L_ij^T x_i for i = nb-1 .. j+1 (descending),
+//     x_j = inv(L_jj)^T x_j.
+// Publication: each sweep starts by filling the exchange buffer `ybuf`
+// (kNB x 16 values per diagonal block and 16-column chunk of right-hand
+// sides) with kYEmpty, a signaling-NaN bit pattern that no arithmetic
+// produces. A finished block of y is written there with device-scope
+// (write-through) atomic stores; a workgroup that needs it polls its own
+// entries with device-scope atomic loads until none holds kYEmpty, so the
+// poll and the read of y are the same loads and no flag, cache write-back
+// or invalidate is involved. Entries outside the block's rows or the
+// chunk's columns are never written and never polled.
+// Tasks of a launch are ordered so that every block a workgroup waits for
+// is produced by a workgroup of lower index in the same launch (the host
+// checks this when it builds the plan): workgroups are dispatched in index
+// order, so the lowest-index unfinished workgroup is always resident or
+// next to be dispatched, and it waits for nothing unfinished -- the launch
+// cannot deadlock. Waits are bounded all the same: a wait that exceeds
 // kChainMaxPolls polls (or sees another wait fail) counts an error in `err`
 // and goes on, so every workgroup ends; the host reports the error.
-// Publication: y is written with device-scope (write-through) atomic
-// stores, every thread waits for its stores to complete, barrier, then
-// thread 0 stores the flag; a waiter polls the flag with device-scope
-// atomic loads and reads y with device-scope atomic loads (served from the
-// coherence point, never from a stale cache line), so no cache write-back
-// or invalidate is needed. Flags hold the number of the 16-column
-// right-hand-side chunk they are final for, and are cleared by a fill
-// kernel at the start of each sweep. Launched with no flags, the kernel
-// waits for nothing: the forward contribution blocks run that way, in a
-// launch after their level's chained launch.
 // ---------------------------------------------------------------------------
 struct ChainTask {
    int node;
    int blk;    // diagonal block index (< nb) or nb for a contribution block
    int nb;     // diagonal blocks of the front
    int r0, r1; // rows of the front this workgroup owns (at most kNB)
-   int flag0;  // index of the front's first flag
+   int flag0;  // index of the front's first diagonal block in the exchange buffer
+   int cb0;    // backward: first partial-sum slot of the front's contribution-block product
+   int cbn;    //   and slots per 64-column chunk (0: no contribution block)
 };
 
 constexpr int kChainMaxPolls = 1 << 20;
-
-// Thread 0: waits until *flag >= target.
-__device__ inline void chain_wait(const int* flag, int target, int* err) {
-   for (int s = 0;; ++s) {
-      if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
-      if (s >= kChainMaxPolls ||
-          ((s & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
-         __hip_atomic_fetch_add(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-         break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-   }
-}
-
+constexpr unsigned long long kYEmpty = 0x7FF4DEADBEEF0000ull;
 constexpr int kChainThreads = 512;  // 8 waves: one 16-row tile of the block each
 
 // Raw buffer loads: byte offset from `base`; an offset at or beyond `bytes`
@@ -1268,7 +1259,8 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
                                                          const T* __restrict__ arena, const T* __restrict__ dinvs,
                                                          const int* __restrict__ rlist, double* __restrict__ xloc,
                                                          double* __restrict__ xsol, int ldsol, int nrhs,
-                                                         int* __restrict__ flags, int* __restrict__ err) {
+                                                         double* __restrict__ ybuf, const double* __restrict__ part,
+                                                         int* __restrict__ err) {
    constexpr int KU = kNB / 4;                         // k-steps of the 16x16x4 MFMA over 128
    constexpr int YU = kNB * 16 / kChainThreads;        // staged y values per thread
    constexpr uint32_t ES = sizeof(T);
@@ -1280,8 +1272,10 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    const int r = lane & 15, kq = lane >> 4;
    const int i = wave * 16 + r;  // block row of this lane's operand loads
+   const int nch = (nrhs + 15) / 16;
    double* xl = xloc + nd.solve_off * nrhs;
-   int* fl = flags ? flags + t.flag0 : nullptr;
+   // exchange slot of diagonal block b, right-hand-side chunk c
+   auto yslot = [&](int b, int c) { return ybuf + ((int64_t)(t.flag0 + b) * nch + c) * (kNB * 16); };
    const bool diag = t.blk < t.nb;
    const int ntile = BWD ? t.nb - 1 - t.blk : min(t.blk, t.nb);
    // the factor columns of the front, [0, n) (the host keeps n * ld * 8 below
@@ -1334,7 +1328,7 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
          for (int u = 0; u < KU; ++u) av[u] = buf_load(frs, lo + (uint32_t)u * step, (T*)nullptr);
       }
    };
-   for (int c0 = 0, ci = 1; c0 < nrhs; c0 += 16, ++ci) {
+   for (int c0 = 0, ci = 0; c0 < nrhs; c0 += 16, ++ci) {
       const int nc = min(16, nrhs - c0);
       double xv[4];
 #pragma unroll
@@ -1342,21 +1336,64 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
          const int o = wave * 16 + Mfma<T>::row(lane, v);
          xv[v] = *live_or_zero(o < rows && r < nc, xl + t.r0 + o + (int64_t)(c0 + r) * m);
       }
+      if (BWD && t.cbn > 0) {
+         // the contribution-block product: partial sums of 64-column chunks
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int o = wave * 16 + Mfma<T>::row(lane, v);
+            if (o < rows && r < nc) {
+               const int col = t.r0 + o;
+               const double* pp = part + ((int64_t)(t.cb0 + (col >> 6) * t.cbn) * 64 + (col & 63)) * nrhs + c0 + r;
+               const int64_t st = (int64_t)64 * nrhs;
+               double sum = 0.0;  // slots summed in order, loads eight at a time
+               int k = 0;
+               for (; k + 8 <= t.cbn; k += 8) {
+                  double pv[8];
+#pragma unroll
+                  for (int e = 0; e < 8; ++e) pv[e] = pp[(k + e) * st];
+#pragma unroll
+                  for (int e = 0; e < 8; ++e) sum += pv[e];
+               }
+               for (; k < t.cbn; ++k) sum += pp[k * st];
+               xv[v] -= sum;
+            }
+         }
+      }
       if (ntile > 0) load_tile(0);
       for (int s = 0; s < ntile; ++s) {
          const int kblk = BWD ? t.nb - 1 - s : s, kb0 = kblk * kNB, kbk = min(kNB, n - kb0);
-         if (flags) {
-            if (threadIdx.x == 0) chain_wait(fl + kblk, ci, err);
-            __syncthreads();
-         }
-         double yv[YU];  // all loads in flight together
+         double yv[YU];
+         {
+            // poll this thread's entries of block kblk until all are written
+            const double* src = yslot(kblk, ci);
+            unsigned need = 0;
 #pragma unroll
-         for (int u = 0; u < YU; ++u) {
-            const int e = threadIdx.x + u * kChainThreads, l = e >> 4, c = e & 15;
-            double* p = xl + kb0 + l + (int64_t)(c0 + c) * m;
-            yv[u] = 0.0;
-            if (l < kbk && c < nc) yv[u] = flags ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+            for (int u = 0; u < YU; ++u) {
+               const int e = threadIdx.x + u * kChainThreads, l = e >> 4, c = e & 15;
+               yv[u] = 0.0;
+               if (l < kbk && c < nc) need |= 1u << u;
+            }
+            for (int p = 0; need; ++p) {
+#pragma unroll
+               for (int u = 0; u < YU; ++u)
+                  if (need & (1u << u)) {
+                     const double v = __hip_atomic_load(src + threadIdx.x + u * kChainThreads, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+                     if (__builtin_bit_cast(unsigned long long, v) != kYEmpty) {
+                        yv[u] = v;
+                        need &= ~(1u << u);
+                     }
+                  }
+               if (!need) break;
+               if (p >= kChainMaxPolls ||
+                   ((p & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                  __hip_atomic_fetch_add(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  break;
+               }
+               __builtin_amdgcn_s_sleep(1);
+            }
          }
+         __syncthreads();  // every wave is done with the previous ys
 #pragma unroll
          for (int u = 0; u < YU; ++u) ys[threadIdx.x + u * kChainThreads] = (T)yv[u];
          __syncthreads();
@@ -1368,14 +1405,6 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
          for (int v = 0; v < 4; ++v) xv[v] -= (double)acc[v];
          if (s + 1 < ntile) load_tile(s + 1);
       }
-      if (!diag) {
-#pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int o = wave * 16 + Mfma<T>::row(lane, v);
-            if (o < rows && r < nc) xl[t.r0 + o + (int64_t)(c0 + r) * m] = xv[v];
-         }
-         continue;
-      }
       if (!kInvEarly) load_inv();
       __syncthreads();  // every wave is done with ys
 #pragma unroll
@@ -1384,28 +1413,114 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
       acc_t acc = acc_t{0, 0, 0, 0};
 #pragma unroll
       for (int u = 0; u < KU; ++u) acc = Mfma<T>::op(mv[u], ys[(4 * u + kq) * 16 + r], acc);
+      double* dst = yslot(t.blk, ci);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
          const int o = wave * 16 + Mfma<T>::row(lane, v);
          if (o < rows && r < nc)
-            __hip_atomic_store(xl + t.r0 + o + (int64_t)(c0 + r) * m, (double)acc[v], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(dst + o * 16 + r, (double)acc[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
-      __builtin_amdgcn_s_waitcnt(0);  // this thread's stores of y are complete
-      __syncthreads();
-      if (threadIdx.x == 0) __hip_atomic_store(fl + t.blk, ci, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      // the solution vector (read by the next level's gathers, after this launch)
+      // the front's solve vector and the solution (read by later launches)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
          const int o = wave * 16 + Mfma<T>::row(lane, v);
-         if (o < rows && r < nc) xsol[sidx[v] + (int64_t)(c0 + r) * ldsol] = (double)acc[v];
+         if (o < rows && r < nc) {
+            xl[t.r0 + o + (int64_t)(c0 + r) * m] = (double)acc[v];
+            xsol[sidx[v] + (int64_t)(c0 + r) * ldsol] = (double)acc[v];
+         }
       }
    }
 }
 
-static __global__ void k_fill_i32(int* p, int n, int v) {
-   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-   if (i < n) p[i] = v;
+// Forward contribution blocks of the chained fronts: x_r -= L_rk y_k over
+// all nb diagonal blocks, y read from the exchange buffer (complete after
+// the level's chained launch; contiguous per block, entries outside the
+// block's rows or the chunk's columns hold kYEmpty and are replaced by 0). Two workgroups of 4 waves per task, 64 rows
+// each (more workgroups resident than one of 8 waves); the tile and the y
+// block of step s + 1 are loaded while step s is computed. Same operand
+// layouts and k order as k_chain's forward tiles.
+constexpr int kFcbThreads = 256;
+
+template <typename T>
+__global__ __launch_bounds__(kFcbThreads) void k_fcb(const ChainTask* __restrict__ tasks,
+                                                     const NodeDesc* __restrict__ nodes,
+                                                     const T* __restrict__ arena, double* __restrict__ xloc,
+                                                     const double* __restrict__ ybuf, int nrhs) {
+   constexpr int KU = kNB / 4;
+   constexpr int YU = kNB * 16 / kFcbThreads;
+   constexpr uint32_t ES = sizeof(T);
+   typedef typename Mfma<T>::acc_t acc_t;
+   __shared__ T ys[kNB * 16];
+   const ChainTask t = tasks[blockIdx.x >> 1];
+   const NodeDesc nd = nodes[t.node];
+   const int half = (blockIdx.x & 1) * 64;
+   const int m = nd.m, n = nd.n, ld = nd.ld, nb = t.nb;
+   const int rows = min(64, t.r1 - t.r0 - half);  // rows of this workgroup (may be <= 0)
+   const int r0 = t.r0 + half;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int r = lane & 15, kq = lane >> 4;
+   const int i = wave * 16 + r;
+   if (rows <= 0) return;
+   double* xl = xloc + nd.solve_off * nrhs;
+   const int nch = (nrhs + 15) / 16;
+   const uint32_t fbytes = (uint32_t)n * (uint32_t)ld * ES;
+   const __amdgpu_buffer_rsrc_t frs = buf_rsrc(arena + nd.off, fbytes);
+   T av[2][KU];
+   double yv[2][YU];
+   int nc = 0, c0 = 0, ci = 0;
+   auto load = [&](auto slot, int s) {
+      constexpr int S = decltype(slot)::value;
+      const int kb0 = s * kNB, kbk = min(kNB, n - kb0);
+      uint32_t lo = (uint32_t)(r0 + i + (kb0 + kq) * ld) * ES;
+      const uint32_t step = (uint32_t)(4 * ld) * ES;
+      if (i >= rows) lo = fbytes;
+#pragma unroll
+      for (int u = 0; u < KU; ++u) av[S][u] = buf_load(frs, lo + (uint32_t)u * step, (T*)nullptr);
+      const double* ysrc = ybuf + ((int64_t)(t.flag0 + s) * nch + ci) * (kNB * 16);
+#pragma unroll
+      for (int u = 0; u < YU; ++u) {
+         const int e = threadIdx.x + u * kFcbThreads, l = e >> 4, c = e & 15;
+         const double v = ysrc[e];
+         yv[S][u] = (l < kbk && c < nc) ? v : 0.0;
+      }
+   };
+   double xv[4];
+   auto step = [&](auto slot, int s) {
+      constexpr int S = decltype(slot)::value;
+      if (s + 1 < nb) load(std::integral_constant<int, 1 - S>{}, s + 1);
+      __syncthreads();  // every wave is done with the previous ys
+#pragma unroll
+      for (int u = 0; u < YU; ++u) ys[threadIdx.x + u * kFcbThreads] = (T)yv[S][u];
+      __syncthreads();
+      acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int u = 0; u < KU; ++u) acc = Mfma<T>::op(av[S][u], ys[(4 * u + kq) * 16 + r], acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) xv[v] -= (double)acc[v];
+   };
+   for (c0 = 0, ci = 0; c0 < nrhs; c0 += 16, ++ci) {
+      nc = min(16, nrhs - c0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int o = wave * 16 + Mfma<T>::row(lane, v);
+         xv[v] = *live_or_zero(o < rows && r < nc, xl + r0 + o + (int64_t)(c0 + r) * m);
+      }
+      load(std::integral_constant<int, 0>{}, 0);
+      for (int s = 0; s < nb; s += 2) {
+         step(std::integral_constant<int, 0>{}, s);
+         if (s + 1 < nb) step(std::integral_constant<int, 1>{}, s + 1);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int o = wave * 16 + Mfma<T>::row(lane, v);
+         if (o < rows && r < nc) xl[r0 + o + (int64_t)(c0 + r) * m] = xv[v];
+      }
+   }
+}
+
+static __global__ void k_fill_y(unsigned long long* p, int64_t n) {
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) p[i] = kYEmpty;
 }
 
 // acc += A(16 rows x K) B(K x 16) for small fronts: A from global memory

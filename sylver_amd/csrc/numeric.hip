@@ -459,7 +459,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       }
    if (chain) {
       push(P->fwdprog, L_CLEAR, 0, P->chain_flags);
-      push(P->bwdprog, L_CLEAR, P->chain_flags, P->chain_flags);
+      push(P->bwdprog, L_CLEAR, 0, P->chain_flags);
    }
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
@@ -499,7 +499,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             if (flag0[i] < 0) continue;
             const int nb = (d.n + kNB - 1) / kNB;
             for (int j = 0; j < nb; ++j)
-               P->ctask.push_back(ChainTask{i, j, nb, j * kNB, std::min(d.n, (j + 1) * kNB), flag0[i]});
+               P->ctask.push_back(ChainTask{i, j, nb, j * kNB, std::min(d.n, (j + 1) * kNB), flag0[i], 0, 0});
          }
          push(P->fwdprog, L_FCHAIN, q0, P->ctask.size() - q0);
          const int64_t q1 = P->ctask.size();
@@ -508,7 +508,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             if (flag0[i] < 0) continue;
             const int nb = (d.n + kNB - 1) / kNB;
             for (int r = d.n; r < d.m; r += kNB)
-               P->ctask.push_back(ChainTask{i, nb, nb, r, std::min(d.m, r + kNB), flag0[i]});
+               P->ctask.push_back(ChainTask{i, nb, nb, r, std::min(d.m, r + kNB), flag0[i], 0, 0});
          }
          push(P->fwdprog, L_FCB, q1, P->ctask.size() - q1);
       }
@@ -551,18 +551,24 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       }
       // contribution-block part: columns [0, n) against rows [n, m), split
       // into row chunks (partial sums per slot) and reduced in slot order
+      // (chained fronts sum their slots at the start of their chained
+      // launch; the others take a reduction launch)
       int64_t c0 = P->stask.size();
       std::vector<SolveTask> red;
+      std::vector<int> cb0(lv.size(), 0), cbn(lv.size(), 0);
       int slots = 0;
-      for (int i : lv) {
+      for (size_t li = 0; li < lv.size(); ++li) {
+         const int i = lv[li];
          const NodeDesc& d = P->hnodes[i];
          if (d.m - d.n <= 0 || d.n == 0 || small_solve(d)) continue;
+         cb0[li] = slots;
+         cbn[li] = (d.m - d.n + kCbRowChunk - 1) / kCbRowChunk;
          for (int j = 0; j < d.n; j += kColChunk) {
             const int s0 = slots;
             for (int r = d.n; r < d.m; r += kCbRowChunk)
                P->stask.push_back(SolveTask{i, r, std::min(d.m, r + kCbRowChunk), j,
                                             std::min(d.n, j + kColChunk), slots++});
-            red.push_back(SolveTask{i, s0, slots - s0, j, std::min(d.n, j + kColChunk), 0});
+            if (flag0[i] < 0) red.push_back(SolveTask{i, s0, slots - s0, j, std::min(d.n, j + kColChunk), 0});
          }
       }
       P->cb_slots = std::max<int64_t>(P->cb_slots, slots);
@@ -573,12 +579,14 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       if (chain) {
          // one launch for the level: per front its diagonal blocks from the last
          const int64_t q0 = P->ctask.size();
-         for (int i : lv) {
+         for (size_t li = 0; li < lv.size(); ++li) {
+            const int i = lv[li];
             const NodeDesc& d = P->hnodes[i];
             if (flag0[i] < 0) continue;
             const int nb = (d.n + kNB - 1) / kNB;
             for (int j = nb - 1; j >= 0; --j)
-               P->ctask.push_back(ChainTask{i, j, nb, j * kNB, std::min(d.n, (j + 1) * kNB), flag0[i]});
+               P->ctask.push_back(ChainTask{i, j, nb, j * kNB, std::min(d.n, (j + 1) * kNB), flag0[i], cb0[li],
+                                            cbn[li]});
          }
          push(P->bwdprog, L_BCHAIN, q0, P->ctask.size() - q0);
       }
@@ -804,21 +812,25 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
          case L_BRED:
             if (L.count > 0) k_bwd_cbr<<<g, bl, 0, s>>>(tk, P->d_nodes, F->d_part_cb, F->d_xloc, nrhs);
             break;
-         case L_CLEAR:
-            k_fill_i32<<<dim3((L.count + kThreads - 1) / kThreads), bl, 0, s>>>(F->d_chain + 1 + L.off, L.count, 0);
+         case L_CLEAR: {
+            const int64_t ny = P->chain_flags * ((nrhs + 15) / 16) * kNB * 16;
+            k_fill_y<<<dim3((unsigned)((ny + kThreads - 1) / kThreads)), bl, 0, s>>>(
+                reinterpret_cast<unsigned long long*>(F->d_ybuf), ny);
             break;
+         }
          case L_FCHAIN:
-            k_chain<T, false><<<g, dim3(kChainThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena, dinv, P->d_rlist, F->d_xloc,
-                                               xsol, n, nrhs, F->d_chain + 1, F->d_chain);
+            k_chain<T, false><<<g, dim3(kChainThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena, dinv,
+                                                                P->d_rlist, F->d_xloc, xsol, n, nrhs, F->d_ybuf,
+                                                                nullptr, F->d_chain);
             break;
          case L_FCB:
-            k_chain<T, false><<<g, dim3(kChainThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena, dinv,
-                                                                P->d_rlist, F->d_xloc, xsol, n, nrhs, nullptr,
-                                                                F->d_chain);
+            k_fcb<T><<<dim3(2 * L.count), dim3(kFcbThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena,
+                                                                     F->d_xloc, F->d_ybuf, nrhs);
             break;
          case L_BCHAIN:
-            k_chain<T, true><<<g, dim3(kChainThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena, dinvT, P->d_rlist, F->d_xloc,
-                                              xsol, n, nrhs, F->d_chain + 1 + P->chain_flags, F->d_chain);
+            k_chain<T, true><<<g, dim3(kChainThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena, dinvT,
+                                                               P->d_rlist, F->d_xloc, xsol, n, nrhs, F->d_ybuf,
+                                                               F->d_part_cb, F->d_chain);
             break;
          default:
             break;
@@ -836,9 +848,11 @@ void ensure_solve_ws(const DevicePlan* P, DeviceFactors* F, int nrhs) {
    F->d_xloc = dalloc<double>((size_t)P->solve_rows * nrhs);
    F->d_part_cb = dalloc<double>((size_t)std::max<int64_t>(P->cb_slots, 1) * 64 * nrhs);
    if (!F->d_chain) {
-      F->d_chain = dalloc<int>((size_t)(1 + 2 * P->chain_flags));
-      HIPCHECK(hipMemset(F->d_chain, 0, (size_t)(1 + 2 * P->chain_flags) * sizeof(int)));
+      F->d_chain = dalloc<int>(1);
+      HIPCHECK(hipMemset(F->d_chain, 0, sizeof(int)));
    }
+   dfree(F->d_ybuf);
+   F->d_ybuf = dalloc<double>((size_t)std::max<int64_t>(P->chain_flags, 1) * ((nrhs + 15) / 16) * kNB * 16);
    F->d_b = dalloc<double>((size_t)P->n * nrhs);
    F->d_x = dalloc<double>((size_t)P->n * nrhs);
    F->d_r = dalloc<double>((size_t)P->n * nrhs);

@@ -67,7 +67,7 @@ enum LaunchKind {
    L_FCHAIN,  // level-wide forward solve of the level's larger fronts (k_chain)
    L_BCHAIN,
    L_FCB,     // forward contribution blocks of the chained fronts (k_chain, no waits)
-   L_CLEAR,   // flag clear at the start of a sweep
+   L_CLEAR,   // exchange-buffer fill at the start of a chained sweep
    L_SYNC  // no kernel: only the stream wait / event record of the entry
 };
 
@@ -128,7 +128,7 @@ struct DevicePlan {
    std::vector<SolveTask> stask;
    std::vector<int4> gch;  // forward gather: {child, i0, i1} ranges per task
    std::vector<ChainTask> ctask;
-   int64_t chain_flags = 0;  // diagonal blocks of the fronts solved by k_chain
+   int64_t chain_flags = 0;  // diagonal blocks of the fronts solved by k_chain (exchange slots)
    ColTask* d_zero = nullptr;
    InitTask* d_init = nullptr;
    EaTask* d_ea = nullptr;
@@ -184,7 +184,8 @@ struct DeviceFactors {
    double* d_r = nullptr;
    double* d_sol = nullptr;
    double* d_part_cb = nullptr;  // backward CB partial sums: cb_slots x 64 x nrhs
-   int* d_chain = nullptr;       // k_chain: [0] wait errors, then forward and backward flags
+   int* d_chain = nullptr;       // k_chain: count of waits that gave up
+   double* d_ybuf = nullptr;     // k_chain: exchange buffer of the diagonal blocks of y
    unsigned long long* d_norms = nullptr;
    // GMRES-IR workspace (allocated on first use)
    int gm_nrhs = 0, gm_m = 0;
@@ -224,7 +225,7 @@ struct DeviceFactors {
       for (void* p : {arena, dinv, dinvT, (void*)d_aval, (void*)d_scaling, (void*)d_csr_val,
                       (void*)d_info, (void*)d_xloc, (void*)d_b, (void*)d_x, (void*)d_r,
                       (void*)d_sol, (void*)d_norms, (void*)d_V, (void*)d_W, (void*)d_tmp,
-                      (void*)d_part, (void*)d_coef, (void*)d_chain})
+                      (void*)d_part, (void*)d_coef, (void*)d_chain, (void*)d_ybuf})
          dfree(p);
       for (auto e : xev) (void)hipEventDestroy(e);
       if (stream2) (void)hipStreamDestroy(stream2);
