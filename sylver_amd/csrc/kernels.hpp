@@ -1201,7 +1201,9 @@ __global__ __launch_bounds__(kThreads) void k_solve_step1(const NodeDesc nd, int
 //   Forward, diagonal block j (rows [128j, min(n, 128j + 128))):
 //     x_j -= L_jk y_k for k = 0 .. j-1 (ascending), y_j = inv(L_jj) x_j.
 //   Forward, contribution block (rows at or below n): x -= L_rk y_k for
-//     every k, in a launch of its own after the level's chained launch
+//     every k -- in the chained launch, after every diagonal block of the
+//     level, where they all fit on the device at once (their work then
+//     hides behind the chains), else in a launch of their own after it
 //     (k_fcb).
 //   Backward, diagonal block j: x_j -= the partial sums of L21^T x2 (the
 //     contribution-block product, computed by the launch before, summed in
@@ -1404,6 +1406,14 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
 #pragma unroll
          for (int v = 0; v < 4; ++v) xv[v] -= (double)acc[v];
          if (s + 1 < ntile) load_tile(s + 1);
+      }
+      if (!diag) {
+#pragma unroll
+         for (int v = 0; v < 4; ++v) {
+            const int o = wave * 16 + Mfma<T>::row(lane, v);
+            if (o < rows && r < nc) xl[t.r0 + o + (int64_t)(c0 + r) * m] = xv[v];
+         }
+         continue;
       }
       if (!kInvEarly) load_inv();
       __syncthreads();  // every wave is done with ys

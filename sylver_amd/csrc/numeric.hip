@@ -448,6 +448,12 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    // the chained kernel addresses them with 32-bit offsets).
    const char* steps_env = std::getenv("SYLVER_AMD_SOLVE_STEPS");
    const bool chain = !(steps_env && steps_env[0] == '1');
+   int num_cus = 0;
+   {
+      int dev = 0;
+      HIPCHECK(hipGetDevice(&dev));
+      HIPCHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+   }
    std::vector<int> flag0(P->hnodes.size(), -1);
    P->chain_flags = 0;
    if (chain)
@@ -491,18 +497,22 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       }
       if (chain) {
          // one launch for the level's diagonal blocks (a chain per front,
-         // block order), then one for the contribution blocks, which need
-         // the whole of y and so wait for nothing in their own launch
+         // block order); the contribution blocks follow them in the same
+         // launch when every workgroup of it can be resident at once (one
+         // per CU), else they take a launch of their own after it
          const int64_t q0 = P->ctask.size();
+         int64_t nwg = 0;
          for (int i : lv) {
             const NodeDesc& d = P->hnodes[i];
             if (flag0[i] < 0) continue;
             const int nb = (d.n + kNB - 1) / kNB;
+            nwg += nb + (d.m - d.n + kNB - 1) / kNB;
             for (int j = 0; j < nb; ++j)
                P->ctask.push_back(ChainTask{i, j, nb, j * kNB, std::min(d.n, (j + 1) * kNB), flag0[i], 0, 0});
          }
-         push(P->fwdprog, L_FCHAIN, q0, P->ctask.size() - q0);
-         const int64_t q1 = P->ctask.size();
+         const bool cb_inside = nwg <= num_cus;
+         if (!cb_inside) push(P->fwdprog, L_FCHAIN, q0, P->ctask.size() - q0);
+         const int64_t q1 = cb_inside ? q0 : (int64_t)P->ctask.size();
          for (int i : lv) {
             const NodeDesc& d = P->hnodes[i];
             if (flag0[i] < 0) continue;
@@ -510,7 +520,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             for (int r = d.n; r < d.m; r += kNB)
                P->ctask.push_back(ChainTask{i, nb, nb, r, std::min(d.m, r + kNB), flag0[i], 0, 0});
          }
-         push(P->fwdprog, L_FCB, q1, P->ctask.size() - q1);
+         push(P->fwdprog, cb_inside ? L_FCHAIN : L_FCB, q1, P->ctask.size() - q1);
       }
       // fronts outside the chained launch: one launch per diagonal block
       int maxblk = 0;

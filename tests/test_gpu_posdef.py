@@ -125,14 +125,18 @@ def test_laplacian_3d_large_front_path(S, O, gpu, small_max):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_chained_solve_matches_steps(S, O, gpu, monkeypatch, prec):
-    """The level-wide chained solve (one launch per level, flags between
-    the workgroups of a front) performs the per-step solve's arithmetic in
-    the same order: solutions bitwise equal, for two right-hand-side chunks
-    (20 columns) and the forward-only / backward-only jobs."""
+@pytest.mark.parametrize("grid", ["3d24", "3d48"])
+def test_chained_solve_matches_steps(S, O, gpu, monkeypatch, prec, grid):
+    """The level-wide chained solve (one launch per level, workgroups of a
+    front linked through the exchange buffer) performs the per-step solve's
+    arithmetic in the same order: solutions bitwise equal, for two
+    right-hand-side chunks (20 columns) and the forward-only /
+    backward-only jobs. 3D 24^3: every level's contribution blocks fit in
+    its chained launch; 3D 48^3: the wider levels (over 256 workgroups)
+    take the separate contribution-block launch."""
     precision = S.PREC_FP64 if prec == "fp64" else S.PREC_FP32
     S.set_ext_options(precision=precision, small_front_max=48, refine=S.REFINE_NONE)
-    A = M.laplacian_3d(24)
+    A = M.laplacian_3d(24) if grid == "3d24" else M.laplacian_3d(48)
     b = M.random_rhs(A.n, 20, seed=3)
     out = {}
     for mode in ("steps", "chain"):
