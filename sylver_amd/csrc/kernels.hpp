@@ -1701,34 +1701,113 @@ static __global__ __launch_bounds__(kThreads) void k_bwd_cbr(const SolveTask* __
 // ---------------------------------------------------------------------------
 // FP64 refinement helpers.
 // ---------------------------------------------------------------------------
-// r = b - A x with A in CSR (both triangles, pivot order); one row per thread.
-static __global__ __launch_bounds__(kThreads) void k_residual(int n, const int64_t* __restrict__ rowptr,
-                                                       const int* __restrict__ col,
-                                                       const double* __restrict__ val,
-                                                       const double* __restrict__ x,
-                                                       const double* __restrict__ b,
-                                                       double* __restrict__ r, int nrhs) {
-   const int i = blockIdx.x * kThreads + threadIdx.x;
-   if (i >= n) return;
-   const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
-   // 16 right-hand sides per pass: the row's entries are read once and the
-   // 16 gathers of x per entry are independent loads (columns past nrhs
-   // re-read the last one and are not stored)
-   for (int c0 = 0; c0 < nrhs; c0 += 16) {
-      const int nc = min(16, nrhs - c0);
-      double s[16];
+// Refinement step, fused: x += d (both n x nrhs, column-major), a copy of x
+// in row-major 16-column chunks (xrm[chunk][i][16], read by k_residual_rm:
+// one row's 16 values in one 128-byte line) and the workgroup's max |x| per
+// column into part[chunk][blockIdx.x][16] (reduced by k_max_parts; no
+// atomics: many workgroups contending for 16 addresses serialize).
+// A workgroup walks 64-row x 16-column tiles of one chunk (grid-stride),
+// transposed through LDS; wave w owns columns w, w+4, w+8, w+12 of the tile.
+static __global__ __launch_bounds__(kThreads) void k_axpy_rm(int n, int nrhs, const double* __restrict__ d,
+                                                      double* __restrict__ x, double* __restrict__ xrm,
+                                                      double* __restrict__ part) {
+   __shared__ double tile[64][17];
+   const int ch = blockIdx.y, c0 = ch * 16;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   double mx[4] = {0.0, 0.0, 0.0, 0.0};
+   for (int i0 = blockIdx.x * 64; i0 < n; i0 += gridDim.x * 64) {
 #pragma unroll
-      for (int c = 0; c < 16; ++c) s[c] = b[i + (int64_t)(c0 + min(c, nc - 1)) * n];
+      for (int k = 0; k < 4; ++k) {
+         const int c = wave + 4 * k, i = i0 + lane;
+         double v = 0.0;
+         if (i < n && c0 + c < nrhs) {
+            const int64_t idx = i + (int64_t)(c0 + c) * n;
+            v = x[idx] + d[idx];
+            x[idx] = v;
+         }
+         tile[lane][c] = v;
+         mx[k] = fmax(mx[k], fabs(v));
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+         const int e = threadIdx.x + k * kThreads, il = e >> 4, c = e & 15;
+         if (i0 + il < n) xrm[((int64_t)ch * n + i0 + il) * 16 + c] = tile[il][c];
+      }
+      __syncthreads();
+   }
+#pragma unroll
+   for (int k = 0; k < 4; ++k) {
+      double m = mx[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+      if (lane == 0) part[((int64_t)ch * gridDim.x + blockIdx.x) * 16 + wave + 4 * k] = m;
+   }
+}
+
+// r = b - A x with A in CSR (both triangles, pivot order) and x row-major
+// (xrm, 16-column chunks); four lanes per row, four right-hand sides each
+// (one 32-byte piece of the row's 128-byte line of x per entry), entries in
+// row order; rows grid-stride, the workgroup's max |r| per column into
+// part[chunk][blockIdx.x][16] as in k_axpy_rm.
+static __global__ __launch_bounds__(kThreads) void k_residual_rm(int n, const int64_t* __restrict__ rowptr,
+                                                          const int* __restrict__ col,
+                                                          const double* __restrict__ val,
+                                                          const double* __restrict__ xrm,
+                                                          const double* __restrict__ b, double* __restrict__ r,
+                                                          int nrhs, double* __restrict__ part) {
+   __shared__ double wmax[4][16];
+   const int q = threadIdx.x & 3, ch = blockIdx.y, cq = ch * 16 + 4 * q;
+   const double* xc = xrm + (int64_t)ch * n * 16 + 4 * q;
+   double mx[4] = {0.0, 0.0, 0.0, 0.0};
+   for (int i = (blockIdx.x * kThreads + threadIdx.x) >> 2; i < n; i += (gridDim.x * kThreads) >> 2) {
+      double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+         if (cq + e < nrhs) s[e] = b[i + (int64_t)(cq + e) * n];
+      const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
       for (int64_t p = p0; p < p1; ++p) {
          const double v = val[p];
-         const double* xj = x + col[p] + (int64_t)c0 * n;
+         const v4d xv = *reinterpret_cast<const v4d*>(xc + (int64_t)col[p] * 16);
 #pragma unroll
-         for (int c = 0; c < 16; ++c) s[c] -= v * xj[(int64_t)min(c, nc - 1) * n];
+         for (int e = 0; e < 4; ++e) s[e] -= v * xv[e];
       }
 #pragma unroll
-      for (int c = 0; c < 16; ++c)
-         if (c < nc) r[i + (int64_t)(c0 + c) * n] = s[c];
+      for (int e = 0; e < 4; ++e)
+         if (cq + e < nrhs) {
+            r[i + (int64_t)(cq + e) * n] = s[e];
+            mx[e] = fmax(mx[e], fabs(s[e]));
+         }
    }
+   // lanes of equal q hold the same four columns
+#pragma unroll
+   for (int e = 0; e < 4; ++e) {
+      double m = mx[e];
+#pragma unroll
+      for (int o = 32; o >= 4; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+      if ((threadIdx.x & 63) < 4) wmax[threadIdx.x >> 6][4 * q + e] = m;
+   }
+   __syncthreads();
+   if (threadIdx.x < 16)
+      part[((int64_t)ch * gridDim.x + blockIdx.x) * 16 + threadIdx.x] =
+          fmax(fmax(wmax[0][threadIdx.x], wmax[1][threadIdx.x]), fmax(wmax[2][threadIdx.x], wmax[3][threadIdx.x]));
+}
+
+// out[c] = max over the nparts partial maxima of column c (part[chunk][p][16],
+// c = 16 chunk + column), as the ordered bits of a non-negative double.
+// One workgroup per column.
+static __global__ __launch_bounds__(kThreads) void k_max_parts(const double* __restrict__ part, int nparts,
+                                                        int nrhs, unsigned long long* __restrict__ out) {
+   const int c = blockIdx.x, ch = c >> 4, cc = c & 15;
+   double m = 0.0;
+   for (int p = threadIdx.x; p < nparts; p += kThreads) m = fmax(m, part[((int64_t)ch * nparts + p) * 16 + cc]);
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+   __shared__ double wm[4];
+   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+   __syncthreads();
+   if (threadIdx.x == 0 && c < nrhs)
+      out[c] = (unsigned long long)__double_as_longlong(fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3])));
 }
 
 // Per-column max |v| into out[c] (as ordered bits of a non-negative double).
@@ -1758,7 +1837,7 @@ static __global__ __launch_bounds__(kThreads) void k_spmv(int n, const int64_t* 
    const int i = blockIdx.x * kThreads + threadIdx.x;
    if (i >= n) return;
    const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
-   for (int c0 = 0; c0 < nrhs; c0 += 16) {  // as k_residual
+   for (int c0 = 0; c0 < nrhs; c0 += 16) {  // 16 right-hand sides per pass
       const int nc = min(16, nrhs - c0);
       double s[16];
 #pragma unroll

@@ -849,6 +849,8 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
    HIPCHECK(hipGetLastError());
 }
 
+constexpr int kMaxParts = 1024;  // workgroups per chunk of the refinement kernels
+
 void ensure_solve_ws(const DevicePlan* P, DeviceFactors* F, int nrhs) {
    if (F->ws_nrhs >= nrhs) return;
    F->clear_solve_graphs();  // captured with the old buffers
@@ -868,6 +870,10 @@ void ensure_solve_ws(const DevicePlan* P, DeviceFactors* F, int nrhs) {
    F->d_r = dalloc<double>((size_t)P->n * nrhs);
    F->d_sol = dalloc<double>((size_t)P->n * nrhs);
    F->d_norms = dalloc<unsigned long long>(3 * (size_t)nrhs);
+   dfree(F->d_xrm);
+   F->d_xrm = dalloc<double>((size_t)P->n * 16 * ((nrhs + 15) / 16));
+   dfree(F->d_maxpart);
+   F->d_maxpart = dalloc<double>((size_t)2 * ((nrhs + 15) / 16) * kMaxParts * 16);
    F->ws_nrhs = nrhs;
 }
 
@@ -1092,16 +1098,24 @@ void solve_once(const DevicePlan* P, DeviceFactors* F, int job, const double* b,
    HIPCHECK(hipGraphLaunch(exec, s));
 }
 
-// Scaled backward error of x for the system A x = b, worst over columns.
-double device_bwd_err(const DevicePlan* P, DeviceFactors* F, const double* b, const double* x,
-                      double* r, int nrhs, hipStream_t s) {
-   const int n = P->n;
-   k_residual<<<grid1(n), kThreads, 0, s>>>(n, P->d_csr_ptr, P->d_csr_col, F->d_csr_val, x, b, r, nrhs);
-   k_fill_u64<<<1, 64, 0, s>>>(F->d_norms, 3 * nrhs, 0ull);
-   dim3 g(std::min<unsigned>(grid1(n), 256u), nrhs);
-   k_colmax<<<g, kThreads, 0, s>>>(n, r, nrhs, F->d_norms);
-   k_colmax<<<g, kThreads, 0, s>>>(n, x, nrhs, F->d_norms + nrhs);
-   k_colmax<<<g, kThreads, 0, s>>>(n, b, nrhs, F->d_norms + 2 * nrhs);
+// One refinement update, fused: x += d, r = b - A x, and the scaled
+// backward error max_c |r_c|_inf / (|A|_inf |x_c|_inf + |b_c|_inf), worst
+// over the columns (max |b| per column already in norms[2 nrhs ..], from
+// the start of the solve).
+double refine_update_err(const DevicePlan* P, DeviceFactors* F, const double* d, int nrhs, hipStream_t s) {
+   const int n = P->n, nch = (nrhs + 15) / 16;
+   // grid-stride launches of at most kMaxParts workgroups per chunk, each
+   // leaving its per-column maxima for one reduction launch
+   const unsigned ga = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, kMaxParts));
+   const unsigned gr =
+       (unsigned)std::max<int64_t>(1, std::min<int64_t>((4 * (int64_t)n + kThreads - 1) / kThreads, kMaxParts));
+   double* px = F->d_maxpart;
+   double* pr = F->d_maxpart + (int64_t)nch * kMaxParts * 16;
+   k_axpy_rm<<<dim3(ga, nch), kThreads, 0, s>>>(n, nrhs, d, F->d_x, F->d_xrm, px);
+   k_residual_rm<<<dim3(gr, nch), kThreads, 0, s>>>(n, P->d_csr_ptr, P->d_csr_col, F->d_csr_val, F->d_xrm, F->d_b,
+                                                    F->d_r, nrhs, pr);
+   k_max_parts<<<dim3(16 * nch), kThreads, 0, s>>>(pr, (int)gr, nrhs, F->d_norms);
+   k_max_parts<<<dim3(16 * nch), kThreads, 0, s>>>(px, (int)ga, nrhs, F->d_norms + nrhs);
    std::vector<unsigned long long> h(3 * nrhs);
    HIPCHECK(hipMemcpyAsync(h.data(), F->d_norms, h.size() * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, s));
@@ -1269,6 +1283,10 @@ SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int
          // step that does not halve the backward error switches the remaining
          // steps to GMRES-IR corrections.
          k_fill<<<grid1(nn), kThreads, 0, s>>>(nn, 0.0, F->d_x);
+         // max |b| per column, once
+         k_fill_u64<<<1, 64, 0, s>>>(F->d_norms + 2 * nrhs, nrhs, 0ull);
+         k_colmax<<<dim3(std::min<unsigned>(grid1(n), 256u), nrhs), kThreads, 0, s>>>(n, F->d_b, nrhs,
+                                                                                     F->d_norms + 2 * nrhs);
          k_copy<<<grid1(nn), kThreads, 0, s>>>(nn, F->d_b, F->d_r);
          double err = 1e300;
          int it = 0;
@@ -1278,8 +1296,7 @@ SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int
                gmres_correction<float>(P, F, F->d_r, F->d_sol, nrhs, 20, 1e-10, s);
             else
                solve_once<float>(P, F, 0, F->d_r, F->d_sol, nrhs, s);
-            k_axpy<<<grid1(nn), kThreads, 0, s>>>(nn, 1.0, F->d_sol, F->d_x);
-            double e = device_bwd_err(P, F, F->d_b, F->d_x, F->d_r, nrhs, s);
+            double e = refine_update_err(P, F, F->d_sol, nrhs, s);
             if (!gm && xo.refine == SYLVER_AMD_REFINE_AUTO && it > 0 && !(e < 0.5 * err)) gm = true;
             err = e;
             if (!(e > xo.refine_tol)) {

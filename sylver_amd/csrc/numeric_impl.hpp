@@ -187,6 +187,8 @@ struct DeviceFactors {
    int* d_chain = nullptr;       // k_chain: count of waits that gave up
    double* d_ybuf = nullptr;     // k_chain: exchange buffer of the diagonal blocks of y
    unsigned long long* d_norms = nullptr;
+   double* d_xrm = nullptr;      // refinement: x in row-major 16-column chunks
+   double* d_maxpart = nullptr;  // refinement: per-workgroup column maxima of x and r
    // GMRES-IR workspace (allocated on first use)
    int gm_nrhs = 0, gm_m = 0;
    double* d_V = nullptr;      // (gm_m + 1) Krylov vectors, n x nrhs each
@@ -225,7 +227,8 @@ struct DeviceFactors {
       for (void* p : {arena, dinv, dinvT, (void*)d_aval, (void*)d_scaling, (void*)d_csr_val,
                       (void*)d_info, (void*)d_xloc, (void*)d_b, (void*)d_x, (void*)d_r,
                       (void*)d_sol, (void*)d_norms, (void*)d_V, (void*)d_W, (void*)d_tmp,
-                      (void*)d_part, (void*)d_coef, (void*)d_chain, (void*)d_ybuf})
+                      (void*)d_part, (void*)d_coef, (void*)d_chain, (void*)d_ybuf,
+                      (void*)d_xrm, (void*)d_maxpart})
          dfree(p);
       for (auto e : xev) (void)hipEventDestroy(e);
       if (stream2) (void)hipStreamDestroy(stream2);
