@@ -1235,7 +1235,35 @@ struct ChainTask {
    int flag0;  // index of the front's first diagonal block in the exchange buffer
    int cb0;    // backward: first partial-sum slot of the front's contribution-block product
    int cbn;    //   and slots per 64-column chunk (0: no contribution block)
+   int g0, gn; // forward: child ranges {child, i0, i1} landing in [r0, r1) (cgch[g0 .. g0+gn))
 };
+
+// Forward gather of rows [r0, r0 + 128) of a front into xg (128 x 16, row
+// major, rows outside [lo, hi) left zero): b on the pivot rows, zero below,
+// then the children's contribution rows in list order -- the order of
+// k_fwd_gather, so the sums are the same.
+__device__ inline void chain_gather(double* xg, const ChainTask& t, const NodeDesc& nd, const NodeDesc* nodes,
+                                    const int4* cgch, const int* rlist, const int* cmap, const double* xloc,
+                                    const double* b, int ldb, int nrhs, int c0, int nc, int lo, int hi) {
+   for (int e = threadIdx.x; e < kNB * 16; e += blockDim.x) {
+      const int l = e >> 4, c = e & 15, row = t.r0 + l;
+      double v = 0.0;
+      if (l >= lo && l < hi && row < nd.n && c < nc) v = b[rlist[nd.rlist_off + row] - 1 + (int64_t)(c0 + c) * ldb];
+      xg[e] = v;
+   }
+   for (int g = t.g0; g < t.g0 + t.gn; ++g) {
+      const int4 cg = cgch[g];
+      const NodeDesc cn = nodes[cg.x];
+      const int* cm = cmap + cn.cmap_off;
+      const double* cx = xloc + cn.solve_off * nrhs + cn.n;
+      __syncthreads();
+      for (int e = threadIdx.x; e < (cg.z - cg.y) * 16; e += blockDim.x) {
+         const int i = cg.y + (e >> 4), c = e & 15, l = cm[i] - t.r0;
+         if (c < nc && l >= lo && l < hi) xg[l * 16 + c] += cx[i + (int64_t)(c0 + c) * cn.m];
+      }
+   }
+   __syncthreads();
+}
 
 constexpr int kChainMaxPolls = 1 << 20;
 constexpr unsigned long long kYEmpty = 0x7FF4DEADBEEF0000ull;
@@ -1262,12 +1290,15 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
                                                          const int* __restrict__ rlist, double* __restrict__ xloc,
                                                          double* __restrict__ xsol, int ldsol, int nrhs,
                                                          double* __restrict__ ybuf, const double* __restrict__ part,
-                                                         int* __restrict__ err) {
+                                                         int* __restrict__ err, const int4* __restrict__ cgch,
+                                                         const int* __restrict__ cmap, const double* __restrict__ b,
+                                                         int ldb) {
    constexpr int KU = kNB / 4;                         // k-steps of the 16x16x4 MFMA over 128
    constexpr int YU = kNB * 16 / kChainThreads;        // staged y values per thread
    constexpr uint32_t ES = sizeof(T);
    typedef typename Mfma<T>::acc_t acc_t;
    __shared__ T ys[kNB * 16];  // staged y (rounded to T, as the products use it)
+   __shared__ double xg[BWD ? 1 : kNB * 16];  // forward: the block's gathered x
    const ChainTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    const int m = nd.m, n = nd.n, ld = nd.ld, rows = t.r1 - t.r0;
@@ -1296,7 +1327,7 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
    };
    constexpr bool kInvEarly = sizeof(T) == 4;
    if (kInvEarly && diag) load_inv();
-   int sidx[4] = {0, 0, 0, 0};  // solution rows of this lane's outputs (diagonal blocks)
+   int sidx[4] = {0, 0, 0, 0};  // solution rows of this lane's rows (diagonal blocks)
    if (diag)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -1332,11 +1363,20 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
    };
    for (int c0 = 0, ci = 0; c0 < nrhs; c0 += 16, ++ci) {
       const int nc = min(16, nrhs - c0);
+      // the block's x: backward from the solution (the forward's y on the
+      // pivot rows), forward gathered from b and the children
       double xv[4];
+      if constexpr (BWD) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-         const int o = wave * 16 + Mfma<T>::row(lane, v);
-         xv[v] = *live_or_zero(o < rows && r < nc, xl + t.r0 + o + (int64_t)(c0 + r) * m);
+         for (int v = 0; v < 4; ++v) {
+            const int o = wave * 16 + Mfma<T>::row(lane, v);
+            xv[v] = (o < rows && r < nc) ? xsol[sidx[v] + (int64_t)(c0 + r) * ldsol] : 0.0;
+         }
+      } else {
+         if (c0 > 0) __syncthreads();  // the previous chunk is done with xg
+         chain_gather(xg, t, nd, nodes, cgch, rlist, cmap, xloc, b, ldb, nrhs, c0, nc, 0, rows);
+#pragma unroll
+         for (int v = 0; v < 4; ++v) xv[v] = xg[(wave * 16 + Mfma<T>::row(lane, v)) * 16 + r];
       }
       if (BWD && t.cbn > 0) {
          // the contribution-block product: partial sums of 64-column chunks
@@ -1455,12 +1495,16 @@ template <typename T>
 __global__ __launch_bounds__(kFcbThreads) void k_fcb(const ChainTask* __restrict__ tasks,
                                                      const NodeDesc* __restrict__ nodes,
                                                      const T* __restrict__ arena, double* __restrict__ xloc,
-                                                     const double* __restrict__ ybuf, int nrhs) {
+                                                     const double* __restrict__ ybuf, int nrhs,
+                                                     const int4* __restrict__ cgch, const int* __restrict__ rlist,
+                                                     const int* __restrict__ cmap, const double* __restrict__ b,
+                                                     int ldb) {
    constexpr int KU = kNB / 4;
    constexpr int YU = kNB * 16 / kFcbThreads;
    constexpr uint32_t ES = sizeof(T);
    typedef typename Mfma<T>::acc_t acc_t;
    __shared__ T ys[kNB * 16];
+   __shared__ double xg[kNB * 16];
    const ChainTask t = tasks[blockIdx.x >> 1];
    const NodeDesc nd = nodes[t.node];
    const int half = (blockIdx.x & 1) * 64;
@@ -1510,11 +1554,11 @@ __global__ __launch_bounds__(kFcbThreads) void k_fcb(const ChainTask* __restrict
    };
    for (c0 = 0, ci = 0; c0 < nrhs; c0 += 16, ++ci) {
       nc = min(16, nrhs - c0);
+      // this workgroup's 64 rows of the block, gathered
+      __syncthreads();
+      chain_gather(xg, t, nd, nodes, cgch, rlist, cmap, xloc, b, ldb, nrhs, c0, nc, half, half + rows);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-         const int o = wave * 16 + Mfma<T>::row(lane, v);
-         xv[v] = *live_or_zero(o < rows && r < nc, xl + r0 + o + (int64_t)(c0 + r) * m);
-      }
+      for (int v = 0; v < 4; ++v) xv[v] = xg[(half + wave * 16 + Mfma<T>::row(lane, v)) * 16 + r];
       load(std::integral_constant<int, 0>{}, 0);
       for (int s = 0; s < nb; s += 2) {
          step(std::integral_constant<int, 0>{}, s);
@@ -1571,7 +1615,9 @@ __global__ __launch_bounds__(kThreads) void k_solve_small(const int* __restrict_
                                                           const T* __restrict__ dinvs,
                                                           const int* __restrict__ rlist,
                                                           double* __restrict__ xloc,
-                                                          double* __restrict__ xsol, int ldsol, int nrhs) {
+                                                          double* __restrict__ xsol, int ldsol, int nrhs,
+                                                          const double* __restrict__ b, int ldb,
+                                                          const int* __restrict__ cmap) {
    __shared__ double xs[2 * kNB * 16];
    __shared__ double ys[kNB * 16];
    const NodeDesc nd = nodes[list[blockIdx.x]];
@@ -1584,10 +1630,31 @@ __global__ __launch_bounds__(kThreads) void k_solve_small(const int* __restrict_
    for (int c0 = 0; c0 < nrhs; c0 += 16) {
       const int nc = min(16, nrhs - c0);
       __syncthreads();
+      // the front's x, gathered here. Backward: every row from the solution.
+      // Forward: b on the pivot rows, zero below, then the contribution
+      // rows of the children added in list order.
       for (int e = threadIdx.x; e < 2 * kNB * 16; e += kThreads) {
          const int l = e >> 4, c = e & 15;
-         xs[e] = *live_or_zero(l < m && c < nc, xl + l + (int64_t)(c0 + c) * m);
+         double v = 0.0;
+         if (c < nc && l < (BWD ? m : n)) {
+            const int g = rlist[nd.rlist_off + l] - 1;
+            v = BWD ? xsol[g + (int64_t)(c0 + c) * ldsol] : b[g + (int64_t)(c0 + c) * ldb];
+         }
+         xs[e] = v;
       }
+      if (!BWD)
+         for (int ch = nd.first_child; ch != -1;) {
+            const NodeDesc cn = nodes[ch];
+            const int qc = cn.m - cn.n;
+            const int* cm = cmap + cn.cmap_off;
+            const double* cx = xloc + cn.solve_off * nrhs + cn.n;
+            __syncthreads();
+            for (int e = threadIdx.x; e < qc * 16; e += kThreads) {
+               const int i = e >> 4, c = e & 15;
+               if (c < nc) xs[cm[i] * 16 + c] += cx[i + (int64_t)(c0 + c) * cn.m];
+            }
+            ch = cn.next_child;
+         }
       __syncthreads();
       if (BWD && q > 0) {
          // x[0:n] -= L21^T x[n:m]: output tiles over the n columns
@@ -1633,24 +1700,25 @@ __global__ __launch_bounds__(kThreads) void k_solve_small(const int* __restrict_
 
 // Backward contribution-block update split over row chunks: task (node,
 // rows [k0, kb) with kb - k0 <= 128, columns [r0, r1) with r1 - r0 <= 64,
-// slot `write`) stores part[slot] = L[rows, cols]^T x[rows] (64 x nrhs)
-// without touching x; each wave computes one 16-column tile on MFMA with its
+// slot `write`) stores part[slot] = L[rows, cols]^T x[rows] (64 x nrhs),
+// x[rows] read from the solution through the row map; each wave computes
+// one 16-column tile on MFMA with its
 // operand prefetched into registers. The reduction below subtracts the
 // partial sums of a column chunk in slot order.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_bwd_cbp(const SolveTask* __restrict__ tasks,
                                                       const NodeDesc* __restrict__ nodes,
                                                       const T* __restrict__ arena,
-                                                      const double* __restrict__ xloc,
+                                                      const double* __restrict__ xsol, int ldsol,
+                                                      const int* __restrict__ rlist,
                                                       double* __restrict__ part, int nrhs) {
    constexpr int KU = kNB / 4;
    __shared__ double xs[kNB * 16];
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
-   const int m = nd.m, q = t.kb - t.k0;
+   const int q = t.kb - t.k0;
    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    const int r = lane & 15, kq = lane >> 4;
-   const double* xl = xloc + nd.solve_off * nrhs;
    const T* F = arena + nd.off;
    double* out = part + (int64_t)t.write * 64 * nrhs;
    const int j = t.r0 + wave * 16 + r;
@@ -1666,7 +1734,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_cbp(const SolveTask* __restric
 #pragma unroll
       for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
          const int l = e >> 4, c = e & 15;
-         xs[e] = *live_or_zero(l < q && c < nc, xl + t.k0 + l + (int64_t)(c0 + c) * m);
+         xs[e] = (l < q && c < nc) ? xsol[rlist[nd.rlist_off + t.k0 + l] - 1 + (int64_t)(c0 + c) * ldsol] : 0.0;
       }
       __syncthreads();
       typename Mfma<T>::acc_t acc = {0, 0, 0, 0};

@@ -433,8 +433,8 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    auto small_solve = [](const NodeDesc& d) { return d.n <= kNB && d.m <= 2 * kNB; };
    constexpr int kUpdRows = 64;   // rows per forward update task: one 16-row tile per wave
    // a step launch whose tasks all belong to one front gets it by value
-   auto mark_single = [&](std::vector<Launch>& prog, int64_t t0) {
-      if (prog.empty() || prog.back().off != t0) return;
+   auto mark_single = [&](std::vector<Launch>& prog, int kind, int64_t t0) {
+      if (prog.empty() || prog.back().kind != kind || prog.back().off != t0) return;
       Launch& l = prog.back();
       const int node = P->stask[t0].node;
       for (int64_t t = t0; t < t0 + l.count; ++t)
@@ -467,24 +467,29 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       push(P->fwdprog, L_CLEAR, 0, P->chain_flags);
       push(P->bwdprog, L_CLEAR, 0, P->chain_flags);
    }
+   // rows of the children's contribution blocks that land in [r, r1) of
+   // front i, in child list order, appended to `out` as {child, i0, i1}
+   auto child_ranges = [&](int i, int r, int r1, std::vector<int4>& out, int split) {
+      for (int c = P->hnodes[i].first_child; c != -1; c = P->hnodes[c].next_child) {
+         const NodeDesc& cd = P->hnodes[c];
+         const int q = cd.m - cd.n;
+         const int* cm = P->hcmap.data() + cd.cmap_off;
+         const int i0 = (int)(std::lower_bound(cm, cm + q, r) - cm);
+         const int i1 = (int)(std::lower_bound(cm, cm + q, r1) - cm);
+         for (int a = i0; a < i1; a += split) out.push_back(int4{c, a, std::min(i1, a + split), 0});
+      }
+   };
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
       int64_t g0 = P->stask.size();
       for (int i : lv) {
          const NodeDesc& d = P->hnodes[i];
+         // small fronts and chained fronts gather in their own kernels
+         if (small_solve(d) || flag0[i] >= 0) continue;
          for (int r = 0; r < d.m; r += kRowChunk) {
             const int r1 = std::min(d.m, r + kRowChunk);
-            // rows of each child's contribution block that land in [r, r1)
             const int gbeg = (int)P->gch.size();
-            for (int c = d.first_child; c != -1; c = P->hnodes[c].next_child) {
-               const NodeDesc& cd = P->hnodes[c];
-               const int q = cd.m - cd.n;
-               const int* cm = P->hcmap.data() + cd.cmap_off;
-               const int i0 = (int)(std::lower_bound(cm, cm + q, r) - cm);
-               const int i1 = (int)(std::lower_bound(cm, cm + q, r1) - cm);
-               for (int a = i0; a < i1; a += kThreads)
-                  P->gch.push_back(int4{c, a, std::min(i1, a + kThreads), 0});
-            }
+            child_ranges(i, r, r1, P->gch, kThreads);
             P->stask.push_back(SolveTask{i, gbeg, (int)P->gch.size() - gbeg, r, r1, 0});
          }
       }
@@ -507,8 +512,11 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             if (flag0[i] < 0) continue;
             const int nb = (d.n + kNB - 1) / kNB;
             nwg += nb + (d.m - d.n + kNB - 1) / kNB;
-            for (int j = 0; j < nb; ++j)
-               P->ctask.push_back(ChainTask{i, j, nb, j * kNB, std::min(d.n, (j + 1) * kNB), flag0[i], 0, 0});
+            for (int j = 0; j < nb; ++j) {
+               const int r0 = j * kNB, r1 = std::min(d.n, (j + 1) * kNB), gb = (int)P->cgch.size();
+               child_ranges(i, r0, r1, P->cgch, 1 << 30);
+               P->ctask.push_back(ChainTask{i, j, nb, r0, r1, flag0[i], 0, 0, gb, (int)P->cgch.size() - gb});
+            }
          }
          const bool cb_inside = nwg <= num_cus;
          if (!cb_inside) push(P->fwdprog, L_FCHAIN, q0, P->ctask.size() - q0);
@@ -518,14 +526,18 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             if (flag0[i] < 0) continue;
             const int nb = (d.n + kNB - 1) / kNB;
             for (int r = d.n; r < d.m; r += kNB)
-               P->ctask.push_back(ChainTask{i, nb, nb, r, std::min(d.m, r + kNB), flag0[i], 0, 0});
+            {
+               const int r1 = std::min(d.m, r + kNB), gb = (int)P->cgch.size();
+               child_ranges(i, r, r1, P->cgch, 1 << 30);
+               P->ctask.push_back(ChainTask{i, nb, nb, r, r1, flag0[i], 0, 0, gb, (int)P->cgch.size() - gb});
+            }
          }
          push(P->fwdprog, cb_inside ? L_FCHAIN : L_FCB, q1, P->ctask.size() - q1);
       }
       // fronts outside the chained launch: one launch per diagonal block
       int maxblk = 0;
       for (int i : lv)
-         if (flag0[i] < 0) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
+         if (flag0[i] < 0 && !small_solve(P->hnodes[i])) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
       // one launch per diagonal block: the diagonal solve (write task) and
       // the update of the rows below, every task forming y = inv(L_kk) x_k
       for (int sblk = 0; sblk < maxblk; ++sblk) {
@@ -541,7 +553,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
                P->stask.push_back(SolveTask{i, k0, kb, r, std::min(d.m, r + kUpdRows), 0});
          }
          push(P->fwdprog, L_FDIAG, t0, P->stask.size() - t0);
-         mark_single(P->fwdprog, t0);
+         mark_single(P->fwdprog, L_FDIAG, t0);
       }
    }
    for (int L = nlev - 1; L >= 0; --L) {
@@ -549,6 +561,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
       int64_t g0 = P->stask.size();
       for (int i : lv) {
          const NodeDesc& d = P->hnodes[i];
+         if (small_solve(d) || flag0[i] >= 0) continue;  // these read the solution directly
          for (int r = 0; r < d.m; r += kRowChunk)
             P->stask.push_back(SolveTask{i, 0, 0, r, std::min(d.m, r + kRowChunk), 0});
       }
@@ -596,13 +609,13 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             const int nb = (d.n + kNB - 1) / kNB;
             for (int j = nb - 1; j >= 0; --j)
                P->ctask.push_back(ChainTask{i, j, nb, j * kNB, std::min(d.n, (j + 1) * kNB), flag0[i], cb0[li],
-                                            cbn[li]});
+                                            cbn[li], 0, 0});
          }
          push(P->bwdprog, L_BCHAIN, q0, P->ctask.size() - q0);
       }
       int maxblk = 0;
       for (int i : lv)
-         if (flag0[i] < 0) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
+         if (flag0[i] < 0 && !small_solve(P->hnodes[i])) maxblk = std::max(maxblk, (P->hnodes[i].n + kNB - 1) / kNB);
       // one launch per diagonal block (from the last): the transposed
       // diagonal solve (write task) and the update of the earlier columns by
       // the block's rows, every task forming x_k' = inv(L_kk)^T x_k
@@ -619,7 +632,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
                P->stask.push_back(SolveTask{i, k0, kb, j, std::min(k0, j + kColChunk), 0});
          }
          push(P->bwdprog, L_BDIAG, t0, P->stask.size() - t0);
-         mark_single(P->bwdprog, t0);
+         mark_single(P->bwdprog, L_BDIAG, t0);
       }
    }
 
@@ -704,6 +717,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    P->d_gemm = dupload(P->gemm);
    P->d_stask = dupload(P->stask);
    P->d_ctask = dupload(P->ctask);
+   P->d_cgch = dupload(P->cgch);
    return P;
 }
 
@@ -809,15 +823,16 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
             }
             break;
          case L_BUPD:
-            if (L.count > 0) k_bwd_cbp<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, F->d_xloc, F->d_part_cb, nrhs);
+            if (L.count > 0)
+               k_bwd_cbp<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, xsol, n, P->d_rlist, F->d_part_cb, nrhs);
             break;
          case L_FSMALL:
             k_solve_small<T, false><<<g, bl, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, dinv, P->d_rlist,
-                                                     F->d_xloc, xsol, n, nrhs);
+                                                     F->d_xloc, xsol, n, nrhs, b, n, P->d_cmap);
             break;
          case L_BSMALL:
             k_solve_small<T, true><<<g, bl, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, dinvT, P->d_rlist,
-                                                    F->d_xloc, xsol, n, nrhs);
+                                                    F->d_xloc, xsol, n, nrhs, nullptr, 0, P->d_cmap);
             break;
          case L_BRED:
             if (L.count > 0) k_bwd_cbr<<<g, bl, 0, s>>>(tk, P->d_nodes, F->d_part_cb, F->d_xloc, nrhs);
@@ -831,16 +846,18 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
          case L_FCHAIN:
             k_chain<T, false><<<g, dim3(kChainThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena, dinv,
                                                                 P->d_rlist, F->d_xloc, xsol, n, nrhs, F->d_ybuf,
-                                                                nullptr, F->d_chain);
+                                                                nullptr, F->d_chain, P->d_cgch, P->d_cmap, b, n);
             break;
          case L_FCB:
             k_fcb<T><<<dim3(2 * L.count), dim3(kFcbThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena,
-                                                                     F->d_xloc, F->d_ybuf, nrhs);
+                                                                     F->d_xloc, F->d_ybuf, nrhs, P->d_cgch,
+                                                                     P->d_rlist, P->d_cmap, b, n);
             break;
          case L_BCHAIN:
             k_chain<T, true><<<g, dim3(kChainThreads), 0, s>>>(P->d_ctask + L.off, P->d_nodes, arena, dinvT,
                                                                P->d_rlist, F->d_xloc, xsol, n, nrhs, F->d_ybuf,
-                                                               F->d_part_cb, F->d_chain);
+                                                               F->d_part_cb, F->d_chain, P->d_cgch, P->d_cmap,
+                                                               nullptr, 0);
             break;
          default:
             break;

@@ -128,6 +128,7 @@ struct DevicePlan {
    std::vector<SolveTask> stask;
    std::vector<int4> gch;  // forward gather: {child, i0, i1} ranges per task
    std::vector<ChainTask> ctask;
+   std::vector<int4> cgch;   // forward gather ranges of the chained tasks
    int64_t chain_flags = 0;  // diagonal blocks of the fronts solved by k_chain (exchange slots)
    ColTask* d_zero = nullptr;
    InitTask* d_init = nullptr;
@@ -138,6 +139,7 @@ struct DevicePlan {
    SolveTask* d_stask = nullptr;
    int4* d_gch = nullptr;
    ChainTask* d_ctask = nullptr;
+   int4* d_cgch = nullptr;
    std::vector<Launch> fprog, fwdprog, bwdprog;
    int64_t num_flops = 0;
    int64_t cb_slots = 0;  // partial-sum slots of the largest backward CB launch
@@ -153,7 +155,7 @@ struct DevicePlan {
       for (void* p : {(void*)d_sptr, (void*)d_sparent, (void*)d_rptr, (void*)d_rlist,
                       (void*)d_nptr, (void*)d_nlist, (void*)d_cmap, (void*)d_cmap_off,
                       (void*)d_nodes, (void*)d_zero, (void*)d_init, (void*)d_ea,
-                      (void*)d_nlistv, (void*)d_diag, (void*)d_gemm, (void*)d_stask, (void*)d_gch, (void*)d_ctask,
+                      (void*)d_nlistv, (void*)d_diag, (void*)d_gemm, (void*)d_stask, (void*)d_gch, (void*)d_ctask, (void*)d_cgch,
                       (void*)d_csr_ptr, (void*)d_csr_col, (void*)d_csr_src})
          dfree(p);
    }
