@@ -431,6 +431,9 @@ def main():
                     help="only the per-launch timed (eager, one stream) factorization of the FP32 path")
     ap.add_argument("--variants", default="model", choices=["model", "default"],
                     help="per-front kernel variants: the learned cost model's (trained first) or the default rule")
+    ap.add_argument("--variants-file", default=None,
+                    help="per-front variants of both tiers as .npz: loaded when the file exists (no training), "
+                         "else saved there after training (profiled runs reuse the same choices)")
     ap.add_argument("--costmodel-full", action="store_true",
                     help="learned cost model on the configured stream (400 + 100 matrices, full sizes)")
     ap.add_argument("--no-secondary", action="store_true",
@@ -467,19 +470,31 @@ def main():
     # per-front kernel variants proposed by the learned cost model, trained
     # first on its own matrix stream (also the config's report below)
     cm_report, model = None, None
-    if args.variants == "model":
+    vcache = {}  # tier -> per-front variant array
+    loaded = bool(args.variants_file) and os.path.exists(args.variants_file)
+    if loaded:
+        with np.load(args.variants_file) as z:
+            vcache = {0: z["v0"].astype(np.int8), 1: z["v1"].astype(np.int8)}
+    elif args.variants == "model":
         cm_report, model = bench_costmodel(S, M, full=args.costmodel_full, return_model=True)
     from sylver_amd import costmodel as CM
     fm, fn, fc = CM.tree_fronts(S, akeep)
 
+    def variants_for(tier):
+        if tier not in vcache:
+            if model is None:
+                return None
+            band = CM.in_band(fm)
+            pick = np.zeros(len(fm), bool)
+            pick[band] = model.pick_one_wg(CM.features(fm[band], fn[band], fc[band], tier))
+            vcache[tier] = CM.choose_variants(fm, pick)
+        return vcache[tier]
+
     def use_variants(tier):
-        if model is None:
-            S.set_front_variants(akeep, None)
-            return
-        band = CM.in_band(fm)
-        pick = np.zeros(len(fm), bool)
-        pick[band] = model.pick_one_wg(CM.features(fm[band], fn[band], fc[band], tier))
-        S.set_front_variants(akeep, CM.choose_variants(fm, pick))
+        S.set_front_variants(akeep, variants_for(tier))
+
+    if args.variants_file and not loaded and model is not None:
+        np.savez(args.variants_file, v0=variants_for(0), v1=variants_for(1))
 
     if args.profile_only:  # rocprofv3 runs of the roofline figure: the same eager launches
         use_variants(1)
@@ -576,7 +591,8 @@ def main():
                    "n": A.n, "nrhs": args.nrhs, "nodes": int(inf.num_sup),
                    "maxfront": int(inf.maxfront), "flops": flops,
                    "front_variants": ("learned cost model (per front, trained in this run)" if model is not None
-                                      else "default rule (one workgroup for nrow <= 256)")},
+                                      else "learned cost model (per front, choices loaded from --variants-file)"
+                                      if loaded else "default rule (one workgroup for nrow <= 256)")},
         "mixed": {"factor_s": round(fmx_ms / 1e3, 5), "solve_ir_s": round(smx_ms / 1e3, 5),
                   "factor_gflops": round(flops / (fmx_ms * 1e6), 2),
                   "ir_iters": int(rmx[-1]["refine_iters"]), "bwd_err": errmx},
