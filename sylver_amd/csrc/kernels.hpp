@@ -422,19 +422,33 @@ __global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ 
    const int kb = t.kb;
    const int kb16 = (kb + 15) & ~15;
    T* blk = arena + nd.off + t.k0 + (int64_t)t.k0 * nd.ld;
-   // 16 loads in flight per thread per batch
-   for (int e0 = 0; e0 < kb16 * kb16; e0 += 16 * kThreads) {
-      T v[16];
+   // element e of the kb16 x kb16 block: row e % kb16, column e / kb16 (shifts
+   // for the full 128-wide block, the common case)
+   const bool full = kb16 == kNB;
+   auto split = [&](int e, int& i, int& j) {
+      if (full) {
+         i = e & (kNB - 1);
+         j = e >> 7;
+      } else {
+         i = e % kb16;
+         j = e / kb16;
+      }
+   };
+   // 32 loads in flight per thread per batch
+   for (int e0 = 0; e0 < kb16 * kb16; e0 += 32 * kThreads) {
+      T v[32];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
+      for (int u = 0; u < 32; ++u) {
          const int e = e0 + u * kThreads + threadIdx.x;
-         const int i = e % kb16, j = e / kb16;
+         int i, j;
+         split(e, i, j);
          v[u] = *live_or_zero(e < kb16 * kb16 && i < kb && j < kb && i >= j, blk + i + (int64_t)j * nd.ld);
       }
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
+      for (int u = 0; u < 32; ++u) {
          const int e = e0 + u * kThreads + threadIdx.x;
-         const int i = e % kb16, j = e / kb16;
+         int i, j;
+         split(e, i, j);
          if (e < kb16 * kb16) L[i + j * LD] = (i < kb && j < kb) ? v[u] : ((i == j) ? T(1) : T(0));
       }
    }
@@ -445,18 +459,53 @@ __global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ 
          if (threadIdx.x == 0) info[t.node] = t.k0 + bad;
          return;
       }
-      for (int e = threadIdx.x; e < kb * kb; e += kThreads) {
-         const int i = e % kb, j = e / kb;
-         if (i >= j) blk[i + (int64_t)j * nd.ld] = L[i + j * LD];
+      // 16 LDS reads, then 16 stores, per batch (a loop of single
+      // iterations waits on every LDS read before its store)
+      for (int e0 = 0; e0 < kb16 * kb16; e0 += 16 * kThreads) {
+         T v[16];
+#pragma unroll
+         for (int u = 0; u < 16; ++u) {
+            const int e = min(e0 + u * kThreads + (int)threadIdx.x, kb16 * kb16 - 1);
+            int i, j;
+            split(e, i, j);
+            v[u] = L[i + j * LD];
+         }
+#pragma unroll
+         for (int u = 0; u < 16; ++u) {
+            const int e = e0 + u * kThreads + threadIdx.x;
+            int i, j;
+            split(e, i, j);
+            if (e < kb16 * kb16 && i >= j && i < kb && j < kb) blk[i + (int64_t)j * nd.ld] = v[u];
+         }
       }
    }
    lds_trtri_blocked<T, LD>(L, Tmp, kb16);
    T* out = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
    T* outT = dinvT + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
-   for (int e = threadIdx.x; e < kb * kb; e += kThreads) {
-      const int i = e % kb, j = e / kb;
-      out[i + j * kb] = (i >= j) ? L[i + j * LD] : T(0);
-      outT[i + j * kb] = (j >= i) ? L[j + i * LD] : T(0);
+   if (kb == kNB) {
+      // row index from the low bits, stores coalesced per column (ld kb =
+      // kNB); LDS reads batched ahead of the stores as above
+      for (int e0 = 0; e0 < kNB * kNB; e0 += 8 * kThreads) {
+         T v[8], w[8];
+#pragma unroll
+         for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * kThreads + threadIdx.x, i = e & (kNB - 1), j = e >> 7;
+            v[u] = L[i + j * LD];
+            w[u] = L[j + i * LD];
+         }
+#pragma unroll
+         for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * kThreads + threadIdx.x, i = e & (kNB - 1), j = e >> 7;
+            out[e] = (i >= j) ? v[u] : T(0);
+            outT[e] = (j >= i) ? w[u] : T(0);
+         }
+      }
+   } else {
+      for (int e = threadIdx.x; e < kb * kb; e += kThreads) {
+         const int i = e % kb, j = e / kb;
+         out[i + j * kb] = (i >= j) ? L[i + j * LD] : T(0);
+         outT[i + j * kb] = (j >= i) ? L[j + i * LD] : T(0);
+      }
    }
 }
 
