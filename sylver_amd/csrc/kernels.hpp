@@ -217,7 +217,11 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
 
 // Cholesky of a 16x16 block at D (ld LD) by one wave: lane i < 16 holds row
 // i in registers; the current column j is read from lanes j..15 with
-// v_readlane, so a step waits on no memory round trip.
+// v_readlane, so a step waits on no memory round trip. Step j, rows i > j:
+// l_ij = a_ij / sqrt(d), a_ic -= (l_ij / sqrt(d)) a_cj for c > j -- one FMA
+// per element with the broadcast a_cj. Every lane updates every element:
+// the ones above the diagonal (and lanes past 16) take values that are
+// never read or stored, which saves a compare and a select per element.
 // Returns 0 or the 1-based column of the first non-positive pivot.
 template <typename T, int LD>
 __device__ int wave_potrf16(T* D) {
@@ -236,12 +240,11 @@ __device__ int wave_potrf16(T* D) {
       if (bad == 0 && !(d > T(0))) bad = j + 1;
       if (bad == 0) {
          const T inv = rsqrt_nr(d);
-         const T sd = d * inv;
-         if (lane == j) r[j] = sd;
-         if (lane > j) r[j] *= inv;
+         const T lj = r[j] * inv;  // l_ij for rows below j
+         const T t = lj * inv;
+         r[j] = (lane == j) ? d * inv : lj;
 #pragma unroll
-         for (int c = j + 1; c < 16; ++c)
-            if (lane >= c) r[c] -= r[j] * (col[c] * inv);
+         for (int c = j + 1; c < 16; ++c) r[c] = fma(-t, col[c], r[c]);
       }
    }
    if (own)
