@@ -1765,6 +1765,9 @@ __global__ __launch_bounds__(kThreads) void k_bwd_cbp(const SolveTask* __restric
                                                       const int* __restrict__ rlist,
                                                       double* __restrict__ part, int nrhs) {
    constexpr int KU = kNB / 4;
+   constexpr int VK = 16 / (int)sizeof(T);
+   constexpr uint32_t ES = sizeof(T);
+   constexpr int XU = kNB * 16 / kThreads;
    __shared__ double xs[kNB * 16];
    const SolveTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
@@ -1774,24 +1777,52 @@ __global__ __launch_bounds__(kThreads) void k_bwd_cbp(const SolveTask* __restric
    const T* F = arena + nd.off;
    double* out = part + (int64_t)t.write * 64 * nrhs;
    const int j = t.r0 + wave * 16 + r;
+   // A[jj][k] = L[k0 + k][j], k in the order bwd_k: a lane's VK consecutive
+   // k are 16 contiguous bytes of column j. Rows past the chunk read finite
+   // entries of L that meet zero rows of x.
    T av[KU];
+   if ((int64_t)nd.n * nd.ld * ES < (int64_t(1) << 31)) {
+      const uint32_t fbytes = (uint32_t)nd.n * (uint32_t)nd.ld * ES;
+      const __amdgpu_buffer_rsrc_t frs = buf_rsrc(F, fbytes);
+      const uint32_t lo = j < t.r1 ? (uint32_t)(t.k0 + kq * VK + j * nd.ld) * ES : fbytes;
 #pragma unroll
-   for (int u = 0; u < KU; ++u) {
-      const int k = 4 * u + kq;
-      av[u] = *live_or_zero(j < t.r1 && k < q, F + t.k0 + k + (int64_t)j * nd.ld);
+      for (int g = 0; g < KU / VK; ++g) {
+         const auto raw = __builtin_amdgcn_raw_buffer_load_b128(frs, (int)(lo + (uint32_t)(g * 4 * VK) * ES), 0, 0);
+         T piece[VK];
+         __builtin_memcpy(piece, &raw, 16);
+#pragma unroll
+         for (int e = 0; e < VK; ++e) av[g * VK + e] = piece[e];
+      }
+   } else {
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+         const int k = bwd_k<T>(u, kq);
+         av[u] = *live_or_zero(j < t.r1 && k < q, F + t.k0 + k + (int64_t)j * nd.ld);
+      }
    }
    for (int c0 = 0; c0 < nrhs; c0 += 16) {
       const int nc = min(16, nrhs - c0);
+      // x of the chunk's rows from the solution through the row map: all
+      // row-map loads, then all solution loads, in flight together
+      int gi[XU];
+      double xv[XU];
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+         const int l = (threadIdx.x + u * kThreads) >> 4;
+         gi[u] = l < q ? rlist[nd.rlist_off + t.k0 + l] - 1 : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+         const int c = (threadIdx.x + u * kThreads) & 15;
+         xv[u] = (gi[u] >= 0 && c < nc) ? xsol[gi[u] + (int64_t)(c0 + c) * ldsol] : 0.0;
+      }
       __syncthreads();
 #pragma unroll
-      for (int e = threadIdx.x; e < kNB * 16; e += kThreads) {
-         const int l = e >> 4, c = e & 15;
-         xs[e] = (l < q && c < nc) ? xsol[rlist[nd.rlist_off + t.k0 + l] - 1 + (int64_t)(c0 + c) * ldsol] : 0.0;
-      }
+      for (int u = 0; u < XU; ++u) xs[threadIdx.x + u * kThreads] = xv[u];
       __syncthreads();
       typename Mfma<T>::acc_t acc = {0, 0, 0, 0};
 #pragma unroll
-      for (int u = 0; u < KU; ++u) acc = Mfma<T>::op(av[u], (T)xs[(4 * u + kq) * 16 + r], acc);
+      for (int u = 0; u < KU; ++u) acc = Mfma<T>::op(av[u], (T)xs[bwd_k<T>(u, kq) * 16 + r], acc);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
          const int jj = wave * 16 + Mfma<T>::row(lane, v);
