@@ -1294,14 +1294,22 @@ struct ChainTask {
 // major, rows outside [lo, hi) left zero): b on the pivot rows, zero below,
 // then the children's contribution rows in list order -- the order of
 // k_fwd_gather, so the sums are the same.
+template <int NT>  // threads of the workgroup
 __device__ inline void chain_gather(double* xg, const ChainTask& t, const NodeDesc& nd, const NodeDesc* nodes,
                                     const int4* cgch, const int* rlist, const int* cmap, const double* xloc,
                                     const double* b, int ldb, int nrhs, int c0, int nc, int lo, int hi) {
-   for (int e = threadIdx.x; e < kNB * 16; e += blockDim.x) {
-      const int l = e >> 4, c = e & 15, row = t.r0 + l;
-      double v = 0.0;
-      if (l >= lo && l < hi && row < nd.n && c < nc) v = b[rlist[nd.rlist_off + row] - 1 + (int64_t)(c0 + c) * ldb];
-      xg[e] = v;
+   // row-map loads first, then the values, all in flight together
+   constexpr int GU = kNB * 16 / NT;
+   int gi[GU];
+#pragma unroll
+   for (int u = 0; u < GU; ++u) {
+      const int e = threadIdx.x + u * NT, l = e >> 4, c = e & 15, row = t.r0 + l;
+      gi[u] = (l >= lo && l < hi && row < nd.n && c < nc) ? rlist[nd.rlist_off + row] - 1 : -1;
+   }
+#pragma unroll
+   for (int u = 0; u < GU; ++u) {
+      const int e = threadIdx.x + u * NT, c = e & 15;
+      xg[e] = gi[u] >= 0 ? b[gi[u] + (int64_t)(c0 + c) * ldb] : 0.0;
    }
    for (int g = t.g0; g < t.g0 + t.gn; ++g) {
       const int4 cg = cgch[g];
@@ -1309,7 +1317,7 @@ __device__ inline void chain_gather(double* xg, const ChainTask& t, const NodeDe
       const int* cm = cmap + cn.cmap_off;
       const double* cx = xloc + cn.solve_off * nrhs + cn.n;
       __syncthreads();
-      for (int e = threadIdx.x; e < (cg.z - cg.y) * 16; e += blockDim.x) {
+      for (int e = threadIdx.x; e < (cg.z - cg.y) * 16; e += NT) {
          const int i = cg.y + (e >> 4), c = e & 15, l = cm[i] - t.r0;
          if (c < nc && l >= lo && l < hi) xg[l * 16 + c] += cx[i + (int64_t)(c0 + c) * cn.m];
       }
@@ -1426,7 +1434,7 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
          }
       } else {
          if (c0 > 0) __syncthreads();  // the previous chunk is done with xg
-         chain_gather(xg, t, nd, nodes, cgch, rlist, cmap, xloc, b, ldb, nrhs, c0, nc, 0, rows);
+         chain_gather<kChainThreads>(xg, t, nd, nodes, cgch, rlist, cmap, xloc, b, ldb, nrhs, c0, nc, 0, rows);
 #pragma unroll
          for (int v = 0; v < 4; ++v) xv[v] = xg[(wave * 16 + Mfma<T>::row(lane, v)) * 16 + r];
       }
@@ -1608,7 +1616,7 @@ __global__ __launch_bounds__(kFcbThreads) void k_fcb(const ChainTask* __restrict
       nc = min(16, nrhs - c0);
       // this workgroup's 64 rows of the block, gathered
       __syncthreads();
-      chain_gather(xg, t, nd, nodes, cgch, rlist, cmap, xloc, b, ldb, nrhs, c0, nc, half, half + rows);
+      chain_gather<kFcbThreads>(xg, t, nd, nodes, cgch, rlist, cmap, xloc, b, ldb, nrhs, c0, nc, half, half + rows);
 #pragma unroll
       for (int v = 0; v < 4; ++v) xv[v] = xg[(half + wave * 16 + Mfma<T>::row(lane, v)) * 16 + r];
       load(std::integral_constant<int, 0>{}, 0);
@@ -1685,14 +1693,20 @@ __global__ __launch_bounds__(kThreads) void k_solve_small(const int* __restrict_
       // the front's x, gathered here. Backward: every row from the solution.
       // Forward: b on the pivot rows, zero below, then the contribution
       // rows of the children added in list order.
-      for (int e = threadIdx.x; e < 2 * kNB * 16; e += kThreads) {
-         const int l = e >> 4, c = e & 15;
+      // all row-map loads, then all value loads, in flight together
+      constexpr int SU = 2 * kNB * 16 / kThreads;
+      int gi[SU];
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+         const int e = threadIdx.x + u * kThreads, l = e >> 4, c = e & 15;
+         gi[u] = (c < nc && l < (BWD ? m : n)) ? rlist[nd.rlist_off + l] - 1 : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+         const int c = (threadIdx.x + u * kThreads) & 15;
          double v = 0.0;
-         if (c < nc && l < (BWD ? m : n)) {
-            const int g = rlist[nd.rlist_off + l] - 1;
-            v = BWD ? xsol[g + (int64_t)(c0 + c) * ldsol] : b[g + (int64_t)(c0 + c) * ldb];
-         }
-         xs[e] = v;
+         if (gi[u] >= 0) v = BWD ? xsol[gi[u] + (int64_t)(c0 + c) * ldsol] : b[gi[u] + (int64_t)(c0 + c) * ldb];
+         xs[threadIdx.x + u * kThreads] = v;
       }
       if (!BWD)
          for (int ch = nd.first_child; ch != -1;) {
