@@ -1677,9 +1677,13 @@ __global__ __launch_bounds__(kThreads) void k_solve_small(const int* __restrict_
                                                           double* __restrict__ xloc,
                                                           double* __restrict__ xsol, int ldsol, int nrhs,
                                                           const double* __restrict__ b, int ldb,
-                                                          const int* __restrict__ cmap) {
-   __shared__ double xs[2 * kNB * 16];
-   __shared__ double ys[kNB * 16];
+                                                          const int* __restrict__ cmap, int mrows) {
+   // LDS sized by the launch to its largest front: x (mrows x 16, mrows =
+   // the largest m rounded up to 16, at most 2 kNB) then y (kNB x 16 at
+   // most, the largest n); small leaf fronts then fit many workgroups per CU
+   extern __shared__ double sdyn[];
+   double* xs = sdyn;
+   double* ys = sdyn + mrows * 16;
    const NodeDesc nd = nodes[list[blockIdx.x]];
    const int m = nd.m, n = nd.n, q = m - n;
    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1695,18 +1699,19 @@ __global__ __launch_bounds__(kThreads) void k_solve_small(const int* __restrict_
       // rows of the children added in list order.
       // all row-map loads, then all value loads, in flight together
       constexpr int SU = 2 * kNB * 16 / kThreads;
+      const int lim = mrows * 16;
       int gi[SU];
 #pragma unroll
       for (int u = 0; u < SU; ++u) {
          const int e = threadIdx.x + u * kThreads, l = e >> 4, c = e & 15;
-         gi[u] = (c < nc && l < (BWD ? m : n)) ? rlist[nd.rlist_off + l] - 1 : -1;
+         gi[u] = (e < lim && c < nc && l < (BWD ? m : n)) ? rlist[nd.rlist_off + l] - 1 : -1;
       }
 #pragma unroll
       for (int u = 0; u < SU; ++u) {
-         const int c = (threadIdx.x + u * kThreads) & 15;
+         const int e = threadIdx.x + u * kThreads, c = e & 15;
          double v = 0.0;
          if (gi[u] >= 0) v = BWD ? xsol[gi[u] + (int64_t)(c0 + c) * ldsol] : b[gi[u] + (int64_t)(c0 + c) * ldb];
-         xs[threadIdx.x + u * kThreads] = v;
+         if (e < lim) xs[e] = v;
       }
       if (!BWD)
          for (int ch = nd.first_child; ch != -1;) {

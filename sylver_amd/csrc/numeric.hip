@@ -432,6 +432,15 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    // 256 rows
    auto small_solve = [](const NodeDesc& d) { return d.n <= kNB && d.m <= 2 * kNB; };
    constexpr int kUpdRows = 64;   // rows per forward update task: one 16-row tile per wave
+   // a small-front solve launch records its largest m and n (its LDS size)
+   auto small_sizes = [&](std::vector<Launch>& prog, int kind, int64_t s0) {
+      if (prog.empty() || prog.back().kind != kind || prog.back().off != s0) return;
+      Launch& l = prog.back();
+      for (int64_t k = s0; k < s0 + l.count; ++k) {
+         l.smax_m = std::max(l.smax_m, P->hnodes[P->nlistv[k]].m);
+         l.smax_n = std::max(l.smax_n, P->hnodes[P->nlistv[k]].n);
+      }
+   };
    // a step launch whose tasks all belong to one front gets it by value
    auto mark_single = [&](std::vector<Launch>& prog, int kind, int64_t t0) {
       if (prog.empty() || prog.back().kind != kind || prog.back().off != t0) return;
@@ -499,6 +508,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
          for (int i : lv)
             if (small_solve(P->hnodes[i])) P->nlistv.push_back(i);
          push(P->fwdprog, L_FSMALL, s0, P->nlistv.size() - s0);
+         small_sizes(P->fwdprog, L_FSMALL, s0);
       }
       if (chain) {
          // one launch for the level's diagonal blocks (a chain per front,
@@ -571,6 +581,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
          for (int i : lv)
             if (small_solve(P->hnodes[i])) P->nlistv.push_back(i);
          push(P->bwdprog, L_BSMALL, s0, P->nlistv.size() - s0);
+         small_sizes(P->bwdprog, L_BSMALL, s0);
       }
       // contribution-block part: columns [0, n) against rows [n, m), split
       // into row chunks (partial sums per slot) and reduced in slot order
@@ -783,6 +794,12 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
    HIPCHECK(hipGetLastError());
 }
 
+// dynamic LDS of a small-front solve launch: x (largest m) and y (largest n)
+// rows of 16 doubles, each rounded up to 16 rows
+size_t small_lds(const Launch& L) {
+   return (size_t)(((L.smax_m + 15) & ~15) + ((L.smax_n + 15) & ~15)) * 16 * sizeof(double);
+}
+
 template <typename T>
 void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<Launch>& prog,
                        const double* b, double* xsol, int nrhs, hipStream_t s) {
@@ -827,12 +844,14 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
                k_bwd_cbp<T><<<g, bl, 0, s>>>(tk, P->d_nodes, arena, xsol, n, P->d_rlist, F->d_part_cb, nrhs);
             break;
          case L_FSMALL:
-            k_solve_small<T, false><<<g, bl, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, dinv, P->d_rlist,
-                                                     F->d_xloc, xsol, n, nrhs, b, n, P->d_cmap);
+            k_solve_small<T, false><<<g, bl, small_lds(L), s>>>(P->d_nlistv + L.off, P->d_nodes, arena, dinv,
+                                                                P->d_rlist, F->d_xloc, xsol, n, nrhs, b, n,
+                                                                P->d_cmap, (L.smax_m + 15) & ~15);
             break;
          case L_BSMALL:
-            k_solve_small<T, true><<<g, bl, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, dinvT, P->d_rlist,
-                                                    F->d_xloc, xsol, n, nrhs, nullptr, 0, P->d_cmap);
+            k_solve_small<T, true><<<g, bl, small_lds(L), s>>>(P->d_nlistv + L.off, P->d_nodes, arena, dinvT,
+                                                               P->d_rlist, F->d_xloc, xsol, n, nrhs, nullptr, 0,
+                                                               P->d_cmap, (L.smax_m + 15) & ~15);
             break;
          case L_BRED:
             if (L.count > 0) k_bwd_cbr<<<g, bl, 0, s>>>(tk, P->d_nodes, F->d_part_cb, F->d_xloc, nrhs);

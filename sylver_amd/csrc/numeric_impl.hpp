@@ -82,6 +82,8 @@ struct Launch {
    double words = 0;  // algorithmic memory traffic in front elements
    double bytes = 0;  // algorithmic memory traffic independent of the element size
    int node1 = -1;    // solve steps: the one front all tasks belong to (or -1)
+   int smax_m = 0;    // small-front solves: largest m and n of the launch's fronts
+   int smax_n = 0;
 };
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
