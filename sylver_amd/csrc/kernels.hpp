@@ -323,6 +323,20 @@ __device__ int lds_potrf_blocked(T* L, int kb16, int* sh_bad) {
          if (i >= j) L[i + j * LD] -= acc[v];
       }
    };
+   // two tiles at once: both products are formed before either is applied,
+   // so their LDS and MFMA latencies overlap (tiles are disjoint)
+   auto update_tile2 = [&](int o, int t0, int ti, int tj, int ti2, int tj2) {
+      acc_t acc = acc_t{0, 0, 0, 0}, acc2 = acc_t{0, 0, 0, 0};
+      acc = lds_mma16<T, true>(L + t0 + ti * 16 + o * LD, LD, L + t0 + tj * 16 + o * LD, LD, acc);
+      acc2 = lds_mma16<T, true>(L + t0 + ti2 * 16 + o * LD, LD, L + t0 + tj2 * 16 + o * LD, LD, acc2);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+         const int i = t0 + ti * 16 + Mfma<T>::row(lane, v), j = t0 + tj * 16 + (lane & 15);
+         const int i2 = t0 + ti2 * 16 + Mfma<T>::row(lane, v), j2 = t0 + tj2 * 16 + (lane & 15);
+         if (i >= j) L[i + j * LD] -= acc[v];
+         if (i2 >= j2) L[i2 + j2 * LD] -= acc2[v];
+      }
+   };
    if (wave == 0) factor_diag(0);
    __syncthreads();
    for (int s = 0; s < nbk; ++s) {
@@ -357,15 +371,22 @@ __device__ int lds_potrf_blocked(T* L, int kb16, int* sh_bad) {
          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
          factor_diag(t0);
       }
-      // the other tiles (1..ntiles-1) go to whichever wave is free
+      // the other tiles (1..ntiles-1) go to whichever wave is free, two at a time
       for (;;) {
          int tile = 0;
-         if (lane == 0) tile = atomicAdd(&next_tile, 1);
+         if (lane == 0) tile = atomicAdd(&next_tile, 2);
          tile = __shfl(tile, 0);
          if (tile >= ntiles) break;
          int ti = 0;
          while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
-         update_tile(o, t0, ti, tile - ti * (ti + 1) / 2);
+         const int tj = tile - ti * (ti + 1) / 2;
+         if (tile + 1 < ntiles) {
+            // the next tile in the same packed order: next column, or the next row's first
+            const int ti2 = tj + 1 <= ti ? ti : ti + 1, tj2 = tj + 1 <= ti ? tj + 1 : 0;
+            update_tile2(o, t0, ti, tj, ti2, tj2);
+         } else {
+            update_tile(o, t0, ti, tj);
+         }
       }
       __syncthreads();
    }
