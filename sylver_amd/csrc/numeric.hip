@@ -248,20 +248,20 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
          // beyond them on the side stream, where they overlap the next pair's
          // diagonal/TRSM chain.
          const bool first_of_pair = ((k0 / kNB) % 2) == 0;
-         // lower-triangular tiles of columns [c0, c1) (rows from c0 down)
-         auto tiles = [&](const NodeDesc& d, int kstart, int K, int c0, int c1) {
+         // lower-triangular TS x TS tiles of columns [c0, c1) (rows from c0 down)
+         auto tiles = [&](const NodeDesc& d, int kstart, int K, int c0, int c1, int TS) {
             const int c0a = c0 & ~7;
             const int R = d.m - c0a;
-            const int ntr = (R + 127) / 128, ntc = (c1 - c0a + 127) / 128;
+            const int ntr = (R + TS - 1) / TS, ntc = (c1 - c0a + TS - 1) / TS;
             for (int tj = 0; tj < ntc; ++tj)
                for (int ti = tj; ti < ntr; ++ti) {
                   GemmTask g{};
-                  g.a_off = d.off + c0a + ti * 128 + (int64_t)kstart * d.ld;
-                  g.b_off = d.off + c0a + tj * 128 + (int64_t)kstart * d.ld;
-                  g.c_off = d.off + (c0a + ti * 128) + (int64_t)(c0a + tj * 128) * d.ld;
+                  g.a_off = d.off + c0a + ti * TS + (int64_t)kstart * d.ld;
+                  g.b_off = d.off + c0a + tj * TS + (int64_t)kstart * d.ld;
+                  g.c_off = d.off + (c0a + ti * TS) + (int64_t)(c0a + tj * TS) * d.ld;
                   g.lda = g.ldb = g.ldc = d.ld;
-                  g.M = std::min(128, R - ti * 128);
-                  g.N = std::min(128, c1 - (c0a + tj * 128));
+                  g.M = std::min(TS, R - ti * TS);
+                  g.N = std::min(TS, c1 - (c0a + tj * TS));
                   g.K = K;
                   g.pad0 = (tj == 0) ? c0 - c0a : 0;
                   P->gemm.push_back(g);
@@ -284,19 +284,21 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
                   const int kb = std::min(kNB, d.n - k0);
                   if (first_of_pair && k0 + kNB < d.n) {
                      if (part == 1) continue;
+                     // the next panel's columns: 64 x 64 tiles (critical path)
                      const int k1 = k0 + kNB, kb1 = std::min(kNB, d.n - k1);
-                     for (int r = k1; r < d.m; r += 128) {
-                        GemmTask g{};
-                        g.a_off = d.off + r + (int64_t)k0 * d.ld;
-                        g.b_off = d.off + k1 + (int64_t)k0 * d.ld;
-                        g.c_off = d.off + r + (int64_t)k1 * d.ld;
-                        g.lda = g.ldb = g.ldc = d.ld;
-                        g.M = std::min(128, d.m - r);
-                        g.N = kb1;
-                        g.K = kb;
-                        g.pad0 = 0;
-                        P->gemm.push_back(g);
-                     }
+                     for (int cj = 0; cj < kb1; cj += 64)
+                        for (int r = k1 + cj; r < d.m; r += 64) {
+                           GemmTask g{};
+                           g.a_off = d.off + r + (int64_t)k0 * d.ld;
+                           g.b_off = d.off + k1 + cj + (int64_t)k0 * d.ld;
+                           g.c_off = d.off + r + (int64_t)(k1 + cj) * d.ld;
+                           g.lda = g.ldb = g.ldc = d.ld;
+                           g.M = std::min(64, d.m - r);
+                           g.N = std::min(64, kb1 - cj);
+                           g.K = kb;
+                           g.pad0 = 0;
+                           P->gemm.push_back(g);
+                        }
                      continue;
                   }
                   const int kstart = first_of_pair ? k0 : k0 - kNB;
@@ -304,14 +306,15 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
                   const int r0 = k0 + kb;
                   if (d.m <= r0) continue;
                   const int nx = std::min(d.n, r0 + 2 * kNB);  // end of the next pair's columns
-                  if (part == 0 && nx > r0) tiles(d, kstart, K, r0, nx);
-                  if (part == 1 && d.m > nx) tiles(d, kstart, K, nx, d.m);
+                  if (part == 0 && nx > r0) tiles(d, kstart, K, r0, nx, 64);  // critical path
+                  if (part == 1 && d.m > nx) tiles(d, kstart, K, nx, d.m, 128);
                }
                pad8(u0);
                const int64_t cnt = P->gemm.size() - u0;
                if (cnt == 0) continue;
                push(P->fprog, wide ? L_UPDATE_WIDE : L_UPDATE, u0, cnt);
                Launch& L = P->fprog.back();
+               L.tile = part == 0 ? 64 : 128;
                if (part == 0) {
                   // the next pair's columns were last touched by the side stream
                   if (side_last >= 0) {
@@ -773,11 +776,16 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
             k_update<T, 1><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, dinv);
             break;
          case L_UPDATE:
-            k_update<T, 0><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, nullptr);
+            if (L.tile == 64)
+               k_update<T, 0, 64><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, nullptr);
+            else
+               k_update<T, 0><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, nullptr);
             break;
          case L_UPDATE_WIDE:
             if (bf16_wide)
                k_update_bf16<<<g, b, 0, s>>>(P->d_gemm + L.off, reinterpret_cast<float*>(arena));
+            else if (L.tile == 64)
+               k_update<T, 0, 64><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, nullptr);
             else
                k_update<T, 0><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, nullptr);
             break;

@@ -82,6 +82,7 @@ struct Launch {
    double words = 0;  // algorithmic memory traffic in front elements
    double bytes = 0;  // algorithmic memory traffic independent of the element size
    int node1 = -1;    // solve steps: the one front all tasks belong to (or -1)
+   int tile = 128;    // Schur updates: tile size of the launch's tasks (128 or 64)
    int smax_m = 0;    // small-front solves: largest m and n of the launch's fronts
    int smax_n = 0;
 };
