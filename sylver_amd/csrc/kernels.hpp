@@ -637,13 +637,14 @@ __device__ inline int xcd_task(int b, int G) { return (b & 7) * (G >> 3) + (b >>
 // all kb columns, so each workgroup owns its rows and has read all of A
 // before it writes). Tasks whose offsets or leading dimensions are not
 // multiples of the vector width take scalar loads.
-// TB: tile size (128; 64 for the short launches on the critical path,
-// whose tiles then spread over four times as many workgroups).
-template <typename T, int MODE, int TB = 128>
+// TB x TBN tiles: 128 x 128; 64 x 64 for the short Schur-update launches on
+// the critical path and 64 x 128 for the TRSM (whose tile must span all
+// kb columns), so those launches spread over more workgroups.
+template <typename T, int MODE, int TB = 128, int TBN = TB>
 __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict__ tasks,
                                                      T* __restrict__ arena,
                                                      const T* __restrict__ bsrc) {
-   constexpr int BM = TB, BN = TB, BK = sizeof(T) == 8 ? 32 : 16, PAD = 4;  // FP64: longer MFMA chains per stage
+   constexpr int BM = TB, BN = TBN, BK = sizeof(T) == 8 ? 32 : 16, PAD = 4;  // FP64: longer MFMA chains per stage
    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
    typedef typename V16<T>::vt vt;
    constexpr int VN = V16<T>::N;

@@ -223,7 +223,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             const NodeDesc& d = P->hnodes[i];
             if (k0 >= d.n) continue;
             int kb = std::min(kNB, d.n - k0);
-            for (int r = k0 + kb; r < d.m; r += 128) {
+            for (int r = k0 + kb; r < d.m; r += 64) {  // 64-row tiles (k_update<T, 1, 64, 128>)
                GemmTask g{};
                g.a_off = d.off + r + (int64_t)k0 * d.ld;
                g.b_off = d.dinv_off + (int64_t)(k0 / kNB) * kNB * kNB;
@@ -231,7 +231,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
                g.lda = d.ld;
                g.ldb = kb;
                g.ldc = d.ld;
-               g.M = std::min(128, d.m - r);
+               g.M = std::min(64, d.m - r);
                g.N = kb;
                g.K = kb;
                P->gemm.push_back(g);
@@ -773,7 +773,7 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
             k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, dinvT, F->d_info, 1);
             break;
          case L_TRSM:
-            k_update<T, 1><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, dinv);
+            k_update<T, 1, 64, 128><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, dinv);
             break;
          case L_UPDATE:
             if (L.tile == 64)
