@@ -533,76 +533,6 @@ __global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ 
    }
 }
 
-// ---------------------------------------------------------------------------
-// C = beta*C + alpha*A*B^T on one BM x BN tile per workgroup (MFMA 16x16x4,
-// 2x2 waves). Used for the TRSM of a panel by the inverted diagonal block
-// (in place: BN covers all columns, so each workgroup owns its rows) and for
-// the Schur-complement update of the trailing block (lower tiles only).
-// ---------------------------------------------------------------------------
-template <typename T, int BM, int BN>
-__global__ __launch_bounds__(kThreads) void k_gemm_nt(const GemmTask* __restrict__ tasks,
-                                                      const T* A, const T* B, T* C, T alpha,
-                                                      T beta) {
-   constexpr int BK = 16;
-   constexpr int PAD = 4;
-   constexpr int WM = BM / 2, WN = BN / 2;
-   constexpr int FM = WM / 16, FN = WN / 16;
-   typedef typename Mfma<T>::acc_t acc_t;
-   __shared__ T As[BK][BM + PAD];
-   __shared__ T Bs[BK][BN + PAD];
-   const GemmTask t = tasks[blockIdx.x];
-   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-   const int wm = wave >> 1, wn = wave & 1;
-   acc_t acc[FM][FN];
-   for (int a = 0; a < FM; ++a)
-      for (int b = 0; b < FN; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
-   const T* a = A + t.a_off;
-   const T* b = B + t.b_off;
-   for (int k0 = 0; k0 < t.K; k0 += BK) {
-      for (int e = tid; e < BK * BM; e += kThreads) {
-         int i = e % BM, kk = e / BM;
-         T v = T(0);
-         if (i < t.M && k0 + kk < t.K) v = a[i + (int64_t)(k0 + kk) * t.lda];
-         As[kk][i] = v;
-      }
-      for (int e = tid; e < BK * BN; e += kThreads) {
-         int j = e % BN, kk = e / BN;
-         T v = T(0);
-         if (j < t.N && k0 + kk < t.K) v = b[j + (int64_t)(k0 + kk) * t.ldb];
-         Bs[kk][j] = v;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < BK; kk += 4) {
-         T af[FM], bf[FN];
-#pragma unroll
-         for (int f = 0; f < FM; ++f) af[f] = As[kk + (lane >> 4)][wm * WM + f * 16 + (lane & 15)];
-#pragma unroll
-         for (int f = 0; f < FN; ++f) bf[f] = Bs[kk + (lane >> 4)][wn * WN + f * 16 + (lane & 15)];
-#pragma unroll
-         for (int x = 0; x < FM; ++x)
-#pragma unroll
-            for (int y = 0; y < FN; ++y) acc[x][y] = Mfma<T>::op(af[x], bf[y], acc[x][y]);
-      }
-      __syncthreads();
-   }
-   T* c = C + t.c_off;
-#pragma unroll
-   for (int x = 0; x < FM; ++x)
-#pragma unroll
-      for (int y = 0; y < FN; ++y) {
-         const int j = wn * WN + y * 16 + (lane & 15);
-         if (j >= t.N) continue;
-#pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int i = wm * WM + x * 16 + Mfma<T>::row(lane, v);
-            if (i >= t.M) continue;
-            T* p = c + i + (int64_t)j * t.ldc;
-            T old = (beta == T(0)) ? T(0) : *p;
-            *p = beta * old + alpha * acc[x][y][v];
-         }
-      }
-}
 
 // ---------------------------------------------------------------------------
 // Schur-complement update of the large-front path: C -= A * B^T on one
