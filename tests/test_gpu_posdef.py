@@ -185,6 +185,32 @@ def test_fp32_factor_and_refinement(S, O, gpu):
     assert O.bwd_err(A, x, b) <= BWD_TOL
 
 
+def test_fp32_refinement_two_rhs_chunks(S, O, gpu):
+    """FP32 factor + FP64 refinement with 20 right-hand sides: two 16-column
+    chunks through the chained solves, the row-major copy of x, the residual
+    and the per-workgroup column maxima; every column reaches the bound and
+    the refined solution matches the FP64 path's within it."""
+    A = M.laplacian_3d(20)
+    b = M.random_rhs(A.n, 20, seed=4)
+    S.set_ext_options(precision=S.PREC_FP32, refine=S.REFINE_IR, refine_tol=1e-15)
+    akeep, opts, sym = _analyse(S, A)
+    fkeep, inf = _factor(S, A, akeep, opts)
+    assert inf.flag == 0
+    x = b.copy(order="F")
+    inf = S.spldlt_solve(x, akeep, fkeep, options=opts)
+    assert inf.flag == 0
+    assert inf.tail().refine_iters >= 2
+    for c in range(b.shape[1]):
+        assert O.bwd_err(A, x[:, c], b[:, c]) <= BWD_TOL, c
+    S.set_ext_options(precision=S.PREC_FP64, refine=S.REFINE_AUTO)
+    fk64, inf = _factor(S, A, akeep, opts)
+    assert inf.flag == 0
+    x64 = b.copy(order="F")
+    assert S.spldlt_solve(x64, akeep, fk64, options=opts).flag == 0
+    scale = np.abs(x64).max()
+    assert np.abs(x - x64).max() <= 1e-12 * scale
+
+
 def test_not_posdef_flag(S, gpu):
     # tests/sylver_test_mod.F90 simple_mat_indef factorized as posdef -> -6
     A = M.CSCLower(4, [1, 4, 5, 7, 8], [1, 2, 4, 2, 3, 4, 4], [1.0, 2.0, 3.0, 1.0, 1.0, 4.0, 1.0])
