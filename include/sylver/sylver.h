@@ -73,12 +73,14 @@ typedef struct {
    int scheduler;
 } sylver_options_t;
 
-/* Reference: sylver.h:73 -> sylver_mod.F90:8-26 (StarPU init). Here: selects
- * HIP device 0 and creates the library's stream. ncpu/ngpu are accepted and
- * recorded; one GPU is used. */
+/* Reference: sylver.h:73 -> sylver_mod.F90:8-26 (StarPU init). Here there is
+ * no runtime to start: ncpu/ngpu are recorded, and the numeric phase runs on
+ * the current HIP device (one GPU). Streams, graphs and device buffers belong
+ * to the akeep/fkeep handles that spldlt_analyse/spldlt_factorize create. */
 void sylver_init(int ncpu, int ngpu);
 
-/* Reference: sylver.h:75. Releases the stream and cached device state. */
+/* Reference: sylver.h:75. Waits for outstanding device work; device memory is
+ * released by spldlt_free_akeep / spldlt_free_fkeep, not here. */
 void sylver_finalize(void);
 
 /* Reference: sylver.h:77 -> sylver_ciface.F90 default options

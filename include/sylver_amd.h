@@ -20,7 +20,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
-#include "sylver.h"
+#include "sylver/sylver.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -56,6 +56,17 @@ enum {
    SYLVER_WARNING_ANAL_SINGULAR = 6,
    SYLVER_WARNING_FACT_SINGULAR = 7,
    SYLVER_WARNING_MATCH_ORD_NO_SCALE = 8
+};
+
+/* Flags of this library beyond the reference's set (the reference has no
+ * refinement): spldlt_solve with low-precision factors returns
+ * SYLVER_AMD_WARNING_REFINE when refinement stopped after refine_max_iter
+ * steps above refine_tol (x is the last iterate, inform tail bwd_err its
+ * backward error), and SYLVER_AMD_ERROR_REFINE when a step produced a
+ * non-finite backward error (x is not to be used). */
+enum {
+   SYLVER_AMD_WARNING_REFINE = 9,
+   SYLVER_AMD_ERROR_REFINE = -60
 };
 
 /* ---- 1. internal numeric-tree seam ----

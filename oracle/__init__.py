@@ -39,7 +39,7 @@ def lib():
         L.oracle_front_potrf.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int]
         L.oracle_bwd_err.argtypes = [C.c_int, i64p, ip, dp, C.c_int, dp, C.c_int, dp, C.c_int]
         L.oracle_bwd_err.restype = C.c_double
-        L.oracle_factor_indef.argtypes = [C.c_int, C.c_int, ip, ip, i64p, ip, i64p, i64p, dp,
+        L.oracle_factor_indef.argtypes = [C.c_int, C.c_int, ip, ip, i64p, ip, i64p, i64p, dp, dp,
                                           C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]
         L.oracle_factor_indef.restype = C.c_void_p
         L.oracle_indef_stats.argtypes = [C.c_void_p, ip]
@@ -151,13 +151,16 @@ STAT_KEYS = ("flag", "num_delay", "num_neg", "num_two", "num_zero", "maxfront",
 class IndefFactor:
     """Oracle APP LDL^T factorization of a whole tree (see oracle.h)."""
 
-    def __init__(self, sym, aval, nb=128, u=0.01, small=1e-20, action=True, failed_tpp=True):
+    def __init__(self, sym, aval, nb=128, u=0.01, small=1e-20, action=True, failed_tpp=True,
+                 scaling=None):
         self.sym = sym
         aval = np.ascontiguousarray(aval, np.float64)
+        sc = None if scaling is None else np.ascontiguousarray(scaling, np.float64)
         self.h = lib().oracle_factor_indef(
             sym["n"], sym["nnodes"], _p(sym["sptr"], C.c_int), _p(sym["sparent"], C.c_int),
             _p(sym["rptr"], C.c_int64), _p(sym["rlist"], C.c_int), _p(sym["nptr"], C.c_int64),
-            _p(sym["nlist"], C.c_int64), _p(aval, C.c_double), nb, u, small, int(action),
+            _p(sym["nlist"], C.c_int64), _p(aval, C.c_double),
+            None if sc is None else _p(sc, C.c_double), nb, u, small, int(action),
             int(failed_tpp))
         out = np.zeros(8, np.int32)
         lib().oracle_indef_stats(self.h, _p(out, C.c_int))

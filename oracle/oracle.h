@@ -83,11 +83,13 @@ double oracle_bwd_err(int n, const int64_t *ptr, const int *row, const double *v
  * the TPP second pass are this oracle's own statement of published methods
  * (Bunch-Parlett complete pivoting; threshold partial pivoting with the
  * Duff-Reid 2x2 test): the reference delegates both to SPRAL, which is not
- * in its tree. Returns an opaque handle (free with oracle_indef_free). */
+ * in its tree. scaling (pivot order, may be NULL) scales the scattered
+ * entries as init_a_block does. Returns an opaque handle (free with
+ * oracle_indef_free). */
 void *oracle_factor_indef(int n, int nnodes, const int *sptr, const int *sparent,
                           const int64_t *rptr, const int *rlist, const int64_t *nptr,
-                          const int64_t *nlist, const double *aval, int nb, double u,
-                          double small, int action, int failed_tpp);
+                          const int64_t *nlist, const double *aval, const double *scaling,
+                          int nb, double u, double small, int action, int failed_tpp);
 /* out[8] = {flag, num_delay, num_neg, num_two, num_zero, maxfront,
  * not_first_pass, not_second_pass}. */
 void oracle_indef_stats(const void *h, int *out);

@@ -499,8 +499,8 @@ struct IndefFactor {
 
 extern "C" void* oracle_factor_indef(int n, int nnodes, const int* sptr, const int* sparent,
                                      const int64_t* rptr, const int* rlist, const int64_t* nptr,
-                                     const int64_t* nlist, const double* aval, int nb, double u,
-                                     double small, int action, int failed_tpp) {
+                                     const int64_t* nlist, const double* aval, const double* scaling,
+                                     int nb, double u, double small, int action, int failed_tpp) {
    IndefFactor* F = new IndefFactor;
    F->n = n;
    F->nnodes = nnodes;
@@ -529,10 +529,13 @@ extern "C" void* oracle_factor_indef(int n, int nnodes, const int* sptr, const i
          const int64_t src = nlist[2 * (p - 1)] - 1;
          const int64_t dest = nlist[2 * (p - 1) + 1] - 1;
          const int c = (int)(dest / nrow);
-         int r = (int)(dest % nrow);
-         if (r >= ncol) r += ndin;
-         S.at(r, c) = aval[src];
-         S.at(c, r) = aval[src];
+         const int r0 = (int)(dest % nrow);
+         const int r = r0 >= ncol ? r0 + ndin : r0;
+         // scaled entry: rscale * a * cscale by symbolic row-list entries
+         // (init_a_block, src/kernels/assemble.hxx:166-177)
+         const double v = scaling ? scaling[rl[r0] - 1] * aval[src] * scaling[rl[c] - 1] : aval[src];
+         S.at(r, c) = v;
+         S.at(c, r) = v;
       }
       for (int i = 0; i < ncol; ++i) map[rl[i]] = i;
       for (int i = ncol; i < nrow; ++i) map[rl[i]] = i + ndin;

@@ -135,7 +135,7 @@ KIND_NAMES = ["zero", "scatter_a", "extend_add", "small_front", "diag_potrf", "p
 
 _lib = None
 
-# Every symbol declared in include/sylver.h and include/sylver_amd.h.
+# Every symbol declared in include/sylver/sylver.h and include/sylver_amd.h.
 EXPORTED = [
     "sylver_init", "sylver_finalize", "sylver_default_options", "spldlt_analyse",
     "spldlt_factorize", "spldlt_solve", "spldlt_free_akeep", "spldlt_free_fkeep",

@@ -1,6 +1,6 @@
 """Build the native library in-tree: ``sylver_amd/libsylver_amd.so``.
 
-One shared object holds the C ABI (include/sylver.h, include/sylver_amd.h),
+One shared object holds the C ABI (include/sylver/sylver.h, include/sylver_amd.h),
 the CPU symbolic analysis and the gfx950 HIP kernels. Built with hipcc for
 gfx950 only; run ``python -m sylver_amd.build``.
 """
@@ -35,16 +35,24 @@ def hipcc():
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
+    """Compiles every source (in parallel, one hipcc per source) and links."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(CSRC, "_build", src + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
-        cmd = [hipcc(), *HIP_FLAGS, "-I", os.path.join(HERE, "..", "include"), "-c", path, "-o", obj]
+        cmds.append([hipcc(), *HIP_FLAGS, "-I", os.path.join(HERE, "..", "include"), "-c", path, "-o", obj])
+        objs.append(obj)
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    with ThreadPoolExecutor(max_workers=min(len(cmds), 8)) as ex:
+        list(ex.map(run, cmds))
     tmp = OUT + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]
     if verbose:

@@ -344,8 +344,8 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
       const int nl = (int)lv.size();
       // zero the level's fronts (contiguous)
       k_fill<<<grid_of(lv_arena), kThreads, 0, s>>>(lv_arena, 0.0, S.arena);
-      k_ind_init<<<nl, kThreads, 0, s>>>(S.d_list, S.d_nodes, P->d_nlist, P->d_rlist, F->d_aval, S.arena,
-                                         S.d_vars, S.d_pres);
+      k_ind_init<<<nl, kThreads, 0, s>>>(S.d_list, S.d_nodes, P->d_nlist, P->d_rlist, F->d_aval,
+                                         F->has_scaling ? F->d_scaling : nullptr, S.arena, S.d_vars, S.d_pres);
       if (n_delay > 0) {
          int maxd = 1;
          for (int k = 0; k < n_delay; ++k) {

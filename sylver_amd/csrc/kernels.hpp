@@ -2063,6 +2063,76 @@ static __global__ __launch_bounds__(kThreads) void k_fill(int64_t n, double v, d
    if (i < n) y[i] = v;
 }
 
+// Caller's order -> pivot order, optionally scaled (the solve driver's
+// x2(i,r) = x(invp(i),r) * scaling(i), src/spldlt_factorize_mod.F90:951-966):
+// out[i + c n] = x[invp[i]-1 + c ldx] (* s[i]).
+static __global__ __launch_bounds__(kThreads) void k_perm_in(int n, int nrhs, const int* __restrict__ invp,
+                                                      const double* __restrict__ s,
+                                                      const double* __restrict__ x, int64_t ldx,
+                                                      double* __restrict__ out) {
+   const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+   if (e >= (int64_t)n * nrhs) return;
+   const int c = (int)(e / n), i = (int)(e - (int64_t)c * n);
+   double v = x[(int64_t)(invp[i] - 1) + c * ldx];
+   if (s) v *= s[i];
+   out[e] = v;
+}
+
+// Pivot order -> caller's order, optionally scaled (:1039-1053):
+// out[invp[i]-1 + c ldo] = x[i + c n] (* s[i]).
+static __global__ __launch_bounds__(kThreads) void k_perm_out(int n, int nrhs, const int* __restrict__ invp,
+                                                       const double* __restrict__ s,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ out, int64_t ldo) {
+   const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+   if (e >= (int64_t)n * nrhs) return;
+   const int c = (int)(e / n), i = (int)(e - (int64_t)c * n);
+   double v = x[e];
+   if (s) v *= s[i];
+   out[(int64_t)(invp[i] - 1) + c * ldo] = v;
+}
+
+// y[i + c n] = s[i] x[i + c n] (rows scaled; y may alias x).
+static __global__ __launch_bounds__(kThreads) void k_scale_rows(int n, int nrhs, const double* __restrict__ s,
+                                                         const double* x, double* y) {
+   const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+   if (e >= (int64_t)n * nrhs) return;
+   y[e] = s[e % n] * x[e];
+}
+
+// ||A||_inf with full symmetric row sums (tests/common.hxx:1003-1050): the
+// CSR holds both triangles, so a row's sum is |A| summed over the CSR row.
+// Grid-stride over rows; each workgroup leaves its maximum in part[blockIdx.x].
+static __global__ __launch_bounds__(kThreads) void k_csr_rowabs(int n, const int64_t* __restrict__ rowptr,
+                                                         const double* __restrict__ val,
+                                                         double* __restrict__ part) {
+   double m = 0.0;
+   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+      double s = 0.0;
+      for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) s += fabs(val[p]);
+      m = fmax(m, s);
+   }
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+   __shared__ double wm[4];
+   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+   __syncthreads();
+   if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+}
+
+// out[0] = max of part[0..cnt) (one workgroup).
+static __global__ __launch_bounds__(kThreads) void k_max_reduce(const double* __restrict__ part, int cnt,
+                                                         double* __restrict__ out) {
+   double m = 0.0;
+   for (int p = threadIdx.x; p < cnt; p += kThreads) m = fmax(m, part[p]);
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+   __shared__ double wm[4];
+   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+   __syncthreads();
+   if (threadIdx.x == 0) out[0] = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+}
+
 // CSR values gathered from the cleaned input values.
 static __global__ __launch_bounds__(kThreads) void k_gather_vals(int64_t nnz, const int64_t* __restrict__ src,
                                                           const double* __restrict__ aval,

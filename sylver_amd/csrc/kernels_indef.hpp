@@ -1558,20 +1558,25 @@ static __global__ __launch_bounds__(kThreads) void k_ind_init(const int* __restr
                                                        const int64_t* __restrict__ nlist,
                                                        const int* __restrict__ rlist,
                                                        const double* __restrict__ aval,
+                                                       const double* __restrict__ scaling,
                                                        double* __restrict__ arena,
                                                        int* __restrict__ vars, int* __restrict__ pres) {
    const IndefDesc nd = nodes[list[blockIdx.x]];
    double* F = arena + nd.off;
+   const int* rl = rlist + nd.rlist_off;
    for (int64_t p = threadIdx.x; p < nd.num_a; p += kThreads) {
       const int64_t src = nlist[2 * (nd.amap_off + p)] - 1;
       const int64_t dest = nlist[2 * (nd.amap_off + p) + 1] - 1;
       const int c = (int)(dest / nd.nrow);
-      int r = (int)(dest % nd.nrow);
-      if (r >= nd.ncol) r += nd.ndelay_in;
-      F[r + (int64_t)c * nd.ld] = aval[src];
+      const int r0 = (int)(dest % nd.nrow);
+      const int r = r0 >= nd.ncol ? r0 + nd.ndelay_in : r0;
+      double v = aval[src];
+      // with a scaling: rscale * a * cscale, rows and columns by their
+      // symbolic row-list entries (init_a_block, src/kernels/assemble.hxx:166-177)
+      if (scaling) v = scaling[rl[r0] - 1] * v * scaling[rl[c] - 1];
+      F[r + (int64_t)c * nd.ld] = v;
    }
    int* var = vars + nd.var_off;
-   const int* rl = rlist + nd.rlist_off;
    for (int r = threadIdx.x; r < nd.m; r += kThreads) {
       if (r < nd.ncol)
          var[r] = rl[r];

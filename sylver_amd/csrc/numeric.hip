@@ -55,8 +55,11 @@ int device_count() {
 // ---------------------------------------------------------------------------
 // Plan: pattern-dependent device state.
 // ---------------------------------------------------------------------------
-void destroy_plan(DevicePlan* p) { delete p; }
 void destroy_factors(DeviceFactors* f) { delete f; }
+
+void device_sync() {
+   if (device_count() > 0) (void)hipDeviceSynchronize();
+}
 
 namespace {
 
@@ -732,6 +735,7 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    P->d_stask = dupload(P->stask);
    P->d_ctask = dupload(P->ctask);
    P->d_cgch = dupload(P->cgch);
+   if ((int)a.invp.size() == a.n && a.n > 0) P->d_invp = dupload(a.invp);
    return P;
 }
 
@@ -913,6 +917,10 @@ void ensure_solve_ws(const DevicePlan* P, DeviceFactors* F, int nrhs) {
    F->d_x = dalloc<double>((size_t)P->n * nrhs);
    F->d_r = dalloc<double>((size_t)P->n * nrhs);
    F->d_sol = dalloc<double>((size_t)P->n * nrhs);
+   dfree(F->d_xin);
+   F->d_xin = dalloc<double>((size_t)P->n * nrhs);
+   dfree(F->d_sc);
+   F->d_sc = dalloc<double>((size_t)P->n * nrhs);
    F->d_norms = dalloc<unsigned long long>(3 * (size_t)nrhs);
    dfree(F->d_xrm);
    F->d_xrm = dalloc<double>((size_t)P->n * 16 * ((nrhs + 15) / 16));
@@ -926,7 +934,7 @@ void ensure_solve_ws(const DevicePlan* P, DeviceFactors* F, int nrhs) {
 // ---------------------------------------------------------------------------
 // Factorization driver
 // ---------------------------------------------------------------------------
-FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors** fact, bool posdef,
+FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fact, bool posdef,
                            const double* aval, const double* scaling, const sylver_options_t& opts) {
    FactorResult res;
    res.matrix_rank = a.n;
@@ -936,21 +944,16 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
       return res;
    }
    const sylver_amd_options_t& xo = ext_options();
-   (void)opts;
    try {
       int small_max = xo.small_front_max > 0 ? xo.small_front_max : 256;
       int wide_ncol = xo.bf16_min_ncol > 0 ? xo.bf16_min_ncol : 1024;
+      // a new plan for new knobs or variants; factorizations built on the old
+      // one keep it alive through their own reference
       if (*plan && ((*plan)->small_max != small_max || (*plan)->wide_ncol != wide_ncol ||
-                    (*plan)->variant_gen != a.variant_gen)) {
-         if (*fact && (*fact)->plan == *plan) {
-            destroy_factors(*fact);
-            *fact = nullptr;
-         }
-         destroy_plan(*plan);
-         *plan = nullptr;
-      }
-      if (!*plan) *plan = build_plan(a, small_max, wide_ncol);
-      DevicePlan* P = *plan;
+                    (*plan)->variant_gen != a.variant_gen))
+         plan->reset();
+      if (!*plan) plan->reset(build_plan(a, small_max, wide_ncol));
+      DevicePlan* P = plan->get();
       int prec = xo.precision;
       if (prec != SYLVER_AMD_PREC_FP64 && prec != SYLVER_AMD_PREC_FP32 && prec != SYLVER_AMD_PREC_MIXED)
          prec = SYLVER_AMD_PREC_FP64;
@@ -958,18 +961,26 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
       const bool low = prec != SYLVER_AMD_PREC_FP64;
       if (!posdef) prec = SYLVER_AMD_PREC_FP64;  // the indefinite path factors in FP64
       DeviceFactors* F = *fact;
-      if (F && (F->plan != P || F->precision != prec || F->posdef != posdef)) {
+      if (F && (F->plan.get() != P || F->precision != prec || F->posdef != posdef)) {
          destroy_factors(F);
          F = nullptr;
+         *fact = nullptr;
       }
       if (!F) {
          F = new DeviceFactors();
          *fact = F;
-         F->plan = P;
+         F->plan = *plan;
          F->precision = prec;
          size_t es = low ? sizeof(float) : sizeof(double);
+         HIPCHECK(hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking));
+         HIPCHECK(hipStreamCreateWithFlags(&F->stream2, hipStreamNonBlocking));
          if (posdef) {  // the indefinite path lays out its fronts level by level
-            HIPCHECK(hipMalloc(&F->arena, std::max<int64_t>(P->arena_elems, 1) * es));
+            const size_t ab = std::max<int64_t>(P->arena_elems, 1) * es;
+            HIPCHECK(hipMalloc(&F->arena, ab));
+            // once, outside any capture: padding rows (ld > m) and entries
+            // above the diagonal that no kernel writes stay zero, so the
+            // solves' full-height block loads never meet stale memory
+            HIPCHECK(hipMemsetAsync(F->arena, 0, ab, F->stream));
             HIPCHECK(hipMalloc(&F->dinv, std::max<int64_t>(P->dinv_elems, 1) * es));
             HIPCHECK(hipMalloc(&F->dinvT, std::max<int64_t>(P->dinv_elems, 1) * es));
          }
@@ -977,8 +988,6 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
          F->d_scaling = dalloc<double>(std::max(a.n, 1));
          F->d_csr_val = dalloc<double>(std::max<int64_t>(P->csr_nnz, 1));
          F->d_info = dalloc<int>(std::max(P->nnodes, 1));
-         HIPCHECK(hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking));
-         HIPCHECK(hipStreamCreateWithFlags(&F->stream2, hipStreamNonBlocking));
          F->xev.resize(P->nevents);
          for (auto& e : F->xev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       }
@@ -996,11 +1005,16 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
       HIPCHECK(hipMemsetAsync(F->d_info, 0, std::max(P->nnodes, 1) * sizeof(int), s));
       HIPCHECK(hipEventRecord(e1, s));
       const bool record = xo.record_timing != 0 && posdef;
-      bool use_graph = xo.use_graph != 0 && !F->has_scaling && !record && posdef;
+      const bool use_graph = xo.use_graph != 0 && !record && posdef;
       std::vector<hipEvent_t> events;
       if (record) {
          events.resize(2 * P->fprog.size());
          for (auto& e : events) HIPCHECK(hipEventCreate(&e));
+      }
+      // the captured program bakes in whether the A-scatter scales
+      if (use_graph && F->graph && F->graph_scaled != F->has_scaling) {
+         (void)hipGraphExecDestroy(F->graph);
+         F->graph = nullptr;
       }
       if (use_graph && !F->graph) {
          hipGraph_t g;
@@ -1012,6 +1026,7 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
          HIPCHECK(hipStreamEndCapture(s, &g));
          HIPCHECK(hipGraphInstantiate(&F->graph, g, nullptr, nullptr, 0));
          (void)hipGraphDestroy(g);
+         F->graph_scaled = F->has_scaling;
       }
       if (!posdef) {
          factor_indef(a, P, F, opts, res);
@@ -1022,7 +1037,7 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
       } else {
          run_factor_program<double>(P, F, s, record ? &events : nullptr);
       }
-      HIPCHECK(hipEventRecord(e2, s));
+   HIPCHECK(hipEventRecord(e2, s));
       // FP64 values of the whole matrix for the refinement residual
       if (P->csr_nnz > 0)
          k_gather_vals<<<grid1(P->csr_nnz), kThreads, 0, s>>>(P->csr_nnz, P->d_csr_src, F->d_aval,
@@ -1065,19 +1080,17 @@ FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors**
                res.flag = SYLVER_ERROR_NOT_POS_DEF;
                break;
             }
-      // ||A||_inf with full symmetric row sums (scaled matrix when scaling given)
-      if (!a.ptr.empty()) {
-         std::vector<double> rs(a.n, 0.0);
-         for (int j = 0; j < a.n; ++j)
-            for (int64_t p = a.ptr[j] - 1; p < a.ptr[j + 1] - 1; ++p) {
-               int i = a.row[p] - 1;
-               double v = std::fabs(aval[p]);
-               rs[i] += v;
-               if (i != j) rs[j] += v;
-            }
-         double an = 0;
-         for (double v : rs) an = std::max(an, v);
-         F->anorm = an;
+      // ||A||_inf of the unscaled matrix (full symmetric row sums) for the
+      // refinement's backward error, from the device CSR
+      F->anorm = 0;
+      if (P->csr_nnz > 0) {
+         const int parts = (int)std::min<int64_t>(kMaxParts, std::max<int64_t>(1, (a.n + kThreads - 1) / kThreads));
+         double* part = dalloc<double>(parts + 1);
+         k_csr_rowabs<<<parts, kThreads, 0, s>>>(a.n, P->d_csr_ptr, F->d_csr_val, part);
+         k_max_reduce<<<1, kThreads, 0, s>>>(part, parts, part + parts);
+         HIPCHECK(hipMemcpyAsync(&F->anorm, part + parts, sizeof(double), hipMemcpyDeviceToHost, s));
+         HIPCHECK(hipStreamSynchronize(s));
+         dfree(part);
       }
    } catch (const HipError& e) {
       res.flag = SYLVER_ERROR_CUDA_UNKNOWN;
@@ -1140,6 +1153,21 @@ void solve_once(const DevicePlan* P, DeviceFactors* F, int job, const double* b,
    (void)hipGraphDestroy(graph);
    F->sgraphs.push_back(DeviceFactors::SolveGraph{job, nrhs, es, b, xsol, exec});
    HIPCHECK(hipGraphLaunch(exec, s));
+}
+
+// The refinement's preconditioner: d = S (L L^T)^{-1} S r with the
+// low-precision factors of S A S (S = I without a scaling). `r` is not
+// modified; d may not alias r.
+template <typename T>
+void precond(const DevicePlan* P, DeviceFactors* F, const double* r, double* d, int nrhs, hipStream_t s) {
+   if (!F->has_scaling) {
+      solve_once<T>(P, F, 0, r, d, nrhs, s);
+      return;
+   }
+   const int64_t nn = (int64_t)P->n * nrhs;
+   k_scale_rows<<<grid1(nn), kThreads, 0, s>>>(P->n, nrhs, F->d_scaling, r, F->d_sc);
+   solve_once<T>(P, F, 0, F->d_sc, d, nrhs, s);
+   k_scale_rows<<<grid1(nn), kThreads, 0, s>>>(P->n, nrhs, F->d_scaling, d, d);
 }
 
 // One refinement update, fused: x += d, r = b - A x, and the scaled
@@ -1217,7 +1245,7 @@ int gmres_correction(const DevicePlan* P, DeviceFactors* F, const double* r, dou
    const int64_t nn = (int64_t)n * nrhs;
    ensure_gmres_ws(P, F, nrhs, m);
    double* V = F->d_V;
-   solve_once<T>(P, F, 0, r, V, nrhs, s);
+   precond<T>(P, F, r, V, nrhs, s);
    std::vector<double> h, beta(nrhs), coef(nrhs);
    device_dots(F, n, nrhs, V, V, 0, 1, h, s);
    for (int c = 0; c < nrhs; ++c) {
@@ -1236,7 +1264,7 @@ int gmres_correction(const DevicePlan* P, DeviceFactors* F, const double* r, dou
       double* Vj = V + (int64_t)j * nn;
       double* Vn = V + (int64_t)(j + 1) * nn;
       k_spmv<<<grid1(n), kThreads, 0, s>>>(n, P->d_csr_ptr, P->d_csr_col, F->d_csr_val, Vj, F->d_tmp, nrhs);
-      solve_once<T>(P, F, 0, F->d_tmp, F->d_W, nrhs, s);
+      precond<T>(P, F, F->d_tmp, F->d_W, nrhs, s);
       std::vector<double> hsum((size_t)(j + 1) * nrhs, 0.0);
       for (int pass = 0; pass < 2; ++pass) {
          device_dots(F, n, nrhs, F->d_W, V, nn, j + 1, h, s);
@@ -1291,28 +1319,42 @@ int gmres_correction(const DevicePlan* P, DeviceFactors* F, const double* r, dou
 
 }  // namespace
 
-SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int job, int nrhs,
-                         double* x, int ldx) {
+SolveResult device_solve(DeviceFactors* F, int job, int nrhs, double* x, int ldx, bool user_order) {
    SolveResult res;
-   if (!P || !F) {
+   if (!F || !F->plan) {
       res.flag = SYLVER_ERROR_CALL_SEQUENCE;
       return res;
    }
+   const DevicePlan* P = F->plan.get();
    const sylver_amd_options_t& xo = ext_options();
    try {
-      const int n = a.n;
+      const int n = P->n;
       ensure_solve_ws(P, F, nrhs);
       hipStream_t s = F->stream;
       hipEvent_t e0, e1;
       HIPCHECK(hipEventCreate(&e0));
       HIPCHECK(hipEventCreate(&e1));
-      HIPCHECK(hipMemcpy2DAsync(F->d_b, (size_t)n * sizeof(double), x, (size_t)ldx * sizeof(double),
-                                (size_t)n * sizeof(double), nrhs, hipMemcpyHostToDevice, s));
-      HIPCHECK(hipEventRecord(e0, s));
-      const bool low = F->precision != SYLVER_AMD_PREC_FP64;
-      const bool refine = low && job == 0 && xo.refine != SYLVER_AMD_REFINE_NONE && !F->has_scaling;
       const int64_t nn = (int64_t)n * nrhs;
-      if (F->ind && !F->posdef) {
+      // the caller's x, staged (ld n); permuted (and for jobs 0/1 scaled)
+      // into pivot order on the device
+      const bool permute = user_order && P->d_invp;
+      const bool scaled = user_order && F->has_scaling;
+      const bool low = F->precision != SYLVER_AMD_PREC_FP64;
+      const bool refine = low && job == 0 && xo.refine != SYLVER_AMD_REFINE_NONE && F->posdef;
+      HIPCHECK(hipMemcpy2DAsync(permute ? F->d_xin : F->d_b, (size_t)n * sizeof(double), x,
+                                (size_t)ldx * sizeof(double), (size_t)n * sizeof(double), nrhs,
+                                hipMemcpyHostToDevice, s));
+      HIPCHECK(hipEventRecord(e0, s));
+      // refinement works on the unscaled system and applies S inside its
+      // preconditioner; the plain solves take S on the way in and out
+      const bool scale_in = scaled && !refine && (job == 0 || job == 1);
+      const bool scale_out = scaled && !refine && (job == 0 || job == 3 || job == 4);
+      if (permute)
+         k_perm_in<<<grid1(nn), kThreads, 0, s>>>(n, nrhs, P->d_invp, scale_in ? F->d_scaling : nullptr, F->d_xin,
+                                                  n, F->d_b);
+      else if (scale_in)
+         k_scale_rows<<<grid1(nn), kThreads, 0, s>>>(n, nrhs, F->d_scaling, F->d_b, F->d_b);
+      if (F->ind && !F->posdef && !refine) {
          solve_indef(P, F, job, F->d_b, F->d_sol, nrhs, s);
          res.refine_mode = SYLVER_AMD_REFINE_NONE;
       } else if (!refine) {
@@ -1323,7 +1365,7 @@ SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int
          res.refine_mode = SYLVER_AMD_REFINE_NONE;
       } else {
          // FP64 refinement: x0 = 0, r0 = b; classical steps
-         // x_{k+1} = x_k + solve(r_k), r_{k+1} = b - A x_{k+1}; in AUTO mode a
+         // x_{k+1} = x_k + M^{-1} r_k, r_{k+1} = b - A x_{k+1}; in AUTO mode a
          // step that does not halve the backward error switches the remaining
          // steps to GMRES-IR corrections.
          k_fill<<<grid1(nn), kThreads, 0, s>>>(nn, 0.0, F->d_x);
@@ -1334,16 +1376,23 @@ SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int
          k_copy<<<grid1(nn), kThreads, 0, s>>>(nn, F->d_b, F->d_r);
          double err = 1e300;
          int it = 0;
-         bool gm = xo.refine == SYLVER_AMD_REFINE_GMRES;
+         bool gm = xo.refine == SYLVER_AMD_REFINE_GMRES, converged = false, broken = false;
          for (; it < xo.refine_max_iter; ++it) {
             if (gm)
                gmres_correction<float>(P, F, F->d_r, F->d_sol, nrhs, 20, 1e-10, s);
             else
-               solve_once<float>(P, F, 0, F->d_r, F->d_sol, nrhs, s);
+               precond<float>(P, F, F->d_r, F->d_sol, nrhs, s);
             double e = refine_update_err(P, F, F->d_sol, nrhs, s);
+            if (!std::isfinite(e)) {
+               err = e;
+               broken = true;
+               ++it;
+               break;
+            }
             if (!gm && xo.refine == SYLVER_AMD_REFINE_AUTO && it > 0 && !(e < 0.5 * err)) gm = true;
             err = e;
-            if (!(e > xo.refine_tol)) {
+            if (e <= xo.refine_tol) {
+               converged = true;
                ++it;
                break;
             }
@@ -1352,9 +1401,21 @@ SolveResult device_solve(const Analysis& a, DevicePlan* P, DeviceFactors* F, int
          res.refine_mode = gm ? SYLVER_AMD_REFINE_GMRES : SYLVER_AMD_REFINE_IR;
          res.refine_iters = it;
          res.bwd_err = err;
+         if (broken)
+            res.flag = SYLVER_AMD_ERROR_REFINE;
+         else if (!converged)
+            res.flag = SYLVER_AMD_WARNING_REFINE;
+      }
+      double* out = F->d_sol;
+      if (permute) {
+         k_perm_out<<<grid1(nn), kThreads, 0, s>>>(n, nrhs, P->d_invp, scale_out ? F->d_scaling : nullptr, F->d_sol,
+                                                   F->d_xin, n);
+         out = F->d_xin;
+      } else if (scale_out) {
+         k_scale_rows<<<grid1(nn), kThreads, 0, s>>>(n, nrhs, F->d_scaling, F->d_sol, F->d_sol);
       }
       HIPCHECK(hipEventRecord(e1, s));
-      HIPCHECK(hipMemcpy2DAsync(x, (size_t)ldx * sizeof(double), F->d_sol, (size_t)n * sizeof(double),
+      HIPCHECK(hipMemcpy2DAsync(x, (size_t)ldx * sizeof(double), out, (size_t)n * sizeof(double),
                                 (size_t)n * sizeof(double), nrhs, hipMemcpyDeviceToHost, s));
       HIPCHECK(hipStreamSynchronize(s));
       float t = 0;
@@ -1544,7 +1605,7 @@ int time_fronts(int K, const int* m, const int* n, const signed char* variant, i
    for (int i = 0; i < K; ++i)
       if (n[i] < 1 || m[i] < n[i]) return SYLVER_ERROR_CALL_SEQUENCE;
    if (device_count() <= 0) return SYLVER_ERROR_CUDA_UNKNOWN;
-   DevicePlan* P = nullptr;
+   PlanPtr P;
    DeviceFactors* F = nullptr;
    try {
       // a forest of K roots: rows 1..m_i of a matrix of order max m_i
@@ -1563,7 +1624,7 @@ int time_fronts(int K, const int* m, const int* n, const signed char* variant, i
          for (int r = 1; r <= m[i]; ++r) a.rlist.push_back(r);
       }
       if (variant) a.variant.assign(variant, variant + K);
-      P = build_plan(a, 256, 1 << 30);
+      P.reset(build_plan(a, 256, 1 << 30));
       const bool low = precision != SYLVER_AMD_PREC_FP64;
       const size_t es = low ? sizeof(float) : sizeof(double);
       F = new DeviceFactors();
@@ -1585,17 +1646,16 @@ int time_fronts(int K, const int* m, const int* n, const signed char* variant, i
          ld[i] = P->hnodes[i].ld;
          off[i] = P->hnodes[i].off;
       }
-      *ms = low ? time_fronts_t<float>(P, F, nrow, ld, off, reps) : time_fronts_t<double>(P, F, nrow, ld, off, reps);
+      *ms = low ? time_fronts_t<float>(P.get(), F, nrow, ld, off, reps)
+                : time_fronts_t<double>(P.get(), F, nrow, ld, off, reps);
       std::vector<int> info(K);
       HIPCHECK(hipMemcpy(info.data(), F->d_info, K * sizeof(int), hipMemcpyDeviceToHost));
       destroy_factors(F);
-      destroy_plan(P);
       for (int v : info)
          if (v != 0) return SYLVER_ERROR_NOT_POS_DEF;
       return 0;
    } catch (const HipError&) {
       destroy_factors(F);
-      destroy_plan(P);
       return SYLVER_ERROR_CUDA_UNKNOWN;
    }
 }
@@ -1613,7 +1673,7 @@ int batched_front_potrf_v(int nfronts, const int* nrow, const int* ncol, const i
    for (int f = 0; f < nfronts; ++f)
       if (nrow[f] < ncol[f] || ncol[f] < 1) return SYLVER_ERROR_CALL_SEQUENCE;
    if (device_count() <= 0) return SYLVER_ERROR_CUDA_UNKNOWN;
-   DevicePlan* P = nullptr;
+   PlanPtr P;
    DeviceFactors* F = nullptr;
    try {
       Analysis a;
@@ -1633,7 +1693,7 @@ int batched_front_potrf_v(int nfronts, const int* nrow, const int* ncol, const i
          for (int r = 1; r <= nrow[i]; ++r) a.rlist.push_back(r);
       }
       if (variant) a.variant.assign(variant, variant + nfronts);
-      P = build_plan(a, 256, 1 << 30);
+      P.reset(build_plan(a, 256, 1 << 30));
       const bool low = dtype != 0;
       const size_t es = low ? sizeof(float) : sizeof(double);
       F = new DeviceFactors();
@@ -1654,9 +1714,9 @@ int batched_front_potrf_v(int nfronts, const int* nrow, const int* ncol, const i
       HIPCHECK(hipEventCreate(&e1));
       HIPCHECK(hipEventRecord(e0, F->stream));
       if (low)
-         run_factor_program<float>(P, F, F->stream, nullptr, true);
+         run_factor_program<float>(P.get(), F, F->stream, nullptr, true);
       else
-         run_factor_program<double>(P, F, F->stream, nullptr, true);
+         run_factor_program<double>(P.get(), F, F->stream, nullptr, true);
       HIPCHECK(hipEventRecord(e1, F->stream));
       HIPCHECK(hipStreamSynchronize(F->stream));
       float t = 0;
@@ -1667,12 +1727,10 @@ int batched_front_potrf_v(int nfronts, const int* nrow, const int* ncol, const i
       HIPCHECK(hipMemcpy(info, F->d_info, nfronts * sizeof(int), hipMemcpyDeviceToDevice));
       F->arena = nullptr;
       destroy_factors(F);
-      destroy_plan(P);
       return 0;
    } catch (const HipError&) {
       if (F) F->arena = nullptr;
       destroy_factors(F);
-      destroy_plan(P);
       return SYLVER_ERROR_CUDA_UNKNOWN;
    }
 }

@@ -33,7 +33,7 @@ struct FactorResult {
 };
 
 struct SolveResult {
-   int flag = 0;
+   int flag = 0;          // 0, or SYLVER_AMD_WARNING_REFINE / an error flag
    int hip_error = 0;
    double solve_ms = 0;
    int refine_iters = 0;
@@ -47,24 +47,34 @@ sylver_amd_options_t& ext_options();
 // Number of HIP devices (0 on a host without a GPU).
 int device_count();
 
-// Plan lifetime (owned by the akeep).
-void destroy_plan(DevicePlan* plan);
+// Plan lifetime: shared by the akeep (the plan of its current pattern and
+// variants) and every factorization built on it, so that a plan rebuilt for
+// new variants never frees the one older factors still use.
+using PlanPtr = std::shared_ptr<DevicePlan>;
 // Factor lifetime (owned by the fkeep).
 void destroy_factors(DeviceFactors* f);
 
 // Numeric factorization on the device. `aval` are the cleaned matrix values
-// (host, FP64, addressed by nlist); `scaling` may be null. Creates or reuses
-// *plan and *fact.
-FactorResult device_factor(const Analysis& a, DevicePlan** plan, DeviceFactors** fact,
+// (host, FP64, addressed by nlist); `scaling` (pivot order, may be null) is
+// applied as S A S in the A-scatter. Creates or reuses *plan and *fact.
+FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fact,
                            bool posdef, const double* aval, const double* scaling,
                            const sylver_options_t& opts);
 
 // Full solve (job 0) or partial solves (1 fwd, 2 diag, 3 bwd, 4 diag+bwd) of
-// nrhs right-hand sides held in pivot order on the host (column-major,
-// ld = ldx). With low-precision factors and job 0, FP64 refinement against
-// the original matrix is applied.
-SolveResult device_solve(const Analysis& a, DevicePlan* plan, DeviceFactors* fact,
-                         int job, int nrhs, double* x, int ldx);
+// nrhs right-hand sides on the host (column-major, ld = ldx), with the
+// factors' own plan. user_order: x is in the caller's variable order and the
+// permutation to pivot order, the scaling (x2 = S x before the forward solve,
+// x = S x2 after the backward one, as the reference's solve driver does) and
+// the inverse permutation run on the device; otherwise x is in pivot order
+// and neither is applied (the internal seam). With low-precision factors
+// and job 0, FP64 refinement against the original (unscaled) matrix is
+// applied, preconditioned by S (L L^T)^{-1} S.
+SolveResult device_solve(DeviceFactors* fact, int job, int nrhs, double* x, int ldx, bool user_order);
+
+// Waits for all device work of the process's current device (no-op without a
+// device).
+void device_sync();
 
 // Statistics and read-back.
 int factors_stats(const DeviceFactors* f, sylver_amd_stats_t* st);

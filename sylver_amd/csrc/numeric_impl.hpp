@@ -153,19 +153,21 @@ struct DevicePlan {
    int* d_csr_col = nullptr;
    int64_t* d_csr_src = nullptr;
    int64_t clean_nnz = 0;
+   // pivot order -> caller's order: invp[i] = 1-based variable at pivot i
+   int* d_invp = nullptr;
 
    ~DevicePlan() {
       for (void* p : {(void*)d_sptr, (void*)d_sparent, (void*)d_rptr, (void*)d_rlist,
                       (void*)d_nptr, (void*)d_nlist, (void*)d_cmap, (void*)d_cmap_off,
                       (void*)d_nodes, (void*)d_zero, (void*)d_init, (void*)d_ea,
                       (void*)d_nlistv, (void*)d_diag, (void*)d_gemm, (void*)d_stask, (void*)d_gch, (void*)d_ctask, (void*)d_cgch,
-                      (void*)d_csr_ptr, (void*)d_csr_col, (void*)d_csr_src})
+                      (void*)d_csr_ptr, (void*)d_csr_col, (void*)d_csr_src, (void*)d_invp})
          dfree(p);
    }
 };
 
 struct DeviceFactors {
-   const DevicePlan* plan = nullptr;
+   PlanPtr plan;  // the plan the factors (and their captured graphs) were built with
    int precision = SYLVER_AMD_PREC_FP64;
    bool posdef = true;
    void* arena = nullptr;
@@ -180,6 +182,7 @@ struct DeviceFactors {
    hipStream_t stream2 = nullptr;    // look-ahead: far trailing updates
    std::vector<hipEvent_t> xev;      // cross-stream events of the factor program
    hipGraphExec_t graph = nullptr;
+   bool graph_scaled = false;  // the captured A-scatter applies the scaling
    bool has_scaling = false;
    // solve workspaces
    int ws_nrhs = 0;
@@ -188,6 +191,8 @@ struct DeviceFactors {
    double* d_x = nullptr;
    double* d_r = nullptr;
    double* d_sol = nullptr;
+   double* d_xin = nullptr;      // the caller's x (caller's order), staged for the permutation
+   double* d_sc = nullptr;       // scaled right-hand sides of the preconditioner (S r)
    double* d_part_cb = nullptr;  // backward CB partial sums: cb_slots x 64 x nrhs
    int* d_chain = nullptr;       // k_chain: count of waits that gave up
    double* d_ybuf = nullptr;     // k_chain: exchange buffer of the diagonal blocks of y
@@ -233,7 +238,7 @@ struct DeviceFactors {
                       (void*)d_info, (void*)d_xloc, (void*)d_b, (void*)d_x, (void*)d_r,
                       (void*)d_sol, (void*)d_norms, (void*)d_V, (void*)d_W, (void*)d_tmp,
                       (void*)d_part, (void*)d_coef, (void*)d_chain, (void*)d_ybuf,
-                      (void*)d_xrm, (void*)d_maxpart})
+                      (void*)d_xrm, (void*)d_maxpart, (void*)d_xin, (void*)d_sc})
          dfree(p);
       for (auto e : xev) (void)hipEventDestroy(e);
       if (stream2) (void)hipStreamDestroy(stream2);

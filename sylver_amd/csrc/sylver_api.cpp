@@ -8,7 +8,7 @@
 #include <new>
 #include <vector>
 
-#include "../../include/sylver.h"
+#include "../../include/sylver/sylver.h"
 #include "../../include/sylver_amd.h"
 #include "analyse.hpp"
 #include "numeric.hpp"
@@ -20,12 +20,8 @@ namespace {
 struct Akeep {
    uint32_t magic = 0x414b4550;  // "AKEP"
    Analysis a;
-   DevicePlan* plan = nullptr;
-   ~Akeep() {
-      destroy_plan(plan);
-      plan = nullptr;
-      magic = 0;
-   }
+   PlanPtr plan;  // shared with the factorizations built on it
+   ~Akeep() { magic = 0; }
 };
 
 struct Fkeep {
@@ -34,7 +30,6 @@ struct Fkeep {
    DeviceFactors* f = nullptr;
    bool posdef = true;
    bool factored = false;
-   bool has_scaling = false;
    ~Fkeep() {
       destroy_factors(f);
       f = nullptr;
@@ -79,7 +74,10 @@ void sylver_init(int ncpu, int ngpu) {
    g_init = true;
 }
 
-void sylver_finalize(void) { g_init = false; }
+void sylver_finalize(void) {
+   device_sync();
+   g_init = false;
+}
 
 // Defaults: src/sylver_datatypes_mod.F90:97-198.
 void sylver_default_options(sylver_options_t* o) {
@@ -200,17 +198,21 @@ void spldlt_factorize(bool posdef, long const* ptr, int const* row, double const
             clean[a.conv_dst[k]] += val[a.conv_src[k]];
          aval = clean.data();
       }
-      // Scaling: options->scaling < 0 means "use the caller's scale array"
-      // (the SSIDS convention the reference inherits). Computed scalings
-      // (MC64/auction/equilibration) are not built on this path.
+      // Scaling (src/spldlt_factorize_mod.F90:707-835): options->scaling <= 0
+      // applies the caller's scale array when one is given (case(:0));
+      // it is kept in pivot order, fkeep%scaling(i) = scale(invp(i)).
       std::vector<double> sc;
       const double* scaling = nullptr;
-      if (options->scaling < 0 && scale) {
-         sc.resize(a.n);
-         for (int i = 0; i < a.n; ++i) sc[i] = scale[a.invp[i] - 1];
-         scaling = sc.data();
+      if (options->scaling <= 0) {
+         if (scale) {
+            sc.resize(a.n);
+            for (int i = 0; i < a.n; ++i) sc[i] = scale[a.invp[i] - 1];
+            scaling = sc.data();
+         }
+      } else {
+         inform->flag = SYLVER_ERROR_UNIMPLEMENTED;
+         return;
       }
-      fk->has_scaling = scaling != nullptr;
       FactorResult r = device_factor(a, &ak->plan, &fk->f, posdef, aval, scaling, *options);
       inform->flag = r.flag;
       inform->cuda_error = r.hip_error;
@@ -276,17 +278,10 @@ void spldlt_solve(int job, int nrhs, double* x, int ldx, void* akeep, void* fkee
       return;
    }
    try {
-      int n = a.n;
-      std::vector<double> x2((size_t)n * nrhs);
-      for (int r = 0; r < nrhs; ++r)
-         for (int i = 0; i < n; ++i) x2[(size_t)r * n + i] = x[(size_t)r * ldx + a.invp[i] - 1];
-      SolveResult s = device_solve(a, ak->plan, fk->f, job, nrhs, x2.data(), n);
+      // permutation, scaling and their inverses run on the device
+      SolveResult s = device_solve(fk->f, job, nrhs, x, ldx, true);
       inform->flag = s.flag;
       inform->cuda_error = s.hip_error;
-      if (s.flag >= 0)
-         for (int r = 0; r < nrhs; ++r)
-            for (int i = 0; i < n; ++i)
-               x[(size_t)r * ldx + a.invp[i] - 1] = x2[(size_t)r * n + i];
       tail.solve_ms = s.solve_ms;
       tail.refine_iters = s.refine_iters;
       tail.refine_mode = s.refine_mode;
@@ -378,7 +373,7 @@ int sylver_amd_fkeep_device_symbolic(void const* fkeep, int* sptr, int* sparent,
                                      long* cmap_off) {
    const Fkeep* fk = as_fkeep(const_cast<void*>(fkeep));
    if (!fk || !fk->ak || !fk->ak->plan) return SYLVER_ERROR_CALL_SEQUENCE;
-   return plan_device_symbolic(fk->ak->plan, sptr, sparent, reinterpret_cast<int64_t*>(rptr),
+   return plan_device_symbolic(fk->ak->plan.get(), sptr, sparent, reinterpret_cast<int64_t*>(rptr),
                                rlist, reinterpret_cast<int64_t*>(nptr),
                                reinterpret_cast<int64_t*>(nlist), cmap,
                                reinterpret_cast<int64_t*>(cmap_off));
@@ -628,8 +623,11 @@ void spldlt_destroy_numeric_tree_dbl(bool posdef, void* t) {
 
 static int seam_solve(const void* tree, int job, int nrhs, double* x, int ldx) {
    auto* t = static_cast<const SeamNumeric*>(tree);
-   if (!t || !t->sym) return SYLVER_ERROR_CALL_SEQUENCE;
-   SolveResult s = device_solve(t->sym->ak.a, t->sym->ak.plan, t->f, job, nrhs, x, ldx);
+   if (!t || !t->sym || !t->f) return SYLVER_ERROR_CALL_SEQUENCE;
+   if (nrhs < 1 || ldx < t->sym->ak.a.n || (!x && t->sym->ak.a.n > 0)) return SYLVER_ERROR_X_SIZE;
+   // the numeric tree's solves: x already in pivot order, no scaling (the
+   // reference's driver applies both around these calls)
+   SolveResult s = device_solve(t->f, job, nrhs, x, ldx, false);
    return s.flag;
 }
 

@@ -1,7 +1,8 @@
 """CPU tests of the library boundary and its symbolic analysis (no GPU):
 - the shared library loads and exports every function include/*.h declares;
 - sylver_inform_t / sylver_options_t have the reference's layout (checked
-  against offsetof() from the C compiler on include/sylver.h);
+  against offsetof() from the C compiler on include/sylver/sylver.h, the
+  reference's include path);
 - default options are the reference's (src/sylver_datatypes_mod.F90:97-198);
 - the supernodal structure contains the exact Cholesky structure (brute
   force symbolic elimination on small graphs) for several orderings;
@@ -25,7 +26,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared_functions():
     names = []
-    for h in ("sylver.h", "sylver_amd.h"):
+    for h in ("sylver/sylver.h", "sylver_amd.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"^\s*(?:extern\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", txt, re.M):
@@ -47,7 +48,7 @@ def test_struct_layout_matches_header(S):
     src = r"""
 #include <stdio.h>
 #include <stddef.h>
-#include "sylver.h"
+#include "sylver/sylver.h"
 #define P(T, f) printf(#T " " #f " %zu\n", offsetof(T, f));
 int main(void) {
   P(sylver_inform_t, flag) P(sylver_inform_t, num_factor) P(sylver_inform_t, num_flops)

@@ -119,6 +119,41 @@ def test_multifrontal_nd_orders(S, O, grid):
             assert np.abs(np.tril(O.node_factor(sym, off, lcol, s)) - L[np.ix_(rl, cols)]).max() < 1e-13
 
 
+def test_multifrontal_scaled_equals_dense_cholesky_of_sas(S, O):
+    """With a scaling s (pivot order in the oracle, the reference's
+    fkeep%scaling), the factors are the Cholesky factor of P S A S P^T: a
+    wrong row/column index into s fails this."""
+    A = M.laplacian_3d(5, 4, 3)
+    sym = _analyse(S, A, order=M.grid_nd_order(5, 4, 3))
+    s = np.random.default_rng(7).uniform(0.25, 4.0, A.n)
+    inv = sym["invp"] - 1
+    st, off, lcol = O.factor_posdef(sym, A.val, scaling=s[inv], nb=4)
+    assert st == 0
+    D = A.to_dense() * np.outer(s, s)
+    L = np.linalg.cholesky(D[np.ix_(inv, inv)])
+    for k in range(sym["nnodes"]):
+        rl = sym["rlist"][sym["rptr"][k] - 1:sym["rptr"][k + 1] - 1] - 1
+        cols = np.arange(sym["sptr"][k] - 1, sym["sptr"][k + 1] - 1)
+        assert np.abs(np.tril(O.node_factor(sym, off, lcol, k)) - L[np.ix_(rl, cols)]).max() < 1e-13
+
+
+def test_indef_scaled_solves_unscaled_system(S, O):
+    """Indefinite oracle with a scaling: inertia unchanged (S A S is congruent
+    to A) and x = S (S A S)^{-1} S b solves A x = b; with s applied to the
+    wrong index the solve fails this."""
+    A = M.kkt(6, 40, seed=4)
+    sym = _analyse(S, A)
+    s = np.random.default_rng(8).uniform(0.25, 4.0, A.n)
+    inv = sym["invp"] - 1
+    F = O.IndefFactor(sym, A.val, scaling=s[inv])
+    assert F.stats["flag"] == 0 and F.stats["num_neg"] == 40
+    b = np.random.default_rng(9).uniform(-1, 1, (A.n, 2))
+    xp = F.solve(b[inv] * s[inv][:, None]) * s[inv][:, None]
+    x = np.empty_like(xp)
+    x[inv] = xp
+    assert O.bwd_err(A, x, b) < 1e-15
+
+
 @pytest.mark.parametrize("name", ["simple_mat", "c_example", "lap2d_4", "lap3d_3"])
 def test_solve_known_answers(S, O, name):
     c = _case(name)
