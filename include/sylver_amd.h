@@ -305,6 +305,27 @@ int sylver_amd_mlp_train(float *params, float const *x, float const *y, int n, i
                          float lr, unsigned long long seed, float *loss_hist);
 int sylver_amd_mlp_predict(float const *params, float const *x, int n, float *out);
 
+/* ---- Scalings and matching-based ordering (host code) ----
+ * The reference calls SPRAL for these (src/spldlt_factorize_mod.F90:734-835:
+ * hungarian_scale_sym, auction_scale_sym; src/spldlt_analyse_mod.F90:789-820:
+ * match_order_metis); spldlt_factorize selects them with options.scaling =
+ * 1 / 2 and spldlt_analyse with options.ordering = 2. Exposed with SPRAL's
+ * argument meaning for direct use and testing. The matrix is the lower
+ * triangle, 1-based CSC; outputs are in the caller's variable order:
+ * scaling[n] > 0 with every |scaling_i a_ij scaling_j| <= 1; match[n] (may
+ * be NULL) the variable matched to each variable, -1 for none (0-based).
+ * hungarian: returns 0, 1 (structurally singular, scaled anyway when
+ * scale_if_singular) or -2 (structurally singular, not scaled). auction:
+ * returns 0 or 1 (some variables unmatched within its iteration limit).
+ * match_order: order[n] = 1-based pivot position of each variable, matched
+ * pairs adjacent; returns 0 or 1 (structurally singular). */
+int sylver_amd_hungarian_scale_sym(int n, long const *ptr, int const *row, double const *val,
+                                   int scale_if_singular, double *scaling, int *match);
+int sylver_amd_auction_scale_sym(int n, long const *ptr, int const *row, double const *val,
+                                 double *scaling, int *match);
+int sylver_amd_match_order(int n, long const *ptr, int const *row, double const *val, int *order,
+                           double *scaling);
+
 /* Number of HIP devices visible (0 when none); never fails. */
 int sylver_amd_device_count(void);
 

@@ -154,6 +154,7 @@ EXPORTED = [
     "sylver_amd_last_batched_ms", "sylver_amd_set_front_variants", "sylver_amd_time_fronts",
     "sylver_amd_batched_front_potrf_v",
     "sylver_amd_mlp_init", "sylver_amd_mlp_grad", "sylver_amd_mlp_train", "sylver_amd_mlp_predict",
+    "sylver_amd_hungarian_scale_sym", "sylver_amd_auction_scale_sym", "sylver_amd_match_order",
 ]
 
 
@@ -206,6 +207,9 @@ def lib():
     L.sylver_amd_mlp_grad.argtypes = [fp, fp, fp, C.c_int, fp, fp]
     L.sylver_amd_mlp_train.argtypes = [fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp]
     L.sylver_amd_mlp_predict.argtypes = [fp, fp, C.c_int, fp]
+    L.sylver_amd_hungarian_scale_sym.argtypes = [C.c_int, lp, ip, dp, C.c_int, dp, ip]
+    L.sylver_amd_auction_scale_sym.argtypes = [C.c_int, lp, ip, dp, dp, ip]
+    L.sylver_amd_match_order.argtypes = [C.c_int, lp, ip, dp, ip, dp]
     _lib = L
     return L
 
@@ -589,3 +593,34 @@ def mlp_predict(params, x):
         raise RuntimeError(f"sylver_amd_mlp_predict failed: {rc}")
     return out
 
+
+
+# ---- scalings and matching-based ordering (host code; see include/sylver_amd.h) ----
+def hungarian_scale_sym(A, scale_if_singular=True):
+    """Returns (flag, scaling, match) in the caller's variable order."""
+    s = np.zeros(A.n)
+    m = np.zeros(A.n, np.int32)
+    flag = lib().sylver_amd_hungarian_scale_sym(
+        A.n, _p(np.ascontiguousarray(A.ptr, np.int64), C.c_long), _p(np.ascontiguousarray(A.row, np.int32), C.c_int),
+        _p(np.ascontiguousarray(A.val, np.float64), C.c_double), int(scale_if_singular), _p(s, C.c_double),
+        _p(m, C.c_int))
+    return flag, s, m
+
+
+def auction_scale_sym(A):
+    s = np.zeros(A.n)
+    m = np.zeros(A.n, np.int32)
+    flag = lib().sylver_amd_auction_scale_sym(
+        A.n, _p(np.ascontiguousarray(A.ptr, np.int64), C.c_long), _p(np.ascontiguousarray(A.row, np.int32), C.c_int),
+        _p(np.ascontiguousarray(A.val, np.float64), C.c_double), _p(s, C.c_double), _p(m, C.c_int))
+    return flag, s, m
+
+
+def match_order(A):
+    """Returns (flag, order (1-based positions), scaling)."""
+    order = np.zeros(A.n, np.int32)
+    s = np.zeros(A.n)
+    flag = lib().sylver_amd_match_order(
+        A.n, _p(np.ascontiguousarray(A.ptr, np.int64), C.c_long), _p(np.ascontiguousarray(A.row, np.int32), C.c_int),
+        _p(np.ascontiguousarray(A.val, np.float64), C.c_double), _p(order, C.c_int), _p(s, C.c_double))
+    return flag, order, s

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsylver_amd.so")
 
-SOURCES = ["numeric.hip", "indef.hip", "costmodel.hip", "analyse.cpp", "sylver_api.cpp"]
+SOURCES = ["numeric.hip", "indef.hip", "costmodel.hip", "analyse.cpp", "scaling.cpp", "sylver_api.cpp"]
 
 HIP_FLAGS = [
     "-O3",

@@ -6,6 +6,7 @@
 // same outputs and conventions. build_map and the symbolic tree follow the
 // reference files cited at each function.
 #include "analyse.hpp"
+#include "scaling.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -21,6 +22,7 @@ constexpr int kErrANOOR = -2;
 constexpr int kErrAPtr = -3;
 constexpr int kErrAAllOOR = -4;
 constexpr int kErrOrder = -8;
+constexpr int kErrVal = -9;
 constexpr int kWarnIdxOOR = 1;
 constexpr int kWarnDupIdx = 2;
 constexpr int kWarnDupAndOOR = 3;
@@ -466,7 +468,15 @@ void build_symbolic_tree(Analysis& a) {
 // ---------------------------------------------------------------------------
 // Full analysis.
 // ---------------------------------------------------------------------------
-int analyse(int n, int* order, const int64_t* ptr, const int* row, bool check,
+std::vector<double> clean_values(const Analysis& a, const double* val) {
+   const int64_t nz = a.n > 0 ? a.ptr[a.n] - 1 : 0;
+   if (!a.has_conv_map) return std::vector<double>(val, val + nz);
+   std::vector<double> clean(nz, 0.0);
+   for (size_t k = 0; k < a.conv_src.size(); ++k) clean[a.conv_dst[k]] += val[a.conv_src[k]];
+   return clean;
+}
+
+int analyse(int n, int* order, const int64_t* ptr, const int* row, const double* val, bool check,
             int ordering, int nemin, Analysis& a) {
    a = Analysis();
    a.n = n;
@@ -567,6 +577,16 @@ int analyse(int n, int* order, const int64_t* ptr, const int* row, bool check,
          if (o < 1 || o > n || seen[o - 1]) return kErrOrder;
          seen[o - 1] = 1;
          ord[i] = o;
+      }
+   } else if (ordering == 2) {
+      // matching-based ordering (src/spldlt_analyse_mod.F90:789-820); the
+      // matching's scaling is kept for spldlt_factorize with scaling = 3
+      if (n > 0 && val == nullptr) return kErrVal;
+      if (n > 0) {
+         const std::vector<double> cv = clean_values(a, val);
+         a.scaling.assign(n, 1.0);
+         if (match_order(n, cptr, crow, cv.data(), ord.data(), a.scaling.data()) == 1 && flag == 0)
+            flag = kWarnAnalSingular;
       }
    } else {
       if (n > 0) nested_dissection(n, cptr, crow, ord.data());

@@ -64,6 +64,9 @@ struct Analysis {
    // offset and leading dimension per front; empty = the plan's own arena.
    std::vector<int64_t> fixed_off;
    std::vector<int> fixed_ld;
+   // Scaling from the matching-based ordering (ordering = 2), caller's
+   // variable order; used by spldlt_factorize with scaling = 3.
+   std::vector<double> scaling;
    // Statistics reported through sylver_inform_t
    int64_t num_factor = 0;  // entries in L (lower)
    int64_t num_flops = 0;   // sum over nodes of sum_{r=m-n+1}^{m} r^2
@@ -75,9 +78,14 @@ struct Analysis {
    int nemin = 32;
 };
 
-// Returns the inform flag (0 on success, <0 error, >0 warning).
-int analyse(int n, int* order, const int64_t* ptr, const int* row, bool check,
+// Returns the inform flag (0 on success, <0 error, >0 warning). `val` is
+// only read for the matching-based ordering (ordering = 2).
+int analyse(int n, int* order, const int64_t* ptr, const int* row, const double* val, bool check,
             int ordering, int nemin, Analysis& out);
+
+// Values of the cleaned pattern (duplicates summed, out-of-range entries
+// dropped) from the caller's val.
+std::vector<double> clean_values(const Analysis& a, const double* val);
 
 // General nested dissection ordering of the graph of a symmetric matrix given
 // by its lower CSC pattern (1-based). order[v] = 1-based position.

@@ -2120,6 +2120,50 @@ static __global__ __launch_bounds__(kThreads) void k_csr_rowabs(int n, const int
    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
 }
 
+// Norm equilibration (MC77-like, infinity norm; Ruiz 2001), one sweep's row
+// maxima of |S A S| over the full CSR in pivot order: each entry's scaled
+// value is formed as the reference's driver loop forms it for the stored
+// lower-triangle entry (row r >= column c in the caller's order):
+// (s_r * |a|) * s_c, so that both triangles round alike.
+static __global__ __launch_bounds__(kThreads) void k_equil_rowmax(int n, const int64_t* __restrict__ rowptr,
+                                                           const int* __restrict__ col,
+                                                           const double* __restrict__ val,
+                                                           const int* __restrict__ invp,
+                                                           const double* __restrict__ s,
+                                                           double* __restrict__ rmax) {
+   const int i = blockIdx.x * kThreads + threadIdx.x;
+   if (i >= n) return;
+   const int oi = invp[i];
+   const double si = s[i];
+   double m = 0.0;
+   for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+      const int j = col[p];
+      const double a = fabs(val[p]), sj = s[j];
+      const double v = oi >= invp[j] ? (si * a) * sj : (sj * a) * si;
+      m = fmax(m, v);
+   }
+   rmax[i] = m;
+}
+
+// s_i /= sqrt(rmax_i) where rmax_i > 0; per-workgroup max |1 - rmax_i| into
+// part[blockIdx.x] (the convergence test).
+static __global__ __launch_bounds__(kThreads) void k_equil_update(int n, const double* __restrict__ rmax,
+                                                           double* __restrict__ s, double* __restrict__ part) {
+   const int i = blockIdx.x * kThreads + threadIdx.x;
+   double dev = 0.0;
+   if (i < n) {
+      const double r = rmax[i];
+      if (r > 0.0) s[i] = s[i] / sqrt(r);
+      dev = fabs(1.0 - r);
+   }
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) dev = fmax(dev, __shfl_xor(dev, o, 64));
+   __shared__ double wm[4];
+   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = dev;
+   __syncthreads();
+   if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+}
+
 // out[0] = max of part[0..cnt) (one workgroup).
 static __global__ __launch_bounds__(kThreads) void k_max_reduce(const double* __restrict__ part, int cnt,
                                                          double* __restrict__ out) {

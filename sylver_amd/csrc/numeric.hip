@@ -929,13 +929,39 @@ void ensure_solve_ws(const DevicePlan* P, DeviceFactors* F, int nrhs) {
    F->ws_nrhs = nrhs;
 }
 
+// Norm equilibration of the CSR values into F->d_scaling (pivot order),
+// following SPRAL's equilib_scale_sym as the reference calls it
+// (src/spldlt_factorize_mod.F90:808-835): s = 1; per sweep the largest
+// scaled entry of every row, s_i /= sqrt(max_i), stop after the sweep whose
+// maxima are all within tol of 1, at most max_it sweeps.
+void equilibrate(const DevicePlan* P, DeviceFactors* F, hipStream_t s, int max_it = 10, double tol = 1e-8) {
+   const int n = P->n;
+   k_fill<<<grid1(n), kThreads, 0, s>>>(n, 1.0, F->d_scaling);
+   const int parts = (int)grid1(n);
+   double* buf = dalloc<double>((size_t)n + parts + 1);
+   double* rmax = buf;
+   double* part = buf + n;
+   for (int it = 0; it < max_it; ++it) {
+      k_equil_rowmax<<<grid1(n), kThreads, 0, s>>>(n, P->d_csr_ptr, P->d_csr_col, F->d_csr_val, P->d_invp,
+                                                   F->d_scaling, rmax);
+      k_equil_update<<<parts, kThreads, 0, s>>>(n, rmax, F->d_scaling, part);
+      k_max_reduce<<<1, kThreads, 0, s>>>(part, parts, part + parts);
+      double dev = 0;
+      HIPCHECK(hipMemcpyAsync(&dev, part + parts, sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      if (dev < tol) break;
+   }
+   dfree(buf);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // Factorization driver
 // ---------------------------------------------------------------------------
 FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fact, bool posdef,
-                           const double* aval, const double* scaling, const sylver_options_t& opts) {
+                           const double* aval, const double* scaling, const sylver_options_t& opts,
+                           bool equilib, double* scaling_out) {
    FactorResult res;
    res.matrix_rank = a.n;
    if (device_count() <= 0) {
@@ -1003,6 +1029,18 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
       F->has_scaling = scaling != nullptr;
       if (scaling) HIPCHECK(hipMemcpyAsync(F->d_scaling, scaling, a.n * sizeof(double), hipMemcpyHostToDevice, s));
       HIPCHECK(hipMemsetAsync(F->d_info, 0, std::max(P->nnodes, 1) * sizeof(int), s));
+      // FP64 values of the whole matrix (CSR, pivot order) for the
+      // equilibration, the norm and the refinement residual
+      if (P->csr_nnz > 0)
+         k_gather_vals<<<grid1(P->csr_nnz), kThreads, 0, s>>>(P->csr_nnz, P->d_csr_src, F->d_aval, nullptr,
+                                                               nullptr, F->d_csr_val);
+      if (equilib) {
+         if (P->csr_nnz <= 0 || !P->d_invp) throw HipError{hipErrorInvalidValue, __LINE__};
+         equilibrate(P, F, s);
+         F->has_scaling = true;
+         if (scaling_out)
+            HIPCHECK(hipMemcpyAsync(scaling_out, F->d_scaling, a.n * sizeof(double), hipMemcpyDeviceToHost, s));
+      }
       HIPCHECK(hipEventRecord(e1, s));
       const bool record = xo.record_timing != 0 && posdef;
       const bool use_graph = xo.use_graph != 0 && !record && posdef;
@@ -1037,11 +1075,7 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
       } else {
          run_factor_program<double>(P, F, s, record ? &events : nullptr);
       }
-   HIPCHECK(hipEventRecord(e2, s));
-      // FP64 values of the whole matrix for the refinement residual
-      if (P->csr_nnz > 0)
-         k_gather_vals<<<grid1(P->csr_nnz), kThreads, 0, s>>>(P->csr_nnz, P->d_csr_src, F->d_aval,
-                                                               nullptr, nullptr, F->d_csr_val);
+      HIPCHECK(hipEventRecord(e2, s));
       std::vector<int> info(std::max(P->nnodes, 1));
       HIPCHECK(hipMemcpyAsync(info.data(), F->d_info, info.size() * sizeof(int), hipMemcpyDeviceToHost, s));
       HIPCHECK(hipStreamSynchronize(s));

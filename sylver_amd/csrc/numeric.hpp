@@ -56,10 +56,15 @@ void destroy_factors(DeviceFactors* f);
 
 // Numeric factorization on the device. `aval` are the cleaned matrix values
 // (host, FP64, addressed by nlist); `scaling` (pivot order, may be null) is
-// applied as S A S in the A-scatter. Creates or reuses *plan and *fact.
+// applied as S A S in the A-scatter. equilib: the scaling is instead
+// computed on the device by norm equilibration of the values (MC77-like:
+// at most 10 sweeps, stop when every scaled row's largest entry is within
+// 1e-8 of 1) and, when scaling_out is given, returned there (pivot order).
+// Creates or reuses *plan and *fact.
 FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fact,
                            bool posdef, const double* aval, const double* scaling,
-                           const sylver_options_t& opts);
+                           const sylver_options_t& opts, bool equilib = false,
+                           double* scaling_out = nullptr);
 
 // Full solve (job 0) or partial solves (1 fwd, 2 diag, 3 bwd, 4 diag+bwd) of
 // nrhs right-hand sides on the host (column-major, ld = ldx), with the

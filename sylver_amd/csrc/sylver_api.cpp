@@ -12,6 +12,7 @@
 #include "../../include/sylver_amd.h"
 #include "analyse.hpp"
 #include "numeric.hpp"
+#include "scaling.hpp"
 
 using namespace sylver_amd;
 
@@ -111,7 +112,6 @@ void sylver_default_options(sylver_options_t* o) {
 void spldlt_analyse(int n, int* order, long const* ptr, int const* row, double const* val,
                     void** akeep, bool check, sylver_options_t const* options,
                     sylver_inform_t* inform) {
-   (void)val;
    sylver_inform_t dummy;
    if (!inform) inform = &dummy;
    clear_inform(inform);
@@ -128,12 +128,10 @@ void spldlt_analyse(int n, int* order, long const* ptr, int const* row, double c
    try {
       ak = new Akeep();
       int nemin = options->nemin < 1 ? 32 : options->nemin;
-      int flag = analyse(n, options->ordering == 0 ? order : (order ? order : nullptr),
-                         reinterpret_cast<const int64_t*>(ptr), row, check,
-                         options->ordering, nemin, ak->a);
-      if (options->ordering != 0 && order == nullptr) {
-         // ordering computed internally; nothing to return to the caller
-      }
+      // ordering 0: the caller's order; 1: nested dissection; 2: matching-based
+      // (src/spldlt_analyse_mod.F90:740-820)
+      int flag = analyse(n, options->ordering == 0 ? order : nullptr, reinterpret_cast<const int64_t*>(ptr),
+                         row, val, check, options->ordering, nemin, ak->a);
       inform->flag = flag;
       if (flag < 0) {
          delete ak;
@@ -192,29 +190,55 @@ void spldlt_factorize(bool posdef, long const* ptr, int const* row, double const
       std::vector<double> clean;
       const double* aval = val;
       if (a.has_conv_map) {
-         int64_t nz = a.n > 0 ? a.ptr[a.n] - 1 : 0;
-         clean.assign(nz, 0.0);
-         for (size_t k = 0; k < a.conv_src.size(); ++k)
-            clean[a.conv_dst[k]] += val[a.conv_src[k]];
+         clean = clean_values(a, val);
          aval = clean.data();
       }
-      // Scaling (src/spldlt_factorize_mod.F90:707-835): options->scaling <= 0
-      // applies the caller's scale array when one is given (case(:0));
-      // it is kept in pivot order, fkeep%scaling(i) = scale(invp(i)).
-      std::vector<double> sc;
+      // Scaling (src/spldlt_factorize_mod.F90:707-835), kept in pivot order
+      // (fkeep%scaling(i) = scaling(invp(i))); computed scalings are copied
+      // to the caller's scale array (caller's order) when one is given.
+      //   <= 0  the caller's scale array, when one is given (case(:0))
+      //      1  matching-based, shortest augmenting paths (MC64-like)
+      //      2  matching-based, auction
+      //      3  the matching-based ordering's scaling saved by the analysis
+      //   >= 4  norm equilibration (MC77-like), computed on the device
+      int warn = 0;
+      if (!a.scaling.empty() && options->scaling != 3) warn = SYLVER_WARNING_MATCH_ORD_NO_SCALE;
+      std::vector<double> sc, user(a.n);
       const double* scaling = nullptr;
-      if (options->scaling <= 0) {
-         if (scale) {
-            sc.resize(a.n);
-            for (int i = 0; i < a.n; ++i) sc[i] = scale[a.invp[i] - 1];
-            scaling = sc.data();
+      bool equilib = false;
+      auto to_pivot = [&](const double* s_user) {
+         sc.resize(a.n);
+         for (int i = 0; i < a.n; ++i) sc[i] = s_user[a.invp[i] - 1];
+         scaling = sc.data();
+      };
+      const int mode = options->scaling;
+      if (mode <= 0) {
+         if (scale) to_pivot(scale);
+      } else if (mode == 1 || mode == 2) {
+         int st = mode == 1 ? hungarian_scale_sym(a.n, a.ptr.data(), a.row.data(), aval, options->action,
+                                                  user.data(), nullptr)
+                            : auction_scale_sym(a.n, a.ptr.data(), a.row.data(), aval, user.data(), nullptr);
+         if (st == -2) {
+            inform->flag = SYLVER_ERROR_SINGULAR;
+            return;
          }
+         to_pivot(user.data());
+         if (scale) std::memcpy(scale, user.data(), a.n * sizeof(double));
+      } else if (mode == 3) {
+         if (a.scaling.empty()) {
+            inform->flag = SYLVER_ERROR_NO_SAVED_SCALING;
+            return;
+         }
+         to_pivot(a.scaling.data());
       } else {
-         inform->flag = SYLVER_ERROR_UNIMPLEMENTED;
-         return;
+         equilib = true;
+         sc.resize(a.n);
       }
-      FactorResult r = device_factor(a, &ak->plan, &fk->f, posdef, aval, scaling, *options);
-      inform->flag = r.flag;
+      FactorResult r = device_factor(a, &ak->plan, &fk->f, posdef, aval, scaling, *options, equilib,
+                                     equilib ? sc.data() : nullptr);
+      if (equilib && scale && r.flag >= 0)
+         for (int i = 0; i < a.n; ++i) scale[a.invp[i] - 1] = sc[i];
+      inform->flag = (r.flag == 0 && warn) ? warn : r.flag;
       inform->cuda_error = r.hip_error;
       inform->maxdepth = a.maxdepth;
       inform->maxfront = a.maxfront;
@@ -480,6 +504,37 @@ int sylver_amd_batched_front_potrf(int nfronts, int const* nrow, int const* ncol
                                  data, dtype, info, stream);
    } catch (...) {
       return SYLVER_ERROR_UNKNOWN;
+   }
+}
+
+int sylver_amd_hungarian_scale_sym(int n, long const* ptr, int const* row, double const* val,
+                                   int scale_if_singular, double* scaling, int* match) {
+   if (n < 0 || (n > 0 && (!ptr || !row || !val || !scaling))) return SYLVER_ERROR_CALL_SEQUENCE;
+   try {
+      return hungarian_scale_sym(n, reinterpret_cast<const int64_t*>(ptr), row, val, scale_if_singular != 0,
+                                 scaling, match);
+   } catch (...) {
+      return SYLVER_ERROR_ALLOCATION;
+   }
+}
+
+int sylver_amd_auction_scale_sym(int n, long const* ptr, int const* row, double const* val, double* scaling,
+                                 int* match) {
+   if (n < 0 || (n > 0 && (!ptr || !row || !val || !scaling))) return SYLVER_ERROR_CALL_SEQUENCE;
+   try {
+      return auction_scale_sym(n, reinterpret_cast<const int64_t*>(ptr), row, val, scaling, match);
+   } catch (...) {
+      return SYLVER_ERROR_ALLOCATION;
+   }
+}
+
+int sylver_amd_match_order(int n, long const* ptr, int const* row, double const* val, int* order,
+                           double* scaling) {
+   if (n < 0 || (n > 0 && (!ptr || !row || !val || !order || !scaling))) return SYLVER_ERROR_CALL_SEQUENCE;
+   try {
+      return match_order(n, reinterpret_cast<const int64_t*>(ptr), row, val, order, scaling);
+   } catch (...) {
+      return SYLVER_ERROR_ALLOCATION;
    }
 }
 

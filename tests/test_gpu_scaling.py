@@ -196,3 +196,93 @@ def test_plan_rebuild_keeps_older_factors(S, O, gpu):
     S.spldlt_free_fkeep(fk1)
     S.spldlt_free_fkeep(fk2)
     S.spldlt_free_akeep(akeep)
+
+
+def test_equilibration_on_device_matches_oracle(S, O, gpu):
+    """options.scaling >= 4: the norm equilibration runs on the device
+    (sweeps over the pivot-order CSR); the scale returned to the caller
+    equals the oracle's restatement bit for bit, and the scaled posdef
+    factorization solves the unscaled system."""
+    from oracle import scaling as OS
+    A = M.laplacian_3d(12)
+    rng = np.random.default_rng(11)
+    f = 10.0 ** rng.uniform(-3, 3, A.n)  # a badly scaled SPD matrix
+    col = np.repeat(np.arange(A.n), np.diff(A.ptr))
+    row = A.row - 1
+    B = M.CSCLower(A.n, A.ptr, A.row, A.val * f[row] * f[col])
+    akeep, opts, sym = _analyse(S, B)
+    opts.scaling = 4
+    s = np.zeros(B.n)
+    fk, inf = S.spldlt_factorize(True, B.val, akeep, scale=s, options=opts)
+    assert inf.flag == 0
+    want, sweeps = OS.equilib_scale_sym(B)
+    assert np.array_equal(s, want), np.abs(s - want).max()
+    b = M.random_rhs(B.n, 2, seed=12)
+    x, inf = _solve(S, akeep, fk, opts, b)
+    assert inf.flag == 0
+    assert O.bwd_err(B, x, b) <= BWD_TOL
+    S.spldlt_free_fkeep(fk)
+    S.spldlt_free_akeep(akeep)
+
+
+@pytest.mark.parametrize("mode", [1, 2, 4])
+def test_computed_scalings_indefinite(S, O, gpu, mode):
+    """KKT with badly scaled constraint rows, scaled by the matching-based
+    scalings (1 shortest augmenting paths, 2 auction) or equilibration (4):
+    exact inertia, backward error of the original system <= 1e-14, and the
+    returned scale equals the host routine's (modes 1, 2)."""
+    A = M.kkt(8, 120, seed=7)
+    rng = np.random.default_rng(13)
+    f = np.ones(A.n)
+    f[512:] = 10.0 ** rng.uniform(-4, 4, 120)
+    col = np.repeat(np.arange(A.n), np.diff(A.ptr))
+    row = A.row - 1
+    B = M.CSCLower(A.n, A.ptr, A.row, A.val * f[row] * f[col])
+    akeep, opts, sym = _analyse(S, B)
+    opts.scaling = mode
+    s = np.zeros(B.n)
+    fk, inf = S.spldlt_factorize(False, B.val, akeep, scale=s, options=opts)
+    assert inf.flag >= 0, inf.flag
+    assert inf.num_neg == 120
+    if mode == 1:
+        assert np.array_equal(s, S.hungarian_scale_sym(B)[1])
+    elif mode == 2:
+        assert np.array_equal(s, S.auction_scale_sym(B)[1])
+    assert np.all(s > 0)
+    b = M.random_rhs(B.n, 2, seed=14)
+    x, inf = _solve(S, akeep, fk, opts, b)
+    assert inf.flag >= 0
+    assert O.bwd_err(B, x, b) <= 1e-14
+    S.spldlt_free_fkeep(fk)
+    S.spldlt_free_akeep(akeep)
+
+
+def test_matching_order_and_saved_scaling(S, O, gpu):
+    """ordering = 2 (matching-based) keeps its scaling for scaling = 3;
+    scaling = 3 without it is SYLVER_ERROR_NO_SAVED_SCALING; another scaling
+    after a matching order returns the warning
+    SYLVER_WARNING_MATCH_ORD_NO_SCALE (src/spldlt_factorize_mod.F90:722-725)."""
+    A = M.kkt(8, 150, seed=9)
+    opts = S.default_options()
+    akeep, inf, _ = S.spldlt_analyse(A.n, A.ptr, A.row, A.val, options=opts)
+    opts.scaling = 3
+    fk, inf = S.spldlt_factorize(False, A.val, akeep, options=opts)
+    assert inf.flag == -15
+    S.spldlt_free_akeep(akeep)
+    opts = S.default_options()
+    opts.ordering = 2
+    akeep, inf, _ = S.spldlt_analyse(A.n, A.ptr, A.row, A.val, options=opts)
+    assert inf.flag >= 0
+    opts.scaling = 3
+    fk, inf = S.spldlt_factorize(False, A.val, akeep, options=opts)
+    assert inf.flag == 0, inf.flag
+    assert inf.num_neg == 150
+    b = M.random_rhs(A.n, 2, seed=15)
+    x, inf = _solve(S, akeep, fk, opts, b)
+    assert O.bwd_err(A, x, b) <= 1e-14
+    opts.scaling = 0
+    fk2, inf = S.spldlt_factorize(False, A.val, akeep, options=opts)
+    assert inf.flag == 8
+    S.spldlt_free_fkeep(fk)
+    S.spldlt_free_fkeep(fk2)
+    S.spldlt_free_akeep(akeep)
