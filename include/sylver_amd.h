@@ -245,6 +245,17 @@ int sylver_amd_fkeep_indef_node(void const *fkeep, int node, int *dims, int *row
  * (the counters of the reference's inform_c, src/sylver_ciface.hxx:73-86). */
 int sylver_amd_fkeep_indef_stats(void const *fkeep, int *out);
 
+/* Host-side check of the device launch plan (no device needed): builds the
+ * plan of the analysis with the given knobs (0 = defaults) and num_cus
+ * compute units, and checks every index its launches use against the
+ * buffers they address (fronts in the arena, inverse blocks, task lists,
+ * child maps, exchange and partial-sum slots). out[8] = {violations, factor
+ * launches, forward-solve launches, backward-solve launches, arena
+ * elements, exchange slots, levels, partial-sum slots}; why (why_len bytes,
+ * may be NULL) receives the first violation. Returns 0 or an error flag. */
+int sylver_amd_plan_check(void const *akeep, int small_front_max, int bf16_min_ncol, int num_cus,
+                          long *out, char *why, int why_len);
+
 /* Batched dense front factorization: for each front f, the nrow[f] x nrow[f]
  * column-major block at data + offset[f] (elements, ld = nrow[f], device
  * memory) is partially Cholesky factorized over its first ncol[f] columns:

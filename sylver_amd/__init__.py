@@ -155,6 +155,7 @@ EXPORTED = [
     "sylver_amd_batched_front_potrf_v",
     "sylver_amd_mlp_init", "sylver_amd_mlp_grad", "sylver_amd_mlp_train", "sylver_amd_mlp_predict",
     "sylver_amd_hungarian_scale_sym", "sylver_amd_auction_scale_sym", "sylver_amd_match_order",
+    "sylver_amd_plan_check",
 ]
 
 
@@ -210,6 +211,7 @@ def lib():
     L.sylver_amd_hungarian_scale_sym.argtypes = [C.c_int, lp, ip, dp, C.c_int, dp, ip]
     L.sylver_amd_auction_scale_sym.argtypes = [C.c_int, lp, ip, dp, dp, ip]
     L.sylver_amd_match_order.argtypes = [C.c_int, lp, ip, dp, ip, dp]
+    L.sylver_amd_plan_check.argtypes = [vp, C.c_int, C.c_int, C.c_int, lp, C.c_char_p, C.c_int]
     _lib = L
     return L
 
@@ -624,3 +626,17 @@ def match_order(A):
         A.n, _p(np.ascontiguousarray(A.ptr, np.int64), C.c_long), _p(np.ascontiguousarray(A.row, np.int32), C.c_int),
         _p(np.ascontiguousarray(A.val, np.float64), C.c_double), _p(order, C.c_int), _p(s, C.c_double))
     return flag, order, s
+
+
+def plan_check(akeep, small_front_max=0, bf16_min_ncol=0, num_cus=256):
+    """Host-side index check of the device launch plan (no device needed).
+    Returns (stats dict, first violation text)."""
+    out = np.zeros(8, np.int64)
+    why = C.create_string_buffer(256)
+    rc = lib().sylver_amd_plan_check(akeep, int(small_front_max), int(bf16_min_ncol), int(num_cus),
+                                     _p(out, C.c_long), why, 256)
+    if rc != 0:
+        raise RuntimeError(f"sylver_amd_plan_check failed: {rc} {why.value.decode()}")
+    keys = ("violations", "factor_launches", "fwd_launches", "bwd_launches", "arena_elems",
+            "exchange_slots", "levels", "cb_slots")
+    return dict(zip(keys, out.tolist())), why.value.decode()

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -63,8 +64,10 @@ void device_sync() {
 
 namespace {
 
-DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
-   auto* P = new DevicePlan();
+// The plan's host side: layout, maps, task lists and launch programs. No
+// HIP call (it also runs, for its checks, on a host without a device).
+DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int num_cus) {
+   std::unique_ptr<DevicePlan> P(new DevicePlan());
    P->n = a.n;
    P->nnodes = a.nnodes;
    P->small_max = small_max;
@@ -463,12 +466,6 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    // the chained kernel addresses them with 32-bit offsets).
    const char* steps_env = std::getenv("SYLVER_AMD_SOLVE_STEPS");
    const bool chain = !(steps_env && steps_env[0] == '1');
-   int num_cus = 0;
-   {
-      int dev = 0;
-      HIPCHECK(hipGetDevice(&dev));
-      HIPCHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
-   }
    std::vector<int> flag0(P->hnodes.size(), -1);
    P->chain_flags = 0;
    if (chain)
@@ -710,12 +707,23 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
             }
          }
       P->csr_nnz = rp[n];
-      P->d_csr_ptr = dupload(rp);
-      P->d_csr_col = dupload(col);
-      P->d_csr_src = dupload(src);
+      P->hcsr_ptr = std::move(rp);
+      P->hcsr_col = std::move(col);
+      P->hcsr_src = std::move(src);
    }
+   return P.release();
+}
 
-   // ---- device copies ----
+// Device copies of the analysis outputs and of the plan's task lists.
+void upload_plan(DevicePlan* P, const Analysis& a) {
+   if (P->csr_nnz > 0) {
+      P->d_csr_ptr = dupload(P->hcsr_ptr);
+      P->d_csr_col = dupload(P->hcsr_col);
+      P->d_csr_src = dupload(P->hcsr_src);
+      std::vector<int64_t>().swap(P->hcsr_ptr);
+      std::vector<int>().swap(P->hcsr_col);
+      std::vector<int64_t>().swap(P->hcsr_src);
+   }
    P->d_sptr = dupload(a.sptr);
    P->d_sparent = dupload(a.sparent);
    P->d_rptr = dupload(a.rptr);
@@ -736,7 +744,193 @@ DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
    P->d_ctask = dupload(P->ctask);
    P->d_cgch = dupload(P->cgch);
    if ((int)a.invp.size() == a.n && a.n > 0) P->d_invp = dupload(a.invp);
-   return P;
+}
+
+// Checks every index a launch of the plan will use against the buffer it
+// addresses (arena, inverse blocks, task lists, maps, exchange slots),
+// without a device. Returns the number of violations; the first is
+// described in *why.
+int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* why) {
+   int64_t bad = 0;
+   auto fail = [&](const std::string& m) {
+      if (bad++ == 0 && why) *why = m;
+   };
+   const int nn = P->nnodes;
+   const int64_t AE = P->arena_elems, DE = P->dinv_elems;
+   auto in_arena = [&](int64_t off, int rows, int cols, int ld) {
+      if (rows <= 0 || cols <= 0) return true;
+      return off >= 0 && off + (int64_t)(rows - 1) + (int64_t)(cols - 1) * ld < AE;
+   };
+   for (int i = 0; i < nn; ++i) {
+      const NodeDesc& d = P->hnodes[i];
+      if (d.m < d.n || d.n < 0 || d.ld < d.m) fail("node " + std::to_string(i) + ": shape");
+      if (!in_arena(d.off, d.m, d.m, d.ld)) fail("node " + std::to_string(i) + ": front outside the arena");
+      if (d.parent < 0 || d.parent > nn) fail("node " + std::to_string(i) + ": parent");
+      if (d.solve_off < 0 || d.solve_off + d.m > P->solve_rows) fail("node " + std::to_string(i) + ": solve rows");
+      if (d.cmap_off < 0 || d.cmap_off + (d.m - d.n) > (int64_t)P->hcmap.size())
+         fail("node " + std::to_string(i) + ": child map");
+      if (d.n > 0) {
+         const int64_t nblk = (d.n + kNB - 1) / kNB, last = d.n - (nblk - 1) * kNB;
+         if (d.dinv_off < 0 || d.dinv_off + (nblk - 1) * kNB * kNB + last * last > DE)
+            fail("node " + std::to_string(i) + ": inverse blocks");
+      }
+      if (d.parent < nn) {
+         const NodeDesc& p = P->hnodes[d.parent];
+         for (int k = 0; k < d.m - d.n; ++k) {
+            const int c = P->hcmap[d.cmap_off + k];
+            if (c < 0 || c >= p.m) fail("node " + std::to_string(i) + ": child map entry");
+         }
+      }
+      if (d.num_a > 0 && (d.amap_off < 0 || 2 * (d.amap_off + d.num_a) > (int64_t)a.nlist.size()))
+         fail("node " + std::to_string(i) + ": A map");
+   }
+   auto check_gemm = [&](const GemmTask& g, bool trsm) {
+      if (g.M <= 0) return;
+      if (!in_arena(g.a_off, g.M, g.K, g.lda)) fail("gemm: A outside the arena");
+      if (trsm) {
+         if (g.b_off < 0 || g.b_off + (int64_t)g.K * g.ldb > DE) fail("trsm: inverse block outside");
+      } else if (!in_arena(g.b_off, g.N, g.K, g.ldb)) {
+         fail("gemm: B outside the arena");
+      }
+      if (!in_arena(g.c_off, g.M, g.N, g.ldc)) fail("gemm: C outside the arena");
+   };
+   auto check_launches = [&](const std::vector<Launch>& prog, bool factor) {
+      for (const Launch& L : prog) {
+         if (L.wait_ev >= P->nevents || L.rec_ev >= P->nevents) fail("launch: event index");
+         if (L.count < 0 || L.off < 0) fail("launch: range");
+         const int64_t e = L.off + L.count;
+         switch (L.kind) {
+            case L_ZERO:
+               if (e > (int64_t)P->zero.size()) fail("zero: range");
+               else
+                  for (int64_t t = L.off; t < e; ++t) {
+                     const ColTask& c = P->zero[t];
+                     if (c.node < 0 || c.node >= nn || c.c0 < 0 || c.c1 > P->hnodes[c.node].m || c.c0 >= c.c1)
+                        fail("zero task");
+                  }
+               break;
+            case L_INIT:
+               if (e > (int64_t)P->init.size()) fail("init: range");
+               else
+                  for (int64_t t = L.off; t < e; ++t) {
+                     const InitTask& c = P->init[t];
+                     if (c.node < 0 || c.node >= nn || c.p0 < 0 || c.p1 > P->hnodes[c.node].num_a) fail("init task");
+                  }
+               break;
+            case L_EA:
+               if (e > (int64_t)P->ea.size()) fail("extend-add: range");
+               else
+                  for (int64_t t = L.off; t < e; ++t) {
+                     const EaTask& c = P->ea[t];
+                     if (c.child < 0 || c.child >= nn || P->hnodes[c.child].parent >= nn || c.j0 < 0 ||
+                         c.j1 > P->hnodes[c.child].m - P->hnodes[c.child].n)
+                        fail("extend-add task");
+                  }
+               break;
+            case L_SMALL:
+            case L_FSMALL:
+            case L_BSMALL:
+               if (e > (int64_t)P->nlistv.size()) fail("node list: range");
+               else
+                  for (int64_t t = L.off; t < e; ++t)
+                     if (P->nlistv[t] < 0 || P->nlistv[t] >= nn) fail("node list entry");
+               break;
+            case L_DIAG:
+            case L_DINV:
+               if (e > (int64_t)P->diag.size()) fail("diag: range");
+               else
+                  for (int64_t t = L.off; t < e; ++t) {
+                     const DiagTask& c = P->diag[t];
+                     if (c.node < 0 || c.node >= nn || c.k0 < 0 || c.kb < 1 || c.kb > kNB ||
+                         c.k0 + c.kb > P->hnodes[c.node].n)
+                        fail("diag task");
+                  }
+               break;
+            case L_TRSM:
+            case L_UPDATE:
+            case L_UPDATE_WIDE:
+               if (e > (int64_t)P->gemm.size()) fail("gemm: range");
+               else
+                  for (int64_t t = L.off; t < e; ++t) check_gemm(P->gemm[t], L.kind == L_TRSM);
+               break;
+            case L_FGATHER:
+            case L_FDIAG:
+            case L_BGATHER:
+            case L_BDIAG:
+            case L_BUPD:
+            case L_BRED:
+               if (e > (int64_t)P->stask.size()) fail("solve task: range");
+               else
+                  for (int64_t t = L.off; t < e; ++t) {
+                     const SolveTask& c = P->stask[t];
+                     if (c.node < 0 || c.node >= nn) {
+                        fail("solve task node");
+                        continue;
+                     }
+                     const NodeDesc& d = P->hnodes[c.node];
+                     if (L.kind == L_FGATHER && (c.r0 < 0 || c.r1 > d.m || c.k0 < 0 || c.k0 + c.kb > (int)P->gch.size()))
+                        fail("forward gather task");
+                     if ((L.kind == L_FDIAG || L.kind == L_BDIAG) &&
+                         (c.k0 < 0 || c.kb < 1 || c.k0 + c.kb > d.n || c.r0 < 0 || c.r1 > d.m))
+                        fail("diagonal solve task");
+                     if (L.kind == L_BGATHER && (c.r0 < 0 || c.r1 > d.m)) fail("backward gather task");
+                     if (L.kind == L_BUPD && (c.k0 < d.n || c.kb > d.m || c.r0 < 0 || c.r1 > d.n || c.write < 0 ||
+                                              c.write >= P->cb_slots))
+                        fail("contribution-block solve task");
+                     if (L.kind == L_BRED && (c.k0 < 0 || c.k0 + c.kb > P->cb_slots || c.r1 > d.n))
+                        fail("reduction task");
+                  }
+               if (L.node1 >= nn) fail("solve launch node");
+               break;
+            case L_FCHAIN:
+            case L_BCHAIN:
+            case L_FCB:
+               if (e > (int64_t)P->ctask.size()) fail("chain: range");
+               else
+                  for (int64_t t = L.off; t < e; ++t) {
+                     const ChainTask& c = P->ctask[t];
+                     if (c.node < 0 || c.node >= nn) {
+                        fail("chain task node");
+                        continue;
+                     }
+                     const NodeDesc& d = P->hnodes[c.node];
+                     if (c.blk < 0 || c.blk > c.nb || c.nb != (d.n + kNB - 1) / kNB || c.r0 < 0 || c.r1 > d.m ||
+                         c.r1 - c.r0 > kNB || c.flag0 < 0 || c.flag0 + c.nb > P->chain_flags ||
+                         c.g0 < 0 || c.g0 + c.gn > (int64_t)P->cgch.size() || c.cb0 < 0 || c.cb0 + c.cbn * ((d.n + 63) / 64) > P->cb_slots)
+                        fail("chain task");
+                  }
+               break;
+            case L_CLEAR:
+            case L_SYNC:
+               break;
+            default:
+               fail("launch kind");
+         }
+         if (factor && L.stream != 0 && L.stream != 1) fail("launch stream");
+      }
+   };
+   check_launches(P->fprog, true);
+   check_launches(P->fwdprog, false);
+   check_launches(P->bwdprog, false);
+   for (const auto* gv : {&P->gch, &P->cgch})
+      for (const int4& g : *gv) {
+         if (g.x < 0 || g.x >= nn) {
+            fail("gather range: child");
+            continue;
+         }
+         const NodeDesc& c = P->hnodes[g.x];
+         if (g.y < 0 || g.y > g.z || g.z > c.m - c.n) fail("gather range: rows");
+      }
+   return bad;
+}
+
+DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
+   int dev = 0, num_cus = 0;
+   HIPCHECK(hipGetDevice(&dev));
+   HIPCHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus));
+   upload_plan(P.get(), a);
+   return P.release();
 }
 
 inline unsigned grid1(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
@@ -955,6 +1149,22 @@ void equilibrate(const DevicePlan* P, DeviceFactors* F, hipStream_t s, int max_i
 }
 
 }  // namespace
+
+int plan_check(const Analysis& a, int small_max, int wide_ncol, int num_cus, int64_t* out, std::string* why) {
+   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus));
+   const int64_t bad = check_plan_indices(P.get(), a, why);
+   if (out) {
+      out[0] = bad;
+      out[1] = (int64_t)P->fprog.size();
+      out[2] = (int64_t)P->fwdprog.size();
+      out[3] = (int64_t)P->bwdprog.size();
+      out[4] = P->arena_elems;
+      out[5] = P->chain_flags;
+      out[6] = P->nlevels;
+      out[7] = P->cb_slots;
+   }
+   return 0;
+}
 
 // ---------------------------------------------------------------------------
 // Factorization driver

@@ -77,6 +77,13 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
 // applied, preconditioned by S (L L^T)^{-1} S.
 SolveResult device_solve(DeviceFactors* fact, int job, int nrhs, double* x, int ldx, bool user_order);
 
+// Builds the host side of the plan of `a` (no device needed) and checks
+// every index its launches will use against the buffers they address.
+// out[8] = {violations, factor launches, forward-solve launches,
+// backward-solve launches, arena elements, exchange slots, levels,
+// partial-sum slots}; *why describes the first violation.
+int plan_check(const Analysis& a, int small_max, int wide_ncol, int num_cus, int64_t* out, std::string* why);
+
 // Waits for all device work of the process's current device (no-op without a
 // device).
 void device_sync();

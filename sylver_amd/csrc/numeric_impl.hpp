@@ -155,6 +155,9 @@ struct DevicePlan {
    int64_t clean_nnz = 0;
    // pivot order -> caller's order: invp[i] = 1-based variable at pivot i
    int* d_invp = nullptr;
+   // host CSR until the upload
+   std::vector<int64_t> hcsr_ptr, hcsr_src;
+   std::vector<int> hcsr_col;
 
    ~DevicePlan() {
       for (void* p : {(void*)d_sptr, (void*)d_sparent, (void*)d_rptr, (void*)d_rlist,

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/sylver/sylver.h"
@@ -535,6 +536,30 @@ int sylver_amd_match_order(int n, long const* ptr, int const* row, double const*
       return match_order(n, reinterpret_cast<const int64_t*>(ptr), row, val, order, scaling);
    } catch (...) {
       return SYLVER_ERROR_ALLOCATION;
+   }
+}
+
+int sylver_amd_plan_check(void const* akeep, int small_front_max, int bf16_min_ncol, int num_cus, long* out,
+                          char* why, int why_len) {
+   const Akeep* ak = as_akeep(const_cast<void*>(akeep));
+   if (!ak || !out || num_cus < 1) return SYLVER_ERROR_CALL_SEQUENCE;
+   try {
+      std::string msg;
+      int64_t o[8];
+      plan_check(ak->a, small_front_max > 0 ? small_front_max : 256, bf16_min_ncol > 0 ? bf16_min_ncol : 1024,
+                 num_cus, o, &msg);
+      for (int k = 0; k < 8; ++k) out[k] = (long)o[k];
+      if (why && why_len > 0) {
+         std::strncpy(why, msg.c_str(), (size_t)why_len - 1);
+         why[why_len - 1] = 0;
+      }
+      return 0;
+   } catch (const std::exception& e) {
+      if (why && why_len > 0) {
+         std::strncpy(why, e.what(), (size_t)why_len - 1);
+         why[why_len - 1] = 0;
+      }
+      return SYLVER_ERROR_UNKNOWN;
    }
 }
 
