@@ -268,6 +268,19 @@ int sylver_amd_batched_front_potrf(int nfronts, int const *nrow_host, int const 
                                    long const *offset_host, void *data_dev, int dtype,
                                    int *info_dev, void *stream);
 
+/* sylver_amd_batched_front_potrf without any device allocation. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The per-front sizes and offsets are staged in a fixed array of
+ * device memory (1 MiB per device, held until the process ends), so nfronts
+ * is at most 65536 (larger: SYLVER_ERROR_CALL_SEQUENCE). Same arguments,
+ * kernel and results as sylver_amd_batched_front_potrf; synchronous on
+ * stream; does not time itself (sylver_amd_last_batched_ms is unchanged).
+ * Calls that overlap on one device share that array and race. */
+int sylver_amd_batched_front_potrf_noalloc(int nfronts, int const *nrow_host, int const *ncol_host,
+                                           long const *offset_host, void *data_dev, int dtype,
+                                           int *info_dev, void *stream);
+
 /* Synthetic input of the batched small-front workload (no counterpart in
  * the reference): fills each front f (nrow[f] x nrow[f], ld nrow[f], at
  * data + offset[f] elements, device memory) with a symmetric matrix whose

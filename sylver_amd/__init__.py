@@ -152,7 +152,7 @@ EXPORTED = [
     "sylver_amd_batched_front_potrf", "sylver_amd_batched_front_fill", "sylver_amd_device_count", "sylver_amd_dev_alloc",
     "sylver_amd_dev_free", "sylver_amd_memcpy_h2d", "sylver_amd_memcpy_d2h",
     "sylver_amd_last_batched_ms", "sylver_amd_set_front_variants", "sylver_amd_time_fronts",
-    "sylver_amd_batched_front_potrf_v",
+    "sylver_amd_batched_front_potrf_v", "sylver_amd_batched_front_potrf_noalloc",
     "sylver_amd_mlp_init", "sylver_amd_mlp_grad", "sylver_amd_mlp_train", "sylver_amd_mlp_predict",
     "sylver_amd_hungarian_scale_sym", "sylver_amd_auction_scale_sym", "sylver_amd_match_order",
     "sylver_amd_plan_check",
@@ -192,6 +192,7 @@ def lib():
     L.sylver_amd_fkeep_indef_node.argtypes = [vp, C.c_int, ip, ip, dp, dp]
     L.sylver_amd_fkeep_indef_stats.argtypes = [vp, ip]
     L.sylver_amd_batched_front_potrf.argtypes = [C.c_int, ip, ip, lp, vp, C.c_int, ip, vp]
+    L.sylver_amd_batched_front_potrf_noalloc.argtypes = [C.c_int, ip, ip, lp, vp, C.c_int, ip, vp]
     L.sylver_amd_batched_front_fill.argtypes = [C.c_int, ip, lp, vp, C.c_int, C.c_ulonglong]
     L.sylver_amd_device_count.argtypes = []
     L.sylver_amd_dev_alloc.argtypes = [pvp, C.c_size_t]
@@ -501,6 +502,18 @@ def batched_front_potrf(fronts_flat: "DeviceArray", nrow, ncol, offsets, dtype):
     out = info.numpy()
     info.free()
     return rc, out, float(lib().sylver_amd_last_batched_ms())
+
+
+def batched_front_potrf_noalloc(fronts_flat: "DeviceArray", nrow, ncol, offsets, dtype, info: "DeviceArray"):
+    """sylver_amd_batched_front_potrf without device allocations (sizes
+    staged in the library's fixed per-device array); info is the caller's
+    device array of at least len(nrow) int32. Returns the flag."""
+    nrow = np.ascontiguousarray(nrow, np.int32)
+    ncol = np.ascontiguousarray(ncol, np.int32)
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    return lib().sylver_amd_batched_front_potrf_noalloc(
+        len(nrow), _p(nrow, C.c_int), _p(ncol, C.c_int), _p(offsets, C.c_long), fronts_flat.ptr,
+        0 if np.dtype(dtype) == np.float64 else 1, C.cast(info.ptr, C.POINTER(C.c_int)), None)
 
 
 def batched_front_potrf_v(fronts_flat: "DeviceArray", nrow, ncol, offsets, dtype, variant=None):

@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -2022,6 +2023,83 @@ int batched_front_potrf(int nfronts, const int* nrow, const int* ncol, const int
       (void)hipEventDestroy(e1);
       dfree(dnd);
       dfree(dl);
+      return 0;
+   } catch (const HipError&) {
+      return SYLVER_ERROR_CUDA_UNKNOWN;
+   }
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free batched factorization. The per-front sizes and offsets are
+// staged in a fixed device array instead of allocated descriptors: int64
+// offsets in the first kNoAllocMaxFronts slots, then nrow and ncol as ints.
+// Each device holds its own copy for the life of the process; every call on
+// a device reuses it, so calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) int64_t scratch[2 * kNoAllocMaxFronts];  // 1 MiB
+
+// Device address of `scratch` per device, taken once per device with that
+// device current (the array's name in host code is not a device address).
+static std::mutex g_scratch_mu;
+static int64_t* g_scratch_ptr[kMaxScratchDevices] = {};
+
+static int64_t* scratch_ptr(int dev) {
+   std::lock_guard<std::mutex> lock(g_scratch_mu);
+   if (!g_scratch_ptr[dev]) {
+      void* p = nullptr;
+      HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      if (!p) throw HipError{hipErrorInvalidSymbol, __LINE__};
+      g_scratch_ptr[dev] = static_cast<int64_t*>(p);
+   }
+   return g_scratch_ptr[dev];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_small_front_sized(const int64_t* __restrict__ off,
+                                                                const int* __restrict__ nrow,
+                                                                const int* __restrict__ ncol,
+                                                                T* __restrict__ data, int* __restrict__ info) {
+   const int f = blockIdx.x;
+   NodeDesc nd{};
+   nd.m = nrow[f];
+   nd.n = ncol[f];
+   nd.ld = nd.m;
+   nd.off = off[f];
+   nd.first_child = nd.next_child = -1;
+   if (threadIdx.x == 0) info[f] = 0;  // front_potrf_block opens with a barrier
+   front_potrf_block<T>(nd, data + nd.off, info + f);
+}
+
+int batched_front_potrf_noalloc(int nfronts, const int* nrow, const int* ncol, const int64_t* off, void* data,
+                                int dtype, int* info, void* stream) {
+   if (nfronts < 0 || nfronts > kNoAllocMaxFronts || (nfronts > 0 && (!nrow || !ncol || !off || !data || !info)))
+      return SYLVER_ERROR_CALL_SEQUENCE;
+   if (dtype != 0 && dtype != 1) return SYLVER_ERROR_CALL_SEQUENCE;
+   for (int f = 0; f < nfronts; ++f)
+      if (nrow[f] < ncol[f] || ncol[f] < 0 || off[f] < 0) return SYLVER_ERROR_CALL_SEQUENCE;
+   if (nfronts == 0) return 0;
+   if (device_count() <= 0) return SYLVER_ERROR_CUDA_UNKNOWN;
+   try {
+      int dev = 0;
+      HIPCHECK(hipGetDevice(&dev));
+      if (dev < 0 || dev >= kMaxScratchDevices) return SYLVER_ERROR_CUDA_UNKNOWN;
+      int64_t* s_off = scratch_ptr(dev);
+      int* s_nrow = reinterpret_cast<int*>(s_off + kNoAllocMaxFronts);
+      int* s_ncol = s_nrow + kNoAllocMaxFronts;
+      hipStream_t s = static_cast<hipStream_t>(stream);
+      HIPCHECK(hipMemcpyAsync(s_off, off, nfronts * sizeof(int64_t), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemcpyAsync(s_nrow, nrow, nfronts * sizeof(int), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemcpyAsync(s_ncol, ncol, nfronts * sizeof(int), hipMemcpyHostToDevice, s));
+      if (dtype == 0)
+         k_small_front_sized<double><<<nfronts, kThreads, 0, s>>>(s_off, s_nrow, s_ncol,
+                                                                   static_cast<double*>(data), info);
+      else
+         k_small_front_sized<float><<<nfronts, kThreads, 0, s>>>(s_off, s_nrow, s_ncol,
+                                                                  static_cast<float*>(data), info);
+      HIPCHECK(hipGetLastError());
+      // the host arrays are the caller's and the scratch is shared: finish
+      // before returning
+      HIPCHECK(hipStreamSynchronize(s));
       return 0;
    } catch (const HipError&) {
       return SYLVER_ERROR_CUDA_UNKNOWN;

@@ -110,6 +110,12 @@ int batched_front_potrf(int nfronts, const int* nrow, const int* ncol, const int
                         void* data, int dtype, int* info, void* stream);
 int batched_front_potrf_v(int nfronts, const int* nrow, const int* ncol, const int64_t* off, void* data,
                           int dtype, const signed char* variant, int* info);
+// batched_front_potrf without allocations: sizes and offsets staged in a
+// fixed per-device array of kNoAllocMaxFronts fronts.
+constexpr int kNoAllocMaxFronts = 1 << 16;
+constexpr int kMaxScratchDevices = 64;
+int batched_front_potrf_noalloc(int nfronts, const int* nrow, const int* ncol, const int64_t* off, void* data,
+                                int dtype, int* info, void* stream);
 int time_fronts(int K, const int* m, const int* n, const signed char* variant, int precision, int reps,
                 double* ms);
 // Cost model (costmodel.hip): an 8 -> 64 -> 64 -> 2 MLP in FP32.

@@ -508,6 +508,17 @@ int sylver_amd_batched_front_potrf(int nfronts, int const* nrow, int const* ncol
    }
 }
 
+int sylver_amd_batched_front_potrf_noalloc(int nfronts, int const* nrow, int const* ncol,
+                                           long const* offset, void* data, int dtype, int* info,
+                                           void* stream) {
+   try {
+      return batched_front_potrf_noalloc(nfronts, nrow, ncol, reinterpret_cast<const int64_t*>(offset),
+                                         data, dtype, info, stream);
+   } catch (...) {
+      return SYLVER_ERROR_UNKNOWN;
+   }
+}
+
 int sylver_amd_hungarian_scale_sym(int n, long const* ptr, int const* row, double const* val,
                                    int scale_if_singular, double* scaling, int* match) {
    if (n < 0 || (n > 0 && (!ptr || !row || !val || !scaling))) return SYLVER_ERROR_CALL_SEQUENCE;
