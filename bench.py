@@ -49,20 +49,26 @@ def run_path(S, A, akeep, opts, b, precision, steps, warmup, refine_tol):
     fkeep = None
     recs = []
     for it in range(warmup + steps):
+        x = b.copy(order="F")
+        # wall clock around each call, as the reference driver times them
+        # (drivers/spldlt_test.F90:133-148): calls are synchronous at return
         t0 = time.perf_counter()
         fkeep, inf = S.spldlt_factorize(True, A.val, akeep, fkeep, options=opts)
+        t1 = time.perf_counter()
         if inf.flag < 0:
             raise RuntimeError(f"factorize failed: flag {inf.flag}")
         ft = inf.tail()
-        x = b.copy(order="F")
+        t2 = time.perf_counter()
         sinf = S.spldlt_solve(x, akeep, fkeep, options=opts)
+        t3 = time.perf_counter()
         if sinf.flag < 0:
             raise RuntimeError(f"solve failed: flag {sinf.flag}")
-        wall = time.perf_counter() - t0
+        wall = (t1 - t0) + (t3 - t2)
         st = sinf.tail()
         if it >= warmup:
             recs.append(dict(factor_ms=ft.factor_ms, upload_ms=ft.upload_ms, solve_ms=st.solve_ms,
-                             refine_iters=st.refine_iters, wall_ms=1e3 * wall, x=x))
+                             refine_iters=st.refine_iters, wall_ms=1e3 * wall, factor_call_ms=1e3 * (t1 - t0),
+                             solve_call_ms=1e3 * (t3 - t2), x=x))
         log(f"  prec={precision} step {it}: factor {ft.factor_ms:.2f} ms, solve {st.solve_ms:.2f} ms,"
             f" IR iters {st.refine_iters}, wall {1e3 * wall:.1f} ms")
     return fkeep, recs
@@ -516,6 +522,15 @@ def main():
     use_variants(1)
     fkmx, rmx = run_path(S, A, akeep, opts, b, prec, args.steps, args.warmup, 1e-15)
     errmx = M.backward_error(A, rmx[-1]["x"], b)
+    # the other low-precision tier beside the headline one (FP32 only, or FP32
+    # with BF16 updates on fronts of more than 1024 pivots)
+    alt_prec = S.PREC_MIXED if prec == S.PREC_FP32 else S.PREC_FP32
+    fkalt, ralt = run_path(S, A, akeep, opts, b, alt_prec, args.steps, args.warmup, 1e-15)
+    erralt = M.backward_error(A, ralt[-1]["x"], b)
+    S.spldlt_free_fkeep(fkalt)
+    for name, e in (("fp64", err64), ("headline", errmx), ("other tier", erralt)):
+        if not (e <= 1e-15):
+            raise RuntimeError(f"{name} path: scaled backward error {e} above 1e-15")
 
     f64_ms = float(np.mean([r["factor_ms"] for r in r64]))
     s64_ms = float(np.mean([r["solve_ms"] for r in r64]))
@@ -600,6 +615,23 @@ def main():
         "fp64": {"factor_s": round(f64_ms / 1e3, 5), "solve_s": round(s64_ms / 1e3, 5),
                  "factor_gflops": round(flops / (f64_ms * 1e6), 2), "bwd_err": err64,
                  "frac_fp64_peak": round(flops / (f64_ms * 1e6) / 1e3 / PEAK["fp64"], 4)},
+        "other_tier": {"tier": "bf16/fp32+fp64-ir (BF16 updates on fronts with > 1024 pivots)"
+                       if alt_prec == S.PREC_MIXED else "fp32+fp64-ir",
+                       "factor_s": round(float(np.mean([r["factor_ms"] for r in ralt])) / 1e3, 5),
+                       "solve_ir_s": round(float(np.mean([r["solve_ms"] for r in ralt])) / 1e3, 5),
+                       "ir_iters": int(ralt[-1]["refine_iters"]), "bwd_err": erralt,
+                       "step_gflops": round(flops / (float(np.mean([r["factor_ms"] + r["solve_ms"]
+                                                                     for r in ralt])) * 1e6), 2)},
+        "call_wall_ms": {
+            "convention": "wall clock around each spldlt_factorize / spldlt_solve call (synchronous at return;"
+                          " values and x handed over from host memory), as drivers/spldlt_test.F90:133-148",
+            "headline": {"factorize": round(float(np.mean([r["factor_call_ms"] for r in rmx])), 3),
+                         "solve": round(float(np.mean([r["solve_call_ms"] for r in rmx])), 3)},
+            "fp64": {"factorize": round(float(np.mean([r["factor_call_ms"] for r in r64])), 3),
+                     "solve": round(float(np.mean([r["solve_call_ms"] for r in r64])), 3)},
+            "headline_step_gflops": round(flops / (float(np.mean([r["wall_ms"] for r in rmx])) * 1e6), 2),
+            "speedup_vs_fp64": round(float(np.mean([r["wall_ms"] for r in r64])) /
+                                     float(np.mean([r["wall_ms"] for r in rmx])), 3)},
         "speedup_mixed_vs_fp64": round((f64_ms + s64_ms) / step_ms, 3),
         "upload_ms": round(float(np.mean([r["upload_ms"] for r in rmx])), 3),
         "wall_ms_per_step_pcie_incl": round(float(np.mean([r["wall_ms"] for r in rmx])), 3),

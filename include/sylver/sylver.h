@@ -55,11 +55,14 @@ typedef struct {
    int unit_diagnostics;
    int unit_error;
    int unit_warning;
-   int ordering;     /* 0: user order, 1: nested dissection (built here; the reference calls METIS) */
+   int ordering;     /* 0: user order, 1: nested dissection (built here; the reference calls METIS),
+                       2: matching-based (compressed 2x2 candidates, then nested dissection) */
    int nemin;        /* supernode amalgamation parameter (default 32) */
    bool prune_tree;
    long min_gpu_work;
-   int scaling;      /* only 0 (none) and user scaling through `scale` are supported */
+   int scaling;      /* <= 0: the caller's `scale` if given; 1: matching (shortest augmenting
+                       paths); 2: matching (auction); 3: the scaling saved by ordering = 2;
+                       >= 4: norm equilibration */
    int pivot_method; /* 1 APP aggressive, 2 APP block (default), 3 TPP */
    double small;
    double u;

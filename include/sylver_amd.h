@@ -353,6 +353,31 @@ int sylver_amd_match_order(int n, long const *ptr, int const *row, double const 
 /* Number of HIP devices visible (0 when none); never fails. */
 int sylver_amd_device_count(void);
 
+/* ---- Rutherford-Boeing input (replaces SPRAL's rb_read, which the
+ * reference's drivers call: drivers/spldlt_test.F90:86-95; SPRAL is not
+ * part of the reference tree). ---- */
+#define SYLVER_AMD_RB_WARNING_MADE_UP 1  /* pattern-only file: values made up (or zeros) */
+#define SYLVER_AMD_RB_ERROR_ARGS -1
+#define SYLVER_AMD_RB_ERROR_OPEN -2      /* file cannot be opened */
+#define SYLVER_AMD_RB_ERROR_FORMAT -3    /* malformed header, format or data */
+#define SYLVER_AMD_RB_ERROR_TYPE -4      /* complex, elemental or non-square matrix */
+#define SYLVER_AMD_RB_ERROR_NO_VALUES -5 /* values = 1 and the file holds a pattern only */
+#define SYLVER_AMD_RB_ERROR_ALLOC -6
+
+/* Reads a Rutherford-Boeing (or Harwell-Boeing) assembled real or pattern
+ * matrix into the lower triangle in 1-based CSC (ptr: n+1 entries, row and
+ * val: ptr[n]-1 entries; malloc'ed, release each with sylver_amd_rb_free).
+ * Symmetric types keep their entries (mirrored below the diagonal if stored
+ * above); unsymmetric types keep row >= column. values: 0 read values if
+ * present (zeros for a pattern), 1 values required, 2 values made up for a
+ * pattern-only file (symmetric, diagonally dominant, from seed), as the
+ * reference driver's rb_options%values = 2. type (may be NULL) receives the
+ * 3-letter matrix type. Returns 0, SYLVER_AMD_RB_WARNING_MADE_UP or a
+ * negative SYLVER_AMD_RB_ERROR_* code. */
+int sylver_amd_rb_read(const char *filename, int values, unsigned long long seed, int *nrow, int *ncol,
+                       long **ptr, int **row, double **val, char *type);
+void sylver_amd_rb_free(void *p);
+
 /* Device memory helpers for callers without their own HIP runtime (the
  * tests and benchmark use them instead of a second framework's allocator).
  * Return 0 or SYLVER_ERROR_CUDA_UNKNOWN. */

@@ -155,7 +155,7 @@ EXPORTED = [
     "sylver_amd_batched_front_potrf_v", "sylver_amd_batched_front_potrf_noalloc",
     "sylver_amd_mlp_init", "sylver_amd_mlp_grad", "sylver_amd_mlp_train", "sylver_amd_mlp_predict",
     "sylver_amd_hungarian_scale_sym", "sylver_amd_auction_scale_sym", "sylver_amd_match_order",
-    "sylver_amd_plan_check",
+    "sylver_amd_plan_check", "sylver_amd_rb_read", "sylver_amd_rb_free",
 ]
 
 
@@ -213,6 +213,10 @@ def lib():
     L.sylver_amd_auction_scale_sym.argtypes = [C.c_int, lp, ip, dp, dp, ip]
     L.sylver_amd_match_order.argtypes = [C.c_int, lp, ip, dp, ip, dp]
     L.sylver_amd_plan_check.argtypes = [vp, C.c_int, C.c_int, C.c_int, lp, C.c_char_p, C.c_int]
+    L.sylver_amd_rb_read.argtypes = [C.c_char_p, C.c_int, C.c_ulonglong, ip, ip, C.POINTER(lp), C.POINTER(ip),
+                                     C.POINTER(dp), C.c_char_p]
+    L.sylver_amd_rb_free.argtypes = [vp]
+    L.sylver_amd_rb_free.restype = None
     _lib = L
     return L
 

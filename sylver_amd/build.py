@@ -14,7 +14,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsylver_amd.so")
 
-SOURCES = ["numeric.hip", "indef.hip", "costmodel.hip", "analyse.cpp", "scaling.cpp", "sylver_api.cpp"]
+SOURCES = ["numeric.hip", "indef.hip", "costmodel.hip", "analyse.cpp", "scaling.cpp", "rb_io.cpp", "sylver_api.cpp"]
+# the reference driver drivers/spldlt_test.F90 on the C interface (gcc)
+DRIVER = os.path.join(HERE, "drivers", "spldlt_test.c")
+DRIVER_OUT = os.path.join(HERE, "spldlt_test")
 
 HIP_FLAGS = [
     "-O3",
@@ -59,6 +62,13 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, OUT)
+    cmd = ["gcc", "-O2", "-std=c99", "-D_POSIX_C_SOURCE=200809L", "-Wall", "-Werror",
+           "-I", os.path.join(HERE, "..", "include"), DRIVER, "-L", HERE, "-lsylver_amd",
+           "-Wl,-rpath,$ORIGIN", "-lm", "-o", DRIVER_OUT + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(DRIVER_OUT + ".tmp", DRIVER_OUT)
     return OUT
 
 

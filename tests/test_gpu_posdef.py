@@ -294,3 +294,39 @@ def test_mixed_bf16_wide_fronts_refined(S, O, gpu):
         assert O.bwd_err(A, x, b) <= BWD_TOL
     finally:
         S.set_ext_options(bf16_min_ncol=1024, refine_max_iter=30)
+
+
+@pytest.fixture(scope="module")
+def lap80(S):
+    """The 80^3 3D Laplacian (n = 512,000) analysed once, 16 RHS U[-1,1]
+    from seed 0: the full-size mixed-precision workload."""
+    A = M.laplacian_3d(80)
+    akeep, opts, sym = _analyse(S, A)
+    b = M.random_rhs(A.n, 16, seed=0)
+    yield A, akeep, opts, sym, b
+    S.spldlt_free_akeep(akeep)
+
+
+@pytest.mark.parametrize("tier", ["fp32", "mixed_bf16"])
+def test_laplacian_80_full_size_refined(S, O, gpu, lap80, tier):
+    """80^3 at full size (7,580-row root, 60-block chains): FP32 fronts, and
+    the mixed tier with BF16 Schur updates on fronts of more than 1024 pivots
+    (the default threshold), each refined in FP64 to scaled backward error
+    <= 1e-15 on all 16 right-hand sides, checked by the oracle."""
+    A, akeep, opts, sym, b = lap80
+    ncol = np.diff(sym["sptr"])
+    assert int(ncol.max()) > 1024  # the BF16 tier has fronts to work on
+    S.set_ext_options(precision=S.PREC_FP32 if tier == "fp32" else S.PREC_MIXED, bf16_min_ncol=1024,
+                      refine=S.REFINE_AUTO, refine_tol=1e-15, refine_max_iter=30)
+    fkeep, inf = _factor(S, A, akeep, opts)
+    try:
+        assert inf.flag == 0
+        x = b.copy(order="F")
+        sinf = S.spldlt_solve(x, akeep, fkeep, options=opts)
+        assert sinf.flag == 0
+        t = sinf.tail()
+        assert 1 <= t.refine_iters <= 30
+        for c in range(b.shape[1]):
+            assert O.bwd_err(A, x[:, c], b[:, c]) <= BWD_TOL, c
+    finally:
+        S.spldlt_free_fkeep(fkeep)
