@@ -257,14 +257,21 @@ def bench_small_laplacian(S, M, oracle_mod):
 
 def bench_kkt(S, M, oracle_mod, side=48, mrows=27648, sample_side=16):
     """KKT [H A^T; A 0], H the 7-point Laplacian on side^3, A mrows x side^3
-    with 3 entries per row at random columns: APP LDL^T in FP64 (u = 0.01,
-    small = 1e-20). Inertia must be (side^3, mrows, 0). The oracle (one
-    thread) and the device are also timed on a sample_side^3 case with
+    with 3 entries per row at random columns: APP LDL^T (u = 0.01, small =
+    1e-20) in FP64, and in the FP32 tier (contribution-block updates on FP32
+    MFMA, pivot tests and D in FP64) whose solve runs FP64 GMRES-IR to
+    backward error <= 1e-15; the FP64 factor also with the matching-based
+    ordering and its scaling. Inertia must be (side^3, mrows, 0). The oracle
+    (one thread) and the device are also timed on a sample_side^3 case with
     mrows = sample_side^3 / 4, where the oracle's delay counts are compared."""
-    def run(A, reps):
+    def run(A, reps, precision=None, ordering=1):
         opts = S.default_options()
+        opts.ordering = ordering
         akeep, inf, _ = S.spldlt_analyse(A.n, A.ptr, A.row, A.val, options=opts)
-        S.set_ext_options(precision=S.PREC_FP64, use_graph=1, record_timing=0)
+        if ordering == 2:
+            opts.scaling = 3
+        S.set_ext_options(precision=S.PREC_FP64 if precision is None else precision, use_graph=1, record_timing=0,
+                          refine=S.REFINE_GMRES, refine_tol=1e-15, refine_max_iter=20)
         fk, fms = None, []
         for it in range(reps):
             fk, finf = S.spldlt_factorize(False, A.val, akeep, fk, options=opts)
@@ -274,20 +281,32 @@ def bench_kkt(S, M, oracle_mod, side=48, mrows=27648, sample_side=16):
                 fms.append(finf.tail().factor_ms)
         return akeep, inf, fk, finf, float(np.median(fms)), opts
 
+    def record(A, akeep, ainf, fk, finf, fms, opts, nrhs=1):
+        b = M.random_rhs(A.n, nrhs, seed=0)
+        x = b.copy(order="F")
+        sinf = S.spldlt_solve(x, akeep, fk, options=opts)
+        err = M.backward_error(A, x, b)
+        st = sinf.tail()
+        rec = {"flops_symbolic": float(ainf.num_flops), "flops_executed": float(finf.num_flops),
+               "factor_ms": round(fms, 3), "factor_gflops": round(float(finf.num_flops) / (fms * 1e6), 2),
+               "solve_ms": round(st.solve_ms, 3), "bwd_err": err,
+               "num_neg": int(finf.num_neg), "inertia_ok": int(finf.num_neg) == mrows and finf.matrix_rank == A.n,
+               "num_delay": int(finf.num_delay), "num_two": int(finf.num_two), "maxfront": int(finf.maxfront)}
+        if st.refine_iters > 0:
+            rec["refine_iters"] = int(st.refine_iters)
+            rec["refine_mode"] = {1: "IR", 2: "GMRES-IR"}.get(int(st.refine_mode), int(st.refine_mode))
+            rec["factor_plus_solve_ms"] = round(fms + st.solve_ms, 3)
+        S.spldlt_free_fkeep(fk)
+        S.spldlt_free_akeep(akeep)
+        return rec
+
     A = M.kkt(side, mrows, seed=0)
-    akeep, ainf, fk, finf, fms, opts = run(A, 2)
-    b = M.random_rhs(A.n, 1, seed=0)
-    x = b.copy(order="F")
-    sinf = S.spldlt_solve(x, akeep, fk, options=opts)
-    err = M.backward_error(A, x, b)
     nh = side ** 3
-    out = {"n": A.n, "nh": nh, "m": mrows, "flops_symbolic": float(ainf.num_flops),
-           "flops_executed": float(finf.num_flops), "factor_ms": round(fms, 3),
-           "factor_gflops": round(float(finf.num_flops) / (fms * 1e6), 2),
-           "solve_ms": round(sinf.tail().solve_ms, 3), "bwd_err": err,
-           "num_neg": int(finf.num_neg), "inertia_ok": int(finf.num_neg) == mrows and finf.matrix_rank == A.n,
-           "num_delay": int(finf.num_delay), "num_two": int(finf.num_two), "maxfront": int(finf.maxfront)}
-    S.spldlt_free_fkeep(fk)
+    out = {"n": A.n, "nh": nh, "m": mrows}
+    out.update(record(A, *run(A, 2)))
+    out["fp32_gmres_ir"] = record(A, *run(A, 2, precision=S.PREC_FP32))
+    out["matching_order_fp64"] = record(A, *run(A, 1, ordering=2))
+    S.set_ext_options(precision=S.PREC_FP64, refine=S.REFINE_AUTO)
     # the bounded sample: device and oracle on the same smaller case
     ms = sample_side ** 3 // 4
     A2 = M.kkt(sample_side, ms, seed=0)
@@ -303,6 +322,7 @@ def bench_kkt(S, M, oracle_mod, side=48, mrows=27648, sample_side=16):
                      "cpu_kind": "port (oracle, one thread)",
                      "stats_equal_oracle": all(st[k] == ref.stats[k] for k in st)}
     S.spldlt_free_fkeep(fk2)
+    S.set_ext_options(precision=S.PREC_FP64, refine=S.REFINE_AUTO)
     return out
 
 

@@ -112,6 +112,9 @@ void destroy_indef(IndefState* s) { delete s; }
 namespace {
 
 inline unsigned grid_of(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kThreads - 1) / kThreads); }
+// Grid of a grid-stride kernel over n items: a dispatch covers at most 2^32
+// work-items (the level arena of a large tree holds more entries than that).
+inline unsigned grid_stride(int64_t n) { return (unsigned)std::min<int64_t>(grid_of(n), int64_t(1) << 16); }
 
 // Grow a device buffer to at least `need` elements, keeping `keep` elements.
 template <typename T>
@@ -260,6 +263,9 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
    }
    const double u = opts.u, small = opts.small;
    const int action = opts.action ? 1 : 0;
+   // FP32 tier: contribution-block updates on FP32 MFMA (operands rounded to
+   // FP32, FP64 fronts); pivot tests, D and its inverse stay FP64
+   const bool low = F->precision != SYLVER_AMD_PREC_FP64;
    // fronts on the multi-workgroup panel: at least panel_min fully-summed
    // columns; in auto mode also fronts of 2^18 entries or more (many
    // rows, where one workgroup's pass over all rows per pivot is slow)
@@ -343,7 +349,7 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
       HIPCHECK(hipMemcpyAsync(S.d_nodes, S.h.data(), nn * sizeof(IndefDesc), hipMemcpyHostToDevice, s));
       const int nl = (int)lv.size();
       // zero the level's fronts (contiguous)
-      k_fill<<<grid_of(lv_arena), kThreads, 0, s>>>(lv_arena, 0.0, S.arena);
+      k_fill<<<grid_stride(lv_arena), kThreads, 0, s>>>(lv_arena, 0.0, S.arena);
       k_ind_init<<<nl, kThreads, 0, s>>>(S.d_list, S.d_nodes, P->d_nlist, P->d_rlist, F->d_aval,
                                          F->has_scaling ? F->d_scaling : nullptr, S.arena, S.d_vars, S.d_pres);
       if (n_delay > 0) {
@@ -418,11 +424,20 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
          for (int tj = 0; tj < nt; ++tj)
             for (int ti = tj; ti < nt; ++ti) {
                GemmTask t{};
-               t.a_off = d.off + d.n + (int64_t)ti * 128;
-               t.b_off = d.ws_off + (int64_t)tj * 128;
+               if (low) {  // FP32 operands in the workspace (float offsets)
+                  const int ldq = ind32_ldq(q);
+                  const int64_t base = ind32_base(d.ws_off);
+                  t.a_off = base + (int64_t)ldq * d.nelim + (int64_t)ti * 128;
+                  t.b_off = base + (int64_t)tj * 128;
+                  t.lda = ldq;
+                  t.ldb = ldq;
+               } else {
+                  t.a_off = d.off + d.n + (int64_t)ti * 128;
+                  t.b_off = d.ws_off + (int64_t)tj * 128;
+                  t.lda = d.ld;
+                  t.ldb = q;
+               }
                t.c_off = d.off + (d.n + (int64_t)ti * 128) + (d.n + (int64_t)tj * 128) * d.ld;
-               t.lda = d.ld;
-               t.ldb = q;
                t.ldc = d.ld;
                t.M = std::min(128, q - ti * 128);
                t.N = std::min(128, q - tj * 128);
@@ -442,8 +457,14 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
          HIPCHECK(hipMemcpyAsync(S.d_tasks, tasks.data(), tasks.size() * sizeof(GemmTask), hipMemcpyHostToDevice, s));
          HIPCHECK(hipMemcpyAsync(S.d_nodes, S.h.data(), nn * sizeof(IndefDesc), hipMemcpyHostToDevice, s));
          dim3 g(64, (unsigned)cb_nodes.size());
-         k_ind_ld<<<g, kThreads, 0, s>>>(S.d_list, S.d_nodes, S.arena, S.d_dinv, S.d_ws);
-         k_update<double, 2><<<(unsigned)tasks.size(), kThreads, 0, s>>>(S.d_tasks, S.arena, S.d_ws);
+         if (low) {
+            k_ind_ld32<<<g, kThreads, 0, s>>>(S.d_list, S.d_nodes, S.arena, S.d_dinv, S.d_ws);
+            k_update_cb32<<<(unsigned)tasks.size(), kThreads, 0, s>>>(S.d_tasks, reinterpret_cast<const float*>(S.d_ws),
+                                                                      S.arena);
+         } else {
+            k_ind_ld<<<g, kThreads, 0, s>>>(S.d_list, S.d_nodes, S.arena, S.d_dinv, S.d_ws);
+            k_update<double, 2><<<(unsigned)tasks.size(), kThreads, 0, s>>>(S.d_tasks, S.arena, S.d_ws);
+         }
          HIPCHECK(hipStreamSynchronize(s));  // the task and list buffers are rewritten next level
       }
       // L into the factor store, trailing blocks into the contribution store
@@ -475,6 +496,8 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
          dim3 g((unsigned)std::min<int64_t>((big + kThreads - 1) / kThreads, 2048), (unsigned)nl);
          k_ind_store<<<g, kThreads, 0, s>>>(S.d_list, S.d_nodes, S.arena, S.lstore, S.cbs, nn);
       }
+      // a launch the runtime refused (grid or resources) must not pass silently
+      HIPCHECK(hipGetLastError());
       HIPCHECK(hipStreamSynchronize(s));  // list and node buffers are rewritten next level
    }
    S.x_rows = x_used;
@@ -586,7 +609,7 @@ void solve_indef(const DevicePlan* P, DeviceFactors* F, int job, const double* b
          blocked((int)L, true);
       }
    } else {
-      k_copy<<<grid_of(nn), kThreads, 0, s>>>(nn, b, sol);
+      k_copy<<<grid_stride(nn), kThreads, 0, s>>>(nn, b, sol);
    }
    if (diag)
       k_ind_diag<<<std::max(S.nnodes, 1), kThreads, 0, s>>>(S.d_nodes, S.nnodes, S.d_vars, S.d_dinv, sol, n,

@@ -570,10 +570,12 @@ __device__ inline int xcd_task(int b, int G) { return (b & 7) * (G >> 3) + (b >>
 // TB x TBN tiles: 128 x 128; 64 x 64 for the short Schur-update launches on
 // the critical path and 64 x 128 for the TRSM (whose tile must span all
 // kb columns), so those launches spread over more workgroups.
-template <typename T, int MODE, int TB = 128, int TBN = TB>
-__global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict__ tasks,
-                                                     T* __restrict__ arena,
-                                                     const T* __restrict__ bsrc) {
+// One tile of a GEMM task: C (ldc, element type TC) -= A B^T, or C = A B^T
+// for MODE 1; A and B of type T (leading dimensions t.lda, t.ldb), products
+// on T's MFMA. `vec`: A, B and their leading dimensions allow vector loads.
+template <typename T, int MODE, int TB, int TBN, typename TC>
+__device__ __forceinline__ void gemm_tile(const GemmTask& t, const T* __restrict__ a, const T* __restrict__ b,
+                                          TC* __restrict__ c, int ldc, bool vec) {
    constexpr int BM = TB, BN = TBN, BK = sizeof(T) == 8 ? 32 : 16, PAD = 4;  // FP64: longer MFMA chains per stage
    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
    typedef typename V16<T>::vt vt;
@@ -583,15 +585,9 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
    typedef typename Mfma<T>::acc_t acc_t;
    __shared__ T As[2][BK][BM + PAD];
    __shared__ T Bs[2][BK][BN + PAD];
-   const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
-   if (t.M <= 0) return;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int wm = wave >> 1, wn = wave & 1;
    const int ld = t.lda, ldb = t.ldb;
-   const T* a = arena + t.a_off;
-   const T* b = (MODE == 0 ? arena : bsrc) + t.b_off;
-   T* c = arena + t.c_off;
-   const bool vec = ((t.a_off | t.b_off | (int64_t)ld | (int64_t)ldb) % VN) == 0;
    acc_t acc[FM][FN];
 #pragma unroll
    for (int x = 0; x < FM; ++x)
@@ -671,7 +667,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
    // column. The C values of one y are all loaded before any is stored.
 #pragma unroll
    for (int y = 0; y < FN; ++y) {
-      T cv[FM][4];
+      TC cv[FM][4];
       if (MODE != 1) {
 #pragma unroll
          for (int x = 0; x < FM; ++x)
@@ -680,7 +676,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
                const int i = wm * WM + x * 16 + (lane & 15);
                const int j = wn * WN + y * 16 + Mfma<T>::row(lane, v);
                const bool ok = i < t.M && j < t.N && j >= t.pad0;
-               cv[x][v] = *live_or_zero(ok, c + i + (int64_t)j * ld);
+               cv[x][v] = *live_or_zero(ok, c + i + (int64_t)j * ldc);
             }
       }
 #pragma unroll
@@ -690,9 +686,34 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
             const int i = wm * WM + x * 16 + (lane & 15);
             const int j = wn * WN + y * 16 + Mfma<T>::row(lane, v);
             if (i < t.M && j < t.N && j >= t.pad0)
-               c[i + (int64_t)j * ld] = (MODE == 1) ? acc[x][y][v] : cv[x][v] - acc[x][y][v];
+               c[i + (int64_t)j * ldc] = (MODE == 1) ? (TC)acc[x][y][v] : cv[x][v] - (TC)acc[x][y][v];
          }
    }
+}
+
+template <typename T, int MODE, int TB = 128, int TBN = TB>
+__global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict__ tasks,
+                                                     T* __restrict__ arena,
+                                                     const T* __restrict__ bsrc) {
+   const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
+   if (t.M <= 0) return;
+   constexpr int VN = V16<T>::N;
+   const bool vec = ((t.a_off | t.b_off | (int64_t)t.lda | (int64_t)t.ldb) % VN) == 0;
+   gemm_tile<T, MODE, TB, TBN, T>(t, arena + t.a_off, (MODE == 0 ? arena : bsrc) + t.b_off, arena + t.c_off, t.lda,
+                                  vec);
+}
+
+// Contribution-block update of the indefinite path's FP32 tier: CB -= L21
+// (L21 D)^T with both operands rounded to FP32 in the workspace (offsets in
+// floats), FP32 MFMA accumulation, the FP64 contribution block in the arena
+// (ldc = t.ldc).
+static __global__ __launch_bounds__(kThreads) void k_update_cb32(const GemmTask* __restrict__ tasks,
+                                                          const float* __restrict__ ws,
+                                                          double* __restrict__ arena) {
+   const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
+   if (t.M <= 0) return;
+   const bool vec = ((t.a_off | t.b_off | (int64_t)t.lda | (int64_t)t.ldb) % V16<float>::N) == 0;
+   gemm_tile<float, 2, 128, 128, double>(t, ws + t.a_off, ws + t.b_off, arena + t.c_off, t.ldc, vec);
 }
 
 // ---------------------------------------------------------------------------
@@ -2054,13 +2075,15 @@ static __global__ __launch_bounds__(kThreads) void k_axpy(int64_t n, double a, c
 
 static __global__ __launch_bounds__(kThreads) void k_copy(int64_t n, const double* __restrict__ x,
                                                    double* __restrict__ y) {
-   int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-   if (i < n) y[i] = x[i];
+   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+      y[i] = x[i];
 }
 
+// Grid-stride: one dispatch covers at most 2^32 work-items, so callers cap
+// the grid for large n.
 static __global__ __launch_bounds__(kThreads) void k_fill(int64_t n, double v, double* __restrict__ y) {
-   int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-   if (i < n) y[i] = v;
+   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+      y[i] = v;
 }
 
 // Caller's order -> pivot order, optionally scaled (the solve driver's

@@ -1684,6 +1684,54 @@ static __global__ __launch_bounds__(kThreads) void k_ind_ld(const int* __restric
    }
 }
 
+// FP32 tier: LD as above and a copy of L21 = L[n:m, 0:nelim], both rounded
+// to FP32, into the node's workspace viewed as floats from float offset
+// ind32_base(ws_off): LD at [0, ldq nelim), L21 at [ldq nelim, 2 ldq nelim),
+// ldq = q rounded up to 4 (16-byte aligned columns for the update's loads).
+// The D^{-1} products are formed in FP64 and rounded once.
+__host__ __device__ inline int64_t ind32_base(int64_t ws_off) { return (2 * ws_off + 3) / 4 * 4; }
+__host__ __device__ inline int ind32_ldq(int q) { return (q + 3) / 4 * 4; }
+
+static __global__ __launch_bounds__(kThreads) void k_ind_ld32(const int* __restrict__ list,
+                                                       const IndefDesc* __restrict__ nodes,
+                                                       const double* __restrict__ arena,
+                                                       const double* __restrict__ dinv,
+                                                       double* __restrict__ wsbase) {
+#pragma clang fp contract(off)
+   const IndefDesc nd = nodes[list[blockIdx.y]];
+   const int q = nd.m - nd.n, ne = nd.nelim, ldq = ind32_ldq(q);
+   const double* F = arena + nd.off;
+   const double* d = dinv + nd.d_off;
+   float* W = reinterpret_cast<float*>(wsbase) + ind32_base(nd.ws_off);
+   float* L21 = W + (int64_t)ldq * ne;
+   for (int j = blockIdx.x; j < ne; j += gridDim.x) {
+      const bool second = j > 0 && __builtin_isinf(d[2 * j]);
+      const bool first = j + 1 < ne && __builtin_isinf(d[2 * j + 2]);
+      for (int i = threadIdx.x; i < ldq; i += kThreads) {
+         if (i >= q) {  // padding rows: zeros, never read into C
+            W[i + (int64_t)j * ldq] = 0.0f;
+            L21[i + (int64_t)j * ldq] = 0.0f;
+            continue;
+         }
+         const double lij = F[nd.n + i + (int64_t)j * nd.ld];
+         double v;
+         if (first) {
+            const double det = __builtin_fma(d[2 * j], d[2 * j + 3], -(d[2 * j + 1] * d[2 * j + 1]));
+            v = dot2(d[2 * j + 3] / det, lij, -d[2 * j + 1] / det, F[nd.n + i + (int64_t)(j + 1) * nd.ld]);
+         } else if (second) {
+            const int k = j - 1;
+            const double det = __builtin_fma(d[2 * k], d[2 * k + 3], -(d[2 * k + 1] * d[2 * k + 1]));
+            v = dot2(-d[2 * k + 1] / det, F[nd.n + i + (int64_t)k * nd.ld], d[2 * k] / det, lij);
+         } else {
+            const double D0 = (d[2 * j] == 0.0) ? 0.0 : 1.0 / d[2 * j];
+            v = D0 * lij;
+         }
+         W[i + (int64_t)j * ldq] = (float)v;
+         L21[i + (int64_t)j * ldq] = (float)lij;
+      }
+   }
+}
+
 // After a level: L of every front (m x nelim) into the factor store, and the
 // trailing block (lower triangle of rows/cols nelim..m-1) into the
 // contribution store for the parent's assembly. Grid (chunks, fronts).

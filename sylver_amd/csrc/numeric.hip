@@ -1196,7 +1196,9 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
          prec = SYLVER_AMD_PREC_FP64;
       // Mixed precision without the BF16 update kernels runs the FP32 path.
       const bool low = prec != SYLVER_AMD_PREC_FP64;
-      if (!posdef) prec = SYLVER_AMD_PREC_FP64;  // the indefinite path factors in FP64
+      // the indefinite path's low tier is FP32 (Schur updates of the
+      // contribution blocks on FP32 MFMA; no BF16 tier)
+      if (!posdef && low) prec = SYLVER_AMD_PREC_FP32;
       DeviceFactors* F = *fact;
       if (F && (F->plan.get() != P || F->precision != prec || F->posdef != posdef)) {
          destroy_factors(F);
@@ -1405,13 +1407,19 @@ void solve_once(const DevicePlan* P, DeviceFactors* F, int job, const double* b,
 // modified; d may not alias r.
 template <typename T>
 void precond(const DevicePlan* P, DeviceFactors* F, const double* r, double* d, int nrhs, hipStream_t s) {
+   auto solve = [&](const double* rhs) {
+      if (F->posdef)
+         solve_once<T>(P, F, 0, rhs, d, nrhs, s);
+      else
+         solve_indef(P, F, 0, rhs, d, nrhs, s);
+   };
    if (!F->has_scaling) {
-      solve_once<T>(P, F, 0, r, d, nrhs, s);
+      solve(r);
       return;
    }
    const int64_t nn = (int64_t)P->n * nrhs;
    k_scale_rows<<<grid1(nn), kThreads, 0, s>>>(P->n, nrhs, F->d_scaling, r, F->d_sc);
-   solve_once<T>(P, F, 0, F->d_sc, d, nrhs, s);
+   solve(F->d_sc);
    k_scale_rows<<<grid1(nn), kThreads, 0, s>>>(P->n, nrhs, F->d_scaling, d, d);
 }
 
@@ -1585,7 +1593,7 @@ SolveResult device_solve(DeviceFactors* F, int job, int nrhs, double* x, int ldx
       const bool permute = user_order && P->d_invp;
       const bool scaled = user_order && F->has_scaling;
       const bool low = F->precision != SYLVER_AMD_PREC_FP64;
-      const bool refine = low && job == 0 && xo.refine != SYLVER_AMD_REFINE_NONE && F->posdef;
+      const bool refine = low && job == 0 && xo.refine != SYLVER_AMD_REFINE_NONE;
       HIPCHECK(hipMemcpy2DAsync(permute ? F->d_xin : F->d_b, (size_t)n * sizeof(double), x,
                                 (size_t)ldx * sizeof(double), (size_t)n * sizeof(double), nrhs,
                                 hipMemcpyHostToDevice, s));

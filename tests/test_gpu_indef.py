@@ -306,3 +306,83 @@ def test_seam_numeric_tree_posdef(S, gpu):
     L.spldlt_destroy_numeric_tree_posdef_dbl(nt)
     L.spldlt_destroy_symbolic_tree.argtypes = [C.c_void_p]
     L.spldlt_destroy_symbolic_tree(tree)
+
+
+def test_kkt_48_matching_order_full_size(S, O, gpu):
+    """The KKT workload at full size (H = 48^3 Laplacian, 27,648 constraint
+    rows) with the matching-based ordering and its saved scaling: one tree
+    level then holds more than 2^32 front entries (its zero fill is a
+    grid-stride launch), fronts reach 56,263 rows; inertia exactly
+    (110,592, 27,648, 0) and the scaled backward error <= 1e-14."""
+    A = M.kkt(48, 27648, seed=0)
+    opts = S.default_options()
+    opts.ordering = 2
+    akeep, inf, _ = S.spldlt_analyse(A.n, A.ptr, A.row, A.val, options=opts)
+    assert inf.flag >= 0
+    opts.scaling = 3
+    fk, inf = S.spldlt_factorize(False, A.val, akeep, options=opts)
+    try:
+        assert inf.flag == 0, inf.flag
+        assert inf.num_neg == 27648 and inf.matrix_rank == A.n
+        assert inf.maxfront > 50000
+        b = M.random_rhs(A.n, 1, seed=0)
+        x = b.copy(order="F")
+        assert S.spldlt_solve(x, akeep, fk, options=opts).flag == 0
+        assert O.bwd_err(A, x, b) <= BWD_TOL
+    finally:
+        S.spldlt_free_fkeep(fk)
+        S.spldlt_free_akeep(akeep)
+
+
+@pytest.mark.parametrize("refine", ["gmres", "auto"])
+def test_fp32_tier_refined_kkt16(S, O, gpu, refine):
+    """Indefinite FP32 tier (contribution-block Schur updates on FP32 MFMA;
+    APP pivot tests, D and its 2x2 inverses in FP64) with FP64 GMRES-IR
+    (forced, or AUTO: IR falling back to GMRES-IR) against the FP64 matrix:
+    KKT with H = 16^3, 1,024 constraint rows. Inertia exactly (4096, 1024,
+    0) and scaled backward error <= 1e-15 on every right-hand side (oracle)."""
+    A = M.kkt(16, 1024, seed=0)
+    akeep, opts, sym = _analyse(S, A)
+    mode = S.REFINE_GMRES if refine == "gmres" else S.REFINE_AUTO
+    S.set_ext_options(precision=S.PREC_FP32, refine=mode, refine_tol=1e-15, refine_max_iter=20)
+    try:
+        fk, inf = S.spldlt_factorize(False, A.val, akeep, options=opts)
+        assert inf.flag == 0, inf.flag
+        assert inf.tail().precision == S.PREC_FP32
+        assert inf.num_neg == 1024 and inf.matrix_rank == A.n
+        b = M.random_rhs(A.n, 3, seed=7)
+        x = b.copy(order="F")
+        sinf = S.spldlt_solve(x, akeep, fk, options=opts)
+        assert sinf.flag == 0, sinf.flag
+        t = sinf.tail()
+        assert t.refine_iters >= 1
+        if refine == "gmres":
+            assert t.refine_mode == S.REFINE_GMRES
+        for c in range(b.shape[1]):
+            assert O.bwd_err(A, x[:, c], b[:, c]) <= 1e-15, c
+        S.spldlt_free_fkeep(fk)
+    finally:
+        S.set_ext_options(precision=S.PREC_FP64, refine=S.REFINE_AUTO)
+        S.spldlt_free_akeep(akeep)
+
+
+def test_fp32_tier_matching_order_kkt(S, O, gpu):
+    """The FP32 tier with the matching-based ordering and its saved scaling
+    (scaling = 3): the refinement runs on the unscaled system."""
+    A = M.kkt(12, 432, seed=2)
+    opts = S.default_options()
+    opts.ordering = 2
+    akeep, inf, _ = S.spldlt_analyse(A.n, A.ptr, A.row, A.val, options=opts)
+    opts.scaling = 3
+    S.set_ext_options(precision=S.PREC_FP32, refine=S.REFINE_GMRES, refine_tol=1e-15, refine_max_iter=20)
+    try:
+        fk, inf = S.spldlt_factorize(False, A.val, akeep, options=opts)
+        assert inf.flag == 0 and inf.num_neg == 432
+        b = M.random_rhs(A.n, 2, seed=3)
+        x = b.copy(order="F")
+        assert S.spldlt_solve(x, akeep, fk, options=opts).flag == 0
+        assert O.bwd_err(A, x, b) <= 1e-15
+        S.spldlt_free_fkeep(fk)
+    finally:
+        S.set_ext_options(precision=S.PREC_FP64, refine=S.REFINE_AUTO)
+        S.spldlt_free_akeep(akeep)
