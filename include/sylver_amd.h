@@ -230,7 +230,9 @@ int sylver_amd_fkeep_node_factor(void const *fkeep, int node, double *l);
  * factorization) is returned when the factorization ran with
  * record_timing == 2 (debug mode). */
 int sylver_amd_fkeep_node_assembled(void const *fkeep, int node, double *front);
-/* Indefinite factors of node `node` (0-based): dims[4] = {m, n, nelim,
+/* Indefinite factors of node `node` (0-based) of an fkeep of
+ * spldlt_factorize(posdef = false) or of a numeric tree of
+ * spldlt_create_numeric_tree_dbl(posdef = false): dims[4] = {m, n, nelim,
  * ndelay_in} (front rows and fully-summed columns including delayed ones,
  * eliminated pivots, delayed columns received); rows (m ints): 1-based
  * pivot-order index of each front row after pivoting (the reference's

@@ -21,7 +21,6 @@ namespace sylver_amd {
 
 constexpr int kThreads = 256;
 constexpr int kNB = 128;      // block size of the large-front path and of the solve
-constexpr int kSB = 16;       // inner block of the one-workgroup front kernel
 constexpr int kMaxRhsChunk = 16;
 
 struct NodeDesc {
@@ -441,6 +440,9 @@ __global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ 
    __shared__ T L[kNB * LD];
    __shared__ T Tmp[kNB * 16];
    __shared__ int sh_bad;
+   // the diagonal blocks are the factorization's critical path: their waves
+   // win instruction issue over the look-ahead updates sharing the CU
+   __builtin_amdgcn_s_setprio(3);
    const DiagTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    const int kb = t.kb;
@@ -695,6 +697,9 @@ template <typename T, int MODE, int TB = 128, int TBN = TB>
 __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict__ tasks,
                                                      T* __restrict__ arena,
                                                      const T* __restrict__ bsrc) {
+   // the panel TRSM and the next pair's columns (64-row tiles) are on the
+   // critical path; the far look-ahead updates (128 x 128) are not
+   if (TB == 64) __builtin_amdgcn_s_setprio(2);
    const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
    if (t.M <= 0) return;
    constexpr int VN = V16<T>::N;
@@ -813,105 +818,144 @@ static __global__ __launch_bounds__(kThreads) void k_update_bf16(const GemmTask*
 
 // ---------------------------------------------------------------------------
 // One workgroup factorizes a whole front (partial Cholesky over its first n
-// columns, Schur complement into the CB): blocked right-looking with inner
-// block kSB; the diagonal block in LDS, the panel TRSM one row per thread,
-// the trailing update on MFMA 16x16 tiles read through L2.
+// columns, Schur complement into the CB), right-looking in panels of kPB
+// columns. Per panel: the diagonal block in LDS (the blocked Cholesky and
+// inverse of the large-front diagonal kernel), the rows below as one product
+// with that inverse (L21 = A21 inv(L11)^T, 16-row strips on MFMA), then the
+// trailing lower triangle -= L21 L21^T on MFMA in 32 x 32 tiles with the
+// panel read through L1/L2: every trailing entry is read and written once
+// per panel, n / kPB passes in all.
 // Restates factor_front_posdef's arithmetic (reference src/factor.hxx:19-148,
 // src/kernels/factor.hxx:34-263) with a different blocking.
 // ---------------------------------------------------------------------------
+constexpr int kPB = 64;
+
 template <typename T>
 __device__ void front_potrf_block(const NodeDesc& nd, T* __restrict__ F, int* info_slot) {
-   constexpr int B = kSB;
-   __shared__ T D[B][B + 1];
+   constexpr int LD = kPB + 1;
+   __shared__ T D[kPB * LD];
+   __shared__ T Tmp[kNB * 16];  // scratch of lds_trtri_blocked (ld kNB)
+   __shared__ int sh_bad;
    typedef typename Mfma<T>::acc_t acc_t;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int m = nd.m, n = nd.n, ld = nd.ld;
-   for (int k = 0; k < n; k += B) {
-      const int kb = min(B, n - k);
+   for (int k0 = 0; k0 < n; k0 += kPB) {
+      const int kb = min(kPB, n - k0), kb16 = (kb + 15) & ~15;
+      T* blk = F + k0 + (int64_t)k0 * ld;
+      __syncthreads();  // the previous panel's update is complete
+      // diagonal block, padded to kb16 with the identity, upper triangle zero
+      for (int e = tid; e < kb16 * kb16; e += kThreads) {
+         const int i = e % kb16, j = e / kb16;
+         const bool live = i < kb && j < kb && i >= j;
+         const T v = *live_or_zero(live, blk + i + (int64_t)j * ld);
+         D[i + j * LD] = live ? v : ((i == j) ? T(1) : T(0));
+      }
       __syncthreads();
+      const int bad = lds_potrf_blocked<T, LD>(D, kb16, &sh_bad);
+      if (bad) {
+         if (tid == 0) *info_slot = k0 + bad;
+         return;
+      }
       for (int e = tid; e < kb * kb; e += kThreads) {
-         int i = e % kb, j = e / kb;
-         D[i][j] = F[(k + i) + (int64_t)(k + j) * ld];
-      }
-      // diagonal block Cholesky
-      for (int j = 0; j < kb; ++j) {
-         __syncthreads();
-         T d = D[j][j];
-         if (!(d > T(0))) {
-            if (tid == 0) *info_slot = k + j + 1;
-            return;
-         }
-         d = dsqrt(d);
-         __syncthreads();
-         if (tid == 0) D[j][j] = d;
-         if (tid > j && tid < kb) D[tid][j] /= d;
-         __syncthreads();
-         {
-            const int i = tid & 15, c = tid >> 4;
-            if (c > j && i >= c && i < kb) D[i][c] -= D[i][j] * D[c][j];
-         }
+         const int i = e % kb, j = e / kb;
+         if (i >= j) blk[i + (int64_t)j * ld] = D[i + j * LD];
       }
       __syncthreads();
-      for (int e = tid; e < kb * kb; e += kThreads) {
-         int i = e % kb, j = e / kb;
-         if (i >= j) F[(k + i) + (int64_t)(k + j) * ld] = D[i][j];
-      }
-      // panel TRSM: rows below the diagonal block, one row per thread
-      for (int r = k + kb + tid; r < m; r += kThreads) {
-         T x[B];
-#pragma unroll
-         for (int c = 0; c < B; ++c) x[c] = *live_or_zero(c < kb, F + r + (int64_t)(k + c) * ld);
-#pragma unroll
-         for (int c = 0; c < B; ++c) {
-            if (c < kb) {
-               T s = x[c];
-#pragma unroll
-               for (int cc = 0; cc < c; ++cc) s -= x[cc] * D[c][cc];
-               x[c] = s / D[c][c];
-            }
-         }
-#pragma unroll
-         for (int c = 0; c < B; ++c)
-            if (c < kb) F[r + (int64_t)(k + c) * ld] = x[c];
-      }
-      __syncthreads();
-      // trailing update of the lower triangle of rows/cols [k+kb, m)
-      const int r0 = k + kb;
-      const int R = m - r0;
+      lds_trtri_blocked<T, LD>(D, Tmp, kb16);  // D = inv(L11), lower
+      const int r0 = k0 + kb, R = m - r0;
       if (R <= 0) continue;
-      const int nt = (R + 15) / 16;
+      // L21 = A21 inv(L11)^T in place. Operands: x = inv(L11)[c][k] (LDS),
+      // y = A21[row][k]; item v of lane l is L21[rs + l%16][16 ct + row(l, v)].
+      // All K operands of a strip or tile are loaded before its first product:
+      // one L2 round trip per tile instead of one per k-step.
+      constexpr int KS = kPB / 4;  // k-steps of a full panel
+      const int nct = kb16 / 16, nks = kb16 / 4;
+      for (int rs = r0 + wave * 16; rs < m; rs += 64) {
+         acc_t acc[kPB / 16];
+#pragma unroll
+         for (int ct = 0; ct < kPB / 16; ++ct) acc[ct] = acc_t{0, 0, 0, 0};
+         const int row = rs + (lane & 15);
+         T y[KS];
+#pragma unroll
+         for (int q = 0; q < KS; ++q) {
+            const int k = 4 * q + (lane >> 4);
+            y[q] = *live_or_zero(row < m && k < kb, F + row + (int64_t)(k0 + k) * ld);
+         }
+#pragma unroll
+         for (int q = 0; q < KS; ++q) {
+            if (q >= nks) break;
+            const int k = 4 * q + (lane >> 4);
+#pragma unroll
+            for (int ct = 0; ct < kPB / 16; ++ct)
+               if (ct < nct) acc[ct] = Mfma<T>::op(D[(ct * 16 + (lane & 15)) + k * LD], y[q], acc[ct]);
+         }
+#pragma unroll
+         for (int ct = 0; ct < kPB / 16; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+               const int c = ct * 16 + Mfma<T>::row(lane, v);
+               if (ct < nct && row < m && c < kb) F[row + (int64_t)(k0 + c) * ld] = acc[ct][v];
+            }
+      }
+      __syncthreads();  // L21 written before the update reads it
+      // trailing lower triangle: C[i][j] -= sum_k L21[i][k] L21[j][k] in
+      // 32 x 32 tiles (2 x 2 MFMA tiles per wave); operands x = L21[j][k],
+      // y = L21[i][k]; item v of lane l is C[i0 + l%16][j0 + row(l, v)].
+      const int nt = (R + 31) / 32;
       const int ntiles = nt * (nt + 1) / 2;
-      const T* P = F + r0 + (int64_t)k * ld;  // panel rows r0.., columns k..k+kb
       for (int tile = wave; tile < ntiles; tile += 4) {
          int ti = (int)((__builtin_sqrtf(8.0f * tile + 1.0f) - 1.0f) * 0.5f);
          while (ti * (ti + 1) / 2 > tile) --ti;
          while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
          const int tj = tile - ti * (ti + 1) / 2;
-         acc_t acc = acc_t{0, 0, 0, 0};
-         const int ia = ti * 16 + (lane & 15);
-         const int ib = tj * 16 + (lane & 15);
-         T av[B / 4], bv[B / 4];
+         const int ri = r0 + ti * 32, rj = r0 + tj * 32;
+         acc_t acc[2][2];
 #pragma unroll
-         for (int u = 0; u < B / 4; ++u) {
-            const int kc = 4 * u + (lane >> 4);
-            av[u] = *live_or_zero(ia < R && kc < kb, P + ia + (int64_t)kc * ld);
-            bv[u] = *live_or_zero(ib < R && kc < kb, P + ib + (int64_t)kc * ld);
+         for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+         // FP64: the operands in two halves of the panel (register budget)
+         constexpr int KC = sizeof(T) == 8 ? KS / 2 : KS;
+         for (int q0 = 0; q0 < nks; q0 += KC) {
+            T xa[2][KC], yb[2][KC];
+#pragma unroll
+            for (int q = 0; q < KC; ++q) {
+               const int k = 4 * (q0 + q) + (lane >> 4);
+#pragma unroll
+               for (int u = 0; u < 2; ++u) {
+                  const int i = ri + u * 16 + (lane & 15), j = rj + u * 16 + (lane & 15);
+                  yb[u][q] = *live_or_zero(i < m && k < kb, F + i + (int64_t)(k0 + k) * ld);
+                  xa[u][q] = *live_or_zero(j < m && k < kb, F + j + (int64_t)(k0 + k) * ld);
+               }
+            }
+#pragma unroll
+            for (int q = 0; q < KC; ++q) {
+               if (q0 + q >= nks) break;
+#pragma unroll
+               for (int a = 0; a < 2; ++a)
+#pragma unroll
+                  for (int b = 0; b < 2; ++b) acc[a][b] = Mfma<T>::op(xa[b][q], yb[a][q], acc[a][b]);
+            }
          }
+         T cv[2][2][4];
 #pragma unroll
-         for (int u = 0; u < B / 4; ++u) acc = Mfma<T>::op(bv[u], av[u], acc);
-         // transposed tile: item v of lane l is the entry (i = l%16, j = row(l, v))
-         const int i = ti * 16 + (lane & 15);
-         T cv[4];
+         for (int a = 0; a < 2; ++a)
 #pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int j = tj * 16 + Mfma<T>::row(lane, v);
-            cv[v] = *live_or_zero(i < R && j <= i, F + (r0 + i) + (int64_t)(r0 + j) * ld);
-         }
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
-         for (int v = 0; v < 4; ++v) {
-            const int j = tj * 16 + Mfma<T>::row(lane, v);
-            if (i < R && j <= i) F[(r0 + i) + (int64_t)(r0 + j) * ld] = cv[v] - acc[v];
-         }
+               for (int v = 0; v < 4; ++v) {
+                  const int i = ri + a * 16 + (lane & 15), j = rj + b * 16 + Mfma<T>::row(lane, v);
+                  cv[a][b][v] = *live_or_zero(i < m && j <= i, F + i + (int64_t)j * ld);
+               }
+#pragma unroll
+         for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+               for (int v = 0; v < 4; ++v) {
+                  const int i = ri + a * 16 + (lane & 15), j = rj + b * 16 + Mfma<T>::row(lane, v);
+                  if (i < m && j <= i) F[i + (int64_t)j * ld] = cv[a][b][v] - acc[a][b][v];
+               }
       }
    }
    __syncthreads();

@@ -51,6 +51,38 @@ Fkeep* as_fkeep(void* p) {
    return (f && f->magic == 0x464b4550) ? f : nullptr;
 }
 
+// Numeric-tree seam objects (src/SymbolicTree.cxx:110-137,
+// src/NumericTreePosdef.cxx:19-83, src/NumericTree.cxx:27-136).
+struct SeamSymbolic {
+   Akeep ak;  // holds an Analysis rebuilt from the caller's arrays
+};
+struct SeamNumeric {
+   uint32_t magic = 0x534e554d;  // "SNUM"
+   SeamSymbolic* sym = nullptr;
+   DeviceFactors* f = nullptr;
+   bool posdef = true;
+   ~SeamNumeric() {
+      destroy_factors(f);
+      magic = 0;
+   }
+};
+
+// The indefinite factors behind a handle: an fkeep of spldlt_factorize or a
+// numeric tree of spldlt_create_numeric_tree_dbl (posdef = false).
+const DeviceFactors* indef_factors_of(const void* p) {
+   if (!p) return nullptr;
+   const uint32_t magic = *static_cast<const uint32_t*>(p);
+   if (magic == 0x464b4550) {
+      const Fkeep* fk = static_cast<const Fkeep*>(p);
+      return (fk->f && !fk->posdef) ? fk->f : nullptr;
+   }
+   if (magic == 0x534e554d) {
+      const SeamNumeric* t = static_cast<const SeamNumeric*>(p);
+      return (t->f && !t->posdef) ? t->f : nullptr;
+   }
+   return nullptr;
+}
+
 void clear_inform(sylver_inform_t* inform) {
    if (!inform) return;
    std::memset(inform, 0, sizeof(*inform));
@@ -412,15 +444,15 @@ int sylver_amd_fkeep_node_factor(void const* fkeep, int node, double* l) {
 
 int sylver_amd_fkeep_indef_node(void const* fkeep, int node, int* dims, int* rows, double* l,
                                 double* dinv) {
-   const Fkeep* fk = as_fkeep(const_cast<void*>(fkeep));
-   if (!fk || !fk->f || fk->posdef) return SYLVER_ERROR_CALL_SEQUENCE;
-   return indef_node(fk->f, node, dims, rows, l, dinv);
+   const DeviceFactors* f = indef_factors_of(fkeep);
+   if (!f) return SYLVER_ERROR_CALL_SEQUENCE;
+   return indef_node(f, node, dims, rows, l, dinv);
 }
 
 int sylver_amd_fkeep_indef_stats(void const* fkeep, int* out) {
-   const Fkeep* fk = as_fkeep(const_cast<void*>(fkeep));
-   if (!fk || !fk->f || fk->posdef || !out) return SYLVER_ERROR_CALL_SEQUENCE;
-   return indef_stats(fk->f, out);
+   const DeviceFactors* f = indef_factors_of(fkeep);
+   if (!f || !out) return SYLVER_ERROR_CALL_SEQUENCE;
+   return indef_stats(f, out);
 }
 
 int sylver_amd_fkeep_node_assembled(void const* fkeep, int node, double* front) {
@@ -590,18 +622,6 @@ double sylver_amd_last_batched_ms(void) { return last_batched_ms(); }
 // Internal numeric-tree seam (src/SymbolicTree.cxx:110-137,
 // src/NumericTreePosdef.cxx:19-83, src/NumericTree.cxx:27-136).
 // ---------------------------------------------------------------------------
-namespace {
-struct SeamSymbolic {
-   Akeep ak;  // holds an Analysis rebuilt from the caller's arrays
-};
-struct SeamNumeric {
-   SeamSymbolic* sym = nullptr;
-   DeviceFactors* f = nullptr;
-   bool posdef = true;
-   ~SeamNumeric() { destroy_factors(f); }
-};
-}  // namespace
-
 void* spldlt_create_symbolic_tree(void* akeep, int n, int nnodes, int const* sptr,
                                   int const* sparent, long const* rptr, int const* rlist,
                                   long const* nptr, long const* nlist, int nsubtrees,
