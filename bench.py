@@ -453,7 +453,7 @@ def main():
                     help="run only one precision path (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (scales PMC byte counters)")
-    ap.add_argument("--profile-only", action="store_true",
+    ap.add_argument("--profile-only", action="store_true",  # with --paths fp64: the FP64 factorization
                     help="only the per-launch timed (eager, one stream) factorization of the FP32 path")
     ap.add_argument("--variants", default="model", choices=["model", "default"],
                     help="per-front kernel variants: the learned cost model's (trained first) or the default rule")
@@ -462,6 +462,9 @@ def main():
                          "else saved there after training (profiled runs reuse the same choices)")
     ap.add_argument("--costmodel-full", action="store_true",
                     help="learned cost model on the configured stream (400 + 100 matrices, full sizes)")
+    ap.add_argument("--only", default=None, choices=["batched", "kkt"],
+                    help="run one secondary workload alone and print its record (profiling runs)")
+    ap.add_argument("--kkt-side", type=int, default=48, help="grid side of H for --only kkt")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the secondary workloads (2D Laplacian, batched fronts, KKT)")
     args = ap.parse_args()
@@ -479,6 +482,15 @@ def main():
     if S.device_count() < 1:
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(3)
+    if args.only == "batched":
+        print(json.dumps(bench_batched_fronts(S, M, model=None, cpu_fronts=200), default=float))
+        return
+    if args.only == "kkt":
+        import oracle as oracle_mod
+        side = args.kkt_side
+        print(json.dumps(bench_kkt(S, M, oracle_mod, side=side, mrows=27648 if side == 48 else side ** 3 // 4),
+                         default=float))
+        return
     g = args.grid
     A = M.laplacian_3d(g)
     b = M.random_rhs(A.n, args.nrhs, seed=0)
@@ -523,8 +535,9 @@ def main():
         np.savez(args.variants_file, v0=variants_for(0), v1=variants_for(1))
 
     if args.profile_only:  # rocprofv3 runs of the roofline figure: the same eager launches
-        use_variants(1)
-        kinds, fms, _ = kernel_profile(S, A, akeep, opts, prec, None)
+        pr = S.PREC_FP64 if args.paths == "fp64" else prec
+        use_variants(0 if pr == S.PREC_FP64 else 1)
+        kinds, fms, _ = kernel_profile(S, A, akeep, opts, pr, None)
         print(json.dumps({"profile_only": True, "kernels": kinds}))
         return
     if args.paths != "both":  # profiling runs: one path, no report
