@@ -164,7 +164,10 @@ typedef struct {
    int record_timing;    /* record per-level device timings (0) */
    int indef_panel_min;  /* indefinite fronts with at least this many fully-summed columns
                             use the multi-workgroup APP panel (0 = auto: 256) */
-   int reserved[6];
+   int small_assembly;   /* one-workgroup posdef fronts: 1 assembled (zero, A entries,
+                            children) inside their factor launch, one launch per level
+                            (default); 0 separate zero / scatter / extend-add launches */
+   int reserved[5];
 } sylver_amd_options_t;
 
 void sylver_amd_default_options(sylver_amd_options_t *opt);

@@ -115,7 +115,7 @@ class ExtOptions(C.Structure):
         ("precision", C.c_int), ("bf16_min_ncol", C.c_int), ("refine", C.c_int),
         ("refine_tol", C.c_double), ("refine_max_iter", C.c_int), ("use_graph", C.c_int),
         ("tile_nb", C.c_int), ("small_front_max", C.c_int), ("record_timing", C.c_int),
-        ("indef_panel_min", C.c_int), ("reserved", C.c_int * 6),
+        ("indef_panel_min", C.c_int), ("small_assembly", C.c_int), ("reserved", C.c_int * 5),
     ]
 
 

@@ -991,6 +991,55 @@ __global__ __launch_bounds__(kThreads) void k_front_fill(const int* __restrict__
    }
 }
 
+// One workgroup assembles and factorizes a whole front: the lower triangle
+// zeroed, the A entries scattered, the children's contribution blocks added
+// in list order (one child at a time, so no two writers meet), then the
+// partial factorization. Per entry the values arrive in the order of the
+// separate k_zero / k_init_a / k_extend_add launches (A value, then the
+// children in list order), so the fronts are bitwise the same; the fronts
+// of a level take one launch instead of four or more.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_small_asm(const int* __restrict__ node_list,
+                                                        const NodeDesc* __restrict__ nodes,
+                                                        const int64_t* __restrict__ nlist,
+                                                        const int* __restrict__ rlist,
+                                                        const double* __restrict__ aval,
+                                                        const double* __restrict__ scaling,
+                                                        const int* __restrict__ cmap,
+                                                        T* __restrict__ arena, int* __restrict__ info) {
+   const int node = node_list[blockIdx.x];
+   const NodeDesc nd = nodes[node];
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   T* F = arena + nd.off;
+   for (int c = wave; c < nd.m; c += 4) {
+      T* col = F + (int64_t)c * nd.ld;
+      for (int r = c + lane; r < nd.m; r += 64) col[r] = T(0);
+   }
+   __syncthreads();
+   for (int64_t p = threadIdx.x; p < nd.num_a; p += kThreads) {
+      const int64_t src = nlist[2 * (nd.amap_off + p)] - 1;
+      const int64_t dest = nlist[2 * (nd.amap_off + p) + 1] - 1;
+      const int c = (int)(dest / nd.m);
+      const int r = (int)(dest - (int64_t)c * nd.m);
+      double v = aval[src];
+      if (scaling) v = scaling[rlist[nd.rlist_off + r] - 1] * v * scaling[rlist[nd.rlist_off + c] - 1];
+      F[(int64_t)c * nd.ld + r] = (T)v;
+   }
+   for (int ch = nd.first_child; ch != -1;) {
+      const NodeDesc c = nodes[ch];
+      __syncthreads();  // the previous writer of these entries is done
+      const int q = c.m - c.n;
+      const int* __restrict__ cm = cmap + c.cmap_off;
+      for (int j = wave; j < q; j += 4) {
+         T* dst = F + (int64_t)cm[j] * nd.ld;
+         const T* src = arena + c.off + (int64_t)(c.n + j) * c.ld + c.n;
+         for (int i = j + lane; i < q; i += 64) dst[cm[i]] += src[i];
+      }
+      ch = c.next_child;
+   }
+   front_potrf_block<T>(nd, F, info + node);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_small_front(const int* __restrict__ node_list,
                                                           const NodeDesc* __restrict__ nodes,

@@ -43,6 +43,7 @@ sylver_amd_options_t& ext_options() {
       d.refine_tol = 1e-15;
       d.refine_max_iter = 30;
       d.use_graph = 1;
+      d.small_assembly = 1;
       return d;
    }();
    return o;
@@ -67,12 +68,15 @@ namespace {
 
 // The plan's host side: layout, maps, task lists and launch programs. No
 // HIP call (it also runs, for its checks, on a host without a device).
-DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int num_cus) {
+DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int num_cus, bool fuse_small) {
    std::unique_ptr<DevicePlan> P(new DevicePlan());
    P->n = a.n;
    P->nnodes = a.nnodes;
    P->small_max = small_max;
    P->wide_ncol = wide_ncol;
+   // fronts at fixed caller offsets (the batched entry points) hold their
+   // values already: nothing to assemble
+   P->fuse_small = fuse_small && a.fixed_off.empty();
    P->variant_gen = a.variant_gen;
    const int nn = a.nnodes;
    // one-workgroup kernel or tiled path for front i
@@ -168,9 +172,18 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
    int side_last = -1;  // last event recorded on the side stream, not yet waited for
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
+      // one-workgroup fronts assembled inside their factor launch
+      auto fused = [&](int i) { return P->fuse_small && P->hnodes[i].n > 0 && one_wg(i); };
+      {
+         int64_t f0 = P->nlistv.size();
+         for (int i : lv)
+            if (fused(i)) P->nlistv.push_back(i);
+         push(P->fprog, L_SMALL_ASM, f0, P->nlistv.size() - f0);
+      }
       // zero
       int64_t z0 = P->zero.size();
       for (int i : lv) {
+         if (fused(i)) continue;
          const NodeDesc& d = P->hnodes[i];
          int chunk = std::max(4, std::min(256, 16384 / std::max(d.m, 1)));
          for (int c = 0; c < d.m; c += chunk) P->zero.push_back(ColTask{i, c, std::min(d.m, c + chunk), 0});
@@ -179,6 +192,7 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
       // scatter A
       int64_t i0 = P->init.size();
       for (int i : lv) {
+         if (fused(i)) continue;
          const NodeDesc& d = P->hnodes[i];
          for (int64_t p = 0; p < d.num_a; p += 4096)
             P->init.push_back(InitTask{i, 0, p, std::min<int64_t>(d.num_a, p + 4096)});
@@ -193,6 +207,7 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
             for (int r = 0; r < rank && c != -1; ++r) c = P->hnodes[c].next_child;
             if (c == -1) continue;
             any = true;
+            if (fused(i)) continue;
             const NodeDesc& dc = P->hnodes[c];
             int q = dc.m - dc.n;
             if (q <= 0) continue;
@@ -207,7 +222,7 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
       std::vector<int> large;
       for (int i : lv) {
          const NodeDesc& d = P->hnodes[i];
-         if (d.n == 0) continue;
+         if (d.n == 0 || fused(i)) continue;
          if (one_wg(i))
             P->nlistv.push_back(i);
          else
@@ -393,6 +408,20 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
                const NodeDesc& d = P->hnodes[P->nlistv[t]];
                L.flops += (double)node_flops(d.m, d.n);
                L.words += (double)d.m * (d.m + 1);  // lower triangle read + written once
+               break;
+            }
+            case L_SMALL_ASM: {
+               const NodeDesc& d = P->hnodes[P->nlistv[t]];
+               L.flops += (double)node_flops(d.m, d.n);
+               // zero fill, then the factorization's read and write of the lower triangle
+               L.words += 1.5 * (double)d.m * (d.m + 1);
+               L.words += (double)d.num_a;
+               L.bytes += 24.0 * (double)d.num_a;
+               for (int c = d.first_child; c != -1; c = P->hnodes[c].next_child) {
+                  const double q = P->hnodes[c].m - P->hnodes[c].n;
+                  L.words += 3.0 * q * (q + 1) / 2;  // read child, read+write parent
+                  L.bytes += 4.0 * q * (q + 1) / 2;  // row map
+               }
                break;
             }
             case L_DIAG:
@@ -836,6 +865,30 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
                   for (int64_t t = L.off; t < e; ++t)
                      if (P->nlistv[t] < 0 || P->nlistv[t] >= nn) fail("node list entry");
                break;
+            case L_SMALL_ASM:
+               if (e > (int64_t)P->nlistv.size()) fail("node list: range");
+               else
+                  for (int64_t t = L.off; t < e; ++t) {
+                     const int i = P->nlistv[t];
+                     if (i < 0 || i >= nn) {
+                        fail("node list entry");
+                        continue;
+                     }
+                     const NodeDesc& d = P->hnodes[i];
+                     if (d.amap_off < 0 || d.num_a < 0) fail("assembled front: A map");
+                     int guard = 0;
+                     for (int c = d.first_child; c != -1 && guard <= nn; c = P->hnodes[c].next_child, ++guard) {
+                        if (c < 0 || c >= nn || P->hnodes[c].parent != i) {
+                           fail("assembled front: child");
+                           break;
+                        }
+                        const int64_t q = P->hnodes[c].m - P->hnodes[c].n;
+                        if (P->hnodes[c].cmap_off < 0 || P->hnodes[c].cmap_off + q > (int64_t)P->hcmap.size())
+                           fail("assembled front: child map");
+                     }
+                     if (guard > nn) fail("assembled front: child list cycle");
+                  }
+               break;
             case L_DIAG:
             case L_DINV:
                if (e > (int64_t)P->diag.size()) fail("diag: range");
@@ -925,11 +978,11 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
    return bad;
 }
 
-DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol) {
+DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol, bool fuse_small) {
    int dev = 0, num_cus = 0;
    HIPCHECK(hipGetDevice(&dev));
    HIPCHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
-   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus));
+   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, fuse_small));
    upload_plan(P.get(), a);
    return P.release();
 }
@@ -967,6 +1020,10 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
             break;
          case L_SMALL:
             k_small_front<T><<<g, b, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, F->d_info);
+            break;
+         case L_SMALL_ASM:
+            k_small_asm<T><<<g, b, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, P->d_nlist, P->d_rlist, F->d_aval,
+                                           F->has_scaling ? F->d_scaling : nullptr, P->d_cmap, arena, F->d_info);
             break;
          case L_DIAG:
             k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, dinvT, F->d_info, 1);
@@ -1152,7 +1209,7 @@ void equilibrate(const DevicePlan* P, DeviceFactors* F, hipStream_t s, int max_i
 }  // namespace
 
 int plan_check(const Analysis& a, int small_max, int wide_ncol, int num_cus, int64_t* out, std::string* why) {
-   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus));
+   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, ext_options().small_assembly != 0));
    const int64_t bad = check_plan_indices(P.get(), a, why);
    if (out) {
       out[0] = bad;
@@ -1186,10 +1243,12 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
       int wide_ncol = xo.bf16_min_ncol > 0 ? xo.bf16_min_ncol : 1024;
       // a new plan for new knobs or variants; factorizations built on the old
       // one keep it alive through their own reference
+      const bool fuse = xo.small_assembly != 0;
       if (*plan && ((*plan)->small_max != small_max || (*plan)->wide_ncol != wide_ncol ||
+                    (*plan)->fuse_small != fuse ||
                     (*plan)->variant_gen != a.variant_gen))
          plan->reset();
-      if (!*plan) plan->reset(build_plan(a, small_max, wide_ncol));
+      if (!*plan) plan->reset(build_plan(a, small_max, wide_ncol, fuse));
       DevicePlan* P = plan->get();
       int prec = xo.precision;
       if (prec != SYLVER_AMD_PREC_FP64 && prec != SYLVER_AMD_PREC_FP32 && prec != SYLVER_AMD_PREC_MIXED)
@@ -1310,11 +1369,13 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
             float t = 0;
             HIPCHECK(hipEventElapsedTime(&t, events[2 * i], events[2 * i + 1]));
             const Launch& L = P->fprog[i];
-            if (L.kind >= 16) continue;  // L_SYNC: no kernel
-            F->kind_ms[L.kind] += t;
-            F->kind_flops[L.kind] += L.flops;
-            F->kind_bytes[L.kind] += L.words * es + L.bytes;
-            F->kind_launches[L.kind] += 1;
+            // the assembled one-workgroup fronts count with the one-workgroup kind
+            const int kind = L.kind == L_SMALL_ASM ? (int)L_SMALL : L.kind;
+            if (kind >= 16) continue;  // L_SYNC: no kernel
+            F->kind_ms[kind] += t;
+            F->kind_flops[kind] += L.flops;
+            F->kind_bytes[kind] += L.words * es + L.bytes;
+            F->kind_launches[kind] += 1;
          }
          for (auto& e : events) (void)hipEventDestroy(e);
       }
@@ -1877,7 +1938,7 @@ int time_fronts(int K, const int* m, const int* n, const signed char* variant, i
          for (int r = 1; r <= m[i]; ++r) a.rlist.push_back(r);
       }
       if (variant) a.variant.assign(variant, variant + K);
-      P.reset(build_plan(a, 256, 1 << 30));
+      P.reset(build_plan(a, 256, 1 << 30, false));
       const bool low = precision != SYLVER_AMD_PREC_FP64;
       const size_t es = low ? sizeof(float) : sizeof(double);
       F = new DeviceFactors();
@@ -1946,7 +2007,7 @@ int batched_front_potrf_v(int nfronts, const int* nrow, const int* ncol, const i
          for (int r = 1; r <= nrow[i]; ++r) a.rlist.push_back(r);
       }
       if (variant) a.variant.assign(variant, variant + nfronts);
-      P.reset(build_plan(a, 256, 1 << 30));
+      P.reset(build_plan(a, 256, 1 << 30, false));
       const bool low = dtype != 0;
       const size_t es = low ? sizeof(float) : sizeof(double);
       F = new DeviceFactors();

@@ -68,6 +68,7 @@ enum LaunchKind {
    L_BCHAIN,
    L_FCB,     // forward contribution blocks of the chained fronts (k_chain, no waits)
    L_CLEAR,   // exchange-buffer fill at the start of a chained sweep
+   L_SMALL_ASM,  // one-workgroup fronts assembled (zero, A, children) and factored in one launch
    L_SYNC  // no kernel: only the stream wait / event record of the entry
 };
 
@@ -105,6 +106,7 @@ struct DevicePlan {
    int variant_gen = -1;  // Analysis::variant_gen the plan was built with
    int n = 0, nnodes = 0, nlevels = 0;
    int small_max = 256;
+   bool fuse_small = false;  // one-workgroup fronts assembled inside their factor launch (L_SMALL_ASM)
    int wide_ncol = 1024;  // fronts with more pivots take the wide-front update launches
    std::vector<NodeDesc> hnodes;
    std::vector<std::vector<int>> levels;

@@ -379,6 +379,7 @@ void sylver_amd_default_options(sylver_amd_options_t* o) {
    o->tile_nb = 0;
    o->small_front_max = 0;
    o->indef_panel_min = 0;
+   o->small_assembly = 1;
    o->record_timing = 0;
 }
 

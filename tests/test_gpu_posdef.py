@@ -330,3 +330,34 @@ def test_laplacian_80_full_size_refined(S, O, gpu, lap80, tier):
             assert O.bwd_err(A, x[:, c], b[:, c]) <= BWD_TOL, c
     finally:
         S.spldlt_free_fkeep(fkeep)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_small_front_assembly_fused_bitwise(S, gpu, prec):
+    """One-workgroup fronts assembled inside their factor launch (zero, A
+    entries, children in list order) give bitwise the factors of the
+    separate zero / scatter / extend-add launches, with and without a user
+    scaling."""
+    A = M.laplacian_3d(22)
+    akeep, opts, sym = _analyse(S, A)
+    rng = np.random.default_rng(5)
+    scale = np.exp(rng.uniform(-1.0, 1.0, A.n))
+    pr = S.PREC_FP64 if prec == "fp64" else S.PREC_FP32
+    try:
+        for sc in (None, scale):
+            got = []
+            for fuse in (0, 1):
+                S.set_ext_options(precision=pr, small_assembly=fuse)
+                o = S.default_options()
+                o.scaling = 0
+                fk, inf = S.spldlt_factorize(True, A.val, akeep, options=o,
+                                             scale=None if sc is None else sc.copy())
+                assert inf.flag == 0
+                got.append([np.tril(S.fkeep_node_factor(fk, s, int(sym["rptr"][s + 1] - sym["rptr"][s]),
+                                                        int(sym["sptr"][s + 1] - sym["sptr"][s])))
+                            for s in range(sym["nnodes"])])
+                S.spldlt_free_fkeep(fk)
+            for a, b in zip(*got):
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    finally:
+        S.set_ext_options(precision=S.PREC_FP64, small_assembly=1)
