@@ -577,11 +577,11 @@ void solve_indef(const DevicePlan* P, DeviceFactors* F, int job, const double* b
    int maxm = 1;
    for (const IndefDesc& d : S.h) maxm = std::max(maxm, d.m);
    grow(S.d_spart, S.spart_cap, (int64_t)((maxm + kThreads - 1) / kThreads + 1) * nrhs * 64, 0, s);
-   // fronts with kIsvMin eliminated columns or more: blocked elimination
+   // fronts with many eliminated columns or a large L: blocked elimination
    auto blocked = [&](int L, bool forward) {
       for (int i : S.levels[L]) {
          const IndefDesc& d = S.h[i];
-         if (d.nelim < kIsvMin) continue;
+         if (!isv_blocked(d.m, d.nelim)) continue;
          if (forward) {
             for (int j0 = 0; j0 < d.nelim; j0 += 64) {
                const int jb = std::min(64, d.nelim - j0);

@@ -1759,6 +1759,12 @@ static __global__ __launch_bounds__(kThreads) void k_ind_store(const int* __rest
 }
 
 constexpr int kIsvMin = 256;  // fronts with this many eliminated columns get the blocked solve
+// ... and so do fronts whose L has this many entries: one workgroup sweeping
+// a tall L column by column waits on a memory round trip per column.
+constexpr int64_t kIsvMinEntries = int64_t(1) << 16;
+__host__ __device__ inline bool isv_blocked(int m, int nelim) {
+   return nelim > 0 && (nelim >= kIsvMin || (int64_t)m * nelim >= kIsvMinEntries);
+}
 
 // ---------------------------------------------------------------------------
 // Solves (NumericTree::solve_fwd / solve_diag_bwd_inner,
@@ -1812,7 +1818,7 @@ static __global__ __launch_bounds__(kThreads) void k_ind_fwd(const int* __restri
       }
       for (int r = threadIdx.x; r < m; r += kThreads) xl[r] = (r < n) ? xp[pre[r]] : xp[r];
       __syncthreads();
-      if (ne >= kIsvMin) continue;  // blocked elimination by k_isv_* (the loop's barriers stay uniform)
+      if (isv_blocked(m, ne)) continue;  // blocked elimination by k_isv_* (the loop's barriers stay uniform)
       for (int j = 0; j < ne; ++j) {
          const double y = xl[j];
          for (int r = j + 1 + threadIdx.x; r < m; r += kThreads)
@@ -1869,7 +1875,7 @@ static __global__ __launch_bounds__(kThreads) void k_ind_bwd(const int* __restri
       const double* x = sol + (int64_t)c * ld;
       for (int r = threadIdx.x; r < m; r += kThreads) xl[r] = x[var[r] - 1];
       __syncthreads();
-      if (ne >= kIsvMin) continue;  // blocked elimination by k_isv_*
+      if (isv_blocked(m, ne)) continue;  // blocked elimination by k_isv_*
       for (int j = ne - 1; j >= 0; --j) {
          double s = 0.0;
          for (int r = j + 1 + threadIdx.x; r < m; r += kThreads) s = __builtin_fma(F[r + (int64_t)j * m], xl[r], s);
@@ -1887,7 +1893,7 @@ static __global__ __launch_bounds__(kThreads) void k_ind_bwd(const int* __restri
 
 
 // ---------------------------------------------------------------------------
-// Blocked solves of a front with many eliminated columns (kIsvMin or more),
+// Blocked solves of a front with many eliminated columns or a large L (isv_blocked),
 // 64 columns per step: the per-node kernels above gather (forward) or load
 // (backward) the work vector xl, these do the elimination, k_isv_out writes
 // the eliminated variables. Forward: the same fnma chain per row, in column
