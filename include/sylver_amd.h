@@ -165,9 +165,16 @@ typedef struct {
    int indef_panel_min;  /* indefinite fronts with at least this many fully-summed columns
                             use the multi-workgroup APP panel (0 = auto: 256) */
    int small_assembly;   /* one-workgroup posdef fronts: 1 assembled (zero, A entries,
-                            children) inside their factor launch, one launch per level
-                            (default); 0 separate zero / scatter / extend-add launches */
-   int reserved[5];
+                            children) inside their factor launch, one launch per level;
+                            0 separate zero / scatter / extend-add launches (default:
+                            at 80^3 the large fronts' extend-add launches stay, and the
+                            fused form measured 0.65 ms slower in both precisions) */
+   int strip_trsm;       /* posdef large fronts, levels of at most 64 of them: 1 the panel
+                            TRSM by substitution in 64-row strips after a factor-only
+                            diagonal launch, the blocks' inverses (for the solves) formed
+                            by the factorization's last launch (default); 0 the TRSM by
+                            the inverse formed in the diagonal launch */
+   int reserved[4];
 } sylver_amd_options_t;
 
 void sylver_amd_default_options(sylver_amd_options_t *opt);
@@ -254,9 +261,10 @@ int sylver_amd_fkeep_indef_stats(void const *fkeep, int *out);
  * plan of the analysis with the given knobs (0 = defaults) and num_cus
  * compute units, and checks every index its launches use against the
  * buffers they address (fronts in the arena, inverse blocks, task lists,
- * child maps, exchange and partial-sum slots). out[8] = {violations, factor
+ * child maps, exchange and partial-sum slots). out[10] = {violations, factor
  * launches, forward-solve launches, backward-solve launches, arena
- * elements, exchange slots, levels, partial-sum slots}; why (why_len bytes,
+ * elements, exchange slots, levels, partial-sum slots, strip-TRSM
+ * workgroups, diagonal-block tasks}; why (why_len bytes,
  * may be NULL) receives the first violation. Returns 0 or an error flag. */
 int sylver_amd_plan_check(void const *akeep, int small_front_max, int bf16_min_ncol, int num_cus,
                           long *out, char *why, int why_len);

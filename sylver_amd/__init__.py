@@ -115,7 +115,8 @@ class ExtOptions(C.Structure):
         ("precision", C.c_int), ("bf16_min_ncol", C.c_int), ("refine", C.c_int),
         ("refine_tol", C.c_double), ("refine_max_iter", C.c_int), ("use_graph", C.c_int),
         ("tile_nb", C.c_int), ("small_front_max", C.c_int), ("record_timing", C.c_int),
-        ("indef_panel_min", C.c_int), ("small_assembly", C.c_int), ("reserved", C.c_int * 5),
+        ("indef_panel_min", C.c_int), ("small_assembly", C.c_int), ("strip_trsm", C.c_int),
+        ("reserved", C.c_int * 4),
     ]
 
 
@@ -648,12 +649,12 @@ def match_order(A):
 def plan_check(akeep, small_front_max=0, bf16_min_ncol=0, num_cus=256):
     """Host-side index check of the device launch plan (no device needed).
     Returns (stats dict, first violation text)."""
-    out = np.zeros(8, np.int64)
+    out = np.zeros(10, np.int64)
     why = C.create_string_buffer(256)
     rc = lib().sylver_amd_plan_check(akeep, int(small_front_max), int(bf16_min_ncol), int(num_cus),
                                      _p(out, C.c_long), why, 256)
     if rc != 0:
         raise RuntimeError(f"sylver_amd_plan_check failed: {rc} {why.value.decode()}")
     keys = ("violations", "factor_launches", "fwd_launches", "bwd_launches", "arena_elems",
-            "exchange_slots", "levels", "cb_slots")
+            "exchange_slots", "levels", "cb_slots", "strip_tasks", "diag_tasks")
     return dict(zip(keys, out.tolist())), why.value.decode()

@@ -64,6 +64,27 @@ __device__ __forceinline__ const P* live_or_zero(bool ok, const P* p) {
    return ok ? p : reinterpret_cast<const P*>(g_zero_pad);
 }
 
+// Raw buffer loads and stores: byte offset from `base`; a load at or beyond
+// `bytes` reads zero and a store there is dropped, which serves as the mask
+// (one 32-bit offset per lane instead of a 64-bit address per element).
+template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const T* base, uint32_t bytes) {
+   return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float*) {
+   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ inline double buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double*) {
+   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ inline void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, float v) {
+   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)off, 0, 0);
+}
+__device__ inline void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, double v) {
+   typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)off, 0, 0);
+}
+
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -392,6 +413,133 @@ __device__ int lds_potrf_blocked(T* L, int kb16, int* sh_bad) {
    return 0;
 }
 
+// Cholesky of a 16-column panel in LDS by one wave: lanes 0..15 hold the
+// rows of the diagonal block, lanes 16..63 48 rows below it (rows o + 16 +
+// 48 w + ...; w = the wave's index among the panel's waves), each row in
+// registers. The column steps are those of wave_potrf16 -- the rows below
+// the diagonal block are simply more rows of the same elimination -- so a
+// panel of up to 64 rows takes the time of the 16x16 block alone. Every wave
+// of a panel factors the diagonal block (the same values, the same
+// operations, so the same results); wave 0 stores it.
+template <typename T, int LD>
+__device__ int wave_panel16(T* L, int o, int kb16, int w) {
+   const int lane = threadIdx.x & 63;
+   const int row = lane < 16 ? o + lane : o + 16 + w * 48 + (lane - 16);
+   const bool live = row < kb16;
+   T r[16];
+#pragma unroll
+   for (int c = 0; c < 16; ++c) r[c] = (live && (lane >= 16 || c <= lane)) ? L[row + (o + c) * LD] : T(0);
+   int bad = 0;
+#pragma unroll
+   for (int j = 0; j < 16; ++j) {
+      T col[16];
+#pragma unroll
+      for (int c = j; c < 16; ++c) col[c] = lane_bcast(r[j], c);
+      const T d = col[j];
+      if (bad == 0 && !(d > T(0))) bad = j + 1;
+      if (bad == 0) {
+         const T inv = rsqrt_nr(d);
+         const T lj = r[j] * inv;
+         const T t = lj * inv;
+         r[j] = (lane == j) ? d * inv : lj;
+#pragma unroll
+         for (int c = j + 1; c < 16; ++c) r[c] = fma(-t, col[c], r[c]);
+      }
+   }
+   if (live) {
+      if (lane < 16) {
+         if (w == 0)
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+               if (c <= lane) L[row + (o + c) * LD] = r[c];
+      } else {
+#pragma unroll
+         for (int c = 0; c < 16; ++c) L[row + (o + c) * LD] = r[c];
+      }
+   }
+   return bad;
+}
+
+// Waves that factor the panel at column o of a kb16-row block (48 rows below
+// the diagonal block each).
+__device__ inline int panel_waves(int kb16, int o) {
+   const int below = kb16 - o - 16;
+   return below > 0 ? (below + 47) / 48 : 1;
+}
+
+// Row and column of lower 16x16 tile `tile` in packed row order.
+__device__ inline void packed_tile(int tile, int& ti, int& tj) {
+   ti = 0;
+   while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+   tj = tile - ti * (ti + 1) / 2;
+}
+
+// Lower 16x16 tiles (ti, tj) and (ti2, tj2) of the trailing block at t0 minus
+// the products of panel o: both products are formed before either is
+// applied, so their LDS and MFMA latencies overlap (the tiles are disjoint).
+template <typename T, int LD>
+__device__ void panel_update2(T* L, int o, int t0, int ti, int tj, int ti2, int tj2, bool two) {
+   typedef typename Mfma<T>::acc_t acc_t;
+   const int lane = threadIdx.x & 63;
+   acc_t acc = acc_t{0, 0, 0, 0}, acc2 = acc_t{0, 0, 0, 0};
+   acc = lds_mma16<T, true>(L + t0 + ti * 16 + o * LD, LD, L + t0 + tj * 16 + o * LD, LD, acc);
+   if (two) acc2 = lds_mma16<T, true>(L + t0 + ti2 * 16 + o * LD, LD, L + t0 + tj2 * 16 + o * LD, LD, acc2);
+#pragma unroll
+   for (int v = 0; v < 4; ++v) {
+      const int i = t0 + ti * 16 + Mfma<T>::row(lane, v), j = t0 + tj * 16 + (lane & 15);
+      const int i2 = t0 + ti2 * 16 + Mfma<T>::row(lane, v), j2 = t0 + tj2 * 16 + (lane & 15);
+      if (i >= j) L[i + j * LD] -= acc[v];
+      if (two && i2 >= j2) L[i2 + j2 * LD] -= acc2[v];
+   }
+}
+
+// Blocked Cholesky of the padded kb16 x kb16 block in LDS in 16-column
+// panels, each factored whole by wave_panel16 (up to 3 waves). Look-ahead:
+// after panel s the next panel's column of tiles is updated first; the waves
+// then factor panel s + 1 while the others (and they, once done) apply the
+// rest of panel s's update, handed out through an LDS counter. Two barriers
+// per panel. All threads call; returns 0 or the 1-based column of the first
+// non-positive pivot.
+template <typename T, int LD>
+__device__ int lds_potrf_panels(T* L, int kb16, int* sh_bad) {
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int nbk = kb16 / 16;
+   __shared__ int next_tile;
+   if (wave < panel_waves(kb16, 0)) {
+      const int bad = wave_panel16<T, LD>(L, 0, kb16, wave);
+      if (wave == 0 && lane == 0) *sh_bad = bad;
+   }
+   __syncthreads();
+   for (int s = 0; s + 1 < nbk; ++s) {
+      const int o = s * 16;
+      if (*sh_bad) return *sh_bad;
+      const int t0 = o + 16, nt = (kb16 - t0) / 16;
+      // the next panel's column of tiles
+      for (int ti = wave; ti < nt; ti += 4) panel_update2<T, LD>(L, o, t0, ti, 0, ti, 0, false);
+      if (threadIdx.x == 0) next_tile = 0;
+      __syncthreads();
+      if (wave < panel_waves(kb16, t0)) {
+         const int bad = wave_panel16<T, LD>(L, t0, kb16, wave);
+         if (wave == 0 && lane == 0) *sh_bad = bad ? t0 + bad : 0;
+      }
+      // the rest of panel s's update: the lower triangle at t0 + 16, two tiles at a time
+      const int nr = nt - 1, nrt = nr * (nr + 1) / 2, t1 = t0 + 16;
+      for (;;) {
+         int tile = 0;
+         if (lane == 0) tile = atomicAdd(&next_tile, 2);
+         tile = __shfl(tile, 0);
+         if (tile >= nrt) break;
+         int ti, tj;
+         packed_tile(tile, ti, tj);
+         // the next tile in the same packed order: next column, or the next row's first
+         const int ti2 = tj + 1 <= ti ? ti : ti + 1, tj2 = tj + 1 <= ti ? tj + 1 : 0;
+         panel_update2<T, LD>(L, o, t1, ti, tj, ti2, tj2, tile + 1 < nrt);
+      }
+      __syncthreads();
+   }
+   return *sh_bad;
+}
+
 // Blocked in-place inverse of the padded lower triangular kb16 x kb16 block
 // (block columns right to left; Tmp holds kNB x 16 elements, ld kNB).
 template <typename T, int LD>
@@ -428,28 +576,53 @@ __device__ void lds_trtri_blocked(T* L, T* Tmp, int kb16) {
 // Diagonal block of the large-front path: POTRF of the kb x kb block at
 // (k0, k0), L written back, its inverse written to dinv (ld = kb) for the
 // TRSM-by-GEMM and the solves. With do_potrf == 0 only the inverse of the
-// already factored block is formed (small fronts, for the solves).
+// already factored block is formed (small fronts and the strip-TRSM blocks,
+// for the solves); with NOINV only the factor (the strip TRSM below does not
+// use the inverse, which a later launch forms for the solves).
 // ---------------------------------------------------------------------------
+// The kb16 x kb16 diagonal block into LDS (ld kNB + 1): the lower triangle,
+// identity beyond kb, zero above. The block is rows row0 .. row0 + kb of the
+// panel buffer `rs` (ld elements per column, `bytes` long); 128 bytes of
+// loads in flight per thread.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ tasks,
-                                                   const NodeDesc* __restrict__ nodes,
-                                                   T* __restrict__ arena, T* __restrict__ dinv,
-                                                   T* __restrict__ dinvT, int* __restrict__ info,
-                                                   int do_potrf) {
+__device__ void load_diag_lds(__amdgpu_buffer_rsrc_t rs, uint32_t bytes, int row0, int ld, int kb, int kb16, T* L) {
    constexpr int LD = kNB + 1;
-   __shared__ T L[kNB * LD];
-   __shared__ T Tmp[kNB * 16];
-   __shared__ int sh_bad;
-   // the diagonal blocks are the factorization's critical path: their waves
-   // win instruction issue over the look-ahead updates sharing the CU
-   __builtin_amdgcn_s_setprio(3);
-   const DiagTask t = tasks[blockIdx.x];
-   const NodeDesc nd = nodes[t.node];
+   constexpr int NU = 128 / sizeof(T);
+   const bool full = kb16 == kNB;  // shifts for the full 128-wide block, the common case
+   for (int e0 = 0; e0 < kb16 * kb16; e0 += NU * kThreads) {
+      T v[NU];
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+         const int e = e0 + u * kThreads + threadIdx.x;
+         const int i = full ? (e & (kNB - 1)) : e % kb16, j = full ? (e >> 7) : e / kb16;
+         const bool ok = e < kb16 * kb16 && i < kb && j < kb && i >= j;
+         v[u] = buf_load(rs, ok ? (uint32_t)(row0 + i + j * ld) * (uint32_t)sizeof(T) : bytes, (T*)nullptr);
+      }
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+         const int e = e0 + u * kThreads + threadIdx.x;
+         const int i = full ? (e & (kNB - 1)) : e % kb16, j = full ? (e >> 7) : e / kb16;
+         if (e < kb16 * kb16) L[i + j * LD] = (i < kb && j < kb) ? v[u] : ((i == j) ? T(1) : T(0));
+      }
+   }
+}
+
+// Buffer over columns k0 .. k0 + kb of a front (the block's panel).
+template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t panel_rsrc(T* arena, const NodeDesc& nd, int k0, int kb, uint32_t& bytes) {
+   bytes = (uint32_t)((int64_t)kb * nd.ld * (int64_t)sizeof(T));
+   return buf_rsrc(arena + nd.off + (int64_t)k0 * nd.ld, bytes);
+}
+
+template <typename T, bool NOINV>
+__device__ void diag_block(const DiagTask& t, const NodeDesc& nd, T* __restrict__ arena, T* __restrict__ dinv,
+                           T* __restrict__ dinvT, int* __restrict__ info, int do_potrf, T* L, T* Tmp,
+                           int* sh_bad) {
+   constexpr int LD = kNB + 1;
    const int kb = t.kb;
    const int kb16 = (kb + 15) & ~15;
    T* blk = arena + nd.off + t.k0 + (int64_t)t.k0 * nd.ld;
-   // element e of the kb16 x kb16 block: row e % kb16, column e / kb16 (shifts
-   // for the full 128-wide block, the common case)
+   // element e of the kb16 x kb16 block: row e % kb16, column e / kb16
    const bool full = kb16 == kNB;
    auto split = [&](int e, int& i, int& j) {
       if (full) {
@@ -460,27 +633,12 @@ __global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ 
          j = e / kb16;
       }
    };
-   // 32 loads in flight per thread per batch
-   for (int e0 = 0; e0 < kb16 * kb16; e0 += 32 * kThreads) {
-      T v[32];
-#pragma unroll
-      for (int u = 0; u < 32; ++u) {
-         const int e = e0 + u * kThreads + threadIdx.x;
-         int i, j;
-         split(e, i, j);
-         v[u] = *live_or_zero(e < kb16 * kb16 && i < kb && j < kb && i >= j, blk + i + (int64_t)j * nd.ld);
-      }
-#pragma unroll
-      for (int u = 0; u < 32; ++u) {
-         const int e = e0 + u * kThreads + threadIdx.x;
-         int i, j;
-         split(e, i, j);
-         if (e < kb16 * kb16) L[i + j * LD] = (i < kb && j < kb) ? v[u] : ((i == j) ? T(1) : T(0));
-      }
-   }
+   uint32_t pbytes;
+   const auto rs = panel_rsrc(arena, nd, t.k0, kb, pbytes);
+   load_diag_lds<T>(rs, pbytes, t.k0, nd.ld, kb, kb16, L);
    __syncthreads();
    if (do_potrf) {
-      const int bad = lds_potrf_blocked<T, LD>(L, kb16, &sh_bad);
+      const int bad = lds_potrf_panels<T, LD>(L, kb16, sh_bad);
       if (bad) {
          if (threadIdx.x == 0) info[t.node] = t.k0 + bad;
          return;
@@ -504,6 +662,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ 
             if (e < kb16 * kb16 && i >= j && i < kb && j < kb) blk[i + (int64_t)j * nd.ld] = v[u];
          }
       }
+      if (NOINV) return;
    }
    lds_trtri_blocked<T, LD>(L, Tmp, kb16);
    T* out = dinv + nd.dinv_off + (int64_t)(t.k0 / kNB) * kNB * kNB;
@@ -535,6 +694,112 @@ __global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ 
    }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ tasks,
+                                                   const NodeDesc* __restrict__ nodes,
+                                                   T* __restrict__ arena, T* __restrict__ dinv,
+                                                   T* __restrict__ dinvT, int* __restrict__ info,
+                                                   int do_potrf) {
+   constexpr int LD = kNB + 1;
+   __shared__ T L[kNB * LD];
+   __shared__ T Tmp[kNB * 16];
+   __shared__ int sh_bad;
+   // the diagonal blocks are the factorization's critical path: their waves
+   // win instruction issue over the look-ahead updates sharing the CU
+   __builtin_amdgcn_s_setprio(3);
+   const DiagTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
+   diag_block<T, false>(t, nd, arena, dinv, dinvT, info, do_potrf, L, Tmp, &sh_bad);
+}
+
+// Factor only (the blocks whose panel the strip TRSM below solves).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_diag_factor(const DiagTask* __restrict__ tasks,
+                                                          const NodeDesc* __restrict__ nodes,
+                                                          T* __restrict__ arena, int* __restrict__ info) {
+   constexpr int LD = kNB + 1;
+   __shared__ T L[kNB * LD];
+   __shared__ int sh_bad;
+   __builtin_amdgcn_s_setprio(3);  // the factorization's critical path
+   const DiagTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
+   diag_block<T, true>(t, nd, arena, nullptr, nullptr, info, 1, L, nullptr, &sh_bad);
+}
+
+// ---------------------------------------------------------------------------
+// Panel TRSM of the large-front path by strips, after k_diag_factor: one
+// workgroup per 64 rows [r0, r0 + 64) below the diagonal block computes
+// X = A21 L11^{-T} by blocked substitution over 16-column blocks b,
+// X_b = (A_b - sum_{k<b} X_k L_bk^T) inv(L_bb)^T, with the 16x16 inverses
+// formed here from L11 (by wave_trtri16, as k_diag forms them). The chain
+// DIAG -> TRSM thus does not wait for the inverse of the whole block (formed
+// for the solves by the factorization's last launch).
+// Each wave owns 16 rows and keeps its 16x16 blocks in registers in the
+// layout of the MFMA output taken transposed: item v of lane l holds
+// M[l % 16][row(l, v)]. A product R = P Q (P in that layout, Q from LDS) is
+// the sum over s of mfma(Q[row(l, s)][l % 16], item s of P) -- the MFMA's
+// k index (s, l / 16) stands for row(l, s) -- and comes out in the same
+// layout, so the chain of products needs no shuffles. Loads and stores go
+// through a buffer over the block's kb panel columns (32-bit offsets).
+// ---------------------------------------------------------------------------
+struct StripTask { int node, k0, kb, r0; };
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_strip(const StripTask* __restrict__ tasks,
+                                                    const NodeDesc* __restrict__ nodes, T* __restrict__ arena) {
+   constexpr int LD = kNB + 1;
+   typedef typename Mfma<T>::acc_t acc_t;
+   __shared__ T L[kNB * LD];
+   __builtin_amdgcn_s_setprio(3);  // the factorization's critical path
+   const StripTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int kb = t.kb, kb16 = (kb + 15) & ~15, nbk = kb16 / 16;
+   const int ld = nd.ld;
+   uint32_t pbytes;
+   const auto rs = panel_rsrc(arena, nd, t.k0, kb, pbytes);
+   const int i = t.r0 + wave * 16 + (lane & 15);  // this lane's row
+   const bool rowok = i < nd.m;
+   auto off = [&](int c) {  // byte offset of (row i, panel column c), or past the panel
+      return (rowok && c < kb) ? (uint32_t)(i + c * ld) * (uint32_t)sizeof(T) : pbytes;
+   };
+   // A21 rows of the wave, 16-column blocks in the register layout
+   T x[8][4];
+#pragma unroll
+   for (int b = 0; b < 8; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) x[b][v] = buf_load(rs, off(b * 16 + Mfma<T>::row(lane, v)), (T*)nullptr);
+   // L11 (factored by the launch before) into LDS, the 16x16 diagonal inverses in place
+   load_diag_lds<T>(rs, pbytes, t.k0, ld, kb, kb16, L);
+   __syncthreads();
+   for (int b = wave; b < nbk; b += 4) wave_trtri16<T, LD>(L + b * 16 + b * 16 * LD);
+   __syncthreads();
+   const int r = lane & 15;
+#pragma unroll
+   for (int b = 0; b < 8; ++b) {
+      if (b < nbk) {
+         acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+         for (int k = 0; k < b; ++k)
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+               acc = Mfma<T>::op(L[(b * 16 + r) + (k * 16 + Mfma<T>::row(lane, s)) * LD], x[k][s], acc);
+         T tb[4];
+#pragma unroll
+         for (int v = 0; v < 4; ++v) tb[v] = x[b][v] - acc[v];
+         acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+         for (int s = 0; s < 4; ++s)
+            acc = Mfma<T>::op(L[(b * 16 + r) + (b * 16 + Mfma<T>::row(lane, s)) * LD], tb[s], acc);
+#pragma unroll
+         for (int v = 0; v < 4; ++v) x[b][v] = acc[v];
+      }
+   }
+#pragma unroll
+   for (int b = 0; b < 8; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) buf_store(rs, off(b * 16 + Mfma<T>::row(lane, v)), x[b][v]);
+}
 
 // ---------------------------------------------------------------------------
 // Schur-complement update of the large-front path: C -= A * B^T on one
@@ -1397,19 +1662,6 @@ constexpr int kChainMaxPolls = 1 << 20;
 constexpr unsigned long long kYEmpty = 0x7FF4DEADBEEF0000ull;
 constexpr int kChainThreads = 512;  // 8 waves: one 16-row tile of the block each
 
-// Raw buffer loads: byte offset from `base`; an offset at or beyond `bytes`
-// reads zero, which serves as the mask of the operand loads below (one
-// 32-bit offset per lane instead of a 64-bit address per element).
-template <typename T>
-__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const T* base, uint32_t bytes) {
-   return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), (short)0, (int)bytes, 0x00020000);
-}
-__device__ inline float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float*) {
-   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
-}
-__device__ inline double buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double*) {
-   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
-}
 
 template <typename T, bool BWD>
 __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __restrict__ tasks,

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -43,7 +44,8 @@ sylver_amd_options_t& ext_options() {
       d.refine_tol = 1e-15;
       d.refine_max_iter = 30;
       d.use_graph = 1;
-      d.small_assembly = 1;
+      d.small_assembly = 0;
+      d.strip_trsm = 1;
       return d;
    }();
    return o;
@@ -66,9 +68,15 @@ void device_sync() {
 
 namespace {
 
+// Block columns of at most this many large fronts (one level's) take the
+// strip TRSM; beyond it (batched fronts) each front's chain is short next to
+// the launch's width.
+constexpr int kStepMaxFronts = 64;
+
 // The plan's host side: layout, maps, task lists and launch programs. No
 // HIP call (it also runs, for its checks, on a host without a device).
-DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int num_cus, bool fuse_small) {
+DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int num_cus, bool fuse_small,
+                            bool strip_trsm) {
    std::unique_ptr<DevicePlan> P(new DevicePlan());
    P->n = a.n;
    P->nnodes = a.nnodes;
@@ -77,6 +85,7 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
    // fronts at fixed caller offsets (the batched entry points) hold their
    // values already: nothing to assemble
    P->fuse_small = fuse_small && a.fixed_off.empty();
+   P->strip_trsm = strip_trsm;
    P->variant_gen = a.variant_gen;
    const int nn = a.nnodes;
    // one-workgroup kernel or tiled path for front i
@@ -160,6 +169,7 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
    for (int i = 0; i < nn; ++i) P->hnodes[i].cmap_off = P->hcmap_off[i];
 
    // ---- factor program ----
+   std::vector<DiagTask> step_dinv;  // diagonal blocks factored by L_DFACT (inverses formed at the end)
    auto push = [](std::vector<Launch>& prog, int kind, int64_t off, int64_t count) {
       if (count > 0) {
          Launch l;
@@ -233,34 +243,67 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
       int maxn = 0;
       for (int i : large) maxn = std::max(maxn, P->hnodes[i].n);
       for (int k0 = 0; k0 < maxn; k0 += kNB) {
-         int64_t d0 = P->diag.size();
+         const bool first_of_pair = ((k0 / kNB) % 2) == 0;
+         int nfr = 0;
+         int64_t nwg = 0;
          for (int i : large) {
             const NodeDesc& d = P->hnodes[i];
             if (k0 >= d.n) continue;
-            P->diag.push_back(DiagTask{i, k0, std::min(kNB, d.n - k0), 0});
+            ++nfr;
+            nwg += 1 + (d.m - k0 - std::min(kNB, d.n - k0) + 63) / 64;
          }
-         push(P->fprog, L_DIAG, d0, P->diag.size() - d0);
-         int64_t t0 = P->gemm.size();
-         for (int i : large) {
-            const NodeDesc& d = P->hnodes[i];
-            if (k0 >= d.n) continue;
-            int kb = std::min(kNB, d.n - k0);
-            for (int r = k0 + kb; r < d.m; r += 64) {  // 64-row tiles (k_update<T, 1, 64, 128>)
-               GemmTask g{};
-               g.a_off = d.off + r + (int64_t)k0 * d.ld;
-               g.b_off = d.dinv_off + (int64_t)(k0 / kNB) * kNB * kNB;
-               g.c_off = g.a_off;
-               g.lda = d.ld;
-               g.ldb = kb;
-               g.ldc = d.ld;
-               g.M = std::min(64, d.m - r);
-               g.N = kb;
-               g.K = kb;
-               P->gemm.push_back(g);
+         // a latency-bound chain (few fronts, the strips resident on the
+         // device at once): the diagonal blocks factored alone, then the panel
+         // TRSM by substitution in strips; else the TRSM by the inverse
+         const bool step = P->strip_trsm && nfr <= kStepMaxFronts && nwg - nfr <= std::max(num_cus, 1);
+         if (step) {
+            int64_t d0 = P->diag.size();
+            for (int i : large) {
+               const NodeDesc& d = P->hnodes[i];
+               if (k0 >= d.n) continue;
+               const DiagTask t{i, k0, std::min(kNB, d.n - k0), 0};
+               P->diag.push_back(t);
+               step_dinv.push_back(t);  // its inverse: with the small fronts' at the end
             }
+            push(P->fprog, L_DFACT, d0, P->diag.size() - d0);
+            int64_t s0 = P->strip.size();
+            for (int i : large) {
+               const NodeDesc& d = P->hnodes[i];
+               if (k0 >= d.n) continue;
+               const int kb = std::min(kNB, d.n - k0);
+               for (int r = k0 + kb; r < d.m; r += 64) P->strip.push_back(StripTask{i, k0, kb, r});
+            }
+            push(P->fprog, L_STRIP, s0, P->strip.size() - s0);
+         } else {
+            int64_t d0 = P->diag.size();
+            for (int i : large) {
+               const NodeDesc& d = P->hnodes[i];
+               if (k0 >= d.n) continue;
+               P->diag.push_back(DiagTask{i, k0, std::min(kNB, d.n - k0), 0});
+            }
+            push(P->fprog, L_DIAG, d0, P->diag.size() - d0);
+            int64_t t0 = P->gemm.size();
+            for (int i : large) {
+               const NodeDesc& d = P->hnodes[i];
+               if (k0 >= d.n) continue;
+               int kb = std::min(kNB, d.n - k0);
+               for (int r = k0 + kb; r < d.m; r += 64) {  // 64-row tiles (k_update<T, 1, 64, 128>)
+                  GemmTask g{};
+                  g.a_off = d.off + r + (int64_t)k0 * d.ld;
+                  g.b_off = d.dinv_off + (int64_t)(k0 / kNB) * kNB * kNB;
+                  g.c_off = g.a_off;
+                  g.lda = d.ld;
+                  g.ldb = kb;
+                  g.ldc = d.ld;
+                  g.M = std::min(64, d.m - r);
+                  g.N = kb;
+                  g.K = kb;
+                  P->gemm.push_back(g);
+               }
+            }
+            while ((P->gemm.size() - t0) % 8 != 0 && P->gemm.size() > (size_t)t0) P->gemm.push_back(GemmTask{});
+            push(P->fprog, L_TRSM, t0, P->gemm.size() - t0);
          }
-         while ((P->gemm.size() - t0) % 8 != 0 && P->gemm.size() > (size_t)t0) P->gemm.push_back(GemmTask{});
-         push(P->fprog, L_TRSM, t0, P->gemm.size() - t0);
          // Schur update. Block columns are paired: after the first of a pair
          // only the next panel's columns are updated (narrow update); after the
          // second, the trailing block receives both panels at once (K = 256),
@@ -269,7 +312,6 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
          // next diagonal factorization) on the factor stream, the columns
          // beyond them on the side stream, where they overlap the next pair's
          // diagonal/TRSM chain.
-         const bool first_of_pair = ((k0 / kNB) % 2) == 0;
          // lower-triangular TS x TS tiles of columns [c0, c1) (rows from c0 down)
          auto tiles = [&](const NodeDesc& d, int kstart, int K, int c0, int c1, int TS) {
             const int c0a = c0 & ~7;
@@ -366,9 +408,11 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
          side_last = -1;
       }
    }
-   // inverted diagonal blocks of the small fronts (the large path writes its own)
+   // inverted diagonal blocks of the small fronts and of the fused block
+   // steps (the separate DIAG launches write their own)
    {
       int64_t d0 = P->diag.size();
+      P->diag.insert(P->diag.end(), step_dinv.begin(), step_dinv.end());
       for (int L = 0; L < nlev; ++L)
          for (int i : P->levels[L]) {
             const NodeDesc& d = P->hnodes[i];
@@ -429,6 +473,20 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
                const double kb = P->diag[t].kb;
                L.flops += (L.kind == L_DIAG ? 2.0 : 1.0) * kb * kb * kb / 3.0;
                L.words += 2.0 * kb * kb;
+               break;
+            }
+            case L_DFACT: {
+               const double kb = P->diag[t].kb;
+               L.flops += kb * kb * kb / 3.0;
+               L.words += 1.5 * kb * kb;
+               break;
+            }
+            case L_STRIP: {
+               const StripTask& c = P->strip[t];
+               const double kb = c.kb;
+               const double rows = std::min(64, P->hnodes[c.node].m - c.r0);
+               L.flops += rows * kb * kb;
+               L.words += 2.0 * rows * kb + 0.5 * kb * kb;
                break;
             }
             case L_TRSM: {
@@ -764,6 +822,7 @@ void upload_plan(DevicePlan* P, const Analysis& a) {
    P->d_cmap_off = dupload(P->hcmap_off);
    P->d_nodes = dupload(P->hnodes);
    P->d_zero = dupload(P->zero);
+   P->d_strip = dupload(P->strip);
    P->d_init = dupload(P->init);
    P->d_ea = dupload(P->ea);
    P->d_nlistv = dupload(P->nlistv);
@@ -825,7 +884,8 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
       if (!in_arena(g.c_off, g.M, g.N, g.ldc)) fail("gemm: C outside the arena");
    };
    auto check_launches = [&](const std::vector<Launch>& prog, bool factor) {
-      for (const Launch& L : prog) {
+      for (size_t li = 0; li < prog.size(); ++li) {
+         const Launch& L = prog[li];
          if (L.wait_ev >= P->nevents || L.rec_ev >= P->nevents) fail("launch: event index");
          if (L.count < 0 || L.off < 0) fail("launch: range");
          const int64_t e = L.off + L.count;
@@ -890,6 +950,7 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
                   }
                break;
             case L_DIAG:
+            case L_DFACT:
             case L_DINV:
                if (e > (int64_t)P->diag.size()) fail("diag: range");
                else
@@ -899,6 +960,30 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
                          c.k0 + c.kb > P->hnodes[c.node].n)
                         fail("diag task");
                   }
+               break;
+            case L_STRIP:
+               if (e > (int64_t)P->strip.size()) {
+                  fail("strip: range");
+               } else {
+                  // the strip's diagonal block is factored by the launch before
+                  const Launch* prev = li > 0 ? &prog[li - 1] : nullptr;
+                  for (int64_t t = L.off; t < e; ++t) {
+                     const StripTask& c = P->strip[t];
+                     if (c.node < 0 || c.node >= nn) {
+                        fail("strip task: node");
+                        continue;
+                     }
+                     const NodeDesc& d = P->hnodes[c.node];
+                     if (c.k0 < 0 || c.kb < 1 || c.kb > kNB || c.k0 + c.kb > d.n) fail("strip task: block");
+                     if (c.r0 < c.k0 + c.kb || c.r0 >= d.m) fail("strip task: rows");
+                     if ((int64_t)c.kb * d.ld * 8 > 0xFFFFFFFFll) fail("strip task: panel over 4 GiB");
+                     bool dfact = false;
+                     if (prev && prev->kind == L_DFACT && prev->off + prev->count <= (int64_t)P->diag.size())
+                        for (int64_t q = prev->off; q < prev->off + prev->count; ++q)
+                           dfact |= P->diag[q].node == c.node && P->diag[q].k0 == c.k0 && P->diag[q].kb == c.kb;
+                     if (!dfact) fail("strip task: its diagonal block is not factored by the launch before");
+                  }
+               }
                break;
             case L_TRSM:
             case L_UPDATE:
@@ -978,11 +1063,11 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
    return bad;
 }
 
-DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol, bool fuse_small) {
+DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol, bool fuse_small, bool strip_trsm) {
    int dev = 0, num_cus = 0;
    HIPCHECK(hipGetDevice(&dev));
    HIPCHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
-   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, fuse_small));
+   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, fuse_small, strip_trsm));
    upload_plan(P.get(), a);
    return P.release();
 }
@@ -1047,6 +1132,12 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
             break;
          case L_DINV:
             k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, dinvT, F->d_info, 0);
+            break;
+         case L_DFACT:
+            k_diag_factor<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, F->d_info);
+            break;
+         case L_STRIP:
+            k_strip<T><<<g, b, 0, s>>>(P->d_strip + L.off, P->d_nodes, arena);
             break;
          default:  // L_SYNC
             break;
@@ -1209,7 +1300,8 @@ void equilibrate(const DevicePlan* P, DeviceFactors* F, hipStream_t s, int max_i
 }  // namespace
 
 int plan_check(const Analysis& a, int small_max, int wide_ncol, int num_cus, int64_t* out, std::string* why) {
-   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, ext_options().small_assembly != 0));
+   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, ext_options().small_assembly != 0,
+                                                 ext_options().strip_trsm != 0));
    const int64_t bad = check_plan_indices(P.get(), a, why);
    if (out) {
       out[0] = bad;
@@ -1220,6 +1312,8 @@ int plan_check(const Analysis& a, int small_max, int wide_ncol, int num_cus, int
       out[5] = P->chain_flags;
       out[6] = P->nlevels;
       out[7] = P->cb_slots;
+      out[8] = (int64_t)P->strip.size();
+      out[9] = (int64_t)P->diag.size();
    }
    return 0;
 }
@@ -1243,12 +1337,12 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
       int wide_ncol = xo.bf16_min_ncol > 0 ? xo.bf16_min_ncol : 1024;
       // a new plan for new knobs or variants; factorizations built on the old
       // one keep it alive through their own reference
-      const bool fuse = xo.small_assembly != 0;
+      const bool fuse = xo.small_assembly != 0, step = xo.strip_trsm != 0;
       if (*plan && ((*plan)->small_max != small_max || (*plan)->wide_ncol != wide_ncol ||
-                    (*plan)->fuse_small != fuse ||
+                    (*plan)->fuse_small != fuse || (*plan)->strip_trsm != step ||
                     (*plan)->variant_gen != a.variant_gen))
          plan->reset();
-      if (!*plan) plan->reset(build_plan(a, small_max, wide_ncol, fuse));
+      if (!*plan) plan->reset(build_plan(a, small_max, wide_ncol, fuse, step));
       DevicePlan* P = plan->get();
       int prec = xo.precision;
       if (prec != SYLVER_AMD_PREC_FP64 && prec != SYLVER_AMD_PREC_FP32 && prec != SYLVER_AMD_PREC_MIXED)
@@ -1370,7 +1464,10 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
             HIPCHECK(hipEventElapsedTime(&t, events[2 * i], events[2 * i + 1]));
             const Launch& L = P->fprog[i];
             // the assembled one-workgroup fronts count with the one-workgroup kind
-            const int kind = L.kind == L_SMALL_ASM ? (int)L_SMALL : L.kind;
+            const int kind = L.kind == L_SMALL_ASM ? (int)L_SMALL
+                             : L.kind == L_DFACT ? (int)L_DIAG
+                             : L.kind == L_STRIP ? (int)L_TRSM
+                                                 : L.kind;
             if (kind >= 16) continue;  // L_SYNC: no kernel
             F->kind_ms[kind] += t;
             F->kind_flops[kind] += L.flops;
@@ -1938,7 +2035,7 @@ int time_fronts(int K, const int* m, const int* n, const signed char* variant, i
          for (int r = 1; r <= m[i]; ++r) a.rlist.push_back(r);
       }
       if (variant) a.variant.assign(variant, variant + K);
-      P.reset(build_plan(a, 256, 1 << 30, false));
+      P.reset(build_plan(a, 256, 1 << 30, false, false));
       const bool low = precision != SYLVER_AMD_PREC_FP64;
       const size_t es = low ? sizeof(float) : sizeof(double);
       F = new DeviceFactors();
@@ -2007,7 +2104,7 @@ int batched_front_potrf_v(int nfronts, const int* nrow, const int* ncol, const i
          for (int r = 1; r <= nrow[i]; ++r) a.rlist.push_back(r);
       }
       if (variant) a.variant.assign(variant, variant + nfronts);
-      P.reset(build_plan(a, 256, 1 << 30, false));
+      P.reset(build_plan(a, 256, 1 << 30, false, false));
       const bool low = dtype != 0;
       const size_t es = low ? sizeof(float) : sizeof(double);
       F = new DeviceFactors();

@@ -69,6 +69,8 @@ enum LaunchKind {
    L_FCB,     // forward contribution blocks of the chained fronts (k_chain, no waits)
    L_CLEAR,   // exchange-buffer fill at the start of a chained sweep
    L_SMALL_ASM,  // one-workgroup fronts assembled (zero, A, children) and factored in one launch
+   L_DFACT,  // diagonal blocks factored without their inverse (k_diag_factor), before L_STRIP
+   L_STRIP,  // panel TRSM by 64-row strips, by substitution (k_strip)
    L_SYNC  // no kernel: only the stream wait / event record of the entry
 };
 
@@ -107,6 +109,7 @@ struct DevicePlan {
    int n = 0, nnodes = 0, nlevels = 0;
    int small_max = 256;
    bool fuse_small = false;  // one-workgroup fronts assembled inside their factor launch (L_SMALL_ASM)
+   bool strip_trsm = false;  // large-front blocks of few fronts: L_DFACT + L_STRIP (no inverse on the chain)
    int wide_ncol = 1024;  // fronts with more pivots take the wide-front update launches
    std::vector<NodeDesc> hnodes;
    std::vector<std::vector<int>> levels;
@@ -129,6 +132,7 @@ struct DevicePlan {
    std::vector<EaTask> ea;
    std::vector<int> nlistv;  // node lists (small fronts, gathers)
    std::vector<DiagTask> diag;
+   std::vector<StripTask> strip;
    std::vector<GemmTask> gemm;
    std::vector<SolveTask> stask;
    std::vector<int4> gch;  // forward gather: {child, i0, i1} ranges per task
@@ -140,6 +144,7 @@ struct DevicePlan {
    EaTask* d_ea = nullptr;
    int* d_nlistv = nullptr;
    DiagTask* d_diag = nullptr;
+   StripTask* d_strip = nullptr;
    GemmTask* d_gemm = nullptr;
    SolveTask* d_stask = nullptr;
    int4* d_gch = nullptr;
@@ -165,7 +170,7 @@ struct DevicePlan {
       for (void* p : {(void*)d_sptr, (void*)d_sparent, (void*)d_rptr, (void*)d_rlist,
                       (void*)d_nptr, (void*)d_nlist, (void*)d_cmap, (void*)d_cmap_off,
                       (void*)d_nodes, (void*)d_zero, (void*)d_init, (void*)d_ea,
-                      (void*)d_nlistv, (void*)d_diag, (void*)d_gemm, (void*)d_stask, (void*)d_gch, (void*)d_ctask, (void*)d_cgch,
+                      (void*)d_nlistv, (void*)d_diag, (void*)d_strip, (void*)d_gemm, (void*)d_stask, (void*)d_gch, (void*)d_ctask, (void*)d_cgch,
                       (void*)d_csr_ptr, (void*)d_csr_col, (void*)d_csr_src, (void*)d_invp})
          dfree(p);
    }

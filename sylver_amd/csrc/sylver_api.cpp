@@ -379,7 +379,8 @@ void sylver_amd_default_options(sylver_amd_options_t* o) {
    o->tile_nb = 0;
    o->small_front_max = 0;
    o->indef_panel_min = 0;
-   o->small_assembly = 1;
+   o->small_assembly = 0;
+   o->strip_trsm = 1;
    o->record_timing = 0;
 }
 
@@ -589,10 +590,10 @@ int sylver_amd_plan_check(void const* akeep, int small_front_max, int bf16_min_n
    if (!ak || !out || num_cus < 1) return SYLVER_ERROR_CALL_SEQUENCE;
    try {
       std::string msg;
-      int64_t o[8];
+      int64_t o[10];
       plan_check(ak->a, small_front_max > 0 ? small_front_max : 256, bf16_min_ncol > 0 ? bf16_min_ncol : 1024,
                  num_cus, o, &msg);
-      for (int k = 0; k < 8; ++k) out[k] = (long)o[k];
+      for (int k = 0; k < 10; ++k) out[k] = (long)o[k];
       if (why && why_len > 0) {
          std::strncpy(why, msg.c_str(), (size_t)why_len - 1);
          why[why_len - 1] = 0;

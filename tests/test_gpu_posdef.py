@@ -360,4 +360,74 @@ def test_small_front_assembly_fused_bitwise(S, gpu, prec):
             for a, b in zip(*got):
                 assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
     finally:
-        S.set_ext_options(precision=S.PREC_FP64, small_assembly=1)
+        S.set_ext_options(precision=S.PREC_FP64, small_assembly=0)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_strip_trsm_vs_separate_launches(S, O, gpu, prec):
+    """The panel TRSM by substitution in 64-row strips (after a factor-only
+    diagonal launch; inverses formed at the end) against the TRSM by the
+    inverse: both within the oracle's bar, and within rounding of each other
+    (the TRSM's summation differs)."""
+    A = M.laplacian_3d(20)
+    pr = S.PREC_FP64 if prec == "fp64" else S.PREC_FP32
+    akeep, opts, sym = _analyse(S, A)
+    st, why = S.plan_check(akeep, small_front_max=48)
+    assert st["violations"] == 0 and st["strip_tasks"] > 0, why
+    b = M.random_rhs(A.n, 3, seed=4)
+    got = {}
+    try:
+        for strip in (1, 0):
+            S.set_ext_options(precision=pr, small_front_max=48, strip_trsm=strip, refine=S.REFINE_NONE)
+            fk, inf = _factor(S, A, akeep, opts)
+            assert inf.flag == 0
+            if prec == "fp64":
+                _compare_factors(S, O, sym, fk, A, FP64_L_TOL)
+            got[strip] = [np.tril(S.fkeep_node_factor(fk, s, int(sym["rptr"][s + 1] - sym["rptr"][s]),
+                                                      int(sym["sptr"][s + 1] - sym["sptr"][s])))
+                          for s in range(sym["nnodes"])]
+            x = b.copy(order="F")
+            assert S.spldlt_solve(x, akeep, fk, options=opts).flag == 0
+            err = max(O.bwd_err(A, x[:, c], b[:, c]) for c in range(3))
+            assert err <= (BWD_TOL if prec == "fp64" else 1e-6), err
+            S.spldlt_free_fkeep(fk)
+        tol = 1e-13 if prec == "fp64" else 2e-5
+        for a, c in zip(got[1], got[0]):
+            scale = max(1.0, np.abs(c).max())
+            assert np.abs(a - c).max() <= tol * scale
+    finally:
+        S.set_ext_options(precision=S.PREC_FP64, small_front_max=0, strip_trsm=1, refine=S.REFINE_AUTO)
+        S.spldlt_free_akeep(akeep)
+
+
+@pytest.mark.parametrize("where", ["first_block", "mid_root"])
+def test_strip_trsm_not_posdef_then_recovers(S, O, gpu, where):
+    """A non-positive pivot in a large front on the strip-TRSM path: the
+    factorization reports NOT_POS_DEF, and the next factorization of an SPD
+    matrix on the same analysis is exact."""
+    A = M.laplacian_3d(20)
+    akeep, opts, sym = _analyse(S, A)
+    root = int(np.argmax(np.diff(sym["sptr"])))  # the widest front (the root separator)
+    p0, p1 = int(sym["sptr"][root]) - 1, int(sym["sptr"][root + 1]) - 1
+    assert p1 - p0 > 256
+    # a failure in block column 0 or 2 of the 480-column root
+    pivot = p0 + 5 if where == "first_block" else p0 + 300
+    var = int(sym["invp"][pivot])  # 1-based variable at that pivot
+    ks = [k for k in range(A.ptr[var - 1] - 1, A.ptr[var] - 1) if A.row[k] == var]
+    assert len(ks) == 1
+    bad = A.val.copy()
+    bad[ks[0]] = -100.0
+    S.set_ext_options(small_front_max=48, strip_trsm=1)
+    try:
+        fk, inf = S.spldlt_factorize(True, bad, akeep, options=opts)
+        assert inf.flag == S.ERROR_NOT_POS_DEF
+        fk, inf = S.spldlt_factorize(True, A.val, akeep, fk, options=opts)
+        assert inf.flag == 0
+        b = M.random_rhs(A.n, 1, seed=2)
+        x = b.copy(order="F")
+        assert S.spldlt_solve(x, akeep, fk, options=opts).flag == 0
+        assert O.bwd_err(A, x[:, 0], b[:, 0]) <= BWD_TOL
+        S.spldlt_free_fkeep(fk)
+    finally:
+        S.set_ext_options(small_front_max=0, strip_trsm=1)
+        S.spldlt_free_akeep(akeep)

@@ -8,7 +8,7 @@ addresses. Covered: the configurations of the parity tests and the
 benchmark's 80^3 headline case, with the knobs that change the plan
 (one-workgroup cutoff, per-front variants, wide-front threshold, the
 compute-unit count that decides which contribution blocks ride in the
-chained solve launches).
+chained solve launches, the strip TRSM on and off).
 """
 import numpy as np
 import pytest
@@ -35,12 +35,19 @@ CASES = {
 
 @pytest.mark.parametrize("case", sorted(CASES))
 @pytest.mark.parametrize("small_max,bf16,cus", [(0, 0, 256), (48, 64, 256), (48, 0, 1), (1024, 0, 8)])
-def test_plan_indices_in_bounds(S, case, small_max, bf16, cus):
+@pytest.mark.parametrize("strip_trsm", [1, 0])
+def test_plan_indices_in_bounds(S, case, small_max, bf16, cus, strip_trsm):
     A = CASES[case]()
     akeep = _akeep(S, A)
-    st, why = S.plan_check(akeep, small_front_max=small_max, bf16_min_ncol=bf16, num_cus=cus)
+    S.set_ext_options(strip_trsm=strip_trsm)
+    try:
+        st, why = S.plan_check(akeep, small_front_max=small_max, bf16_min_ncol=bf16, num_cus=cus)
+    finally:
+        S.set_ext_options(strip_trsm=1)
     assert st["violations"] == 0, why
     assert st["factor_launches"] > 0
+    if not strip_trsm:
+        assert st["strip_tasks"] == 0
     S.spldlt_free_akeep(akeep)
 
 
@@ -63,4 +70,7 @@ def test_plan_indices_headline_80cubed(S):
     st, why = S.plan_check(akeep)
     assert st["violations"] == 0, why
     assert st["arena_elems"] > 1_000_000_000
+    # the top levels' large fronts take the strip TRSM (64-row strips below
+    # each diagonal block)
+    assert st["strip_tasks"] > 5_000
     S.spldlt_free_akeep(akeep)
