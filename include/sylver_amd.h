@@ -174,7 +174,13 @@ typedef struct {
                             diagonal launch, the blocks' inverses (for the solves) formed
                             by the factorization's last launch (default); 0 the TRSM by
                             the inverse formed in the diagonal launch */
-   int reserved[4];
+   int level_assembly;   /* posdef fronts not assembled inside their factor launch: 1 a
+                            level's assembly in one launch, each workgroup owning a run of
+                            one front's columns (zero, original entries, every child's
+                            columns landing there, children in list order -- bitwise the
+                            separate launches) (default); 0 separate zero / scatter
+                            launches and one extend-add launch per child rank */
+   int reserved[3];
 } sylver_amd_options_t;
 
 void sylver_amd_default_options(sylver_amd_options_t *opt);

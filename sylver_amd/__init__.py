@@ -116,7 +116,7 @@ class ExtOptions(C.Structure):
         ("refine_tol", C.c_double), ("refine_max_iter", C.c_int), ("use_graph", C.c_int),
         ("tile_nb", C.c_int), ("small_front_max", C.c_int), ("record_timing", C.c_int),
         ("indef_panel_min", C.c_int), ("small_assembly", C.c_int), ("strip_trsm", C.c_int),
-        ("reserved", C.c_int * 4),
+        ("level_assembly", C.c_int), ("reserved", C.c_int * 3),
     ]
 
 

@@ -42,6 +42,7 @@ struct NodeDesc {
 struct ColTask { int node, c0, c1, pad; };
 struct InitTask { int node, pad; int64_t p0, p1; };
 struct EaTask { int child, j0, j1, pad; };
+struct AsmTask { int node, c0, c1, pad; };
 struct DiagTask { int node, k0, kb, pad; };
 struct GemmTask {
    int64_t a_off, b_off, c_off;
@@ -214,6 +215,106 @@ __global__ __launch_bounds__(kThreads) void k_extend_add(const EaTask* __restric
          for (int u = 0; u < 8; ++u)
             if (i0 + u * 64 + lane < q) dst[ci[u]] = dv[u] + sv[u];
       }
+   }
+}
+
+// ---------------------------------------------------------------------------
+// Assembly of a level in one launch, by the receiving columns: each task
+// owns columns [c0, c1) of one front and writes nothing else. It zeroes
+// their lower triangle, writes the original entries that fall in them
+// (asort: the front's A-map sorted by position, (col)*nrow + row), then adds
+// the contribution-block columns of every child that land in them, children
+// in list order. Per entry these are the operations, in the order, of
+// k_zero, k_init_a and one k_extend_add launch per child rank, so the front
+// is bitwise the same; the barriers between children order their sums
+// (a child's map is injective, two children may meet on an entry). Child
+// maps are increasing (posdef: no delayed rows), so the child columns of a
+// task are a range, found by bisection.
+// ---------------------------------------------------------------------------
+// first index in the increasing a[0..n) with a[i] >= v
+__device__ inline int lower_bound_i(const int* a, int n, int v) {
+   int lo = 0, hi = n;
+   while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (a[mid] < v)
+         lo = mid + 1;
+      else
+         hi = mid;
+   }
+   return lo;
+}
+// first pair p in [lo, hi) of pos (1-based positions, every second entry)
+// with pos >= v
+__device__ inline int64_t lower_bound_pos(const int64_t* pos, int64_t lo, int64_t hi, int64_t v) {
+   while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (pos[2 * mid + 1] < v)
+         lo = mid + 1;
+      else
+         hi = mid;
+   }
+   return lo;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_assemble(const AsmTask* __restrict__ tasks,
+                                                       const NodeDesc* __restrict__ nodes,
+                                                       const int64_t* __restrict__ asort,
+                                                       const int* __restrict__ rlist,
+                                                       const double* __restrict__ aval,
+                                                       const double* __restrict__ scaling,
+                                                       const int* __restrict__ cmap, T* __restrict__ arena) {
+   const AsmTask t = tasks[blockIdx.x];
+   const NodeDesc nd = nodes[t.node];
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   T* F = arena + nd.off;
+   for (int c = t.c0 + wave; c < t.c1; c += 4) {
+      T* col = F + (int64_t)c * nd.ld;
+      for (int r = c + lane; r < nd.m; r += 64) col[r] = T(0);
+   }
+   __syncthreads();
+   if (t.c0 < nd.n) {  // original entries lie in the pivot columns only
+      const int64_t* am = asort + 2 * nd.amap_off;
+      const int64_t p0 = lower_bound_pos(am, 0, nd.num_a, (int64_t)t.c0 * nd.m + 1);
+      const int64_t p1 = lower_bound_pos(am, p0, nd.num_a, (int64_t)t.c1 * nd.m + 1);
+      for (int64_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
+         const int64_t src = am[2 * p] - 1;
+         const int64_t dest = am[2 * p + 1] - 1;
+         const int c = (int)(dest / nd.m);
+         const int r = (int)(dest - (int64_t)c * nd.m);
+         double v = aval[src];
+         if (scaling) v = scaling[rlist[nd.rlist_off + r] - 1] * v * scaling[rlist[nd.rlist_off + c] - 1];
+         F[(int64_t)c * nd.ld + r] = (T)v;
+      }
+   }
+   for (int ch = nd.first_child; ch != -1;) {
+      const NodeDesc cn = nodes[ch];
+      __syncthreads();
+      const int q = cn.m - cn.n;
+      const int* __restrict__ cm = cmap + cn.cmap_off;
+      const int ja = lower_bound_i(cm, q, t.c0), jb = lower_bound_i(cm, q, t.c1);
+      for (int j = ja + wave; j < jb; j += 4) {
+         const int pj = cm[j];
+         T* dst = F + (int64_t)pj * nd.ld;
+         const T* src = arena + cn.off + (int64_t)(cn.n + j) * cn.ld + cn.n;
+         // 8 rows per lane per batch: all loads issued before the stores
+         for (int i0 = j; i0 < q; i0 += 8 * 64) {
+            int ci[8];
+            T sv[8], dv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+               const int i = min(i0 + u * 64 + lane, q - 1);
+               ci[u] = cm[i];
+               sv[u] = src[i];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) dv[u] = dst[ci[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+               if (i0 + u * 64 + lane < q) dst[ci[u]] = dv[u] + sv[u];
+         }
+      }
+      ch = cn.next_child;
    }
 }
 
@@ -1324,17 +1425,6 @@ __global__ __launch_bounds__(kThreads) void k_small_front(const int* __restrict_
 // L[k, j]^T x_k. A task covers rows [r0, r1) of one node (or columns, for the
 // transposed products).
 // ---------------------------------------------------------------------------
-__device__ inline int lower_bound_i(const int* a, int n, int v) {
-   int lo = 0, hi = n;
-   while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (a[mid] < v)
-         lo = mid + 1;
-      else
-         hi = mid;
-   }
-   return lo;
-}
 
 // Forward gather of rows [r0, r1) (at most 256: one row per thread): own
 // rows from b, the others zero, then the contribution rows of the children

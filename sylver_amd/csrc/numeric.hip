@@ -46,9 +46,16 @@ sylver_amd_options_t& ext_options() {
       d.use_graph = 1;
       d.small_assembly = 0;
       d.strip_trsm = 1;
+      d.level_assembly = 1;
       return d;
    }();
    return o;
+}
+
+// Plan-shaping switches of the extension options.
+PlanKnobs plan_knobs() {
+   const sylver_amd_options_t& o = ext_options();
+   return PlanKnobs{o.small_assembly != 0, o.strip_trsm != 0, o.level_assembly != 0};
 }
 
 int device_count() {
@@ -75,8 +82,7 @@ constexpr int kStepMaxFronts = 64;
 
 // The plan's host side: layout, maps, task lists and launch programs. No
 // HIP call (it also runs, for its checks, on a host without a device).
-DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int num_cus, bool fuse_small,
-                            bool strip_trsm) {
+DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int num_cus, const PlanKnobs& k) {
    std::unique_ptr<DevicePlan> P(new DevicePlan());
    P->n = a.n;
    P->nnodes = a.nnodes;
@@ -84,8 +90,9 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
    P->wide_ncol = wide_ncol;
    // fronts at fixed caller offsets (the batched entry points) hold their
    // values already: nothing to assemble
-   P->fuse_small = fuse_small && a.fixed_off.empty();
-   P->strip_trsm = strip_trsm;
+   P->fuse_small = k.fuse_small && a.fixed_off.empty();
+   P->strip_trsm = k.strip_trsm;
+   P->level_asm = k.level_asm;
    P->variant_gen = a.variant_gen;
    const int nn = a.nnodes;
    // one-workgroup kernel or tiled path for front i
@@ -110,6 +117,22 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
       d.next_child = f.next_child;
       if (f.parent < nn) height[f.parent] = std::max(height[f.parent], height[i] + 1);
       P->num_flops += node_flops(f.nrow, f.ncol);
+   }
+   if (P->level_asm) {
+      // each front's A-map sorted by position (the column runs of L_ASM)
+      P->hasort.resize(2 * std::max<int64_t>((int64_t)a.nlist.size() / 2, 1));
+      std::vector<int64_t> idx;
+      for (int i = 0; i < nn; ++i) {
+         const SymbolicFront& f = a.fronts[i];
+         const int64_t* src = a.nlist.data() + 2 * f.amap_off;
+         idx.resize(f.num_a);
+         for (int64_t p = 0; p < f.num_a; ++p) idx[p] = p;
+         std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return src[2 * x + 1] < src[2 * y + 1]; });
+         for (int64_t p = 0; p < f.num_a; ++p) {
+            P->hasort[2 * (f.amap_off + p)] = src[2 * idx[p]];
+            P->hasort[2 * (f.amap_off + p) + 1] = src[2 * idx[p] + 1];
+         }
+      }
    }
    int nlev = 0;
    for (int i = 0; i < nn; ++i) nlev = std::max(nlev, height[i] + 1);
@@ -190,42 +213,60 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
             if (fused(i)) P->nlistv.push_back(i);
          push(P->fprog, L_SMALL_ASM, f0, P->nlistv.size() - f0);
       }
-      // zero
-      int64_t z0 = P->zero.size();
-      for (int i : lv) {
-         if (fused(i)) continue;
-         const NodeDesc& d = P->hnodes[i];
-         int chunk = std::max(4, std::min(256, 16384 / std::max(d.m, 1)));
-         for (int c = 0; c < d.m; c += chunk) P->zero.push_back(ColTask{i, c, std::min(d.m, c + chunk), 0});
-      }
-      push(P->fprog, L_ZERO, z0, P->zero.size() - z0);
-      // scatter A
-      int64_t i0 = P->init.size();
-      for (int i : lv) {
-         if (fused(i)) continue;
-         const NodeDesc& d = P->hnodes[i];
-         for (int64_t p = 0; p < d.num_a; p += 4096)
-            P->init.push_back(InitTask{i, 0, p, std::min<int64_t>(d.num_a, p + 4096)});
-      }
-      push(P->fprog, L_INIT, i0, P->init.size() - i0);
-      // extend-add by child rank
-      for (int rank = 0;; ++rank) {
-         int64_t e0 = P->ea.size();
-         bool any = false;
+      if (P->level_asm) {
+         // one launch: each task owns a run of one front's columns (zero, A
+         // entries, children); runs of about 32K lower-triangle entries
+         int64_t a0 = P->asmt.size();
          for (int i : lv) {
-            int c = P->hnodes[i].first_child;
-            for (int r = 0; r < rank && c != -1; ++r) c = P->hnodes[c].next_child;
-            if (c == -1) continue;
-            any = true;
             if (fused(i)) continue;
-            const NodeDesc& dc = P->hnodes[c];
-            int q = dc.m - dc.n;
-            if (q <= 0) continue;
-            int chunk = std::max(4, std::min(q, 16384 / std::max(q, 1)));
-            for (int j = 0; j < q; j += chunk) P->ea.push_back(EaTask{c, j, std::min(q, j + chunk), 0});
+            const NodeDesc& d = P->hnodes[i];
+            for (int c = 0; c < d.m;) {
+               int c1 = c;
+               int64_t area = 0;
+               while (c1 < d.m && (c1 == c || area + (d.m - c1) <= 32768)) area += d.m - c1++;
+               P->asmt.push_back(AsmTask{i, c, c1, 0});
+               c = c1;
+            }
          }
-         if (!any) break;
-         push(P->fprog, L_EA, e0, P->ea.size() - e0);
+         push(P->fprog, L_ASM, a0, P->asmt.size() - a0);
+      } else {
+         // zero
+         int64_t z0 = P->zero.size();
+         for (int i : lv) {
+            if (fused(i)) continue;
+            const NodeDesc& d = P->hnodes[i];
+            int chunk = std::max(4, std::min(256, 16384 / std::max(d.m, 1)));
+            for (int c = 0; c < d.m; c += chunk) P->zero.push_back(ColTask{i, c, std::min(d.m, c + chunk), 0});
+         }
+         push(P->fprog, L_ZERO, z0, P->zero.size() - z0);
+         // scatter A
+         int64_t i0 = P->init.size();
+         for (int i : lv) {
+            if (fused(i)) continue;
+            const NodeDesc& d = P->hnodes[i];
+            for (int64_t p = 0; p < d.num_a; p += 4096)
+               P->init.push_back(InitTask{i, 0, p, std::min<int64_t>(d.num_a, p + 4096)});
+         }
+         push(P->fprog, L_INIT, i0, P->init.size() - i0);
+         // extend-add by child rank
+         for (int rank = 0;; ++rank) {
+            int64_t e0 = P->ea.size();
+            bool any = false;
+            for (int i : lv) {
+               int c = P->hnodes[i].first_child;
+               for (int r = 0; r < rank && c != -1; ++r) c = P->hnodes[c].next_child;
+               if (c == -1) continue;
+               any = true;
+               if (fused(i)) continue;
+               const NodeDesc& dc = P->hnodes[c];
+               int q = dc.m - dc.n;
+               if (q <= 0) continue;
+               int chunk = std::max(4, std::min(q, 16384 / std::max(q, 1)));
+               for (int j = 0; j < q; j += chunk) P->ea.push_back(EaTask{c, j, std::min(q, j + chunk), 0});
+            }
+            if (!any) break;
+            push(P->fprog, L_EA, e0, P->ea.size() - e0);
+         }
       }
       // small fronts
       int64_t s0 = P->nlistv.size();
@@ -446,6 +487,27 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
                for (int j = e.j0; j < e.j1; ++j) cnt += q - j;
                L.words += 3.0 * cnt;  // read child, read+write parent
                L.bytes += 4.0 * cnt;  // row map
+               break;
+            }
+            case L_ASM: {
+               const AsmTask& c = P->asmt[t];
+               const NodeDesc& d = P->hnodes[c.node];
+               for (int col = c.c0; col < c.c1; ++col) L.words += d.m - col;  // zero fill
+               for (int ch = d.first_child; ch != -1; ch = P->hnodes[ch].next_child) {
+                  const NodeDesc& dc = P->hnodes[ch];
+                  const int q = dc.m - dc.n;
+                  const int* cm = P->hcmap.data() + dc.cmap_off;
+                  const int ja = (int)(std::lower_bound(cm, cm + q, c.c0) - cm);
+                  const int jb = (int)(std::lower_bound(cm, cm + q, c.c1) - cm);
+                  for (int j = ja; j < jb; ++j) {
+                     L.words += 3.0 * (q - j);  // read child, read+write parent
+                     L.bytes += 4.0 * (q - j);  // row map
+                  }
+               }
+               if (c.c0 == 0) {  // the front's original entries, counted once
+                  L.words += (double)d.num_a;
+                  L.bytes += 24.0 * (double)d.num_a;
+               }
                break;
             }
             case L_SMALL: {
@@ -823,6 +885,8 @@ void upload_plan(DevicePlan* P, const Analysis& a) {
    P->d_nodes = dupload(P->hnodes);
    P->d_zero = dupload(P->zero);
    P->d_strip = dupload(P->strip);
+   P->d_asmt = dupload(P->asmt);
+   if (P->level_asm) P->d_asort = dupload(P->hasort);
    P->d_init = dupload(P->init);
    P->d_ea = dupload(P->ea);
    P->d_nlistv = dupload(P->nlistv);
@@ -949,6 +1013,53 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
                      if (guard > nn) fail("assembled front: child list cycle");
                   }
                break;
+            case L_ASM:
+               if (e > (int64_t)P->asmt.size()) {
+                  fail("assembly: range");
+               } else {
+                  // the tasks of a front cover its columns once, in order
+                  std::map<int, int> next_col;
+                  std::vector<char> seen(nn, 0);
+                  for (int64_t t = L.off; t < e; ++t) {
+                     const AsmTask& c = P->asmt[t];
+                     if (c.node < 0 || c.node >= nn) {
+                        fail("assembly task: node");
+                        continue;
+                     }
+                     const NodeDesc& d = P->hnodes[c.node];
+                     if (c.c0 < 0 || c.c0 >= c.c1 || c.c1 > d.m) fail("assembly task: columns");
+                     if (next_col[c.node] != c.c0) fail("assembly task: column runs");
+                     next_col[c.node] = c.c1;
+                     if (d.amap_off < 0 || (d.amap_off + d.num_a) * 2 > (int64_t)P->hasort.size())
+                        fail("assembly task: A-map");
+                     if (seen[c.node]) continue;  // the children once per front
+                     seen[c.node] = 1;
+                     int guard = 0;
+                     for (int ch = d.first_child; ch != -1 && guard <= nn; ch = P->hnodes[ch].next_child, ++guard) {
+                        if (ch < 0 || ch >= nn || P->hnodes[ch].parent != c.node) {
+                           fail("assembly task: child");
+                           break;
+                        }
+                        const NodeDesc& dc = P->hnodes[ch];
+                        const int64_t q = dc.m - dc.n;
+                        if (dc.cmap_off < 0 || dc.cmap_off + q > (int64_t)P->hcmap.size()) {
+                           fail("assembly task: child map");
+                           break;
+                        }
+                        for (int64_t k = 0; k < q; ++k) {
+                           const int v = P->hcmap[dc.cmap_off + k];
+                           if (v < 0 || v >= d.m || (k > 0 && v <= P->hcmap[dc.cmap_off + k - 1])) {
+                              fail("assembly task: child map not increasing within the front");
+                              break;
+                           }
+                        }
+                     }
+                     if (guard > nn) fail("assembly task: child list cycle");
+                  }
+                  for (const auto& kv : next_col)
+                     if (kv.second != P->hnodes[kv.first].m) fail("assembly task: front not covered");
+               }
+               break;
             case L_DIAG:
             case L_DFACT:
             case L_DINV:
@@ -1063,11 +1174,11 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
    return bad;
 }
 
-DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol, bool fuse_small, bool strip_trsm) {
+DevicePlan* build_plan(const Analysis& a, int small_max, int wide_ncol, const PlanKnobs& k) {
    int dev = 0, num_cus = 0;
    HIPCHECK(hipGetDevice(&dev));
    HIPCHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
-   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, fuse_small, strip_trsm));
+   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, k));
    upload_plan(P.get(), a);
    return P.release();
 }
@@ -1091,7 +1202,8 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
       if (L.wait_ev >= 0) HIPCHECK(hipStreamWaitEvent(s, F->xev[L.wait_ev], 0));
       if (events) HIPCHECK(hipEventRecord((*events)[2 * li], s));
       // compute_only: fronts already hold their values (front timing)
-      const int kind = (compute_only && (L.kind == L_ZERO || L.kind == L_INIT || L.kind == L_EA)) ? L_SYNC : L.kind;
+      const int kind =
+          (compute_only && (L.kind == L_ZERO || L.kind == L_INIT || L.kind == L_EA || L.kind == L_ASM)) ? L_SYNC : L.kind;
       switch (kind) {
          case L_ZERO:
             k_zero<T><<<g, b, 0, s>>>(P->d_zero + L.off, P->d_nodes, arena);
@@ -1102,6 +1214,10 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
             break;
          case L_EA:
             k_extend_add<T><<<g, b, 0, s>>>(P->d_ea + L.off, P->d_nodes, P->d_cmap, arena);
+            break;
+         case L_ASM:
+            k_assemble<T><<<g, b, 0, s>>>(P->d_asmt + L.off, P->d_nodes, P->d_asort, P->d_rlist, F->d_aval,
+                                          F->has_scaling ? F->d_scaling : nullptr, P->d_cmap, arena);
             break;
          case L_SMALL:
             k_small_front<T><<<g, b, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, F->d_info);
@@ -1300,8 +1416,7 @@ void equilibrate(const DevicePlan* P, DeviceFactors* F, hipStream_t s, int max_i
 }  // namespace
 
 int plan_check(const Analysis& a, int small_max, int wide_ncol, int num_cus, int64_t* out, std::string* why) {
-   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, ext_options().small_assembly != 0,
-                                                 ext_options().strip_trsm != 0));
+   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, plan_knobs()));
    const int64_t bad = check_plan_indices(P.get(), a, why);
    if (out) {
       out[0] = bad;
@@ -1337,12 +1452,12 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
       int wide_ncol = xo.bf16_min_ncol > 0 ? xo.bf16_min_ncol : 1024;
       // a new plan for new knobs or variants; factorizations built on the old
       // one keep it alive through their own reference
-      const bool fuse = xo.small_assembly != 0, step = xo.strip_trsm != 0;
+      const PlanKnobs knobs = plan_knobs();
       if (*plan && ((*plan)->small_max != small_max || (*plan)->wide_ncol != wide_ncol ||
-                    (*plan)->fuse_small != fuse || (*plan)->strip_trsm != step ||
-                    (*plan)->variant_gen != a.variant_gen))
+                    (*plan)->fuse_small != knobs.fuse_small || (*plan)->strip_trsm != knobs.strip_trsm ||
+                    (*plan)->level_asm != knobs.level_asm || (*plan)->variant_gen != a.variant_gen))
          plan->reset();
-      if (!*plan) plan->reset(build_plan(a, small_max, wide_ncol, fuse, step));
+      if (!*plan) plan->reset(build_plan(a, small_max, wide_ncol, knobs));
       DevicePlan* P = plan->get();
       int prec = xo.precision;
       if (prec != SYLVER_AMD_PREC_FP64 && prec != SYLVER_AMD_PREC_FP32 && prec != SYLVER_AMD_PREC_MIXED)
@@ -2035,7 +2150,7 @@ int time_fronts(int K, const int* m, const int* n, const signed char* variant, i
          for (int r = 1; r <= m[i]; ++r) a.rlist.push_back(r);
       }
       if (variant) a.variant.assign(variant, variant + K);
-      P.reset(build_plan(a, 256, 1 << 30, false, false));
+      P.reset(build_plan(a, 256, 1 << 30, PlanKnobs{}));
       const bool low = precision != SYLVER_AMD_PREC_FP64;
       const size_t es = low ? sizeof(float) : sizeof(double);
       F = new DeviceFactors();
@@ -2104,7 +2219,7 @@ int batched_front_potrf_v(int nfronts, const int* nrow, const int* ncol, const i
          for (int r = 1; r <= nrow[i]; ++r) a.rlist.push_back(r);
       }
       if (variant) a.variant.assign(variant, variant + nfronts);
-      P.reset(build_plan(a, 256, 1 << 30, false, false));
+      P.reset(build_plan(a, 256, 1 << 30, PlanKnobs{}));
       const bool low = dtype != 0;
       const size_t es = low ? sizeof(float) : sizeof(double);
       F = new DeviceFactors();

@@ -70,6 +70,7 @@ enum LaunchKind {
    L_CLEAR,   // exchange-buffer fill at the start of a chained sweep
    L_SMALL_ASM,  // one-workgroup fronts assembled (zero, A, children) and factored in one launch
    L_DFACT,  // diagonal blocks factored without their inverse (k_diag_factor), before L_STRIP
+   L_ASM,    // a level's assembly (zero, A entries, children) by receiving columns (k_assemble)
    L_STRIP,  // panel TRSM by 64-row strips, by substitution (k_strip)
    L_SYNC  // no kernel: only the stream wait / event record of the entry
 };
@@ -94,6 +95,14 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
 
+// Switches that shape the factor program (extension options).
+struct PlanKnobs {
+   bool fuse_small = false;  // one-workgroup fronts assembled inside their factor launch
+   bool strip_trsm = false;  // panel TRSM by substitution after a factor-only diagonal launch
+   bool level_asm = false;   // a level's assembly in one launch by receiving columns
+};
+PlanKnobs plan_knobs();
+
 struct IndefState;  // APP LDL^T factor state (indef.hip)
 void destroy_indef(IndefState* s);
 struct DevicePlan;
@@ -110,6 +119,7 @@ struct DevicePlan {
    int small_max = 256;
    bool fuse_small = false;  // one-workgroup fronts assembled inside their factor launch (L_SMALL_ASM)
    bool strip_trsm = false;  // large-front blocks of few fronts: L_DFACT + L_STRIP (no inverse on the chain)
+   bool level_asm = false;   // assembly by receiving columns, one launch per level (L_ASM)
    int wide_ncol = 1024;  // fronts with more pivots take the wide-front update launches
    std::vector<NodeDesc> hnodes;
    std::vector<std::vector<int>> levels;
@@ -133,6 +143,8 @@ struct DevicePlan {
    std::vector<int> nlistv;  // node lists (small fronts, gathers)
    std::vector<DiagTask> diag;
    std::vector<StripTask> strip;
+   std::vector<AsmTask> asmt;
+   std::vector<int64_t> hasort;  // each front's A-map pairs sorted by position (L_ASM)
    std::vector<GemmTask> gemm;
    std::vector<SolveTask> stask;
    std::vector<int4> gch;  // forward gather: {child, i0, i1} ranges per task
@@ -145,6 +157,8 @@ struct DevicePlan {
    int* d_nlistv = nullptr;
    DiagTask* d_diag = nullptr;
    StripTask* d_strip = nullptr;
+   AsmTask* d_asmt = nullptr;
+   int64_t* d_asort = nullptr;
    GemmTask* d_gemm = nullptr;
    SolveTask* d_stask = nullptr;
    int4* d_gch = nullptr;
@@ -170,7 +184,7 @@ struct DevicePlan {
       for (void* p : {(void*)d_sptr, (void*)d_sparent, (void*)d_rptr, (void*)d_rlist,
                       (void*)d_nptr, (void*)d_nlist, (void*)d_cmap, (void*)d_cmap_off,
                       (void*)d_nodes, (void*)d_zero, (void*)d_init, (void*)d_ea,
-                      (void*)d_nlistv, (void*)d_diag, (void*)d_strip, (void*)d_gemm, (void*)d_stask, (void*)d_gch, (void*)d_ctask, (void*)d_cgch,
+                      (void*)d_nlistv, (void*)d_diag, (void*)d_strip, (void*)d_asmt, (void*)d_asort, (void*)d_gemm, (void*)d_stask, (void*)d_gch, (void*)d_ctask, (void*)d_cgch,
                       (void*)d_csr_ptr, (void*)d_csr_col, (void*)d_csr_src, (void*)d_invp})
          dfree(p);
    }

@@ -381,6 +381,7 @@ void sylver_amd_default_options(sylver_amd_options_t* o) {
    o->indef_panel_min = 0;
    o->small_assembly = 0;
    o->strip_trsm = 1;
+   o->level_assembly = 1;
    o->record_timing = 0;
 }
 

@@ -431,3 +431,36 @@ def test_strip_trsm_not_posdef_then_recovers(S, O, gpu, where):
     finally:
         S.set_ext_options(small_front_max=0, strip_trsm=1)
         S.spldlt_free_akeep(akeep)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("small_max", [0, 48])
+def test_level_assembly_bitwise(S, gpu, prec, small_max):
+    """A level assembled in one launch by receiving columns (zero, original
+    entries, every child's columns, children in list order) gives bitwise
+    the factors of the separate zero / scatter / per-child-rank extend-add
+    launches, with and without a user scaling."""
+    A = M.laplacian_3d(22)
+    akeep, opts, sym = _analyse(S, A)
+    rng = np.random.default_rng(6)
+    scale = np.exp(rng.uniform(-1.0, 1.0, A.n))
+    pr = S.PREC_FP64 if prec == "fp64" else S.PREC_FP32
+    try:
+        for sc in (None, scale):
+            got = []
+            for lasm in (0, 1):
+                S.set_ext_options(precision=pr, small_front_max=small_max, level_assembly=lasm)
+                o = S.default_options()
+                o.scaling = 0
+                fk, inf = S.spldlt_factorize(True, A.val, akeep, options=o,
+                                             scale=None if sc is None else sc.copy())
+                assert inf.flag == 0
+                got.append([np.tril(S.fkeep_node_factor(fk, s, int(sym["rptr"][s + 1] - sym["rptr"][s]),
+                                                        int(sym["sptr"][s + 1] - sym["sptr"][s])))
+                            for s in range(sym["nnodes"])])
+                S.spldlt_free_fkeep(fk)
+            for a, b in zip(*got):
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    finally:
+        S.set_ext_options(precision=S.PREC_FP64, small_front_max=0, level_assembly=1)
+        S.spldlt_free_akeep(akeep)

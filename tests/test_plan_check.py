@@ -8,7 +8,7 @@ addresses. Covered: the configurations of the parity tests and the
 benchmark's 80^3 headline case, with the knobs that change the plan
 (one-workgroup cutoff, per-front variants, wide-front threshold, the
 compute-unit count that decides which contribution blocks ride in the
-chained solve launches, the strip TRSM on and off).
+chained solve launches, the strip TRSM and the per-level assembly on and off).
 """
 import numpy as np
 import pytest
@@ -35,15 +35,15 @@ CASES = {
 
 @pytest.mark.parametrize("case", sorted(CASES))
 @pytest.mark.parametrize("small_max,bf16,cus", [(0, 0, 256), (48, 64, 256), (48, 0, 1), (1024, 0, 8)])
-@pytest.mark.parametrize("strip_trsm", [1, 0])
-def test_plan_indices_in_bounds(S, case, small_max, bf16, cus, strip_trsm):
+@pytest.mark.parametrize("strip_trsm,level_asm", [(1, 1), (0, 0), (1, 0), (0, 1)])
+def test_plan_indices_in_bounds(S, case, small_max, bf16, cus, strip_trsm, level_asm):
     A = CASES[case]()
     akeep = _akeep(S, A)
-    S.set_ext_options(strip_trsm=strip_trsm)
+    S.set_ext_options(strip_trsm=strip_trsm, level_assembly=level_asm)
     try:
         st, why = S.plan_check(akeep, small_front_max=small_max, bf16_min_ncol=bf16, num_cus=cus)
     finally:
-        S.set_ext_options(strip_trsm=1)
+        S.set_ext_options(strip_trsm=1, level_assembly=1)
     assert st["violations"] == 0, why
     assert st["factor_launches"] > 0
     if not strip_trsm:
