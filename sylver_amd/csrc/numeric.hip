@@ -45,17 +45,21 @@ sylver_amd_options_t& ext_options() {
       d.refine_max_iter = 30;
       d.use_graph = 1;
       d.small_assembly = 0;
-      d.strip_trsm = 1;
+      d.strip_trsm = 2;
       d.level_assembly = 1;
       return d;
    }();
    return o;
 }
 
-// Plan-shaping switches of the extension options.
-PlanKnobs plan_knobs() {
+// Plan-shaping switches of the extension options for a factorization in
+// precision prec (strip_trsm 2: the strip TRSM for FP64 factors only, where
+// the diagonal blocks' inverse costs most -- measured at 80^3: FP64 0.7 ms
+// faster, FP32 0.3 ms slower).
+PlanKnobs plan_knobs(int prec) {
    const sylver_amd_options_t& o = ext_options();
-   return PlanKnobs{o.small_assembly != 0, o.strip_trsm != 0, o.level_assembly != 0};
+   const bool strip = o.strip_trsm == 1 || (o.strip_trsm == 2 && prec == SYLVER_AMD_PREC_FP64);
+   return PlanKnobs{o.small_assembly != 0, strip, o.level_assembly != 0};
 }
 
 int device_count() {
@@ -1416,7 +1420,7 @@ void equilibrate(const DevicePlan* P, DeviceFactors* F, hipStream_t s, int max_i
 }  // namespace
 
 int plan_check(const Analysis& a, int small_max, int wide_ncol, int num_cus, int64_t* out, std::string* why) {
-   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, plan_knobs()));
+   std::unique_ptr<DevicePlan> P(build_plan_host(a, small_max, wide_ncol, num_cus, plan_knobs(ext_options().precision)));
    const int64_t bad = check_plan_indices(P.get(), a, why);
    if (out) {
       out[0] = bad;
@@ -1452,16 +1456,16 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
       int wide_ncol = xo.bf16_min_ncol > 0 ? xo.bf16_min_ncol : 1024;
       // a new plan for new knobs or variants; factorizations built on the old
       // one keep it alive through their own reference
-      const PlanKnobs knobs = plan_knobs();
+      int prec = xo.precision;
+      if (prec != SYLVER_AMD_PREC_FP64 && prec != SYLVER_AMD_PREC_FP32 && prec != SYLVER_AMD_PREC_MIXED)
+         prec = SYLVER_AMD_PREC_FP64;
+      const PlanKnobs knobs = plan_knobs(prec);
       if (*plan && ((*plan)->small_max != small_max || (*plan)->wide_ncol != wide_ncol ||
                     (*plan)->fuse_small != knobs.fuse_small || (*plan)->strip_trsm != knobs.strip_trsm ||
                     (*plan)->level_asm != knobs.level_asm || (*plan)->variant_gen != a.variant_gen))
          plan->reset();
       if (!*plan) plan->reset(build_plan(a, small_max, wide_ncol, knobs));
       DevicePlan* P = plan->get();
-      int prec = xo.precision;
-      if (prec != SYLVER_AMD_PREC_FP64 && prec != SYLVER_AMD_PREC_FP32 && prec != SYLVER_AMD_PREC_MIXED)
-         prec = SYLVER_AMD_PREC_FP64;
       // Mixed precision without the BF16 update kernels runs the FP32 path.
       const bool low = prec != SYLVER_AMD_PREC_FP64;
       // the indefinite path's low tier is FP32 (Schur updates of the

@@ -101,7 +101,7 @@ struct PlanKnobs {
    bool strip_trsm = false;  // panel TRSM by substitution after a factor-only diagonal launch
    bool level_asm = false;   // a level's assembly in one launch by receiving columns
 };
-PlanKnobs plan_knobs();
+PlanKnobs plan_knobs(int prec);
 
 struct IndefState;  // APP LDL^T factor state (indef.hip)
 void destroy_indef(IndefState* s);
