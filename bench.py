@@ -255,6 +255,61 @@ def bench_small_laplacian(S, M, oracle_mod):
             "cpu_kind": "port (oracle, all host threads)"}
 
 
+def bench_lu_fronts(S, oracle_mod, nfronts=20000, seed=5, reps=3, cpu_fronts=300):
+    """Dense front LU of the unsymmetric path (the reference's lu_pp_factor /
+    lu_tpp_factor, src/kernels/lu_pp.hxx, lu_tpp.hxx), batched: orders m
+    log-uniform in [8, 128], every column eliminated, entries U[-1, 1]; one
+    launch of sylver_amd_batched_front_lu per mode and precision. Flops
+    2 m^3 / 3 per front; bytes: the front read and written once. The first
+    fronts are checked bit for bit against the oracle, whose time on one host
+    thread over cpu_fronts fronts is the CPU baseline."""
+    rng = np.random.default_rng(seed)
+    m = np.exp(rng.uniform(np.log(8), np.log(128.999), nfronts)).astype(np.int32)
+    off = np.zeros(nfronts, np.int64)
+    off[1:] = np.cumsum(m.astype(np.int64) ** 2)[:-1]
+    total = int(off[-1] + int(m[-1]) ** 2)
+    host = rng.uniform(-1.0, 1.0, total)
+    flops = float((2.0 / 3.0 * m.astype(np.float64) ** 3).sum())
+    out = {"fronts": nfronts, "seed": seed, "orders": "log-uniform [8, 128]", "gflop": round(flops / 1e9, 3)}
+    for mode in ("pp", "tpp"):
+        for dt, name in ((np.float64, "fp64"), (np.float32, "fp32")):
+            w = np.dtype(dt).itemsize
+            nbytes = float((2 * w * m.astype(np.float64) ** 2).sum())
+            best = None
+            for _ in range(reps):
+                arr = S.DeviceArray(host.astype(dt))
+                rc, ne, perms = S.batched_front_lu(arr, m, m, off, dt, mode, 0.01)
+                ms = float(S.lib().sylver_amd_last_batched_ms())
+                if rc != 0:
+                    raise RuntimeError(f"batched LU failed: {rc}")
+                best = ms if best is None else min(best, ms)
+                if _ == 0:
+                    # parity on the first fronts, bit for bit
+                    res = arr.read(0, int(off[16]))
+                    for f in range(16):
+                        a = host[off[f]:off[f] + m[f] ** 2].astype(dt).reshape((m[f], m[f]), order="F")
+                        ne_o, rp_o, cp_o, f_o = oracle_mod.lu_front(a, int(m[f]), mode, 0.01)
+                        got = res[off[f]:off[f] + m[f] ** 2].reshape((m[f], m[f]), order="F")
+                        if ne[f] != ne_o or not np.array_equal(perms[f][0], rp_o) or \
+                                np.asfortranarray(got).tobytes("F") != np.asfortranarray(f_o).tobytes("F"):
+                            raise RuntimeError(f"LU parity failed at front {f} ({mode}, {name})")
+                arr.free()
+            t = best / 1e3
+            out[f"{mode}_{name}"] = {"ms": round(best, 3), "gflops": round(flops / t / 1e9, 1),
+                                     "gbytes_s": round(nbytes / t / 1e9, 1), "parity_first16": "bitwise"}
+    # CPU baseline: the oracle on one host thread, FP64 partial pivoting
+    t0 = time.perf_counter()
+    cf = 0.0
+    for f in range(cpu_fronts):
+        a = host[off[f]:off[f] + m[f] ** 2].reshape((m[f], m[f]), order="F")
+        oracle_mod.lu_front(a, int(m[f]), "pp")
+        cf += 2.0 / 3.0 * float(m[f]) ** 3
+    tc = time.perf_counter() - t0
+    out["cpu_baseline"] = {"kind": "port (oracle, one thread)", "fronts": cpu_fronts,
+                           "gflops": round(cf / tc / 1e9, 3)}
+    return out
+
+
 def bench_kkt(S, M, oracle_mod, side=48, mrows=27648, sample_side=16):
     """KKT [H A^T; A 0], H the 7-point Laplacian on side^3, A mrows x side^3
     with 3 entries per row at random columns: APP LDL^T (u = 0.01, small =
@@ -462,7 +517,7 @@ def main():
                          "else saved there after training (profiled runs reuse the same choices)")
     ap.add_argument("--costmodel-full", action="store_true",
                     help="learned cost model on the configured stream (400 + 100 matrices, full sizes)")
-    ap.add_argument("--only", default=None, choices=["batched", "kkt"],
+    ap.add_argument("--only", default=None, choices=["batched", "kkt", "lu"],
                     help="run one secondary workload alone and print its record (profiling runs)")
     ap.add_argument("--kkt-side", type=int, default=48, help="grid side of H for --only kkt")
     ap.add_argument("--no-secondary", action="store_true",
@@ -484,6 +539,10 @@ def main():
         sys.exit(3)
     if args.only == "batched":
         print(json.dumps(bench_batched_fronts(S, M, model=None, cpu_fronts=200), default=float))
+        return
+    if args.only == "lu":
+        import oracle as oracle_mod
+        print(json.dumps(bench_lu_fronts(S, oracle_mod), default=float))
         return
     if args.only == "kkt":
         import oracle as oracle_mod
@@ -605,6 +664,7 @@ def main():
         for key, fn in (("laplacian_2d_100", lambda: bench_small_laplacian(S, M, oracle_mod)),
                         ("batched_small_fronts", lambda: bench_batched_fronts(S, M, model=model)),
                         ("kkt_app_48cubed_27648", lambda: bench_kkt(S, M, oracle_mod)),
+                        ("dense_front_lu", lambda: bench_lu_fronts(S, oracle_mod)),
                         ("learned_cost_model",
                          lambda: cm_report if cm_report is not None else bench_costmodel(S, M, full=args.costmodel_full))):
             try:

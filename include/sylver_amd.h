@@ -288,6 +288,28 @@ int sylver_amd_batched_front_potrf(int nfronts, int const *nrow_host, int const 
                                    long const *offset_host, void *data_dev, int dtype,
                                    int *info_dev, void *stream);
 
+/* Dense front LU of the unsymmetric path (the reference's kernels
+ * lu_nopiv_factor, src/kernels/lu_nopiv.hxx:13-41; lu_pp_factor,
+ * src/kernels/lu_pp.hxx:38-73; lu_tpp_factor, src/kernels/lu_tpp.hxx:33-118),
+ * batched: for each front f the m[f] x m[f] column-major block at
+ * data + offset[f] (elements, ld = m[f], device memory; 1 <= m[f] <= 128) is
+ * factored over its first k[f] columns in place -- L unit lower and U upper
+ * in the eliminated rows and columns, the Schur complement in the trailing
+ * block. mode: 0 no pivoting, 1 partial pivoting (rows interchanged over the
+ * whole front), 2 threshold partial pivoting with threshold u and column
+ * interchanges among the first k[f] columns (a column that has no
+ * acceptable pivot is left for later and may stay uneliminated). rperm and
+ * cperm (host, sum of m[f] ints, front after front) receive each front's
+ * row and column orders (0-based: row i of the factored front is row
+ * rperm[i] of the input); nelim (host, nfronts ints) the eliminated columns.
+ * dtype: 0 = double, 1 = float. Synchronous on stream (may be NULL); the
+ * kernel's device time is then sylver_amd_last_batched_ms().
+ * Returns 0 or a SYLVER_ERROR_* code. */
+int sylver_amd_batched_front_lu(int nfronts, int const *m_host, int const *k_host,
+                                long const *offset_host, void *data_dev, int dtype, int mode,
+                                double u, int *rperm_host, int *cperm_host, int *nelim_host,
+                                void *stream);
+
 /* sylver_amd_batched_front_potrf without any device allocation. No
  * counterpart in the reference: it is for callers that cannot allocate
  * (code under HIP graph capture, tight serving loops, harnesses that forbid

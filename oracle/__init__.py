@@ -48,6 +48,9 @@ def lib():
         L.oracle_indef_free.argtypes = [C.c_void_p]
         L.oracle_front_ldlt.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_double, C.c_double,
                                         C.c_int, C.c_int, ip, dp, ip]
+        L.oracle_lu_front.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, dp, C.c_int, ip, ip]
+        L.oracle_lu_front_f32.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_float),
+                                          C.c_int, ip, ip]
         _lib = L
     return _lib
 
@@ -202,6 +205,28 @@ def front_ldlt(a, n, nb=128, u=0.01, small=1e-20, action=True, failed_tpp=True):
                                  int(failed_tpp), _p(perm, C.c_int), _p(dinv, C.c_double),
                                  C.byref(nf))
     return ne, nf.value, perm[:n], dinv[:2 * n], a
+
+
+LU_MODES = {"nopiv": 0, "pp": 1, "tpp": 2}
+
+
+def lu_front(a, k, mode="nopiv", u=0.01):
+    """Dense front LU of the square m x m front `a` (a copy) over its first
+    k columns (oracle_lu.cpp). Returns (nelim, rperm, cperm, factored front):
+    L unit lower and U upper in the first nelim rows/columns, the Schur
+    complement in the trailing block, rperm/cperm the row/column orders."""
+    dt = np.float32 if np.asarray(a).dtype == np.float32 else np.float64
+    a = np.asfortranarray(np.array(a, dtype=dt, copy=True))
+    m = a.shape[0]
+    rp = np.zeros(max(m, 1), np.int32)
+    cp = np.zeros(max(m, 1), np.int32)
+    if dt == np.float32:
+        ne = lib().oracle_lu_front_f32(LU_MODES[mode], u, m, k, _p(a, C.c_float), m, _p(rp, C.c_int),
+                                       _p(cp, C.c_int))
+    else:
+        ne = lib().oracle_lu_front(LU_MODES[mode], u, m, k, _p(a, C.c_double), m, _p(rp, C.c_int),
+                                   _p(cp, C.c_int))
+    return ne, rp[:m], cp[:m], a
 
 
 def dinv_blocks(dinv, nelim):

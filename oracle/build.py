@@ -22,9 +22,11 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-march=x86-64-v3"
 def build(verbose: bool = False) -> str:
     """oracle.cpp keeps the compiler's default FMA contraction (it is the
     timed CPU baseline); oracle_indef.cpp is built with -ffp-contract=off
-    because its roundings are written out explicitly."""
+    because its roundings are written out explicitly, and so is
+    oracle_lu.cpp."""
     objs = []
-    for src, extra in (("oracle.cpp", []), ("oracle_indef.cpp", ["-ffp-contract=off"])):
+    for src, extra in (("oracle.cpp", []), ("oracle_indef.cpp", ["-ffp-contract=off"]),
+                       ("oracle_lu.cpp", ["-ffp-contract=off"])):
         obj = os.path.join(HERE, "_build", src.replace(".cpp", ".o"))
         os.makedirs(os.path.dirname(obj), exist_ok=True)
         cmd = ["g++", *FLAGS, *extra, "-c", os.path.join(HERE, src), "-o", obj]

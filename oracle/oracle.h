@@ -105,6 +105,14 @@ void oracle_indef_free(void *h);
 int oracle_front_ldlt(int m, int n, double *a, int nb, double u, double small, int action,
                       int failed_tpp, int *perm, double *dinv, int *nelim_first);
 
+/* Dense front LU of an m x m column-major front over its first k columns
+ * (src/kernels/lu_nopiv.hxx, lu_pp.hxx, lu_tpp.hxx); mode 0 no pivoting, 1
+ * partial pivoting, 2 threshold partial pivoting (u) with column interchanges
+ * among the first k columns. rperm/cperm (m) receive the row/column orders.
+ * Returns nelim (or -1 for bad arguments). */
+int oracle_lu_front(int mode, double u, int m, int k, double *a, int lda, int *rperm, int *cperm);
+int oracle_lu_front_f32(int mode, double u, int m, int k, float *a, int lda, int *rperm, int *cperm);
+
 #ifdef __cplusplus
 }
 #endif

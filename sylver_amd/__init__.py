@@ -153,7 +153,7 @@ EXPORTED = [
     "sylver_amd_batched_front_potrf", "sylver_amd_batched_front_fill", "sylver_amd_device_count", "sylver_amd_dev_alloc",
     "sylver_amd_dev_free", "sylver_amd_memcpy_h2d", "sylver_amd_memcpy_d2h",
     "sylver_amd_last_batched_ms", "sylver_amd_set_front_variants", "sylver_amd_time_fronts",
-    "sylver_amd_batched_front_potrf_v", "sylver_amd_batched_front_potrf_noalloc",
+    "sylver_amd_batched_front_potrf_v", "sylver_amd_batched_front_potrf_noalloc", "sylver_amd_batched_front_lu",
     "sylver_amd_mlp_init", "sylver_amd_mlp_grad", "sylver_amd_mlp_train", "sylver_amd_mlp_predict",
     "sylver_amd_hungarian_scale_sym", "sylver_amd_auction_scale_sym", "sylver_amd_match_order",
     "sylver_amd_plan_check", "sylver_amd_rb_read", "sylver_amd_rb_free",
@@ -194,6 +194,7 @@ def lib():
     L.sylver_amd_fkeep_indef_stats.argtypes = [vp, ip]
     L.sylver_amd_batched_front_potrf.argtypes = [C.c_int, ip, ip, lp, vp, C.c_int, ip, vp]
     L.sylver_amd_batched_front_potrf_noalloc.argtypes = [C.c_int, ip, ip, lp, vp, C.c_int, ip, vp]
+    L.sylver_amd_batched_front_lu.argtypes = [C.c_int, ip, ip, lp, vp, C.c_int, C.c_int, C.c_double, ip, ip, ip, vp]
     L.sylver_amd_batched_front_fill.argtypes = [C.c_int, ip, lp, vp, C.c_int, C.c_ulonglong]
     L.sylver_amd_device_count.argtypes = []
     L.sylver_amd_dev_alloc.argtypes = [pvp, C.c_size_t]
@@ -519,6 +520,30 @@ def batched_front_potrf_noalloc(fronts_flat: "DeviceArray", nrow, ncol, offsets,
     return lib().sylver_amd_batched_front_potrf_noalloc(
         len(nrow), _p(nrow, C.c_int), _p(ncol, C.c_int), _p(offsets, C.c_long), fronts_flat.ptr,
         0 if np.dtype(dtype) == np.float64 else 1, C.cast(info.ptr, C.POINTER(C.c_int)), None)
+
+
+LU_MODES = {"nopiv": 0, "pp": 1, "tpp": 2}
+
+
+def batched_front_lu(fronts_flat: "DeviceArray", m, k, offsets, dtype, mode="nopiv", u=0.01):
+    """Dense front LU of many square fronts in one launch (lu.hip). Returns
+    (flag, nelim, list of (rperm, cperm) per front)."""
+    m = np.ascontiguousarray(m, np.int32)
+    k = np.ascontiguousarray(k, np.int32)
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    tot = int(m.sum())
+    rp = np.zeros(max(tot, 1), np.int32)
+    cp = np.zeros(max(tot, 1), np.int32)
+    ne = np.zeros(max(len(m), 1), np.int32)
+    rc = lib().sylver_amd_batched_front_lu(
+        len(m), _p(m, C.c_int), _p(k, C.c_int), _p(offsets, C.c_long), fronts_flat.ptr,
+        0 if np.dtype(dtype) == np.float64 else 1, LU_MODES[mode], float(u), _p(rp, C.c_int), _p(cp, C.c_int),
+        _p(ne, C.c_int), None)
+    perms, o = [], 0
+    for mm in m:
+        perms.append((rp[o:o + mm].copy(), cp[o:o + mm].copy()))
+        o += int(mm)
+    return rc, ne[:len(m)].copy(), perms
 
 
 def batched_front_potrf_v(fronts_flat: "DeviceArray", nrow, ncol, offsets, dtype, variant=None):

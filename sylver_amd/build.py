@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsylver_amd.so")
 
-SOURCES = ["numeric.hip", "indef.hip", "costmodel.hip", "analyse.cpp", "scaling.cpp", "rb_io.cpp", "sylver_api.cpp"]
+SOURCES = ["numeric.hip", "indef.hip", "costmodel.hip", "lu.hip", "analyse.cpp", "scaling.cpp", "rb_io.cpp",
+           "sylver_api.cpp"]
 # the reference driver drivers/spldlt_test.F90 on the C interface (gcc)
 DRIVER = os.path.join(HERE, "drivers", "spldlt_test.c")
 DRIVER_OUT = os.path.join(HERE, "spldlt_test")

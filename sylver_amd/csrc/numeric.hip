@@ -2061,6 +2061,7 @@ int dev_copy(void* dst, const void* src, size_t bytes, bool h2d) {
 
 static double g_last_batched_ms = 0;
 double last_batched_ms() { return g_last_batched_ms; }
+void set_last_batched_ms(double ms) { g_last_batched_ms = ms; }
 
 // ---------------------------------------------------------------------------
 // Batched dense fronts (no tree): every front through the one-workgroup

@@ -105,6 +105,7 @@ int dev_alloc(void** ptr, size_t bytes);
 int dev_free(void* ptr);
 int dev_copy(void* dst, const void* src, size_t bytes, bool h2d);
 double last_batched_ms();
+void set_last_batched_ms(double ms);
 
 int batched_front_potrf(int nfronts, const int* nrow, const int* ncol, const int64_t* off,
                         void* data, int dtype, int* info, void* stream);
@@ -114,6 +115,9 @@ int batched_front_potrf_v(int nfronts, const int* nrow, const int* ncol, const i
 // fixed per-device array of kNoAllocMaxFronts fronts.
 constexpr int kNoAllocMaxFronts = 1 << 16;
 constexpr int kMaxScratchDevices = 64;
+// Dense front LU (unsymmetric path), batched: lu.hip.
+int batched_front_lu(int nfronts, const int* m, const int* k, const int64_t* off, void* data, int dtype, int mode,
+                     double u, int* rperm, int* cperm, int* nelim, void* stream);
 int batched_front_potrf_noalloc(int nfronts, const int* nrow, const int* ncol, const int64_t* off, void* data,
                                 int dtype, int* info, void* stream);
 int time_fronts(int K, const int* m, const int* n, const signed char* variant, int precision, int reps,

@@ -554,6 +554,16 @@ int sylver_amd_batched_front_potrf_noalloc(int nfronts, int const* nrow, int con
    }
 }
 
+int sylver_amd_batched_front_lu(int nfronts, int const* m, int const* k, long const* offset, void* data, int dtype,
+                                int mode, double u, int* rperm, int* cperm, int* nelim, void* stream) {
+   try {
+      return batched_front_lu(nfronts, m, k, reinterpret_cast<const int64_t*>(offset), data, dtype, mode, u, rperm,
+                              cperm, nelim, stream);
+   } catch (...) {
+      return SYLVER_ERROR_UNKNOWN;
+   }
+}
+
 int sylver_amd_hungarian_scale_sym(int n, long const* ptr, int const* row, double const* val,
                                    int scale_if_singular, double* scaling, int* match) {
    if (n < 0 || (n > 0 && (!ptr || !row || !val || !scaling))) return SYLVER_ERROR_CALL_SEQUENCE;
