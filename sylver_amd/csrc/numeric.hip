@@ -1583,9 +1583,12 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
             HIPCHECK(hipEventElapsedTime(&t, events[2 * i], events[2 * i + 1]));
             const Launch& L = P->fprog[i];
             // the assembled one-workgroup fronts count with the one-workgroup kind
+            // (kinds past the first 16 are counted with their nearest: the
+            // level assembly with the extend-add)
             const int kind = L.kind == L_SMALL_ASM ? (int)L_SMALL
                              : L.kind == L_DFACT ? (int)L_DIAG
                              : L.kind == L_STRIP ? (int)L_TRSM
+                             : L.kind == L_ASM   ? (int)L_EA
                                                  : L.kind;
             if (kind >= 16) continue;  // L_SYNC: no kernel
             F->kind_ms[kind] += t;
