@@ -256,6 +256,14 @@ __device__ inline int64_t lower_bound_pos(const int64_t* pos, int64_t lo, int64_
    return lo;
 }
 
+// Runs of at most kAsmLds entries (AsmTask::pad = 1) are assembled in LDS
+// and written to the front once: the zero fill and the parent's
+// read-modify-write per child entry stay on chip, the front is written once.
+// The run is its columns' lower trapezoid, column c (rows c .. m-1) at
+// offset (c - c0) m - (c - c0)(c + c0 - 1) / 2. Longer runs (a single column
+// of more than kAsmLds rows) work on the front directly.
+constexpr int kAsmLds = 8192;
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_assemble(const AsmTask* __restrict__ tasks,
                                                        const NodeDesc* __restrict__ nodes,
@@ -264,27 +272,41 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const AsmTask* __restrict
                                                        const double* __restrict__ aval,
                                                        const double* __restrict__ scaling,
                                                        const int* __restrict__ cmap, T* __restrict__ arena) {
+   extern __shared__ __align__(16) unsigned char asm_lds[];
+   T* run = reinterpret_cast<T*>(asm_lds);
    const AsmTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const bool staged = t.pad != 0;
    T* F = arena + nd.off;
-   for (int c = t.c0 + wave; c < t.c1; c += 4) {
-      T* col = F + (int64_t)c * nd.ld;
-      for (int r = c + lane; r < nd.m; r += 64) col[r] = T(0);
+   const int m = nd.m, c0 = t.c0;
+   // run offset of column c (staged)
+   auto col_off = [&](int c) { return (c - c0) * m - (int)(((int64_t)(c - c0) * (c + c0 - 1)) / 2); };
+   if (staged) {
+      const int area = col_off(t.c1);
+      for (int e = threadIdx.x; e < area; e += kThreads) run[e] = T(0);
+   } else {
+      for (int c = c0 + wave; c < t.c1; c += 4) {
+         T* col = F + (int64_t)c * nd.ld;
+         for (int r = c + lane; r < m; r += 64) col[r] = T(0);
+      }
    }
    __syncthreads();
-   if (t.c0 < nd.n) {  // original entries lie in the pivot columns only
+   if (c0 < nd.n) {  // original entries lie in the pivot columns only
       const int64_t* am = asort + 2 * nd.amap_off;
-      const int64_t p0 = lower_bound_pos(am, 0, nd.num_a, (int64_t)t.c0 * nd.m + 1);
-      const int64_t p1 = lower_bound_pos(am, p0, nd.num_a, (int64_t)t.c1 * nd.m + 1);
+      const int64_t p0 = lower_bound_pos(am, 0, nd.num_a, (int64_t)c0 * m + 1);
+      const int64_t p1 = lower_bound_pos(am, p0, nd.num_a, (int64_t)t.c1 * m + 1);
       for (int64_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
          const int64_t src = am[2 * p] - 1;
          const int64_t dest = am[2 * p + 1] - 1;
-         const int c = (int)(dest / nd.m);
-         const int r = (int)(dest - (int64_t)c * nd.m);
+         const int c = (int)(dest / m);
+         const int r = (int)(dest - (int64_t)c * m);
          double v = aval[src];
          if (scaling) v = scaling[rlist[nd.rlist_off + r] - 1] * v * scaling[rlist[nd.rlist_off + c] - 1];
-         F[(int64_t)c * nd.ld + r] = (T)v;
+         if (staged)
+            run[col_off(c) + (r - c)] = (T)v;
+         else
+            F[(int64_t)c * nd.ld + r] = (T)v;
       }
    }
    for (int ch = nd.first_child; ch != -1;) {
@@ -292,29 +314,54 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const AsmTask* __restrict
       __syncthreads();
       const int q = cn.m - cn.n;
       const int* __restrict__ cm = cmap + cn.cmap_off;
-      const int ja = lower_bound_i(cm, q, t.c0), jb = lower_bound_i(cm, q, t.c1);
+      const int ja = lower_bound_i(cm, q, c0), jb = lower_bound_i(cm, q, t.c1);
       for (int j = ja + wave; j < jb; j += 4) {
          const int pj = cm[j];
-         T* dst = F + (int64_t)pj * nd.ld;
          const T* src = arena + cn.off + (int64_t)(cn.n + j) * cn.ld + cn.n;
-         // 8 rows per lane per batch: all loads issued before the stores
-         for (int i0 = j; i0 < q; i0 += 8 * 64) {
-            int ci[8];
-            T sv[8], dv[8];
+         if (staged) {
+            T* dst = run + col_off(pj) - pj;  // row r of column pj at dst[r]
+            for (int i0 = j; i0 < q; i0 += 8 * 64) {
+               int ci[8];
+               T sv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-               const int i = min(i0 + u * 64 + lane, q - 1);
-               ci[u] = cm[i];
-               sv[u] = src[i];
+               for (int u = 0; u < 8; ++u) {
+                  const int i = min(i0 + u * 64 + lane, q - 1);
+                  ci[u] = cm[i];
+                  sv[u] = src[i];
+               }
+#pragma unroll
+               for (int u = 0; u < 8; ++u)
+                  if (i0 + u * 64 + lane < q) dst[ci[u]] = dst[ci[u]] + sv[u];
             }
+         } else {
+            T* dst = F + (int64_t)pj * nd.ld;
+            // 8 rows per lane per batch: all loads issued before the stores
+            for (int i0 = j; i0 < q; i0 += 8 * 64) {
+               int ci[8];
+               T sv[8], dv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) dv[u] = dst[ci[u]];
+               for (int u = 0; u < 8; ++u) {
+                  const int i = min(i0 + u * 64 + lane, q - 1);
+                  ci[u] = cm[i];
+                  sv[u] = src[i];
+               }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-               if (i0 + u * 64 + lane < q) dst[ci[u]] = dv[u] + sv[u];
+               for (int u = 0; u < 8; ++u) dv[u] = dst[ci[u]];
+#pragma unroll
+               for (int u = 0; u < 8; ++u)
+                  if (i0 + u * 64 + lane < q) dst[ci[u]] = dv[u] + sv[u];
+            }
          }
       }
       ch = cn.next_child;
+   }
+   if (!staged) return;
+   __syncthreads();
+   // the run to the front, column by column (rows c .. m-1)
+   for (int c = c0 + wave; c < t.c1; c += 4) {
+      const T* s = run + col_off(c) - c;
+      T* col = F + (int64_t)c * nd.ld;
+      for (int r = c + lane; r < m; r += 64) col[r] = s[r];
    }
 }
 

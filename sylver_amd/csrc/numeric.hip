@@ -219,7 +219,9 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
       }
       if (P->level_asm) {
          // one launch: each task owns a run of one front's columns (zero, A
-         // entries, children); runs of about 32K lower-triangle entries
+         // entries, children) of at most kAsmLds lower-triangle entries,
+         // assembled in LDS (pad = 1); a longer single column works on the
+         // front directly (pad = 0)
          int64_t a0 = P->asmt.size();
          for (int i : lv) {
             if (fused(i)) continue;
@@ -227,8 +229,8 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
             for (int c = 0; c < d.m;) {
                int c1 = c;
                int64_t area = 0;
-               while (c1 < d.m && (c1 == c || area + (d.m - c1) <= 32768)) area += d.m - c1++;
-               P->asmt.push_back(AsmTask{i, c, c1, 0});
+               while (c1 < d.m && (c1 == c || area + (d.m - c1) <= kAsmLds)) area += d.m - c1++;
+               P->asmt.push_back(AsmTask{i, c, c1, area <= kAsmLds ? 1 : 0});
                c = c1;
             }
          }
@@ -1032,6 +1034,13 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
                      }
                      const NodeDesc& d = P->hnodes[c.node];
                      if (c.c0 < 0 || c.c0 >= c.c1 || c.c1 > d.m) fail("assembly task: columns");
+                     if (c.pad != 0) {  // assembled in LDS: the run's trapezoid must fit
+                        int64_t area = 0;
+                        for (int col = c.c0; col < c.c1 && col < d.m; ++col) area += d.m - col;
+                        if (area > kAsmLds) fail("assembly task: run larger than its LDS");
+                     } else if (c.c1 != c.c0 + 1) {
+                        fail("assembly task: a direct run is one column");
+                     }
                      if (next_col[c.node] != c.c0) fail("assembly task: column runs");
                      next_col[c.node] = c.c1;
                      if (d.amap_off < 0 || (d.amap_off + d.num_a) * 2 > (int64_t)P->hasort.size())
@@ -1220,8 +1229,9 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
             k_extend_add<T><<<g, b, 0, s>>>(P->d_ea + L.off, P->d_nodes, P->d_cmap, arena);
             break;
          case L_ASM:
-            k_assemble<T><<<g, b, 0, s>>>(P->d_asmt + L.off, P->d_nodes, P->d_asort, P->d_rlist, F->d_aval,
-                                          F->has_scaling ? F->d_scaling : nullptr, P->d_cmap, arena);
+            k_assemble<T><<<g, b, kAsmLds * sizeof(T), s>>>(P->d_asmt + L.off, P->d_nodes, P->d_asort, P->d_rlist,
+                                                             F->d_aval, F->has_scaling ? F->d_scaling : nullptr,
+                                                             P->d_cmap, arena);
             break;
          case L_SMALL:
             k_small_front<T><<<g, b, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, F->d_info);
