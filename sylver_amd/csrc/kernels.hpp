@@ -570,29 +570,36 @@ __device__ int lds_potrf_blocked(T* L, int kb16, int* sh_bad) {
 // of a panel factors the diagonal block (the same values, the same
 // operations, so the same results); wave 0 stores it.
 template <typename T, int LD>
-__device__ int wave_panel16(T* L, int o, int kb16, int w) {
+__device__ __forceinline__ int wave_panel16(T* L, int o, int kb16, int w) {
    const int lane = threadIdx.x & 63;
    const int row = lane < 16 ? o + lane : o + 16 + w * 48 + (lane - 16);
    const bool live = row < kb16;
    T r[16];
 #pragma unroll
    for (int c = 0; c < 16; ++c) r[c] = (live && (lane >= 16 || c <= lane)) ? L[row + (o + c) * LD] : T(0);
+   // Branch-free, so the whole panel is one block the compiler can schedule:
+   // the next pivot (row j + 1's update, then its broadcast) is formed first
+   // and its reciprocal square root overlaps the rest of step j. After a
+   // non-positive pivot the steps go on with meaningless values; the first
+   // such column is what is returned.
    int bad = 0;
+   T d = lane_bcast(r[0], 0);
 #pragma unroll
    for (int j = 0; j < 16; ++j) {
       T col[16];
 #pragma unroll
-      for (int c = j; c < 16; ++c) col[c] = lane_bcast(r[j], c);
-      const T d = col[j];
-      if (bad == 0 && !(d > T(0))) bad = j + 1;
-      if (bad == 0) {
-         const T inv = rsqrt_nr(d);
-         const T lj = r[j] * inv;
-         const T t = lj * inv;
-         r[j] = (lane == j) ? d * inv : lj;
-#pragma unroll
-         for (int c = j + 1; c < 16; ++c) r[c] = fma(-t, col[c], r[c]);
+      for (int c = j + 1; c < 16; ++c) col[c] = lane_bcast(r[j], c);
+      bad = (bad == 0 && !(d > T(0))) ? j + 1 : bad;
+      const T inv = rsqrt_nr(d);
+      const T lj = r[j] * inv;
+      const T t = lj * inv;
+      r[j] = (lane == j) ? d * inv : lj;
+      if (j + 1 < 16) {
+         r[j + 1] = fma(-t, col[j + 1], r[j + 1]);
+         d = lane_bcast(r[j + 1], j + 1);
       }
+#pragma unroll
+      for (int c = j + 2; c < 16; ++c) r[c] = fma(-t, col[c], r[c]);
    }
    if (live) {
       if (lane < 16) {
@@ -653,35 +660,37 @@ __device__ int lds_potrf_panels(T* L, int kb16, int* sh_bad) {
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int nbk = kb16 / 16;
    __shared__ int next_tile;
-   if (wave < panel_waves(kb16, 0)) {
-      const int bad = wave_panel16<T, LD>(L, 0, kb16, wave);
-      if (wave == 0 && lane == 0) *sh_bad = bad;
-   }
-   __syncthreads();
-   for (int s = 0; s + 1 < nbk; ++s) {
-      const int o = s * 16;
-      if (*sh_bad) return *sh_bad;
-      const int t0 = o + 16, nt = (kb16 - t0) / 16;
-      // the next panel's column of tiles
-      for (int ti = wave; ti < nt; ti += 4) panel_update2<T, LD>(L, o, t0, ti, 0, ti, 0, false);
-      if (threadIdx.x == 0) next_tile = 0;
-      __syncthreads();
+   // round s + 1 factors panel s + 1 (at t0); from the second round on it is
+   // preceded by panel s's update of that panel's column and overlapped by
+   // the rest of panel s's update (one copy of the panel code: it runs once
+   // per diagonal block, fetched cold)
+   for (int s = -1; s + 1 < nbk; ++s) {
+      const int o = s * 16, t0 = o + 16, nt = (kb16 - t0) / 16;
+      if (s >= 0) {
+         if (*sh_bad) return *sh_bad;
+         // the next panel's column of tiles
+         for (int ti = wave; ti < nt; ti += 4) panel_update2<T, LD>(L, o, t0, ti, 0, ti, 0, false);
+         if (threadIdx.x == 0) next_tile = 0;
+         __syncthreads();
+      }
       if (wave < panel_waves(kb16, t0)) {
          const int bad = wave_panel16<T, LD>(L, t0, kb16, wave);
          if (wave == 0 && lane == 0) *sh_bad = bad ? t0 + bad : 0;
       }
-      // the rest of panel s's update: the lower triangle at t0 + 16, two tiles at a time
-      const int nr = nt - 1, nrt = nr * (nr + 1) / 2, t1 = t0 + 16;
-      for (;;) {
-         int tile = 0;
-         if (lane == 0) tile = atomicAdd(&next_tile, 2);
-         tile = __shfl(tile, 0);
-         if (tile >= nrt) break;
-         int ti, tj;
-         packed_tile(tile, ti, tj);
-         // the next tile in the same packed order: next column, or the next row's first
-         const int ti2 = tj + 1 <= ti ? ti : ti + 1, tj2 = tj + 1 <= ti ? tj + 1 : 0;
-         panel_update2<T, LD>(L, o, t1, ti, tj, ti2, tj2, tile + 1 < nrt);
+      if (s >= 0) {
+         // the rest of panel s's update: the lower triangle at t0 + 16, two tiles at a time
+         const int nr = nt - 1, nrt = nr * (nr + 1) / 2, t1 = t0 + 16;
+         for (;;) {
+            int tile = 0;
+            if (lane == 0) tile = atomicAdd(&next_tile, 2);
+            tile = __shfl(tile, 0);
+            if (tile >= nrt) break;
+            int ti, tj;
+            packed_tile(tile, ti, tj);
+            // the next tile in the same packed order: next column, or the next row's first
+            const int ti2 = tj + 1 <= ti ? ti : ti + 1, tj2 = tj + 1 <= ti ? tj + 1 : 0;
+            panel_update2<T, LD>(L, o, t1, ti, tj, ti2, tj2, tile + 1 < nrt);
+         }
       }
       __syncthreads();
    }
@@ -730,27 +739,32 @@ __device__ void lds_trtri_blocked(T* L, T* Tmp, int kb16) {
 // ---------------------------------------------------------------------------
 // The kb16 x kb16 diagonal block into LDS (ld kNB + 1): the lower triangle,
 // identity beyond kb, zero above. The block is rows row0 .. row0 + kb of the
-// panel buffer `rs` (ld elements per column, `bytes` long); 128 bytes of
-// loads in flight per thread.
+// panel buffer `rs` (ld elements per column, `bytes` long). Thread t reads
+// row t % 128 of every other column (t / 128, t / 128 + 2, ...): one load
+// per element, a wave's load covers 64 consecutive rows of a column, 128
+// bytes of loads in flight per thread. The code is short on purpose: the
+// kernels that call this run once per diagonal block, right after other
+// kernels, so every instruction they execute is fetched cold.
 template <typename T>
-__device__ void load_diag_lds(__amdgpu_buffer_rsrc_t rs, uint32_t bytes, int row0, int ld, int kb, int kb16, T* L) {
+__device__ __forceinline__ void load_diag_lds(__amdgpu_buffer_rsrc_t rs, uint32_t bytes, int row0, int ld, int kb, int kb16, T* L) {
    constexpr int LD = kNB + 1;
    constexpr int NU = 128 / sizeof(T);
-   const bool full = kb16 == kNB;  // shifts for the full 128-wide block, the common case
-   for (int e0 = 0; e0 < kb16 * kb16; e0 += NU * kThreads) {
+   constexpr uint32_t ES = sizeof(T);
+   const int i = threadIdx.x & (kNB - 1), jt = threadIdx.x >> 7;
+   if (i >= kb16) return;
+   // rows at or beyond kb read from the end of the buffer (zero)
+   const uint32_t vrow = i < kb ? (uint32_t)(row0 + i) * ES : bytes;
+   for (int u0 = 0; 2 * u0 < kb16; u0 += NU) {
       T v[NU];
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
-         const int e = e0 + u * kThreads + threadIdx.x;
-         const int i = full ? (e & (kNB - 1)) : e % kb16, j = full ? (e >> 7) : e / kb16;
-         const bool ok = e < kb16 * kb16 && i < kb && j < kb && i >= j;
-         v[u] = buf_load(rs, ok ? (uint32_t)(row0 + i + j * ld) * (uint32_t)sizeof(T) : bytes, (T*)nullptr);
+         const int j = jt + 2 * (u0 + u);
+         v[u] = buf_load(rs, j < kb ? vrow + (uint32_t)j * (uint32_t)ld * ES : bytes, (T*)nullptr);
       }
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
-         const int e = e0 + u * kThreads + threadIdx.x;
-         const int i = full ? (e & (kNB - 1)) : e % kb16, j = full ? (e >> 7) : e / kb16;
-         if (e < kb16 * kb16) L[i + j * LD] = (i < kb && j < kb) ? v[u] : ((i == j) ? T(1) : T(0));
+         const int j = jt + 2 * (u0 + u);
+         if (j < kb16) L[i + j * LD] = (i < kb && j < kb) ? (i >= j ? v[u] : T(0)) : T(i == j ? 1 : 0);
       }
    }
 }
@@ -769,18 +783,6 @@ __device__ void diag_block(const DiagTask& t, const NodeDesc& nd, T* __restrict_
    constexpr int LD = kNB + 1;
    const int kb = t.kb;
    const int kb16 = (kb + 15) & ~15;
-   T* blk = arena + nd.off + t.k0 + (int64_t)t.k0 * nd.ld;
-   // element e of the kb16 x kb16 block: row e % kb16, column e / kb16
-   const bool full = kb16 == kNB;
-   auto split = [&](int e, int& i, int& j) {
-      if (full) {
-         i = e & (kNB - 1);
-         j = e >> 7;
-      } else {
-         i = e % kb16;
-         j = e / kb16;
-      }
-   };
    uint32_t pbytes;
    const auto rs = panel_rsrc(arena, nd, t.k0, kb, pbytes);
    load_diag_lds<T>(rs, pbytes, t.k0, nd.ld, kb, kb16, L);
@@ -791,23 +793,23 @@ __device__ void diag_block(const DiagTask& t, const NodeDesc& nd, T* __restrict_
          if (threadIdx.x == 0) info[t.node] = t.k0 + bad;
          return;
       }
-      // 16 LDS reads, then 16 stores, per batch (a loop of single
-      // iterations waits on every LDS read before its store)
-      for (int e0 = 0; e0 < kb16 * kb16; e0 += 16 * kThreads) {
-         T v[16];
+      // the factor back to the front, in the layout of the load: 16 LDS
+      // reads, then 16 stores, per batch (a loop of single iterations waits
+      // on every LDS read before its store); masked stores go past the end
+      {
+         constexpr uint32_t ES = sizeof(T);
+         const int i = threadIdx.x & (kNB - 1), jt = threadIdx.x >> 7;
+         const uint32_t vrow = (uint32_t)(t.k0 + i) * ES;
+         for (int u0 = 0; 2 * u0 < kb16; u0 += 16) {
+            T v[16];
 #pragma unroll
-         for (int u = 0; u < 16; ++u) {
-            const int e = min(e0 + u * kThreads + (int)threadIdx.x, kb16 * kb16 - 1);
-            int i, j;
-            split(e, i, j);
-            v[u] = L[i + j * LD];
-         }
+            for (int u = 0; u < 16; ++u) v[u] = L[i + min(jt + 2 * (u0 + u), kNB - 1) * LD];
 #pragma unroll
-         for (int u = 0; u < 16; ++u) {
-            const int e = e0 + u * kThreads + threadIdx.x;
-            int i, j;
-            split(e, i, j);
-            if (e < kb16 * kb16 && i >= j && i < kb && j < kb) blk[i + (int64_t)j * nd.ld] = v[u];
+            for (int u = 0; u < 16; ++u) {
+               const int j = jt + 2 * (u0 + u);
+               const bool ok = i < kb && j < kb && i >= j;
+               buf_store(rs, ok ? vrow + (uint32_t)j * (uint32_t)nd.ld * ES : pbytes, v[u]);
+            }
          }
       }
       if (NOINV) return;

@@ -498,7 +498,11 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
             case L_ASM: {
                const AsmTask& c = P->asmt[t];
                const NodeDesc& d = P->hnodes[c.node];
-               for (int col = c.c0; col < c.c1; ++col) L.words += d.m - col;  // zero fill
+               // a run staged in LDS writes its columns once and reads each
+               // child entry once; a direct run zero-fills its column, then
+               // reads the child and reads and writes the parent per entry
+               const double per_child = c.pad != 0 ? 1.0 : 3.0;
+               for (int col = c.c0; col < c.c1; ++col) L.words += d.m - col;
                for (int ch = d.first_child; ch != -1; ch = P->hnodes[ch].next_child) {
                   const NodeDesc& dc = P->hnodes[ch];
                   const int q = dc.m - dc.n;
@@ -506,7 +510,7 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
                   const int ja = (int)(std::lower_bound(cm, cm + q, c.c0) - cm);
                   const int jb = (int)(std::lower_bound(cm, cm + q, c.c1) - cm);
                   for (int j = ja; j < jb; ++j) {
-                     L.words += 3.0 * (q - j);  // read child, read+write parent
+                     L.words += per_child * (q - j);
                      L.bytes += 4.0 * (q - j);  // row map
                   }
                }
