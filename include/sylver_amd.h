@@ -172,8 +172,8 @@ typedef struct {
    int strip_trsm;       /* posdef large fronts, levels of at most 64 of them: 1 the panel
                             TRSM by substitution in 64-row strips after a factor-only
                             diagonal launch, the blocks' inverses (for the solves) formed
-                            by the factorization's last launch; 0 the TRSM by the inverse
-                            formed in the diagonal launch; 2 (default) 1 for FP64
+                            by the factorization's last launch (default); 0 the TRSM by
+                            the inverse formed in the diagonal launch; 2 1 for FP64
                             factors, 0 for FP32 ones */
    int level_assembly;   /* posdef fronts not assembled inside their factor launch: 1 a
                             level's assembly in one launch, each workgroup owning a run of

@@ -45,7 +45,7 @@ sylver_amd_options_t& ext_options() {
       d.refine_max_iter = 30;
       d.use_graph = 1;
       d.small_assembly = 0;
-      d.strip_trsm = 2;
+      d.strip_trsm = 1;
       d.level_assembly = 1;
       return d;
    }();
@@ -53,9 +53,9 @@ sylver_amd_options_t& ext_options() {
 }
 
 // Plan-shaping switches of the extension options for a factorization in
-// precision prec (strip_trsm 2: the strip TRSM for FP64 factors only, where
-// the diagonal blocks' inverse costs most -- measured at 80^3: FP64 0.7 ms
-// faster, FP32 0.3 ms slower).
+// precision prec (strip_trsm 1, the default: the strip TRSM for every
+// precision -- measured at 80^3: FP64 1.4 ms, FP32 1.0 ms faster than the
+// TRSM by the inverse; 2: FP64 factors only).
 PlanKnobs plan_knobs(int prec) {
    const sylver_amd_options_t& o = ext_options();
    const bool strip = o.strip_trsm == 1 || (o.strip_trsm == 2 && prec == SYLVER_AMD_PREC_FP64);

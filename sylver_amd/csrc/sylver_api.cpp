@@ -380,7 +380,7 @@ void sylver_amd_default_options(sylver_amd_options_t* o) {
    o->small_front_max = 0;
    o->indef_panel_min = 0;
    o->small_assembly = 0;
-   o->strip_trsm = 2;
+   o->strip_trsm = 1;
    o->level_assembly = 1;
    o->record_timing = 0;
 }

@@ -396,7 +396,7 @@ def test_strip_trsm_vs_separate_launches(S, O, gpu, prec):
             scale = max(1.0, np.abs(c).max())
             assert np.abs(a - c).max() <= tol * scale
     finally:
-        S.set_ext_options(precision=S.PREC_FP64, small_front_max=0, strip_trsm=2, refine=S.REFINE_AUTO)
+        S.set_ext_options(precision=S.PREC_FP64, small_front_max=0, strip_trsm=1, refine=S.REFINE_AUTO)
         S.spldlt_free_akeep(akeep)
 
 
@@ -429,7 +429,7 @@ def test_strip_trsm_not_posdef_then_recovers(S, O, gpu, where):
         assert O.bwd_err(A, x[:, 0], b[:, 0]) <= BWD_TOL
         S.spldlt_free_fkeep(fk)
     finally:
-        S.set_ext_options(small_front_max=0, strip_trsm=2)
+        S.set_ext_options(small_front_max=0, strip_trsm=1)
         S.spldlt_free_akeep(akeep)
 
 
