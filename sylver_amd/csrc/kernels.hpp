@@ -42,7 +42,16 @@ struct NodeDesc {
 struct ColTask { int node, c0, c1, pad; };
 struct InitTask { int node, pad; int64_t p0, p1; };
 struct EaTask { int child, j0, j1, pad; };
-struct AsmTask { int node, c0, c1, pad; };
+// A run of one front's columns [c0, c1) for the level assembly (k_assemble):
+// staged != 0 when it is assembled in LDS; its original entries are the
+// front's sorted A-map pairs [p0, p1); the children's contribution columns
+// that land in it are the ranges asmch[g0 .. g0 + gn) = {child, j0, j1, 0}
+// (columns j0 .. j1 - 1 of the child's contribution block), in child order.
+struct AsmTask {
+   int node, c0, c1, staged;
+   int64_t p0, p1;
+   int g0, gn, pad0, pad1;
+};
 struct DiagTask { int node, k0, kb, pad; };
 struct GemmTask {
    int64_t a_off, b_off, c_off;
@@ -229,34 +238,11 @@ __global__ __launch_bounds__(kThreads) void k_extend_add(const EaTask* __restric
 // is bitwise the same; the barriers between children order their sums
 // (a child's map is injective, two children may meet on an entry). Child
 // maps are increasing (posdef: no delayed rows), so the child columns of a
-// task are a range, found by bisection.
+// task are a range; the plan holds every task's ranges (and its range of
+// original entries), so the kernel starts on its loads at once.
 // ---------------------------------------------------------------------------
-// first index in the increasing a[0..n) with a[i] >= v
-__device__ inline int lower_bound_i(const int* a, int n, int v) {
-   int lo = 0, hi = n;
-   while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (a[mid] < v)
-         lo = mid + 1;
-      else
-         hi = mid;
-   }
-   return lo;
-}
-// first pair p in [lo, hi) of pos (1-based positions, every second entry)
-// with pos >= v
-__device__ inline int64_t lower_bound_pos(const int64_t* pos, int64_t lo, int64_t hi, int64_t v) {
-   while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (pos[2 * mid + 1] < v)
-         lo = mid + 1;
-      else
-         hi = mid;
-   }
-   return lo;
-}
 
-// Runs of at most kAsmLds entries (AsmTask::pad = 1) are assembled in LDS
+// Runs of at most kAsmLds entries (AsmTask::staged = 1) are assembled in LDS
 // and written to the front once: the zero fill and the parent's
 // read-modify-write per child entry stay on chip, the front is written once.
 // The run is its columns' lower trapezoid, column c (rows c .. m-1) at
@@ -271,13 +257,14 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const AsmTask* __restrict
                                                        const int* __restrict__ rlist,
                                                        const double* __restrict__ aval,
                                                        const double* __restrict__ scaling,
-                                                       const int* __restrict__ cmap, T* __restrict__ arena) {
+                                                       const int* __restrict__ cmap, const int4* __restrict__ asmch,
+                                                       T* __restrict__ arena) {
    extern __shared__ __align__(16) unsigned char asm_lds[];
    T* run = reinterpret_cast<T*>(asm_lds);
    const AsmTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-   const bool staged = t.pad != 0;
+   const bool staged = t.staged != 0;
    T* F = arena + nd.off;
    const int m = nd.m, c0 = t.c0;
    // run offset of column c (staged)
@@ -292,30 +279,42 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const AsmTask* __restrict
       }
    }
    __syncthreads();
-   if (c0 < nd.n) {  // original entries lie in the pivot columns only
+   {
+      // original entries: four per thread in flight (pairs, then values)
+      constexpr int AU = 4;
       const int64_t* am = asort + 2 * nd.amap_off;
-      const int64_t p0 = lower_bound_pos(am, 0, nd.num_a, (int64_t)c0 * m + 1);
-      const int64_t p1 = lower_bound_pos(am, p0, nd.num_a, (int64_t)t.c1 * m + 1);
-      for (int64_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
-         const int64_t src = am[2 * p] - 1;
-         const int64_t dest = am[2 * p + 1] - 1;
-         const int c = (int)(dest / m);
-         const int r = (int)(dest - (int64_t)c * m);
-         double v = aval[src];
-         if (scaling) v = scaling[rlist[nd.rlist_off + r] - 1] * v * scaling[rlist[nd.rlist_off + c] - 1];
-         if (staged)
-            run[col_off(c) + (r - c)] = (T)v;
-         else
-            F[(int64_t)c * nd.ld + r] = (T)v;
+      for (int64_t pb = t.p0; pb < t.p1; pb += AU * kThreads) {
+         int64_t src[AU], dest[AU];
+#pragma unroll
+         for (int u = 0; u < AU; ++u) {
+            const int64_t p = min(pb + u * kThreads + (int64_t)threadIdx.x, t.p1 - 1);
+            src[u] = am[2 * p] - 1;
+            dest[u] = am[2 * p + 1] - 1;
+         }
+         double v[AU];
+#pragma unroll
+         for (int u = 0; u < AU; ++u) v[u] = aval[src[u]];
+#pragma unroll
+         for (int u = 0; u < AU; ++u) {
+            if (pb + u * kThreads + (int64_t)threadIdx.x >= t.p1) continue;
+            const int c = (int)(dest[u] / m);
+            const int r = (int)(dest[u] - (int64_t)c * m);
+            double w = v[u];
+            if (scaling) w = scaling[rlist[nd.rlist_off + r] - 1] * w * scaling[rlist[nd.rlist_off + c] - 1];
+            if (staged)
+               run[col_off(c) + (r - c)] = (T)w;
+            else
+               F[(int64_t)c * nd.ld + r] = (T)w;
+         }
       }
    }
-   for (int ch = nd.first_child; ch != -1;) {
-      const NodeDesc cn = nodes[ch];
+   for (int g = t.g0; g < t.g0 + t.gn; ++g) {
+      const int4 cr = asmch[g];
+      const NodeDesc cn = nodes[cr.x];
       __syncthreads();
       const int q = cn.m - cn.n;
       const int* __restrict__ cm = cmap + cn.cmap_off;
-      const int ja = lower_bound_i(cm, q, c0), jb = lower_bound_i(cm, q, t.c1);
-      for (int j = ja + wave; j < jb; j += 4) {
+      for (int j = cr.y + wave; j < cr.z; j += 4) {
          const int pj = cm[j];
          const T* src = arena + cn.off + (int64_t)(cn.n + j) * cn.ld + cn.n;
          if (staged) {
@@ -353,7 +352,6 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const AsmTask* __restrict
             }
          }
       }
-      ch = cn.next_child;
    }
    if (!staged) return;
    __syncthreads();

@@ -207,6 +207,35 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
       }
    };
    int side_last = -1;  // last event recorded on the side stream, not yet waited for
+   // an assembly run's original entries (the front's sorted A-map pairs
+   // whose position falls in its columns) and its child column ranges
+   auto asm_ranges = [&](AsmTask& t) {
+      const NodeDesc& d = P->hnodes[t.node];
+      const int64_t* am = P->hasort.data() + 2 * d.amap_off;
+      auto first_at = [&](int64_t pos) {  // first pair with 1-based position >= pos
+         int64_t lo = 0, hi = d.num_a;
+         while (lo < hi) {
+            const int64_t mid = (lo + hi) / 2;
+            if (am[2 * mid + 1] < pos)
+               lo = mid + 1;
+            else
+               hi = mid;
+         }
+         return lo;
+      };
+      t.p0 = first_at((int64_t)t.c0 * d.m + 1);
+      t.p1 = first_at((int64_t)t.c1 * d.m + 1);
+      t.g0 = (int)P->asmch.size();
+      for (int c = d.first_child; c != -1; c = P->hnodes[c].next_child) {
+         const NodeDesc& cd = P->hnodes[c];
+         const int q = cd.m - cd.n;
+         const int* cm = P->hcmap.data() + P->hcmap_off[c];
+         const int j0 = (int)(std::lower_bound(cm, cm + q, t.c0) - cm);
+         const int j1 = (int)(std::lower_bound(cm, cm + q, t.c1) - cm);
+         if (j0 < j1) P->asmch.push_back(int4{c, j0, j1, 0});
+      }
+      t.gn = (int)P->asmch.size() - t.g0;
+   };
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
       // one-workgroup fronts assembled inside their factor launch
@@ -230,7 +259,13 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
                int c1 = c;
                int64_t area = 0;
                while (c1 < d.m && (c1 == c || area + (d.m - c1) <= kAsmLds)) area += d.m - c1++;
-               P->asmt.push_back(AsmTask{i, c, c1, area <= kAsmLds ? 1 : 0});
+               AsmTask t{};
+               t.node = i;
+               t.c0 = c;
+               t.c1 = c1;
+               t.staged = area <= kAsmLds ? 1 : 0;
+               asm_ranges(t);
+               P->asmt.push_back(t);
                c = c1;
             }
          }
@@ -501,23 +536,20 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
                // a run staged in LDS writes its columns once and reads each
                // child entry once; a direct run zero-fills its column, then
                // reads the child and reads and writes the parent per entry
-               const double per_child = c.pad != 0 ? 1.0 : 3.0;
+               const double per_child = c.staged != 0 ? 1.0 : 3.0;
                for (int col = c.c0; col < c.c1; ++col) L.words += d.m - col;
-               for (int ch = d.first_child; ch != -1; ch = P->hnodes[ch].next_child) {
-                  const NodeDesc& dc = P->hnodes[ch];
-                  const int q = dc.m - dc.n;
-                  const int* cm = P->hcmap.data() + dc.cmap_off;
-                  const int ja = (int)(std::lower_bound(cm, cm + q, c.c0) - cm);
-                  const int jb = (int)(std::lower_bound(cm, cm + q, c.c1) - cm);
-                  for (int j = ja; j < jb; ++j) {
+               for (int g = c.g0; g < c.g0 + c.gn; ++g) {
+                  const int4 cr = P->asmch[g];
+                  const int q = P->hnodes[cr.x].m - P->hnodes[cr.x].n;
+                  for (int j = cr.y; j < cr.z; ++j) {
                      L.words += per_child * (q - j);
                      L.bytes += 4.0 * (q - j);  // row map
                   }
                }
-               if (c.c0 == 0) {  // the front's original entries, counted once
-                  L.words += (double)d.num_a;
-                  L.bytes += 24.0 * (double)d.num_a;
-               }
+               // original entries: the position pair and the FP64 value read
+               // (a direct run also writes each into the front)
+               L.bytes += 24.0 * (double)(c.p1 - c.p0);
+               if (c.staged == 0) L.words += (double)(c.p1 - c.p0);
                break;
             }
             case L_SMALL: {
@@ -896,6 +928,7 @@ void upload_plan(DevicePlan* P, const Analysis& a) {
    P->d_zero = dupload(P->zero);
    P->d_strip = dupload(P->strip);
    P->d_asmt = dupload(P->asmt);
+   P->d_asmch = dupload(P->asmch);
    if (P->level_asm) P->d_asort = dupload(P->hasort);
    P->d_init = dupload(P->init);
    P->d_ea = dupload(P->ea);
@@ -1038,7 +1071,7 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
                      }
                      const NodeDesc& d = P->hnodes[c.node];
                      if (c.c0 < 0 || c.c0 >= c.c1 || c.c1 > d.m) fail("assembly task: columns");
-                     if (c.pad != 0) {  // assembled in LDS: the run's trapezoid must fit
+                     if (c.staged != 0) {  // assembled in LDS: the run's trapezoid must fit
                         int64_t area = 0;
                         for (int col = c.c0; col < c.c1 && col < d.m; ++col) area += d.m - col;
                         if (area > kAsmLds) fail("assembly task: run larger than its LDS");
@@ -1047,8 +1080,50 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
                      }
                      if (next_col[c.node] != c.c0) fail("assembly task: column runs");
                      next_col[c.node] = c.c1;
-                     if (d.amap_off < 0 || (d.amap_off + d.num_a) * 2 > (int64_t)P->hasort.size())
+                     if (d.amap_off < 0 || (d.amap_off + d.num_a) * 2 > (int64_t)P->hasort.size()) {
                         fail("assembly task: A-map");
+                     } else if (c.p0 < 0 || c.p0 > c.p1 || c.p1 > d.num_a) {
+                        fail("assembly task: original-entry range");
+                     } else {
+                        // exactly the entries of the run's columns, each in its lower trapezoid
+                        const int64_t* am = P->hasort.data() + 2 * d.amap_off;
+                        const int64_t lo = (int64_t)c.c0 * d.m, hi = (int64_t)c.c1 * d.m;
+                        if ((c.p0 > 0 && am[2 * (c.p0 - 1) + 1] - 1 >= lo) ||
+                            (c.p1 < d.num_a && am[2 * c.p1 + 1] - 1 < hi))
+                           fail("assembly task: original-entry range not exact");
+                        for (int64_t p = c.p0; p < c.p1; ++p) {
+                           const int64_t pos = am[2 * p + 1] - 1;
+                           const int64_t col = pos / d.m, row = pos - col * d.m;
+                           if (pos < lo || pos >= hi || row < col) {
+                              fail("assembly task: original entry outside the run");
+                              break;
+                           }
+                        }
+                     }
+                     // the child ranges: each child of the front with columns in
+                     // the run, in list order, its columns landing in [c0, c1)
+                     if (c.g0 < 0 || c.gn < 0 || (int64_t)c.g0 + c.gn > (int64_t)P->asmch.size()) {
+                        fail("assembly task: child ranges");
+                     } else {
+                        int g = c.g0;
+                        for (int ch = d.first_child, guard = 0; ch >= 0 && ch < nn && guard <= nn;
+                             ch = P->hnodes[ch].next_child, ++guard) {
+                           const NodeDesc& dc = P->hnodes[ch];
+                           const int q = dc.m - dc.n;
+                           if (dc.cmap_off < 0 || dc.cmap_off + q > (int64_t)P->hcmap.size()) break;
+                           const int* cm = P->hcmap.data() + dc.cmap_off;
+                           const int j0 = (int)(std::lower_bound(cm, cm + q, c.c0) - cm);
+                           const int j1 = (int)(std::lower_bound(cm, cm + q, c.c1) - cm);
+                           if (j0 >= j1) continue;
+                           if (g >= c.g0 + c.gn || P->asmch[g].x != ch || P->asmch[g].y != j0 ||
+                               P->asmch[g].z != j1) {
+                              fail("assembly task: child range");
+                              break;
+                           }
+                           ++g;
+                        }
+                        if (g != c.g0 + c.gn) fail("assembly task: child range count");
+                     }
                      if (seen[c.node]) continue;  // the children once per front
                      seen[c.node] = 1;
                      int guard = 0;
@@ -1235,7 +1310,7 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
          case L_ASM:
             k_assemble<T><<<g, b, kAsmLds * sizeof(T), s>>>(P->d_asmt + L.off, P->d_nodes, P->d_asort, P->d_rlist,
                                                              F->d_aval, F->has_scaling ? F->d_scaling : nullptr,
-                                                             P->d_cmap, arena);
+                                                             P->d_cmap, P->d_asmch, arena);
             break;
          case L_SMALL:
             k_small_front<T><<<g, b, 0, s>>>(P->d_nlistv + L.off, P->d_nodes, arena, F->d_info);

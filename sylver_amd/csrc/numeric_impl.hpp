@@ -144,6 +144,7 @@ struct DevicePlan {
    std::vector<DiagTask> diag;
    std::vector<StripTask> strip;
    std::vector<AsmTask> asmt;
+   std::vector<int4> asmch;      // child column ranges of the assembly runs
    std::vector<int64_t> hasort;  // each front's A-map pairs sorted by position (L_ASM)
    std::vector<GemmTask> gemm;
    std::vector<SolveTask> stask;
@@ -158,6 +159,7 @@ struct DevicePlan {
    DiagTask* d_diag = nullptr;
    StripTask* d_strip = nullptr;
    AsmTask* d_asmt = nullptr;
+   int4* d_asmch = nullptr;
    int64_t* d_asort = nullptr;
    GemmTask* d_gemm = nullptr;
    SolveTask* d_stask = nullptr;
@@ -184,7 +186,7 @@ struct DevicePlan {
       for (void* p : {(void*)d_sptr, (void*)d_sparent, (void*)d_rptr, (void*)d_rlist,
                       (void*)d_nptr, (void*)d_nlist, (void*)d_cmap, (void*)d_cmap_off,
                       (void*)d_nodes, (void*)d_zero, (void*)d_init, (void*)d_ea,
-                      (void*)d_nlistv, (void*)d_diag, (void*)d_strip, (void*)d_asmt, (void*)d_asort, (void*)d_gemm, (void*)d_stask, (void*)d_gch, (void*)d_ctask, (void*)d_cgch,
+                      (void*)d_nlistv, (void*)d_diag, (void*)d_strip, (void*)d_asmt, (void*)d_asmch, (void*)d_asort, (void*)d_gemm, (void*)d_stask, (void*)d_gch, (void*)d_ctask, (void*)d_cgch,
                       (void*)d_csr_ptr, (void*)d_csr_col, (void*)d_csr_src, (void*)d_invp})
          dfree(p);
    }
