@@ -59,7 +59,8 @@ sylver_amd_options_t& ext_options() {
 PlanKnobs plan_knobs(int prec) {
    const sylver_amd_options_t& o = ext_options();
    const bool strip = o.strip_trsm == 1 || (o.strip_trsm == 2 && prec == SYLVER_AMD_PREC_FP64);
-   return PlanKnobs{o.small_assembly != 0, strip, o.level_assembly != 0};
+   const int es = prec == SYLVER_AMD_PREC_FP64 ? 8 : 4;  // bytes per factor entry
+   return PlanKnobs{o.small_assembly != 0, strip, o.level_assembly != 0, kAsmLdsBytes / es};
 }
 
 int device_count() {
@@ -97,6 +98,7 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
    P->fuse_small = k.fuse_small && a.fixed_off.empty();
    P->strip_trsm = k.strip_trsm;
    P->level_asm = k.level_asm;
+   P->asm_elems = k.asm_elems;
    P->variant_gen = a.variant_gen;
    const int nn = a.nnodes;
    // one-workgroup kernel or tiled path for front i
@@ -248,7 +250,7 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
       }
       if (P->level_asm) {
          // one launch: each task owns a run of one front's columns (zero, A
-         // entries, children) of at most kAsmLds lower-triangle entries,
+         // entries, children) of at most asm_elems lower-triangle entries,
          // assembled in LDS (pad = 1); a longer single column works on the
          // front directly (pad = 0)
          int64_t a0 = P->asmt.size();
@@ -258,12 +260,12 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
             for (int c = 0; c < d.m;) {
                int c1 = c;
                int64_t area = 0;
-               while (c1 < d.m && (c1 == c || area + (d.m - c1) <= kAsmLds)) area += d.m - c1++;
+               while (c1 < d.m && (c1 == c || area + (d.m - c1) <= P->asm_elems)) area += d.m - c1++;
                AsmTask t{};
                t.node = i;
                t.c0 = c;
                t.c1 = c1;
-               t.staged = area <= kAsmLds ? 1 : 0;
+               t.staged = area <= P->asm_elems ? 1 : 0;
                asm_ranges(t);
                P->asmt.push_back(t);
                c = c1;
@@ -1074,7 +1076,7 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
                      if (c.staged != 0) {  // assembled in LDS: the run's trapezoid must fit
                         int64_t area = 0;
                         for (int col = c.c0; col < c.c1 && col < d.m; ++col) area += d.m - col;
-                        if (area > kAsmLds) fail("assembly task: run larger than its LDS");
+                        if (area > P->asm_elems) fail("assembly task: run larger than its LDS");
                      } else if (c.c1 != c.c0 + 1) {
                         fail("assembly task: a direct run is one column");
                      }
@@ -1308,7 +1310,7 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
             k_extend_add<T><<<g, b, 0, s>>>(P->d_ea + L.off, P->d_nodes, P->d_cmap, arena);
             break;
          case L_ASM:
-            k_assemble<T><<<g, b, kAsmLds * sizeof(T), s>>>(P->d_asmt + L.off, P->d_nodes, P->d_asort, P->d_rlist,
+            k_assemble<T><<<g, b, P->asm_elems * sizeof(T), s>>>(P->d_asmt + L.off, P->d_nodes, P->d_asort, P->d_rlist,
                                                              F->d_aval, F->has_scaling ? F->d_scaling : nullptr,
                                                              P->d_cmap, P->d_asmch, arena);
             break;
@@ -1551,7 +1553,8 @@ FactorResult device_factor(const Analysis& a, PlanPtr* plan, DeviceFactors** fac
       const PlanKnobs knobs = plan_knobs(prec);
       if (*plan && ((*plan)->small_max != small_max || (*plan)->wide_ncol != wide_ncol ||
                     (*plan)->fuse_small != knobs.fuse_small || (*plan)->strip_trsm != knobs.strip_trsm ||
-                    (*plan)->level_asm != knobs.level_asm || (*plan)->variant_gen != a.variant_gen))
+                    (*plan)->level_asm != knobs.level_asm || (*plan)->asm_elems != knobs.asm_elems ||
+                    (*plan)->variant_gen != a.variant_gen))
          plan->reset();
       if (!*plan) plan->reset(build_plan(a, small_max, wide_ncol, knobs));
       DevicePlan* P = plan->get();

@@ -100,6 +100,7 @@ struct PlanKnobs {
    bool fuse_small = false;  // one-workgroup fronts assembled inside their factor launch
    bool strip_trsm = false;  // panel TRSM by substitution after a factor-only diagonal launch
    bool level_asm = false;   // a level's assembly in one launch by receiving columns
+   int asm_elems = kAsmLdsBytes / 4;  // entries of an assembly run staged in LDS
 };
 PlanKnobs plan_knobs(int prec);
 
@@ -120,6 +121,7 @@ struct DevicePlan {
    bool fuse_small = false;  // one-workgroup fronts assembled inside their factor launch (L_SMALL_ASM)
    bool strip_trsm = false;  // large-front blocks of few fronts: L_DFACT + L_STRIP (no inverse on the chain)
    bool level_asm = false;   // assembly by receiving columns, one launch per level (L_ASM)
+   int asm_elems = 0;        // entries of an L_ASM run staged in LDS
    int wide_ncol = 1024;  // fronts with more pivots take the wide-front update launches
    std::vector<NodeDesc> hnodes;
    std::vector<std::vector<int>> levels;
