@@ -493,8 +493,10 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
       }
    }
    // inverted diagonal blocks of the small fronts and of the fused block
-   // steps (the separate DIAG launches write their own)
-   {
+   // steps (the separate DIAG launches write their own), for the solves;
+   // fronts at caller offsets (the batched entry points) are factored, never
+   // solved, so they go without
+   if (a.fixed_off.empty()) {
       int64_t d0 = P->diag.size();
       P->diag.insert(P->diag.end(), step_dinv.begin(), step_dinv.end());
       for (int L = 0; L < nlev; ++L)
