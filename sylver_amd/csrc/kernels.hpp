@@ -657,7 +657,7 @@ __device__ void panel_update2(T* L, int o, int t0, int ti, int tj, int ti2, int 
 // per panel. All threads call; returns 0 or the 1-based column of the first
 // non-positive pivot.
 template <typename T, int LD>
-__device__ int lds_potrf_panels(T* L, int kb16, int* sh_bad) {
+__device__ __forceinline__ int lds_potrf_panels(T* L, int kb16, int* sh_bad) {
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int nbk = kb16 / 16;
    __shared__ int next_tile;
@@ -865,9 +865,8 @@ __global__ __launch_bounds__(kThreads) void k_diag(const DiagTask* __restrict__ 
 
 // Factor only (the blocks whose panel the strip TRSM below solves).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_diag_factor(const DiagTask* __restrict__ tasks,
-                                                          const NodeDesc* __restrict__ nodes,
-                                                          T* __restrict__ arena, int* __restrict__ info) {
+__device__ __forceinline__ void diag_factor_task(const DiagTask* __restrict__ tasks, const NodeDesc* __restrict__ nodes,
+                                                 T* __restrict__ arena, int* __restrict__ info) {
    constexpr int LD = kNB + 1;
    __shared__ T L[kNB * LD];
    __shared__ int sh_bad;
@@ -875,6 +874,21 @@ __global__ __launch_bounds__(kThreads) void k_diag_factor(const DiagTask* __rest
    const DiagTask t = tasks[blockIdx.x];
    const NodeDesc nd = nodes[t.node];
    diag_block<T, true>(t, nd, arena, nullptr, nullptr, info, 1, L, nullptr, &sh_bad);
+}
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_diag_factor(const DiagTask* __restrict__ tasks,
+                                                          const NodeDesc* __restrict__ nodes,
+                                                          T* __restrict__ arena, int* __restrict__ info) {
+   diag_factor_task<T>(tasks, nodes, arena, info);
+}
+// FP32, within 104 VGPRs: with its 66 KB of LDS it then fits on a CU beside
+// two workgroups of the FP32 Schur update (2 x 200 registers per SIMD lane,
+// 2 x 33 KB), so the critical path's diagonal block starts at once instead of
+// waiting for a CU to drain
+static __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_num_vgpr(104))) void k_diag_factor_f32(
+    const DiagTask* __restrict__ tasks, const NodeDesc* __restrict__ nodes, float* __restrict__ arena,
+    int* __restrict__ info) {
+   diag_factor_task<float>(tasks, nodes, arena, info);
 }
 
 // ---------------------------------------------------------------------------

@@ -1347,7 +1347,10 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
             k_diag<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, dinv, dinvT, F->d_info, 0);
             break;
          case L_DFACT:
-            k_diag_factor<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, F->d_info);
+            if constexpr (sizeof(T) == 4)
+               k_diag_factor_f32<<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, F->d_info);
+            else
+               k_diag_factor<T><<<g, b, 0, s>>>(P->d_diag + L.off, P->d_nodes, arena, F->d_info);
             break;
          case L_STRIP:
             k_strip<T><<<g, b, 0, s>>>(P->d_strip + L.off, P->d_nodes, arena);
