@@ -384,45 +384,6 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Cholesky of a 16x16 block at D (ld LD) by one wave: lane i < 16 holds row
-// i in registers; the current column j is read from lanes j..15 with
-// v_readlane, so a step waits on no memory round trip. Step j, rows i > j:
-// l_ij = a_ij / sqrt(d), a_ic -= (l_ij / sqrt(d)) a_cj for c > j -- one FMA
-// per element with the broadcast a_cj. Every lane updates every element:
-// the ones above the diagonal (and lanes past 16) take values that are
-// never read or stored, which saves a compare and a select per element.
-// Returns 0 or the 1-based column of the first non-positive pivot.
-template <typename T, int LD>
-__device__ int wave_potrf16(T* D) {
-   const int lane = threadIdx.x & 63;
-   const bool own = lane < 16;
-   T r[16];
-#pragma unroll
-   for (int c = 0; c < 16; ++c) r[c] = (own && c <= lane) ? D[lane + c * LD] : T(0);
-   int bad = 0;
-#pragma unroll
-   for (int j = 0; j < 16; ++j) {
-      T col[16];
-#pragma unroll
-      for (int c = j; c < 16; ++c) col[c] = lane_bcast(r[j], c);
-      const T d = col[j];
-      if (bad == 0 && !(d > T(0))) bad = j + 1;
-      if (bad == 0) {
-         const T inv = rsqrt_nr(d);
-         const T lj = r[j] * inv;  // l_ij for rows below j
-         const T t = lj * inv;
-         r[j] = (lane == j) ? d * inv : lj;
-#pragma unroll
-         for (int c = j + 1; c < 16; ++c) r[c] = fma(-t, col[c], r[c]);
-      }
-   }
-   if (own)
-#pragma unroll
-      for (int c = 0; c < 16; ++c)
-         if (c <= lane) D[lane + c * LD] = r[c];
-   return bad;
-}
-
 // In-place inverse of a 16x16 lower triangular block by one wave: lane j < 16
 // computes column j of the inverse by forward substitution.
 template <typename T, int LD>
@@ -462,112 +423,14 @@ __device__ inline typename Mfma<T>::acc_t lds_mma16(const T* A, int lda, const T
    return acc;
 }
 
-// Blocked Cholesky of the padded kb16 x kb16 block in LDS. All threads call.
-// Per 16-column step s: the panel rows below the diagonal block (one row per
-// thread, right-looking so each row's dependency chain is 16 steps), then
-// wave 0 updates the next diagonal block and factors it while the other
-// waves apply the rest of the trailing update with the same panel
-// (look-ahead); tiles are handed out through an LDS counter, so wave 0 joins
-// in when its factorization is done.
-template <typename T, int LD>
-__device__ int lds_potrf_blocked(T* L, int kb16, int* sh_bad) {
-   typedef typename Mfma<T>::acc_t acc_t;
-   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-   const int nbk = kb16 / 16;
-   __shared__ T Rcp[16];
-   __shared__ int next_tile;
-   auto factor_diag = [&](int o) {  // wave 0: Cholesky of the 16x16 block at (o, o)
-      const int bad = wave_potrf16<T, LD>(L + o + o * LD);
-      if (lane == 0) *sh_bad = bad ? o + bad : 0;
-      if (lane < 16) Rcp[lane] = T(1) / L[(o + lane) + (o + lane) * LD];
-   };
-   // lower 16x16 tile (ti, tj) of the trailing block at t0 minus panel o's product
-   auto update_tile = [&](int o, int t0, int ti, int tj) {
-      acc_t acc = acc_t{0, 0, 0, 0};
-      acc = lds_mma16<T, true>(L + t0 + ti * 16 + o * LD, LD, L + t0 + tj * 16 + o * LD, LD, acc);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-         const int i = t0 + ti * 16 + Mfma<T>::row(lane, v);
-         const int j = t0 + tj * 16 + (lane & 15);
-         if (i >= j) L[i + j * LD] -= acc[v];
-      }
-   };
-   // two tiles at once: both products are formed before either is applied,
-   // so their LDS and MFMA latencies overlap (tiles are disjoint)
-   auto update_tile2 = [&](int o, int t0, int ti, int tj, int ti2, int tj2) {
-      acc_t acc = acc_t{0, 0, 0, 0}, acc2 = acc_t{0, 0, 0, 0};
-      acc = lds_mma16<T, true>(L + t0 + ti * 16 + o * LD, LD, L + t0 + tj * 16 + o * LD, LD, acc);
-      acc2 = lds_mma16<T, true>(L + t0 + ti2 * 16 + o * LD, LD, L + t0 + tj2 * 16 + o * LD, LD, acc2);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-         const int i = t0 + ti * 16 + Mfma<T>::row(lane, v), j = t0 + tj * 16 + (lane & 15);
-         const int i2 = t0 + ti2 * 16 + Mfma<T>::row(lane, v), j2 = t0 + tj2 * 16 + (lane & 15);
-         if (i >= j) L[i + j * LD] -= acc[v];
-         if (i2 >= j2) L[i2 + j2 * LD] -= acc2[v];
-      }
-   };
-   if (wave == 0) factor_diag(0);
-   __syncthreads();
-   for (int s = 0; s < nbk; ++s) {
-      const int o = s * 16;
-      if (*sh_bad) return *sh_bad;
-      // panel rows below the diagonal block: X = P * inv(D)^T, one row per thread
-      if (tid == 0) next_tile = 1;
-      const int r = o + 16 + tid;
-      if (r < kb16) {
-         T x[16];
-#pragma unroll
-         for (int c = 0; c < 16; ++c) x[c] = L[r + (o + c) * LD];
-#pragma unroll
-         for (int c = 0; c < 16; ++c) {
-            x[c] *= Rcp[c];
-#pragma unroll
-            for (int cc = c + 1; cc < 16; ++cc) x[cc] -= x[c] * L[(o + cc) + (o + c) * LD];
-         }
-#pragma unroll
-         for (int c = 0; c < 16; ++c) L[r + (o + c) * LD] = x[c];
-      }
-      __syncthreads();
-      if (s + 1 == nbk) break;
-      const int t0 = o + 16;
-      const int nt = (kb16 - t0) / 16;
-      const int ntiles = nt * (nt + 1) / 2;
-      if (wave == 0) {
-         // the next diagonal block first, then its factorization
-         update_tile(o, t0, 0, 0);
-         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-         __builtin_amdgcn_wave_barrier();
-         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-         factor_diag(t0);
-      }
-      // the other tiles (1..ntiles-1) go to whichever wave is free, two at a time
-      for (;;) {
-         int tile = 0;
-         if (lane == 0) tile = atomicAdd(&next_tile, 2);
-         tile = __shfl(tile, 0);
-         if (tile >= ntiles) break;
-         int ti = 0;
-         while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
-         const int tj = tile - ti * (ti + 1) / 2;
-         if (tile + 1 < ntiles) {
-            // the next tile in the same packed order: next column, or the next row's first
-            const int ti2 = tj + 1 <= ti ? ti : ti + 1, tj2 = tj + 1 <= ti ? tj + 1 : 0;
-            update_tile2(o, t0, ti, tj, ti2, tj2);
-         } else {
-            update_tile(o, t0, ti, tj);
-         }
-      }
-      __syncthreads();
-   }
-   return 0;
-}
-
 // Cholesky of a 16-column panel in LDS by one wave: lanes 0..15 hold the
 // rows of the diagonal block, lanes 16..63 48 rows below it (rows o + 16 +
 // 48 w + ...; w = the wave's index among the panel's waves), each row in
-// registers. The column steps are those of wave_potrf16 -- the rows below
-// the diagonal block are simply more rows of the same elimination -- so a
-// panel of up to 64 rows takes the time of the 16x16 block alone. Every wave
+// registers. Column step j: l_ij = a_ij / sqrt(d), a_ic -= (l_ij / sqrt(d))
+// a_cj for c > j -- one FMA per element with a_cj broadcast from lane c by
+// v_readlane, so a step waits on no memory round trip. The rows below the
+// diagonal block are simply more rows of the same elimination, so a panel
+// of up to 64 rows takes the time of the 16x16 block alone. Every wave
 // of a panel factors the diagonal block (the same values, the same
 // operations, so the same results); wave 0 stores it.
 template <typename T, int LD>
@@ -1281,7 +1144,7 @@ __device__ void front_potrf_block(const NodeDesc& nd, T* __restrict__ F, int* in
          D[i + j * LD] = live ? v : ((i == j) ? T(1) : T(0));
       }
       __syncthreads();
-      const int bad = lds_potrf_blocked<T, LD>(D, kb16, &sh_bad);
+      const int bad = lds_potrf_panels<T, LD>(D, kb16, &sh_bad);
       if (bad) {
          if (tid == 0) *info_slot = k0 + bad;
          return;
