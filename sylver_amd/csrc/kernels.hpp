@@ -868,20 +868,21 @@ __device__ inline int xcd_task(int b, int G) { return (b & 7) * (G >> 3) + (b >>
 // One tile of a GEMM task: C (ldc, element type TC) -= A B^T, or C = A B^T
 // for MODE 1; A and B of type T (leading dimensions t.lda, t.ldb), products
 // on T's MFMA. `vec`: A, B and their leading dimensions allow vector loads.
-template <typename T, int MODE, int TB, int TBN, typename TC>
+template <typename T, int MODE, int TB, int TBN, typename TC, int NT = kThreads>
 __device__ __forceinline__ void gemm_tile(const GemmTask& t, const T* __restrict__ a, const T* __restrict__ b,
                                           TC* __restrict__ c, int ldc, bool vec) {
    constexpr int BM = TB, BN = TBN, BK = sizeof(T) == 8 ? 32 : 16, PAD = 4;  // FP64: longer MFMA chains per stage
-   constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+   constexpr int WGN = NT / 128;  // waves: 2 along the rows, NT / 128 along the columns
+   constexpr int WM = BM / 2, WN = BN / WGN, FM = WM / 16, FN = WN / 16;
    typedef typename V16<T>::vt vt;
    constexpr int VN = V16<T>::N;
-   constexpr int AV = BM * BK / VN / kThreads;
-   constexpr int BV = BN * BK / VN / kThreads;
+   constexpr int AV = BM * BK / VN / NT;
+   constexpr int BV = BN * BK / VN / NT;
    typedef typename Mfma<T>::acc_t acc_t;
    __shared__ T As[2][BK][BM + PAD];
    __shared__ T Bs[2][BK][BN + PAD];
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-   const int wm = wave >> 1, wn = wave & 1;
+   const int wm = wave / WGN, wn = wave % WGN;
    const int ld = t.lda, ldb = t.ldb;
    acc_t acc[FM][FN];
 #pragma unroll
@@ -893,14 +894,14 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, const T* __restrict
       if (vec) {
 #pragma unroll
          for (int u = 0; u < AV; ++u) {
-            const int v = tid + u * kThreads;
+            const int v = tid + u * NT;
             const int kk = v / (BM / VN), row = (v % (BM / VN)) * VN;
             ra[u] = *live_or_zero(k0 + kk < t.K && row < t.M,
                                   reinterpret_cast<const vt*>(a + row + (int64_t)(k0 + kk) * ld));
          }
 #pragma unroll
          for (int u = 0; u < BV; ++u) {
-            const int v = tid + u * kThreads;
+            const int v = tid + u * NT;
             const int kk = v / (BN / VN), row = (v % (BN / VN)) * VN;
             rb[u] = *live_or_zero(k0 + kk < t.K && row < t.N,
                                   reinterpret_cast<const vt*>(b + row + (int64_t)(k0 + kk) * ldb));
@@ -908,7 +909,7 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, const T* __restrict
       } else {
 #pragma unroll
          for (int u = 0; u < AV; ++u) {
-            const int v = tid + u * kThreads;
+            const int v = tid + u * NT;
             const int kk = v / (BM / VN), row = (v % (BM / VN)) * VN;
 #pragma unroll
             for (int e = 0; e < VN; ++e)
@@ -916,7 +917,7 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, const T* __restrict
          }
 #pragma unroll
          for (int u = 0; u < BV; ++u) {
-            const int v = tid + u * kThreads;
+            const int v = tid + u * NT;
             const int kk = v / (BN / VN), row = (v % (BN / VN)) * VN;
 #pragma unroll
             for (int e = 0; e < VN; ++e)
@@ -930,14 +931,14 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, const T* __restrict
       const int buf = kc & 1;
 #pragma unroll
       for (int u = 0; u < AV; ++u) {
-         const int v = tid + u * kThreads;
+         const int v = tid + u * NT;
          const int kk = v / (BM / VN), row = (v % (BM / VN)) * VN;
 #pragma unroll
          for (int e = 0; e < VN; ++e) As[buf][kk][row + e] = ra[u][e];
       }
 #pragma unroll
       for (int u = 0; u < BV; ++u) {
-         const int v = tid + u * kThreads;
+         const int v = tid + u * NT;
          const int kk = v / (BN / VN), row = (v % (BN / VN)) * VN;
 #pragma unroll
          for (int e = 0; e < VN; ++e) Bs[buf][kk][row + e] = rb[u][e];
@@ -986,10 +987,13 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, const T* __restrict
    }
 }
 
-template <typename T, int MODE, int TB = 128, int TBN = TB>
-__global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict__ tasks,
-                                                     T* __restrict__ arena,
-                                                     const T* __restrict__ bsrc) {
+// NT threads per workgroup: 256 (2 x 2 waves), or 512 (2 x 4 waves, 64 x 32
+// per wave) for the FP64 far updates, whose 135 KB of LDS allow one
+// workgroup per CU: eight waves then keep two per SIMD.
+template <typename T, int MODE, int TB = 128, int TBN = TB, int NT = kThreads>
+__global__ __launch_bounds__(NT) void k_update(const GemmTask* __restrict__ tasks,
+                                               T* __restrict__ arena,
+                                               const T* __restrict__ bsrc) {
    // the panel TRSM and the next pair's columns (64-row tiles) are on the
    // critical path; the far look-ahead updates (128 x 128) are not
    if (TB == 64) __builtin_amdgcn_s_setprio(2);
@@ -997,7 +1001,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const GemmTask* __restrict_
    if (t.M <= 0) return;
    constexpr int VN = V16<T>::N;
    const bool vec = ((t.a_off | t.b_off | (int64_t)t.lda | (int64_t)t.ldb) % VN) == 0;
-   gemm_tile<T, MODE, TB, TBN, T>(t, arena + t.a_off, (MODE == 0 ? arena : bsrc) + t.b_off, arena + t.c_off, t.lda,
+   gemm_tile<T, MODE, TB, TBN, T, NT>(t, arena + t.a_off, (MODE == 0 ? arena : bsrc) + t.b_off, arena + t.c_off, t.lda,
                                   vec);
 }
 
