@@ -988,8 +988,9 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, const T* __restrict
 }
 
 // NT threads per workgroup: 256 (2 x 2 waves), or 512 (2 x 4 waves, 64 x 32
-// per wave) for the FP64 far updates, whose 135 KB of LDS allow one
-// workgroup per CU: eight waves then keep two per SIMD.
+// per wave) for the far updates: FP64's 135 KB of LDS allow one workgroup
+// per CU, and eight waves then keep two per SIMD; FP32 goes from two to four
+// waves per SIMD (measured at 80^3: FP64 factor 3.7 ms, FP32 0.4 ms faster).
 template <typename T, int MODE, int TB = 128, int TBN = TB, int NT = kThreads>
 __global__ __launch_bounds__(NT) void k_update(const GemmTask* __restrict__ tasks,
                                                T* __restrict__ arena,
