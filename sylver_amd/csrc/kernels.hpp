@@ -1006,6 +1006,18 @@ __global__ __launch_bounds__(NT) void k_update(const GemmTask* __restrict__ task
                                   vec);
 }
 
+// FP32 far updates within 96 VGPRs: two of these eight-wave workgroups per CU
+// (4 waves x 96 registers per SIMD lane) leave room for the FP32 diagonal
+// factor kernel (108 registers) beside them.
+static __global__ __launch_bounds__(2 * kThreads) __attribute__((amdgpu_num_vgpr(96))) void k_update_far_f32(
+    const GemmTask* __restrict__ tasks, float* __restrict__ arena) {
+   const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
+   if (t.M <= 0) return;
+   const bool vec = ((t.a_off | t.b_off | (int64_t)t.lda | (int64_t)t.ldb) % 4) == 0;
+   gemm_tile<float, 0, 128, 128, float, 2 * kThreads>(t, arena + t.a_off, arena + t.b_off, arena + t.c_off, t.lda,
+                                                     vec);
+}
+
 // Contribution-block update of the indefinite path's FP32 tier: CB -= L21
 // (L21 D)^T with both operands rounded to FP32 in the workspace (offsets in
 // floats), FP32 MFMA accumulation, the FP64 contribution block in the arena

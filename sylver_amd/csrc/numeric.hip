@@ -1335,6 +1335,8 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
                k_update_bf16<<<g, b, 0, s>>>(P->d_gemm + L.off, reinterpret_cast<float*>(arena));
             else if (L.tile == 64)
                k_update<T, 0, 64><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, nullptr);
+            else if constexpr (sizeof(T) == 4)
+               k_update_far_f32<<<g, dim3(2 * kThreads), 0, s>>>(P->d_gemm + L.off, arena);
             else
                k_update<T, 0, 128, 128, 2 * kThreads><<<g, dim3(2 * kThreads), 0, s>>>(P->d_gemm + L.off, arena, nullptr);
             break;
