@@ -459,11 +459,12 @@ void factor_indef(const Analysis& a, const DevicePlan* P, DeviceFactors* F, cons
          dim3 g(64, (unsigned)cb_nodes.size());
          if (low) {
             k_ind_ld32<<<g, kThreads, 0, s>>>(S.d_list, S.d_nodes, S.arena, S.d_dinv, S.d_ws);
-            k_update_cb32<<<(unsigned)tasks.size(), kThreads, 0, s>>>(S.d_tasks, reinterpret_cast<const float*>(S.d_ws),
+            k_update_cb32<<<(unsigned)tasks.size(), 2 * kThreads, 0, s>>>(S.d_tasks, reinterpret_cast<const float*>(S.d_ws),
                                                                       S.arena);
          } else {
             k_ind_ld<<<g, kThreads, 0, s>>>(S.d_list, S.d_nodes, S.arena, S.d_dinv, S.d_ws);
-            k_update<double, 2><<<(unsigned)tasks.size(), kThreads, 0, s>>>(S.d_tasks, S.arena, S.d_ws);
+            k_update<double, 2, 128, 128, 2 * kThreads><<<(unsigned)tasks.size(), 2 * kThreads, 0, s>>>(S.d_tasks, S.arena,
+                                                                                                       S.d_ws);
          }
          HIPCHECK(hipStreamSynchronize(s));  // the task and list buffers are rewritten next level
       }

@@ -1022,13 +1022,13 @@ static __global__ __launch_bounds__(2 * kThreads) __attribute__((amdgpu_num_vgpr
 // (L21 D)^T with both operands rounded to FP32 in the workspace (offsets in
 // floats), FP32 MFMA accumulation, the FP64 contribution block in the arena
 // (ldc = t.ldc).
-static __global__ __launch_bounds__(kThreads) void k_update_cb32(const GemmTask* __restrict__ tasks,
-                                                          const float* __restrict__ ws,
-                                                          double* __restrict__ arena) {
+static __global__ __launch_bounds__(2 * kThreads) void k_update_cb32(const GemmTask* __restrict__ tasks,
+                                                              const float* __restrict__ ws,
+                                                              double* __restrict__ arena) {
    const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
    if (t.M <= 0) return;
    const bool vec = ((t.a_off | t.b_off | (int64_t)t.lda | (int64_t)t.ldb) % V16<float>::N) == 0;
-   gemm_tile<float, 2, 128, 128, double>(t, ws + t.a_off, ws + t.b_off, arena + t.c_off, t.ldc, vec);
+   gemm_tile<float, 2, 128, 128, double, 2 * kThreads>(t, ws + t.a_off, ws + t.b_off, arena + t.c_off, t.ldc, vec);
 }
 
 // ---------------------------------------------------------------------------
