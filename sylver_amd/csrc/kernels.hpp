@@ -1623,9 +1623,7 @@ __global__ __launch_bounds__(kThreads) void k_solve_step1(const NodeDesc nd, int
 //     x_j -= L_jk y_k for k = 0 .. j-1 (ascending), y_j = inv(L_jj) x_j.
 //   Forward, contribution block (rows at or below n): x -= L_rk y_k for
 //     every k -- in the chained launch, after every diagonal block of the
-//     level, where they all fit on the device at once (their work then
-//     hides behind the chains), else in a launch of their own after it
-//     (k_fcb).
+//     level (their work hides behind the chains).
 //   Backward, diagonal block j: x_j -= the partial sums of L21^T x2 (the
 //     contribution-block product, computed by the launch before, summed in
 //     slot order), x_j -= L_ij^T x_i for i = nb-1 .. j+1 (descending),
@@ -1894,96 +1892,6 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(const ChainTask* __rest
             xl[t.r0 + o + (int64_t)(c0 + r) * m] = (double)acc[v];
             xsol[sidx[v] + (int64_t)(c0 + r) * ldsol] = (double)acc[v];
          }
-      }
-   }
-}
-
-// Forward contribution blocks of the chained fronts: x_r -= L_rk y_k over
-// all nb diagonal blocks, y read from the exchange buffer (complete after
-// the level's chained launch; contiguous per block, entries outside the
-// block's rows or the chunk's columns hold kYEmpty and are replaced by 0). Two workgroups of 4 waves per task, 64 rows
-// each (more workgroups resident than one of 8 waves); the tile and the y
-// block of step s + 1 are loaded while step s is computed. Same operand
-// layouts and k order as k_chain's forward tiles.
-constexpr int kFcbThreads = 256;
-
-template <typename T>
-__global__ __launch_bounds__(kFcbThreads) void k_fcb(const ChainTask* __restrict__ tasks,
-                                                     const NodeDesc* __restrict__ nodes,
-                                                     const T* __restrict__ arena, double* __restrict__ xloc,
-                                                     const double* __restrict__ ybuf, int nrhs,
-                                                     const int4* __restrict__ cgch, const int* __restrict__ rlist,
-                                                     const int* __restrict__ cmap, const double* __restrict__ b,
-                                                     int ldb) {
-   constexpr int KU = kNB / 4;
-   constexpr int YU = kNB * 16 / kFcbThreads;
-   constexpr uint32_t ES = sizeof(T);
-   typedef typename Mfma<T>::acc_t acc_t;
-   __shared__ T ys[kNB * 16];
-   __shared__ double xg[kNB * 16];
-   const ChainTask t = tasks[blockIdx.x >> 1];
-   const NodeDesc nd = nodes[t.node];
-   const int half = (blockIdx.x & 1) * 64;
-   const int m = nd.m, n = nd.n, ld = nd.ld, nb = t.nb;
-   const int rows = min(64, t.r1 - t.r0 - half);  // rows of this workgroup (may be <= 0)
-   const int r0 = t.r0 + half;
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-   const int r = lane & 15, kq = lane >> 4;
-   const int i = wave * 16 + r;
-   if (rows <= 0) return;
-   double* xl = xloc + nd.solve_off * nrhs;
-   const int nch = (nrhs + 15) / 16;
-   const uint32_t fbytes = (uint32_t)n * (uint32_t)ld * ES;
-   const __amdgpu_buffer_rsrc_t frs = buf_rsrc(arena + nd.off, fbytes);
-   T av[2][KU];
-   double yv[2][YU];
-   int nc = 0, c0 = 0, ci = 0;
-   auto load = [&](auto slot, int s) {
-      constexpr int S = decltype(slot)::value;
-      const int kb0 = s * kNB, kbk = min(kNB, n - kb0);
-      uint32_t lo = (uint32_t)(r0 + i + (kb0 + kq) * ld) * ES;
-      const uint32_t step = (uint32_t)(4 * ld) * ES;
-      if (i >= rows) lo = fbytes;
-#pragma unroll
-      for (int u = 0; u < KU; ++u) av[S][u] = buf_load(frs, lo + (uint32_t)u * step, (T*)nullptr);
-      const double* ysrc = ybuf + ((int64_t)(t.flag0 + s) * nch + ci) * (kNB * 16);
-#pragma unroll
-      for (int u = 0; u < YU; ++u) {
-         const int e = threadIdx.x + u * kFcbThreads, l = e >> 4, c = e & 15;
-         const double v = ysrc[e];
-         yv[S][u] = (l < kbk && c < nc) ? v : 0.0;
-      }
-   };
-   double xv[4];
-   auto step = [&](auto slot, int s) {
-      constexpr int S = decltype(slot)::value;
-      if (s + 1 < nb) load(std::integral_constant<int, 1 - S>{}, s + 1);
-      __syncthreads();  // every wave is done with the previous ys
-#pragma unroll
-      for (int u = 0; u < YU; ++u) ys[threadIdx.x + u * kFcbThreads] = (T)yv[S][u];
-      __syncthreads();
-      acc_t acc = acc_t{0, 0, 0, 0};
-#pragma unroll
-      for (int u = 0; u < KU; ++u) acc = Mfma<T>::op(av[S][u], ys[(4 * u + kq) * 16 + r], acc);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) xv[v] -= (double)acc[v];
-   };
-   for (c0 = 0, ci = 0; c0 < nrhs; c0 += 16, ++ci) {
-      nc = min(16, nrhs - c0);
-      // this workgroup's 64 rows of the block, gathered
-      __syncthreads();
-      chain_gather<kFcbThreads>(xg, t, nd, nodes, cgch, rlist, cmap, xloc, b, ldb, nrhs, c0, nc, half, half + rows);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) xv[v] = xg[(half + wave * 16 + Mfma<T>::row(lane, v)) * 16 + r];
-      load(std::integral_constant<int, 0>{}, 0);
-      for (int s = 0; s < nb; s += 2) {
-         step(std::integral_constant<int, 0>{}, s);
-         if (s + 1 < nb) step(std::integral_constant<int, 1>{}, s + 1);
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-         const int o = wave * 16 + Mfma<T>::row(lane, v);
-         if (o < rows && r < nc) xl[r0 + o + (int64_t)(c0 + r) * m] = xv[v];
       }
    }
 }

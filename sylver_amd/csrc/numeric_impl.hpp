@@ -66,7 +66,6 @@ enum LaunchKind {
    L_BSMALL,
    L_FCHAIN,  // level-wide forward solve of the level's larger fronts (k_chain)
    L_BCHAIN,
-   L_FCB,     // forward contribution blocks of the chained fronts (k_chain, no waits)
    L_CLEAR,   // exchange-buffer fill at the start of a chained sweep
    L_SMALL_ASM,  // one-workgroup fronts assembled (zero, A, children) and factored in one launch
    L_DFACT,  // diagonal blocks factored without their inverse (k_diag_factor), before L_STRIP
