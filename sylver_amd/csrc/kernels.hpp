@@ -871,9 +871,11 @@ __device__ inline int xcd_task(int b, int G) { return (b & 7) * (G >> 3) + (b >>
 template <typename T, int MODE, int TB, int TBN, typename TC, int NT = kThreads>
 __device__ __forceinline__ void gemm_tile(const GemmTask& t, const T* __restrict__ a, const T* __restrict__ b,
                                           TC* __restrict__ c, int ldc, bool vec) {
-   // FP64 four-wave tiles stage 32 columns of K (longer MFMA chains per stage);
-   // eight-wave ones 16, half the LDS (measured 0.7 ms faster at 80^3 FP64)
-   constexpr int BM = TB, BN = TBN, BK = sizeof(T) == 8 && NT == kThreads ? 32 : 16, PAD = 4;
+   // K staged per barrier: FP64 four-wave tiles 32 columns (longer MFMA chains
+   // per stage), eight-wave ones 16 (half the LDS); FP32 16, or 32 for the
+   // eight-wave 64-row tiles (so that each thread still moves whole vectors)
+   constexpr int BM = TB, BN = TBN, PAD = 4;
+   constexpr int BK = sizeof(T) == 8 ? (NT == kThreads ? 32 : 16) : (NT > kThreads && TB == 64 ? 32 : 16);
    constexpr int WGN = NT / 128;  // waves: 2 along the rows, NT / 128 along the columns
    constexpr int WM = BM / 2, WN = BN / WGN, FM = WM / 16, FN = WN / 16;
    typedef typename V16<T>::vt vt;

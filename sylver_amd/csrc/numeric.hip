@@ -1329,12 +1329,9 @@ void run_factor_program(const DevicePlan* P, DeviceFactors* F, hipStream_t s,
          case L_UPDATE_WIDE:
             if (L.kind == L_UPDATE_WIDE && bf16_wide)
                k_update_bf16<<<g, b, 0, s>>>(P->d_gemm + L.off, reinterpret_cast<float*>(arena));
-            else if (L.tile == 64) {
-               if constexpr (sizeof(T) == 8)
-                  k_update<T, 0, 64, 64, 2 * kThreads><<<g, dim3(2 * kThreads), 0, s>>>(P->d_gemm + L.off, arena, nullptr);
-               else
-                  k_update<T, 0, 64><<<g, b, 0, s>>>(P->d_gemm + L.off, arena, nullptr);
-            } else if constexpr (sizeof(T) == 4)
+            else if (L.tile == 64)
+               k_update<T, 0, 64, 64, 2 * kThreads><<<g, dim3(2 * kThreads), 0, s>>>(P->d_gemm + L.off, arena, nullptr);
+            else if constexpr (sizeof(T) == 4)
                k_update_far_f32<<<g, dim3(2 * kThreads), 0, s>>>(P->d_gemm + L.off, arena);
             else
                k_update<T, 0, 128, 128, 2 * kThreads><<<g, dim3(2 * kThreads), 0, s>>>(P->d_gemm + L.off, arena, nullptr);
