@@ -464,8 +464,11 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
                Launch& L = P->fprog.back();
                L.tile = part == 0 ? 64 : 128;
                if (part == 0) {
-                  // the next pair's columns were last touched by the side stream
-                  if (side_last >= 0) {
+                  // the next pair's columns were last touched by the side
+                  // stream (the far update of the pair before); the narrow
+                  // update after a pair's first block writes only that
+                  // pair's second block, which the far update never touches
+                  if (side_last >= 0 && !first_of_pair) {
                      L.wait_ev = side_last;
                      side_last = -1;
                   }
