@@ -690,8 +690,35 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
          for (int a = i0; a < i1; a += split) out.push_back(int4{c, a, std::min(i1, a + split), 0});
       }
    };
+   // A level's one-workgroup fronts and its larger fronts are independent
+   // (no front of a level is an ancestor of another): the small-front launch
+   // runs on the side stream beside the rest of the level, forked after the
+   // level before and joined before the level after.
+   auto overlap_small = [&](std::vector<Launch>& prog, size_t b0) {
+      if (b0 == 0 || prog[b0 - 1].stream != 0) return;
+      size_t si = prog.size();
+      int others = 0;
+      for (size_t k = b0; k < prog.size(); ++k) {
+         if (prog[k].kind == L_FSMALL || prog[k].kind == L_BSMALL)
+            si = k;
+         else
+            ++others;
+      }
+      if (si == prog.size() || others == 0) return;
+      if (prog[b0 - 1].rec_ev < 0) prog[b0 - 1].rec_ev = P->nevents++;
+      prog[si].stream = 1;
+      prog[si].wait_ev = prog[b0 - 1].rec_ev;
+      prog[si].rec_ev = P->nevents++;
+      Launch j;
+      j.kind = L_SYNC;
+      j.count = 0;
+      j.off = 0;
+      j.wait_ev = prog[si].rec_ev;
+      prog.push_back(j);
+   };
    for (int L = 0; L < nlev; ++L) {
       const std::vector<int>& lv = P->levels[L];
+      const size_t lb0 = P->fwdprog.size();
       int64_t g0 = P->stask.size();
       for (int i : lv) {
          const NodeDesc& d = P->hnodes[i];
@@ -764,9 +791,11 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
          push(P->fwdprog, L_FDIAG, t0, P->stask.size() - t0);
          mark_single(P->fwdprog, L_FDIAG, t0);
       }
+      overlap_small(P->fwdprog, lb0);
    }
    for (int L = nlev - 1; L >= 0; --L) {
       const std::vector<int>& lv = P->levels[L];
+      const size_t lb0 = P->bwdprog.size();
       int64_t g0 = P->stask.size();
       for (int i : lv) {
          const NodeDesc& d = P->hnodes[i];
@@ -844,6 +873,7 @@ DevicePlan* build_plan_host(const Analysis& a, int small_max, int wide_ncol, int
          push(P->bwdprog, L_BDIAG, t0, P->stask.size() - t0);
          mark_single(P->bwdprog, L_BDIAG, t0);
       }
+      overlap_small(P->bwdprog, lb0);
    }
 
    // Every flag a chained launch waits for must be set by a workgroup of
@@ -1251,8 +1281,19 @@ int64_t check_plan_indices(const DevicePlan* P, const Analysis& a, std::string* 
             default:
                fail("launch kind");
          }
-         if (factor && L.stream != 0 && L.stream != 1) fail("launch stream");
+         if (L.stream != 0 && L.stream != 1) fail("launch stream");
       }
+      // every side-stream launch records an event that a later launch of the
+      // main stream waits for (the program ends joined)
+      std::vector<int> pending;
+      for (const Launch& L : prog) {
+         if (L.wait_ev >= 0) pending.erase(std::remove(pending.begin(), pending.end(), L.wait_ev), pending.end());
+         if (L.stream == 1) {
+            if (L.rec_ev < 0) fail("side-stream launch without an event");
+            pending.push_back(L.rec_ev);
+         }
+      }
+      if (!factor && !pending.empty()) fail("side stream not joined");
    };
    check_launches(P->fprog, true);
    check_launches(P->fwdprog, false);
@@ -1374,7 +1415,7 @@ size_t small_lds(const Launch& L) {
 
 template <typename T>
 void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<Launch>& prog,
-                       const double* b, double* xsol, int nrhs, hipStream_t s) {
+                       const double* b, double* xsol, int nrhs, hipStream_t s_main) {
    const T* arena = static_cast<const T*>(F->arena);
    const T* dinv = static_cast<const T*>(F->dinv);
    const T* dinvT = static_cast<const T*>(F->dinvT);
@@ -1382,6 +1423,8 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
    for (const Launch& L : prog) {
       dim3 g(L.count), bl(kThreads);
       const SolveTask* tk = P->d_stask + L.off;
+      const hipStream_t s = L.stream ? F->stream2 : s_main;
+      if (L.wait_ev >= 0) HIPCHECK(hipStreamWaitEvent(s, F->xev[L.wait_ev], 0));
       switch (L.kind) {
          case L_FGATHER:
             k_fwd_gather<<<g, bl, 0, s>>>(tk, P->d_nodes, P->d_gch, P->d_rlist, P->d_cmap, b, n, F->d_xloc, nrhs);
@@ -1445,9 +1488,10 @@ void run_solve_program(const DevicePlan* P, DeviceFactors* F, const std::vector<
                                                                F->d_part_cb, F->d_chain, P->d_cgch, P->d_cmap,
                                                                nullptr, 0);
             break;
-         default:
+         default:  // L_SYNC
             break;
       }
+      if (L.rec_ev >= 0) HIPCHECK(hipEventRecord(F->xev[L.rec_ev], s));
    }
    HIPCHECK(hipGetLastError());
 }
